@@ -1,0 +1,153 @@
+"""Benchmark: textured-2DGS forward + backward iterations per second on one GPU.
+
+Workload (default): one 1920x1080 view, 500k textured Gaussians, SH degree 3,
+2M texels in area-proportional 1x1..16x16 grids, FP32, L2 loss against a
+seeded random target.  One step = SH colours, projection, binning (depth sort
++ tile sort + bin edges), splat records, raster forward, loss gradient, raster
+backward, gradient finalize, SH adjoint; inputs are resident in HBM before the
+timed region.  Prints one JSON line.
+
+Roofline: the slower of the two raster kernels, timed live with HIP events on
+the stream it runs on, against its algorithmic bytes per launch:
+  forward   I*(4+128) + T*8 + P*(56 + 4C) + X*C*4
+  backward  I*(4+128) + T*8 + P*(60 + 4C) + 2*X*C*4 + N*128
+(I intersects, T tiles, P pixels, X texels, C channels, N Gaussians).
+
+cpu_baseline: the FP32 C oracle (single thread, the reference's semantics) on a
+bounded sample of whole tiles of the same frame, forward + backward, scaled to
+the full frame by the sampled share of intersects.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "fwd+bwd iter/s, 1080p 500k textured Gaussians; BF16+batched vs FP32 plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", type=int, default=2)
+    ap.add_argument("--num-points", type=int, default=None)
+    ap.add_argument("--cpu-baseline-seconds", type=float, default=15.0,
+                    help="target CPU seconds for the oracle sample (0 disables)")
+    return ap.parse_args()
+
+
+def raster_bytes(kind, I, T, P, X, C, N):
+    if kind == "raster_forward":
+        return I * (4 + 128) + T * 8 + P * (56 + 4 * C) + X * C * 4
+    return I * (4 + 128) + T * 8 + P * (60 + 4 * C) + 2 * X * C * 4 + N * 128
+
+
+def cpu_baseline(scene, step, target_seconds):
+    """Time the FP32 oracle on a sample of tiles and scale to one full frame."""
+    import oracle as O
+
+    O.build()
+    ids = step.ids[: step.num_intersects].cpu().numpy()
+    bins = step.tile_bins.cpu().numpy()
+    colors = step.colors.cpu().numpy()
+    counts = (bins[:, 1] - bins[:, 0]).astype(np.int64)
+    total = int(counts.sum())
+    rng = np.random.default_rng(0)
+    order = rng.permutation(len(counts))
+    H, W, C = scene.img_h, scene.img_w, scene.texture.shape[1]
+    sc = O.Scene(means=scene.means, scales=scene.scales, quats=scene.quats, colors=colors,
+                 opacities=scene.opacities, uv0=scene.uv0, umap=scene.umap, vmap=scene.vmap,
+                 texture=scene.texture, texture_dims=scene.texture_dims, viewmat=scene.viewmat,
+                 c2w=scene.c2w, background=scene.background, fx=scene.fx, fy=scene.fy,
+                 cx=scene.cx, cy=scene.cy, img_h=H, img_w=W, block_width=scene.block_width,
+                 settings=scene.settings, ids_sorted=ids, tile_bins=bins)
+    v = np.zeros((H, W, 3), np.float32)
+    vt = np.zeros((H, W, C), np.float32)
+    z = np.zeros((H, W), np.float32)
+    # grow the sample until the measured time reaches the target
+    share, chosen, elapsed = 0.002, 0, 0.0
+    while True:
+        k = max(1, int(len(order) * share))
+        mask = np.zeros(len(counts), np.uint8)
+        mask[order[:k]] = 1
+        sc.tile_mask = mask
+        t0 = time.perf_counter()
+        f = O.texture_forward(sc, "f32")
+        O.texture_backward(sc, f, v, z, z, z, vt, v, "f32")
+        elapsed = time.perf_counter() - t0
+        chosen = int(counts[order[:k]].sum())
+        if elapsed >= 0.3 * target_seconds or share >= 1.0 or chosen == total:
+            break
+        share = min(1.0, share * max(2.0, 0.5 * target_seconds / max(elapsed, 1e-3)))
+    frame_s = elapsed * total / max(chosen, 1)
+    return {"value": 1.0 / frame_s, "unit": "iter/s", "cores": 1, "kind": "port",
+            "sample": f"{k} of {len(counts)} tiles ({chosen} of {total} intersects), "
+                      f"FP32 oracle fwd+bwd {elapsed:.1f} s, scaled by intersect share"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path "
+              "is not built: every box has a single MI355X.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from gstex_cuda_amd import synthetic
+    from gstex_cuda_amd.pipeline import StageTimer, scene_to_step
+
+    dev = torch.device("cuda:0")
+    scene = synthetic.make_scene(args.workload, num_points=args.num_points)
+    step = scene_to_step(scene, dev)
+    target = torch.from_numpy(scene.target).to(dev)
+    for _ in range(args.warmup):
+        step.step(target)
+    torch.cuda.synchronize()
+    timer = StageTimer()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step.step(target, timer)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = args.steps / elapsed
+
+    I, N = step.num_intersects, step.n
+    T = step.tiles[0] * step.tiles[1]
+    P = scene.img_h * scene.img_w
+    X, C = scene.texture.shape
+    kern = {k: timer.mean_ms(k) for k in ("raster_forward", "raster_backward")}
+    dom = max(kern, key=kern.get)
+    nbytes = raster_bytes(dom, I, T, P, X, C, N)
+    achieved = nbytes / (kern[dom] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None, "algorithmic_bytes": int(nbytes),
+            "kernel_ms": {k: round(v, 4) for k, v in kern.items()}}
+    cpu = None
+    if args.cpu_baseline_seconds > 0:
+        cpu = cpu_baseline(scene, step, args.cpu_baseline_seconds)
+    line = {"metric": METRIC, "value": round(value, 4), "unit": "iter/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(1e3 * elapsed / args.steps, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"single view {scene.img_w}x{scene.img_h}, {N} Gaussians, "
+                                   f"SH degree {max(scene.sh_degree, 0)}, {X} texels, "
+                                   f"FP32 plain path",
+                       "num_gaussians": N, "resolution": f"{scene.img_w}x{scene.img_h}",
+                       "texels": X, "channels": C, "intersects": I, "block_width": 16},
+            "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

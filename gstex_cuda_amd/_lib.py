@@ -1,0 +1,104 @@
+"""ctypes binding of ``libgstex_hip.so`` (the C ABI declared in include/gstex_hip.h).
+
+The library is built in-tree by ``gstex_cuda_amd/csrc/Makefile`` (see
+``gstex_cuda_amd.build``).  Loading fails loudly: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgstex_hip.so")
+
+_lock = threading.Lock()
+_lib = None
+
+P = C.c_void_p
+I = C.c_int
+F = C.c_float
+Z = C.c_size_t
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "gstex_last_error": (C.c_char_p, []),
+    "gstex_abi_version": (I, []),
+    "gstex_get_aabb_2d": (I, [I, P, P, F, P, P, F, F, F, F, P, P, P]),
+    "gstex_num_tiles_hit": (I, [I, P, P, I, I, I, P, P]),
+    "gstex_project_gaussians": (I, [I, P, P, F, P, P, F, F, F, F, I, I, I, P, P, P, P, P]),
+    "gstex_project_points": (I, [I, P, P, F, F, F, F, I, P, P, P]),
+    "gstex_compute_sh_forward": (I, [I, I, I, P, P, P, P]),
+    "gstex_compute_sh_backward": (I, [I, I, I, P, P, P, P]),
+    "gstex_map_gaussian_to_intersects": (I, [I, I, P, P, P, P, I, I, I, I, P, P, P]),
+    "gstex_sort_intersects_workspace_size": (Z, [I]),
+    "gstex_sort_intersects": (I, [I, I, P, P, P, P, P, Z, P]),
+    "gstex_get_tile_bin_edges": (I, [I, P, I, I, P, P]),
+    "gstex_bin_gaussians_workspace_size": (Z, [I, I]),
+    "gstex_bin_gaussians": (I, [I, I, P, P, P, P, I, I, I, P, P, P, Z, P]),
+    "gstex_texture_forward_workspace_size": (Z, [I]),
+    "gstex_texture_forward": (I, [I, I, I, I, I, I, I, I,  # tiles, bw, img, texture_info
+                                  P, P, I, P,  # dims, ids, num_ids, bins
+                                  P, P, P, P, I, F, P, P, P, P, P, I,  # params .. texture, X
+                                  P, P, F, F, F, F, I, P,  # viewmat, c2w, intrinsics, settings, bg
+                                  P, P, P, P, P, P, P, P, P,  # nine outputs
+                                  P, Z, P]),
+    "gstex_texture_backward_workspace_size": (Z, [I]),
+    "gstex_texture_backward": (I, [I, I, I, I, I, I,  # H, W, bw, texture_info
+                                   P, P, I, P,
+                                   P, P, P, P, I, F, P, P, P, P, P, I,
+                                   P, P, F, F, F, F, I, P,
+                                   P, P, P, P,  # final_Ts, final_idx, depth_idx, final_s
+                                   P, P, P, P, P, P,  # six output gradients
+                                   P, P, P, P, P, P, P, P, P,  # nine parameter gradients
+                                   P, Z, P]),
+    "gstex_splat_records_size": (Z, [I]),
+    "gstex_build_splat_records": (I, [I, P, P, F, P, P, P, P, P, P, P, P, F, F, F, F, P, P]),
+    "gstex_raster_forward": (I, [I, I, I, I, I, P, I, P, I, P, P, I, P, P, F, F, F, F, P,
+                                 P, P, P, P, P, P, P, P, P, P]),
+    "gstex_raster_backward": (I, [I, I, I, I, I, P, I, P, I, P, P, I, P, P, F, F, F, F, P,
+                                  P, P, P, P,  # final_Ts, final_idx, depth_idx, final_s
+                                  P, P, P, P, P, P,  # six output gradients
+                                  P, P, P]),  # grad_records, v_texture, stream
+    "gstex_finalize_gradients": (I, [I, P, P, P, F, P, P, F, F, P, P, P, P, P, P, P, P, P]),
+    "gstex_fill_f32": (I, [P, Z, F, P]),
+    "gstex_texture_sample_forward": (I, [I, I, I, P, P, I, P, I, P, P]),
+    "gstex_texture_sample_backward": (I, [I, I, I, P, P, I, P, I, P, P, P]),
+}
+
+
+class GstexError(RuntimeError):
+    """A C-ABI call returned a nonzero status."""
+
+
+def lib():
+    """Load ``libgstex_hip.so`` (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise ImportError(
+                    f"{LIB_PATH} is missing: build it with `make -C gstex_cuda_amd/csrc` "
+                    "(or __graft_entry__.build()); there is no CPU fallback")
+            handle = C.CDLL(LIB_PATH)
+            for name, (res, args) in _SIGNATURES.items():
+                fn = getattr(handle, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = handle
+    return _lib
+
+
+def call(name: str, *args):
+    """Invoke ``name`` and raise GstexError with the library's message on failure."""
+    rc = getattr(lib(), name)(*args)
+    if rc != 0:
+        msg = lib().gstex_last_error().decode(errors="replace")
+        raise GstexError(msg or f"{name} failed with status {rc}")
+    return rc
+
+
+def exported_symbols():
+    return list(_SIGNATURES)

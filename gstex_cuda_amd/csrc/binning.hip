@@ -1,0 +1,473 @@
+// Tile binning: intersect keys, a stable on-device LSD radix sort, bin edges,
+// and the depth-first binning pipeline used by the fused render path.
+//
+// Reference semantics: forward.cu:13-98, helpers.cuh:37-51,75-92,
+// utils.py:40-59,106-162 (torch.sort is replaced by the stable radix sort).
+#include "gstex_common.hpp"
+#include "gstex_kernels.hpp"
+
+namespace gstex {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;  // per thread per block
+constexpr int kSortTile = kSortThreads * kSortItems;
+
+// ------------------------------------------------------------------ fills
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(T* __restrict__ p, size_t n, T v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+template <typename T>
+hipError_t fill(T* p, size_t n, T v, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    k_fill<T><<<(unsigned)blocks, 256, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t fill_f32(float* p, size_t n, float v, hipStream_t s) { return fill<float>(p, n, v, s); }
+hipError_t fill_i32(int* p, size_t n, int v, hipStream_t s) { return fill<int>(p, n, v, s); }
+
+// ------------------------------------------------------------------ tile rectangle
+// get_tile_bbox (helpers.cuh:37-51,75-92): [min, max) in tile units, clamped.
+__device__ __forceinline__ void tile_rect(float cx, float cy, float ex, float ey, int bw,
+                                          int tiles_x, int tiles_y, int& x0, int& y0, int& x1,
+                                          int& y1) {
+    const float fbw = (float)bw;
+    const float tcx = cx / fbw, tcy = cy / fbw, tex = ex / fbw, tey = ey / fbw;
+    x0 = min(max(0, f2i_trunc(tcx - tex)), tiles_x);
+    x1 = min(max(0, f2i_trunc(tcx + tex + 1.f)), tiles_x);
+    y0 = min(max(0, f2i_trunc(tcy - tey)), tiles_y);
+    y1 = min(max(0, f2i_trunc(tcy + tey + 1.f)), tiles_y);
+}
+
+__device__ __forceinline__ bool writes_intersects(float ex, float ey) {
+    // forward.cu:32 compares float extents with the double literal 1e-4.
+    return !((double)ex <= 1e-4 && (double)ey <= 1e-4);
+}
+
+// ------------------------------------------------------------------ reference-layout keys
+// One lane per Gaussian, writing (tile << 32 | depth bits, id) row-major over
+// its tile rectangle from cum_tiles_hit[g-1]; outputs are pre-zeroed so slots
+// a Gaussian counts but does not write stay (0, 0).  Writes past the
+// Gaussian's own count window are dropped rather than spilling into the next
+// Gaussian's slots.
+__global__ __launch_bounds__(256) void k_map_intersects(
+    int n, int num_isects, const float* __restrict__ centers, const float* __restrict__ extents,
+    const float* __restrict__ depths, const int* __restrict__ cum, int tiles_x, int tiles_y,
+    int bw, int64_t* __restrict__ keys, int* __restrict__ ids) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const float2 c = reinterpret_cast<const float2*>(centers)[g];
+    const float2 e = reinterpret_cast<const float2*>(extents)[g];
+    if (!writes_intersects(e.x, e.y)) return;
+    int x0, y0, x1, y1;
+    tile_rect(c.x, c.y, e.x, e.y, bw, tiles_x, tiles_y, x0, y0, x1, y1);
+    int cur = g == 0 ? 0 : cum[g - 1];
+    const int end = min(cum[g], num_isects);
+    const int64_t did = (int64_t)__float_as_int(depths[g]);
+    for (int ty = y0; ty < y1; ++ty)
+        for (int tx = x0; tx < x1; ++tx) {
+            if (cur >= 0 && cur < end) {
+                keys[cur] = ((int64_t)(ty * tiles_x + tx) << 32) | did;
+                ids[cur] = g;
+            }
+            ++cur;
+        }
+}
+
+// forward.cu:76-98 on sorted 64-bit keys (tile in the high word).
+__global__ __launch_bounds__(256) void k_bin_edges64(int m, const int64_t* __restrict__ keys,
+                                                     int num_tiles, int* __restrict__ bins) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int cur = (int)(keys[i] >> 32);
+    const bool cur_ok = cur >= 0 && cur < num_tiles;
+    if (i == 0 && cur_ok) bins[2 * cur] = 0;
+    if (i == m - 1 && cur_ok) bins[2 * cur + 1] = m;
+    if (i == 0) return;
+    const int prev = (int)(keys[i - 1] >> 32);
+    if (prev != cur) {
+        if (prev >= 0 && prev < num_tiles) bins[2 * prev + 1] = i;
+        if (cur_ok) bins[2 * cur] = i;
+    }
+}
+
+// Same edges on sorted 32-bit tile ids.
+__global__ __launch_bounds__(256) void k_bin_edges32(int m, const uint32_t* __restrict__ tiles,
+                                                     int num_tiles, int* __restrict__ bins) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int cur = (int)tiles[i];
+    const bool cur_ok = cur >= 0 && cur < num_tiles;
+    if (i == 0 && cur_ok) bins[2 * cur] = 0;
+    if (i == m - 1 && cur_ok) bins[2 * cur + 1] = m;
+    if (i == 0) return;
+    const int prev = (int)tiles[i - 1];
+    if (prev != cur) {
+        if (prev >= 0 && prev < num_tiles) bins[2 * prev + 1] = i;
+        if (cur_ok) bins[2 * cur] = i;
+    }
+}
+
+// ------------------------------------------------------------------ radix sort
+template <typename K>
+__device__ __forceinline__ uint32_t digit_of(K key, int shift, uint32_t mask) {
+    return (uint32_t)(key >> shift) & mask;
+}
+
+// Per-block digit histogram of a 4096-item chunk, stored digit-major
+// (hist[d * nblocks + b]) so one exclusive scan yields scatter bases.
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(int n, const K* __restrict__ keys,
+                                                             K flip, int shift, int bits,
+                                                             uint32_t* __restrict__ hist,
+                                                             int nblocks) {
+    __shared__ uint32_t cnt[256];
+    const int tid = threadIdx.x;
+    cnt[tid] = 0;
+    __syncthreads();
+    const uint32_t mask = (1u << bits) - 1u;
+    const size_t base = (size_t)blockIdx.x * kSortTile;
+#pragma unroll 4
+    for (int r = 0; r < kSortItems; ++r) {
+        const size_t idx = base + (size_t)r * kSortThreads + tid;
+        if (idx < (size_t)n) atomicAdd(&cnt[digit_of(keys[idx] ^ flip, shift, mask)], 1u);
+    }
+    __syncthreads();
+    if (tid < (1 << bits)) hist[(size_t)tid * nblocks + blockIdx.x] = cnt[tid];
+}
+
+// Exclusive scan of a uint32 array in place by one 1024-lane workgroup
+// (sequential segments per lane, then a workgroup scan of segment sums).
+__global__ __launch_bounds__(1024) void k_scan_single(uint32_t* __restrict__ a, int n,
+                                                      uint32_t* __restrict__ total) {
+    __shared__ uint32_t sums[1024];
+    const int tid = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int lo = min(tid * per, n), hi = min(lo + per, n);
+    uint32_t s = 0;
+    for (int i = lo; i < hi; ++i) s += a[i];
+    sums[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t v = tid >= off ? sums[tid - off] : 0u;
+        __syncthreads();
+        sums[tid] += v;
+        __syncthreads();
+    }
+    uint32_t run = tid == 0 ? 0u : sums[tid - 1];
+    for (int i = lo; i < hi; ++i) {
+        uint32_t v = a[i];
+        a[i] = run;
+        run += v;
+    }
+    if (tid == 1023 && total) *total = sums[1023];
+}
+
+// Stable scatter: each round ranks 256 items (one per lane) with wave ballots,
+// combines the four waves' per-digit counts in LDS, and writes each item to
+// base[digit] + rank.  Item order inside a block follows the input order.
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
+    int n, const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+    K* __restrict__ keys_out, uint32_t* __restrict__ vals_out, K flip, int shift, int bits,
+    const uint32_t* __restrict__ hist, int nblocks) {
+    __shared__ uint32_t base_cnt[256];
+    __shared__ uint32_t wave_cnt[kSortThreads / kWave][256];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const uint32_t mask = (1u << bits) - 1u;
+    const int ndig = 1 << bits;
+    if (tid < ndig) base_cnt[tid] = hist[(size_t)tid * nblocks + blockIdx.x];
+    const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    const size_t base = (size_t)blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortItems; ++r) {
+        const size_t idx = base + (size_t)r * kSortThreads + tid;
+        const bool valid = idx < (size_t)n;
+        K key = 0;
+        uint32_t val = 0, d = 0;
+        if (valid) {
+            key = keys_in[idx];
+            val = vals_in[idx];
+            d = digit_of(key ^ flip, shift, mask);
+        }
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < bits; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+        const uint32_t tot = (uint32_t)__popcll(peers);
+#pragma unroll
+        for (int w = 0; w < kSortThreads / kWave; ++w) wave_cnt[w][tid] = 0;
+        __syncthreads();
+        if (valid && rank == 0) wave_cnt[wave][d] = tot;
+        __syncthreads();
+        uint32_t pos = 0;
+        if (valid) {
+            pos = base_cnt[d] + rank;
+            for (int w = 0; w < wave; ++w) pos += wave_cnt[w][d];
+        }
+        __syncthreads();
+        if (tid < ndig) {
+            uint32_t add = 0;
+#pragma unroll
+            for (int w = 0; w < kSortThreads / kWave; ++w) add += wave_cnt[w][tid];
+            base_cnt[tid] += add;
+        }
+        if (valid) {
+            keys_out[pos] = key;
+            vals_out[pos] = val;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename K>
+size_t radix_workspace_bytes(int n) {
+    const size_t nb = (size_t)(n + kSortTile - 1) / kSortTile;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return up(sizeof(K) * (size_t)n) + up(sizeof(uint32_t) * (size_t)n) + up(sizeof(uint32_t) * 256 * nb) + 256;
+}
+
+// Stable LSD sort of (key, value) pairs over key bits [0, end_bit).  With all 64
+// bits of a 64-bit key the sign bit is flipped in the digits, giving the
+// signed (int64) order.
+template <typename K>
+hipError_t radix_sort_pairs(int n, const K* keys_in, const uint32_t* vals_in, K* keys_out,
+                            uint32_t* vals_out, int end_bit, void* ws, size_t ws_bytes,
+                            hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (ws_bytes < radix_workspace_bytes<K>(n)) return hipErrorInvalidValue;
+    const int nblocks = (n + kSortTile - 1) / kSortTile;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    char* p = (char*)ws;
+    K* tk = (K*)p;
+    p += up(sizeof(K) * (size_t)n);
+    uint32_t* tv = (uint32_t*)p;
+    p += up(sizeof(uint32_t) * (size_t)n);
+    uint32_t* hist = (uint32_t*)p;
+    const int passes = (end_bit + 7) / 8;
+    if (passes == 0) {
+        hipError_t e = hipMemcpyAsync(keys_out, keys_in, sizeof(K) * (size_t)n,
+                                      hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
+        return hipMemcpyAsync(vals_out, vals_in, sizeof(uint32_t) * (size_t)n,
+                              hipMemcpyDeviceToDevice, s);
+    }
+    const int bits_per = (end_bit + passes - 1) / passes;  // balanced digit widths <= 8
+    const K flip = (sizeof(K) == 8 && end_bit == 64) ? (K)((K)1 << (8 * sizeof(K) - 1)) : (K)0;
+    const K* sk = keys_in;
+    const uint32_t* sv = vals_in;
+    for (int pass = 0; pass < passes; ++pass) {
+        const int shift = pass * bits_per;
+        const int bits = min(bits_per, end_bit - shift);
+        const bool to_out = ((passes - 1 - pass) % 2) == 0;
+        K* dk = to_out ? keys_out : tk;
+        uint32_t* dv = to_out ? vals_out : tv;
+        k_radix_hist<K><<<nblocks, kSortThreads, 0, s>>>(n, sk, flip, shift, bits, hist, nblocks);
+        k_scan_single<<<1, 1024, 0, s>>>(hist, (1 << bits) * nblocks, nullptr);
+        k_radix_scatter<K><<<nblocks, kSortThreads, 0, s>>>(n, sk, sv, dk, dv, flip, shift, bits,
+                                                             hist, nblocks);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        sk = dk;
+        sv = dv;
+    }
+    return hipSuccess;
+}
+
+size_t sort_isects_workspace_bytes(int n) { return radix_workspace_bytes<uint64_t>(n); }
+
+hipError_t sort_isects(int n, const int64_t* keys_in, const int* ids_in, int64_t* keys_out,
+                       int* ids_out, int end_bit, void* ws, size_t ws_bytes, hipStream_t s) {
+    return radix_sort_pairs<uint64_t>(n, (const uint64_t*)keys_in, (const uint32_t*)ids_in,
+                                      (uint64_t*)keys_out, (uint32_t*)ids_out, end_bit, ws,
+                                      ws_bytes, s);
+}
+
+hipError_t map_intersects(int n, int num_isects, const float* centers, const float* extents,
+                          const float* depths, const int* cum, int tiles_x, int tiles_y, int bw,
+                          int64_t* keys, int* ids, hipStream_t s) {
+    hipError_t e = fill<int64_t>(keys, (size_t)num_isects, (int64_t)0, s);
+    if (e != hipSuccess) return e;
+    e = fill<int>(ids, (size_t)num_isects, 0, s);
+    if (e != hipSuccess) return e;
+    if (n <= 0) return hipSuccess;
+    k_map_intersects<<<(n + 255) / 256, 256, 0, s>>>(n, num_isects, centers, extents, depths, cum,
+                                                    tiles_x, tiles_y, bw, keys, ids);
+    return hipGetLastError();
+}
+
+hipError_t tile_bin_edges(int m, const int64_t* keys_sorted, int num_tiles, int* bins,
+                          hipStream_t s) {
+    hipError_t e = fill<int>(bins, (size_t)num_tiles * 2, 0, s);
+    if (e != hipSuccess || m <= 0) return e;
+    k_bin_edges64<<<(m + 255) / 256, 256, 0, s>>>(m, keys_sorted, num_tiles, bins);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ depth-first binning
+// Equivalent to sorting the reference keys: a stable sort of (tile << 32 |
+// depth bits) over Gaussian-major input orders each tile's entries by
+// (depth bits, Gaussian id), and puts the never-written zero slots (key 0,
+// id 0) first in tile 0.  So: sort the N Gaussians by depth once, emit their
+// tiles in that order after the zero slots, and stable-sort by tile only.
+
+// Per Gaussian: entries it writes (rectangle, capped by its count) and its
+// depth key; Gaussians that write nothing sort last.
+__global__ __launch_bounds__(256) void k_bin_prepare(
+    int n, const float* __restrict__ centers, const float* __restrict__ extents,
+    const float* __restrict__ depths, const int* __restrict__ counts, int tiles_x, int tiles_y,
+    int bw, uint32_t* __restrict__ depth_keys, uint32_t* __restrict__ order,
+    int* __restrict__ written) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const float2 c = reinterpret_cast<const float2*>(centers)[g];
+    const float2 e = reinterpret_cast<const float2*>(extents)[g];
+    int w = 0;
+    if (writes_intersects(e.x, e.y)) {
+        int x0, y0, x1, y1;
+        tile_rect(c.x, c.y, e.x, e.y, bw, tiles_x, tiles_y, x0, y0, x1, y1);
+        w = min((x1 - x0) * (y1 - y0), max(counts[g], 0));
+    }
+    written[g] = w;
+    depth_keys[g] = w > 0 ? (uint32_t)__float_as_uint(depths[g]) : 0xFFFFFFFFu;
+    order[g] = (uint32_t)g;
+}
+
+__global__ __launch_bounds__(256) void k_gather_written(int n, const uint32_t* __restrict__ order,
+                                                        const int* __restrict__ written,
+                                                        uint32_t* __restrict__ out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    out[p] = (uint32_t)written[order[p]];
+}
+
+// Emit (tile, id) for every depth-ordered Gaussian after the zero slots.
+// zero slots occupy [0, m - total_written) and read as (tile 0, id 0).
+__global__ __launch_bounds__(256) void k_bin_emit(
+    int n, int m, const uint32_t* __restrict__ order, const uint32_t* __restrict__ offsets,
+    const uint32_t* __restrict__ total_written, const float* __restrict__ centers,
+    const float* __restrict__ extents, int tiles_x, int tiles_y, int bw, const int* __restrict__ written,
+    uint32_t* __restrict__ tiles_out, uint32_t* __restrict__ ids_out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t g = order[p];
+    const int w = written[g];
+    if (w <= 0) return;
+    const uint32_t tw = *total_written;
+    const uint32_t zbase = tw <= (uint32_t)m ? (uint32_t)m - tw : 0u;
+    const float2 c = reinterpret_cast<const float2*>(centers)[g];
+    const float2 e = reinterpret_cast<const float2*>(extents)[g];
+    int x0, y0, x1, y1;
+    tile_rect(c.x, c.y, e.x, e.y, bw, tiles_x, tiles_y, x0, y0, x1, y1);
+    uint32_t dst = zbase + offsets[p];
+    int k = 0;
+    // dst < m holds when num_intersects = sum(num_tiles_hit); the bound keeps an
+    // inconsistent caller-supplied count from writing past the outputs
+    for (int ty = y0; ty < y1 && k < w; ++ty)
+        for (int tx = x0; tx < x1 && k < w && dst < (uint32_t)m; ++tx, ++k, ++dst) {
+            tiles_out[dst] = (uint32_t)(ty * tiles_x + tx);
+            ids_out[dst] = g;
+        }
+}
+
+__global__ __launch_bounds__(256) void k_zero_slots(int m, const uint32_t* __restrict__ total_written,
+                                                    uint32_t* __restrict__ tiles_out,
+                                                    uint32_t* __restrict__ ids_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t tw = *total_written;
+    const uint32_t z = tw <= (uint32_t)m ? (uint32_t)m - tw : 0u;
+    if ((uint32_t)i < z) {
+        tiles_out[i] = 0u;
+        ids_out[i] = 0u;
+    }
+}
+
+static inline int bits_for(int v) {  // bits needed to represent 0..v-1
+    int b = 0;
+    while ((1ll << b) < (long long)v) ++b;
+    return b;
+}
+
+struct BinLayout {
+    size_t off[8];
+    size_t total;
+};
+
+static BinLayout bin_layout(int n, int m) {
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    BinLayout L;
+    size_t sizes[8] = {
+        up(4 * (size_t)n) * 2,  // depth keys in/out
+        up(4 * (size_t)n) * 2,  // order in/out
+        up(4 * (size_t)n),      // written
+        up(4 * (size_t)n + 4),  // written gathered / offsets (+ total)
+        up(4 * (size_t)m) * 2,  // tile keys in/out
+        up(4 * (size_t)m),      // ids in (pre-sort)
+        up(std::max(radix_workspace_bytes<uint32_t>(n), radix_workspace_bytes<uint32_t>(m))),
+        256,
+    };
+    size_t acc = 0;
+    for (int i = 0; i < 8; ++i) {
+        L.off[i] = acc;
+        acc += sizes[i];
+    }
+    L.total = acc;
+    return L;
+}
+
+size_t bin_gaussians_workspace_bytes(int n, int m) { return bin_layout(n, m).total; }
+
+hipError_t bin_gaussians(int n, int m, const float* centers, const float* extents,
+                         const float* depths, const int* counts, int tiles_x, int tiles_y, int bw,
+                         int* ids_sorted, int* tile_bins, void* ws, size_t ws_bytes,
+                         hipStream_t s) {
+    const int num_tiles = tiles_x * tiles_y;
+    hipError_t e = fill<int>(tile_bins, (size_t)num_tiles * 2, 0, s);
+    if (e != hipSuccess || m <= 0 || n <= 0) return e;
+    BinLayout L = bin_layout(n, m);
+    if (ws_bytes < L.total) return hipErrorInvalidValue;
+    char* b = (char*)ws;
+    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    uint32_t* dkey = (uint32_t*)(b + L.off[0]);
+    uint32_t* dkey2 = (uint32_t*)(b + L.off[0] + up(4 * (size_t)n));
+    uint32_t* ord = (uint32_t*)(b + L.off[1]);
+    uint32_t* ord2 = (uint32_t*)(b + L.off[1] + up(4 * (size_t)n));
+    int* written = (int*)(b + L.off[2]);
+    uint32_t* offs = (uint32_t*)(b + L.off[3]);
+    uint32_t* total = offs + n;
+    uint32_t* tkey = (uint32_t*)(b + L.off[4]);
+    uint32_t* tkey2 = (uint32_t*)(b + L.off[4] + up(4 * (size_t)m));
+    uint32_t* tid_in = (uint32_t*)(b + L.off[5]);
+    void* rws = (void*)(b + L.off[6]);
+    const size_t rws_bytes = up(std::max(radix_workspace_bytes<uint32_t>(n),
+                                         radix_workspace_bytes<uint32_t>(m)));
+    const int gn = (n + 255) / 256, gm = (m + 255) / 256;
+
+    k_bin_prepare<<<gn, 256, 0, s>>>(n, centers, extents, depths, counts, tiles_x, tiles_y, bw,
+                                     dkey, ord, written);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    e = radix_sort_pairs<uint32_t>(n, dkey, ord, dkey2, ord2, 32, rws, rws_bytes, s);
+    if (e != hipSuccess) return e;
+    k_gather_written<<<gn, 256, 0, s>>>(n, ord2, written, offs);
+    k_scan_single<<<1, 1024, 0, s>>>(offs, n, total);
+    k_zero_slots<<<gm, 256, 0, s>>>(m, total, tkey, tid_in);
+    k_bin_emit<<<gn, 256, 0, s>>>(n, m, ord2, offs, total, centers, extents, tiles_x, tiles_y, bw,
+                                  written, tkey, tid_in);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    e = radix_sort_pairs<uint32_t>(m, tkey, tid_in, tkey2, (uint32_t*)ids_sorted,
+                                   bits_for(num_tiles), rws, rws_bytes, s);
+    if (e != hipSuccess) return e;
+    k_bin_edges32<<<gm, 256, 0, s>>>(m, tkey2, num_tiles, tile_bins);
+    return hipGetLastError();
+}
+
+}  // namespace gstex

@@ -1,0 +1,479 @@
+// Exported C ABI (declared in include/gstex_hip.h): argument validation,
+// then the gfx950 launchers.  No entry point allocates device memory.
+#include <stdarg.h>
+#include <stdio.h>
+
+#include "../../include/gstex_hip.h"
+#include "gstex_common.hpp"
+#include "gstex_kernels.hpp"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int check_hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return GSTEX_OK;
+    if (e == hipErrorInvalidValue)
+        return fail(GSTEX_ERR_SPACE, "%s: invalid value (workspace too small?): %s", what,
+                    hipGetErrorString(e));
+    return fail(GSTEX_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+#define REQUIRE(cond, ...)                              \
+    do {                                                \
+        if (!(cond)) return fail(GSTEX_ERR_ARG, __VA_ARGS__); \
+    } while (0)
+
+int tiles_of(int extent, int bw) { return (extent + bw - 1) / bw; }
+
+int check_raster_common(int tiles_x, int tiles_y, int block_width, int img_width, int img_height,
+                        int num_probs, int channels, int num_points, int num_ids, int num_texels,
+                        int settings) {
+    REQUIRE(block_width >= 2 && block_width <= 16,
+            "block_width must be between 2 and 16, got %d", block_width);
+    REQUIRE(img_width > 0 && img_height > 0, "image size must be positive, got %dx%d", img_width,
+            img_height);
+    REQUIRE(tiles_x == tiles_of(img_width, block_width) &&
+                tiles_y == tiles_of(img_height, block_width),
+            "tile_bounds (%d, %d) do not match img_size (%d, %d) and block width %d", tiles_x,
+            tiles_y, img_width, img_height, block_width);
+    REQUIRE(num_probs == 1, "texture_info[1] (num_probs) must be 1, got %d", num_probs);
+    REQUIRE(channels >= 0 && channels <= 64, "texture channels must be in [0, 64], got %d",
+            channels);
+    REQUIRE(num_points >= 0 && num_ids >= 0, "negative sizes");
+    REQUIRE(channels == 0 || num_texels > 0, "texture has no rows");
+    REQUIRE(!(settings & gstex::kSetAlphaViz),
+            "settings bit 16 (alpha visualisation mode) is not supported");
+    return GSTEX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gstex_last_error(void) { return g_err; }
+
+int gstex_abi_version(void) { return 1; }
+
+int gstex_get_aabb_2d(int num_points, const float* means, const float* scales, float glob_scale,
+                      const float* quats, const float* viewmat, float fx, float fy, float cx,
+                      float cy, float* centers, float* extents, void* stream) {
+    REQUIRE(num_points >= 0, "num_points must be >= 0");
+    if (num_points == 0) return GSTEX_OK;
+    REQUIRE(means && scales && quats && viewmat && centers && extents, "null pointer argument");
+    return check_hip(gstex::launch_get_aabb_2d(num_points, means, scales, glob_scale, quats,
+                                               viewmat, fx, fy, cx, cy, centers, extents,
+                                               as_stream(stream)),
+                     "get_aabb_2d");
+}
+
+int gstex_num_tiles_hit(int num_points, const float* centers, const float* extents, int img_height,
+                        int img_width, int block_width, int32_t* num_tiles_hit, void* stream) {
+    REQUIRE(num_points >= 0 && block_width > 0, "invalid sizes");
+    if (num_points == 0) return GSTEX_OK;
+    REQUIRE(centers && extents && num_tiles_hit, "null pointer argument");
+    return check_hip(gstex::launch_num_tiles_hit(num_points, centers, extents, block_width,
+                                                 tiles_of(img_width, block_width),
+                                                 tiles_of(img_height, block_width), num_tiles_hit,
+                                                 as_stream(stream)),
+                     "num_tiles_hit");
+}
+
+int gstex_project_gaussians(int num_points, const float* means, const float* scales,
+                            float glob_scale, const float* quats, const float* viewmat, float fx,
+                            float fy, float cx, float cy, int img_height, int img_width,
+                            int block_width, float* centers, float* extents, float* depths,
+                            int32_t* num_tiles_hit, void* stream) {
+    REQUIRE(num_points >= 0 && block_width > 0, "invalid sizes");
+    if (num_points == 0) return GSTEX_OK;
+    REQUIRE(means && scales && quats && viewmat && centers && extents && depths && num_tiles_hit,
+            "null pointer argument");
+    return check_hip(gstex::launch_project_fused(num_points, means, scales, glob_scale, quats,
+                                                 viewmat, fx, fy, cx, cy, block_width,
+                                                 tiles_of(img_width, block_width),
+                                                 tiles_of(img_height, block_width), centers,
+                                                 extents, depths, num_tiles_hit,
+                                                 as_stream(stream)),
+                     "project_gaussians");
+}
+
+int gstex_project_points(int num_points, const float* points, const float* viewmat, float fx,
+                         float fy, float cx, float cy, int clip, float* pix, float* depths,
+                         void* stream) {
+    REQUIRE(num_points >= 0, "num_points must be >= 0");
+    if (num_points == 0) return GSTEX_OK;
+    REQUIRE(points && viewmat && pix && depths, "null pointer argument");
+    return check_hip(gstex::launch_project_points(num_points, points, viewmat, fx, fy, cx, cy,
+                                                  clip, pix, depths, as_stream(stream)),
+                     "project_points");
+}
+
+int gstex_compute_sh_forward(int num_points, int degree, int degrees_to_use, const float* viewdirs,
+                             const float* coeffs, float* colors, void* stream) {
+    REQUIRE(num_points >= 0, "num_points must be >= 0");
+    REQUIRE(degree >= 0 && degree <= 4, "SH degree must be in [0, 4], got %d", degree);
+    REQUIRE(degrees_to_use >= 0 && degrees_to_use <= degree,
+            "degrees_to_use must be in [0, degree], got %d", degrees_to_use);
+    if (num_points == 0) return GSTEX_OK;
+    REQUIRE(viewdirs && coeffs && colors, "null pointer argument");
+    return check_hip(gstex::launch_sh_forward(num_points, degree, degrees_to_use, viewdirs, coeffs,
+                                              colors, as_stream(stream)),
+                     "compute_sh_forward");
+}
+
+int gstex_compute_sh_backward(int num_points, int degree, int degrees_to_use,
+                              const float* viewdirs, const float* v_colors, float* v_coeffs,
+                              void* stream) {
+    REQUIRE(num_points >= 0, "num_points must be >= 0");
+    REQUIRE(degree >= 0 && degree <= 4, "SH degree must be in [0, 4], got %d", degree);
+    REQUIRE(degrees_to_use >= 0 && degrees_to_use <= degree,
+            "degrees_to_use must be in [0, degree], got %d", degrees_to_use);
+    if (num_points == 0) return GSTEX_OK;
+    REQUIRE(viewdirs && v_colors && v_coeffs, "null pointer argument");
+    return check_hip(gstex::launch_sh_backward(num_points, degree, degrees_to_use, viewdirs,
+                                               v_colors, v_coeffs, as_stream(stream)),
+                     "compute_sh_backward");
+}
+
+int gstex_map_gaussian_to_intersects(int num_points, int num_intersects, const float* centers,
+                                     const float* extents, const float* depths,
+                                     const int32_t* cum_tiles_hit, int tiles_x, int tiles_y,
+                                     int block_width, int wrapped, int64_t* isect_ids,
+                                     int32_t* gaussian_ids, void* stream) {
+    REQUIRE(num_points >= 0 && num_intersects >= 0, "negative sizes");
+    REQUIRE(!wrapped, "wrapped tile binning is not supported");
+    REQUIRE(block_width > 0 && tiles_x >= 0 && tiles_y >= 0, "invalid tile bounds");
+    if (num_intersects == 0) return GSTEX_OK;
+    REQUIRE(isect_ids && gaussian_ids, "null pointer argument");
+    REQUIRE(num_points == 0 || (centers && extents && depths && cum_tiles_hit),
+            "null pointer argument");
+    return check_hip(gstex::map_intersects(num_points, num_intersects, centers, extents, depths,
+                                           cum_tiles_hit, tiles_x, tiles_y, block_width, isect_ids,
+                                           gaussian_ids, as_stream(stream)),
+                     "map_gaussian_to_intersects");
+}
+
+size_t gstex_sort_intersects_workspace_size(int num_intersects) {
+    return num_intersects > 0 ? gstex::sort_isects_workspace_bytes(num_intersects) : 0;
+}
+
+int gstex_sort_intersects(int num_intersects, int end_bit, const int64_t* isect_ids,
+                          const int32_t* gaussian_ids, int64_t* isect_ids_sorted,
+                          int32_t* gaussian_ids_sorted, void* workspace, size_t workspace_bytes,
+                          void* stream) {
+    REQUIRE(num_intersects >= 0, "num_intersects must be >= 0");
+    REQUIRE(end_bit >= 0 && end_bit <= 64, "end_bit must be in [0, 64], got %d", end_bit);
+    if (num_intersects == 0) return GSTEX_OK;
+    REQUIRE(isect_ids && gaussian_ids && isect_ids_sorted && gaussian_ids_sorted && workspace,
+            "null pointer argument");
+    REQUIRE(workspace_bytes >= gstex::sort_isects_workspace_bytes(num_intersects),
+            "workspace too small: %zu < %zu", workspace_bytes,
+            gstex::sort_isects_workspace_bytes(num_intersects));
+    return check_hip(gstex::sort_isects(num_intersects, isect_ids, gaussian_ids, isect_ids_sorted,
+                                        gaussian_ids_sorted, end_bit, workspace, workspace_bytes,
+                                        as_stream(stream)),
+                     "sort_intersects");
+}
+
+int gstex_get_tile_bin_edges(int num_intersects, const int64_t* isect_ids_sorted, int tiles_x,
+                             int tiles_y, int32_t* tile_bins, void* stream) {
+    REQUIRE(num_intersects >= 0 && tiles_x >= 0 && tiles_y >= 0, "negative sizes");
+    if (tiles_x * tiles_y == 0) return GSTEX_OK;
+    REQUIRE(tile_bins && (num_intersects == 0 || isect_ids_sorted), "null pointer argument");
+    return check_hip(gstex::tile_bin_edges(num_intersects, isect_ids_sorted, tiles_x * tiles_y,
+                                           tile_bins, as_stream(stream)),
+                     "get_tile_bin_edges");
+}
+
+size_t gstex_bin_gaussians_workspace_size(int num_points, int num_intersects) {
+    if (num_points <= 0 || num_intersects <= 0) return 0;
+    return gstex::bin_gaussians_workspace_bytes(num_points, num_intersects);
+}
+
+int gstex_bin_gaussians(int num_points, int num_intersects, const float* centers,
+                        const float* extents, const float* depths, const int32_t* num_tiles_hit,
+                        int tiles_x, int tiles_y, int block_width, int32_t* gaussian_ids_sorted,
+                        int32_t* tile_bins, void* workspace, size_t workspace_bytes, void* stream) {
+    REQUIRE(num_points >= 0 && num_intersects >= 0, "negative sizes");
+    REQUIRE(block_width > 0 && tiles_x >= 0 && tiles_y >= 0, "invalid tile bounds");
+    if (tiles_x * tiles_y == 0) return GSTEX_OK;
+    REQUIRE(tile_bins, "null pointer argument");
+    if (num_points > 0 && num_intersects > 0) {
+        REQUIRE(centers && extents && depths && num_tiles_hit && gaussian_ids_sorted && workspace,
+                "null pointer argument");
+        const size_t need = gstex::bin_gaussians_workspace_bytes(num_points, num_intersects);
+        REQUIRE(workspace_bytes >= need, "workspace too small: %zu < %zu", workspace_bytes, need);
+    }
+    return check_hip(gstex::bin_gaussians(num_points, num_intersects, centers, extents, depths,
+                                          num_tiles_hit, tiles_x, tiles_y, block_width,
+                                          gaussian_ids_sorted, tile_bins, workspace,
+                                          workspace_bytes, as_stream(stream)),
+                     "bin_gaussians");
+}
+
+size_t gstex_texture_forward_workspace_size(int num_points) {
+    return gstex::raster_forward_workspace_bytes(num_points > 0 ? num_points : 0);
+}
+
+size_t gstex_texture_backward_workspace_size(int num_points) {
+    return gstex::raster_backward_workspace_bytes(num_points > 0 ? num_points : 0);
+}
+
+int gstex_texture_forward(int tiles_x, int tiles_y, int block_width, int img_width, int img_height,
+                          int num_charts, int num_probs, int channels,
+                          const int32_t* texture_dims, const int32_t* gaussian_ids_sorted,
+                          int num_ids, const int32_t* tile_bins, const float* colors,
+                          const float* opacities, const float* means, const float* scales,
+                          int num_points, float glob_scale, const float* quats, const float* uv0,
+                          const float* umap, const float* vmap, const float* texture,
+                          int num_texels, const float* viewmat, const float* c2w, float fx,
+                          float fy, float cx, float cy, int settings, const float* background,
+                          float* out_img, float* out_depth, float* out_reg, float* out_texture,
+                          float* out_normal, float* final_Ts, int32_t* final_idx,
+                          int32_t* depth_idx, float* out_reg_s, void* workspace,
+                          size_t workspace_bytes, void* stream) {
+    (void)num_charts;
+    int rc = check_raster_common(tiles_x, tiles_y, block_width, img_width, img_height, num_probs,
+                                 channels, num_points, num_ids, num_texels, settings);
+    if (rc) return rc;
+    REQUIRE(out_img && out_depth && out_reg && out_normal && final_Ts && final_idx && depth_idx &&
+                out_reg_s && (channels == 0 || out_texture),
+            "null output pointer");
+    REQUIRE(tile_bins && viewmat && c2w && background, "null pointer argument");
+    REQUIRE(num_ids == 0 || num_points > 0, "gaussian_ids_sorted is non-empty but num_points is 0");
+    REQUIRE(num_ids == 0 || gaussian_ids_sorted, "null pointer argument");
+    if (num_points > 0)
+        REQUIRE(texture_dims && colors && opacities && means && scales && quats && uv0 && umap &&
+                    vmap && (channels == 0 || texture),
+                "null pointer argument");
+    REQUIRE(workspace_bytes >= gstex::raster_forward_workspace_bytes(num_points),
+            "workspace too small: %zu < %zu", workspace_bytes,
+            gstex::raster_forward_workspace_bytes(num_points));
+    gstex::RasterCall c{img_width, img_height, block_width, channels, settings, num_points, num_ids,
+                        num_texels, glob_scale, fx, fy, cx, cy, texture_dims, gaussian_ids_sorted,
+                        tile_bins, colors, opacities, means, scales, quats, uv0, umap, vmap,
+                        texture, viewmat, c2w, background};
+    return check_hip(gstex::raster_forward(c, out_img, out_depth, out_reg, out_texture, out_normal,
+                                           final_Ts, final_idx, depth_idx, out_reg_s, workspace,
+                                           workspace_bytes, as_stream(stream)),
+                     "texture_forward");
+}
+
+int gstex_texture_backward(int img_height, int img_width, int block_width, int num_charts,
+                           int num_probs, int channels, const int32_t* texture_dims,
+                           const int32_t* gaussian_ids_sorted, int num_ids,
+                           const int32_t* tile_bins, const float* colors, const float* opacities,
+                           const float* means, const float* scales, int num_points,
+                           float glob_scale, const float* quats, const float* uv0,
+                           const float* umap, const float* vmap, const float* texture,
+                           int num_texels, const float* viewmat, const float* c2w, float fx,
+                           float fy, float cx, float cy, int settings, const float* background,
+                           const float* final_Ts, const int32_t* final_idx,
+                           const int32_t* depth_idx, const float* final_s, const float* v_output,
+                           const float* v_output_depth, const float* v_output_reg,
+                           const float* v_output_alpha, const float* v_output_texture,
+                           const float* v_output_normal, float* v_colors, float* v_opacity,
+                           float* v_means, float* v_scales, float* v_quats, float* v_uv0,
+                           float* v_umap, float* v_vmap, float* v_texture, void* workspace,
+                           size_t workspace_bytes, void* stream) {
+    (void)num_charts;
+    if (block_width < 2 || block_width > 16)
+        return fail(GSTEX_ERR_ARG, "block_width must be between 2 and 16, got %d", block_width);
+    const int tiles_x = tiles_of(img_width, block_width), tiles_y = tiles_of(img_height, block_width);
+    int rc = check_raster_common(tiles_x, tiles_y, block_width, img_width, img_height, num_probs,
+                                 channels, num_points, num_ids, num_texels, settings);
+    if (rc) return rc;
+    REQUIRE(tile_bins && viewmat && c2w && background && final_Ts && final_idx && depth_idx &&
+                final_s && v_output && v_output_depth && v_output_reg && v_output_alpha &&
+                v_output_normal && (channels == 0 || v_output_texture),
+            "null pointer argument");
+    REQUIRE(num_ids == 0 || num_points > 0, "gaussian_ids_sorted is non-empty but num_points is 0");
+    REQUIRE(num_ids == 0 || gaussian_ids_sorted, "null pointer argument");
+    if (num_points > 0)
+        REQUIRE(texture_dims && colors && opacities && means && scales &&
+                    quats && uv0 && umap && vmap && v_colors && v_opacity && v_means && v_scales &&
+                    v_quats && v_uv0 && v_umap && v_vmap && (channels == 0 || (texture && v_texture)),
+                "null pointer argument");
+    REQUIRE(workspace_bytes >= gstex::raster_backward_workspace_bytes(num_points),
+            "workspace too small: %zu < %zu", workspace_bytes,
+            gstex::raster_backward_workspace_bytes(num_points));
+    gstex::RasterCall c{img_width, img_height, block_width, channels, settings, num_points, num_ids,
+                        num_texels, glob_scale, fx, fy, cx, cy, texture_dims, gaussian_ids_sorted,
+                        tile_bins, colors, opacities, means, scales, quats, uv0, umap, vmap,
+                        texture, viewmat, c2w, background};
+    return check_hip(gstex::raster_backward(c, final_Ts, final_idx, depth_idx, final_s, v_output,
+                                            v_output_depth, v_output_reg, v_output_alpha,
+                                            v_output_texture, v_output_normal, v_colors, v_opacity,
+                                            v_means, v_scales, v_quats, v_uv0, v_umap, v_vmap,
+                                            v_texture, workspace, workspace_bytes,
+                                            as_stream(stream)),
+                     "texture_backward");
+}
+
+size_t gstex_splat_records_size(int num_points) {
+    return num_points > 0 ? (size_t)num_points * gstex::kRecordBytes : 0;
+}
+
+int gstex_build_splat_records(int num_points, const float* means, const float* scales,
+                              float glob_scale, const float* quats, const float* colors,
+                              const float* opacities, const float* uv0, const float* umap,
+                              const float* vmap, const int32_t* texture_dims,
+                              const float* viewmat, float fx, float fy, float cx, float cy,
+                              void* records, void* stream) {
+    REQUIRE(num_points >= 0, "num_points must be >= 0");
+    if (num_points == 0) return GSTEX_OK;
+    REQUIRE(means && scales && quats && colors && opacities && uv0 && umap && vmap &&
+                texture_dims && viewmat && records,
+            "null pointer argument");
+    REQUIRE(((uintptr_t)records & 15) == 0, "records must be 16-byte aligned");
+    return check_hip(gstex::launch_build_records(num_points, means, scales, glob_scale, quats,
+                                                 colors, opacities, uv0, umap, vmap, texture_dims,
+                                                 viewmat, fx, fy, cx, cy, (float*)records,
+                                                 as_stream(stream)),
+                     "build_splat_records");
+}
+
+static gstex::RasterCall stage_call(int img_width, int img_height, int block_width, int channels,
+                                    int settings, int num_points, const int32_t* ids, int num_ids,
+                                    const int32_t* tile_bins, const float* texture, int num_texels,
+                                    const float* viewmat, const float* c2w, float fx, float fy,
+                                    float cx, float cy, const float* background) {
+    return gstex::RasterCall{img_width, img_height, block_width, channels, settings, num_points,
+                             num_ids, num_texels, 1.f, fx, fy, cx, cy, nullptr, ids, tile_bins,
+                             nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                             nullptr, texture, viewmat, c2w, background};
+}
+
+int gstex_raster_forward(int img_width, int img_height, int block_width, int channels,
+                         int settings, const void* records, int num_points,
+                         const int32_t* gaussian_ids_sorted, int num_ids,
+                         const int32_t* tile_bins, const float* texture, int num_texels,
+                         const float* viewmat, const float* c2w, float fx, float fy, float cx,
+                         float cy, const float* background, float* out_img, float* out_depth,
+                         float* out_reg, float* out_texture, float* out_normal, float* final_Ts,
+                         int32_t* final_idx, int32_t* depth_idx, float* out_reg_s, void* stream) {
+    const int tx = img_width > 0 && block_width > 0 ? tiles_of(img_width, block_width) : 0;
+    const int ty = img_height > 0 && block_width > 0 ? tiles_of(img_height, block_width) : 0;
+    int rc = check_raster_common(tx, ty, block_width, img_width, img_height, 1, channels,
+                                 num_points, num_ids, num_texels, settings);
+    if (rc) return rc;
+    REQUIRE(tile_bins && viewmat && c2w && background && (num_ids == 0 || gaussian_ids_sorted) &&
+                (num_points == 0 || records) && (channels == 0 || texture),
+            "null pointer argument");
+    REQUIRE(out_img && out_depth && out_reg && out_normal && final_Ts && final_idx && depth_idx &&
+                out_reg_s && (channels == 0 || out_texture),
+            "null output pointer");
+    REQUIRE(num_ids == 0 || num_points > 0, "gaussian_ids_sorted is non-empty but num_points is 0");
+    const gstex::RasterCall c =
+        stage_call(img_width, img_height, block_width, channels, settings, num_points,
+                   gaussian_ids_sorted, num_ids, tile_bins, texture, num_texels, viewmat, c2w, fx,
+                   fy, cx, cy, background);
+    return check_hip(gstex::raster_forward_stage(c, (const float*)records, out_img, out_depth,
+                                                 out_reg, out_texture, out_normal, final_Ts,
+                                                 final_idx, depth_idx, out_reg_s,
+                                                 as_stream(stream)),
+                     "raster_forward");
+}
+
+int gstex_raster_backward(int img_width, int img_height, int block_width, int channels,
+                          int settings, const void* records, int num_points,
+                          const int32_t* gaussian_ids_sorted, int num_ids,
+                          const int32_t* tile_bins, const float* texture, int num_texels,
+                          const float* viewmat, const float* c2w, float fx, float fy, float cx,
+                          float cy, const float* background, const float* final_Ts,
+                          const int32_t* final_idx, const int32_t* depth_idx,
+                          const float* final_s, const float* v_output,
+                          const float* v_output_depth, const float* v_output_reg,
+                          const float* v_output_alpha, const float* v_output_texture,
+                          const float* v_output_normal, float* grad_records, float* v_texture,
+                          void* stream) {
+    const int tx = img_width > 0 && block_width > 0 ? tiles_of(img_width, block_width) : 0;
+    const int ty = img_height > 0 && block_width > 0 ? tiles_of(img_height, block_width) : 0;
+    int rc = check_raster_common(tx, ty, block_width, img_width, img_height, 1, channels,
+                                 num_points, num_ids, num_texels, settings);
+    if (rc) return rc;
+    REQUIRE(tile_bins && viewmat && c2w && background && (num_ids == 0 || gaussian_ids_sorted) &&
+                (num_points == 0 || (records && grad_records)) &&
+                (channels == 0 || (texture && v_texture && v_output_texture)),
+            "null pointer argument");
+    REQUIRE(final_Ts && final_idx && depth_idx && final_s && v_output && v_output_depth &&
+                v_output_reg && v_output_alpha && v_output_normal,
+            "null pointer argument");
+    REQUIRE(num_ids == 0 || num_points > 0, "gaussian_ids_sorted is non-empty but num_points is 0");
+    const gstex::RasterCall c =
+        stage_call(img_width, img_height, block_width, channels, settings, num_points,
+                   gaussian_ids_sorted, num_ids, tile_bins, texture, num_texels, viewmat, c2w, fx,
+                   fy, cx, cy, background);
+    return check_hip(gstex::raster_backward_stage(c, (const float*)records, final_Ts, final_idx,
+                                                  depth_idx, final_s, v_output, v_output_depth,
+                                                  v_output_reg, v_output_alpha, v_output_texture,
+                                                  v_output_normal, grad_records, v_texture,
+                                                  as_stream(stream)),
+                     "raster_backward");
+}
+
+int gstex_finalize_gradients(int num_points, const float* grad_records, const float* means,
+                             const float* scales, float glob_scale, const float* quats,
+                             const float* viewmat, float fx, float fy, float* v_colors,
+                             float* v_opacity, float* v_means, float* v_scales, float* v_quats,
+                             float* v_uv0, float* v_umap, float* v_vmap, void* stream) {
+    REQUIRE(num_points >= 0, "num_points must be >= 0");
+    if (num_points == 0) return GSTEX_OK;
+    REQUIRE(grad_records && means && scales && quats && viewmat && v_colors && v_opacity &&
+                v_means && v_scales && v_quats && v_uv0 && v_umap && v_vmap,
+            "null pointer argument");
+    return check_hip(gstex::grad_finalize(num_points, grad_records, means, scales, glob_scale,
+                                          quats, viewmat, fx, fy, v_colors, v_opacity, v_means,
+                                          v_scales, v_quats, v_uv0, v_umap, v_vmap,
+                                          as_stream(stream)),
+                     "finalize_gradients");
+}
+
+int gstex_fill_f32(float* dst, size_t n, float v, void* stream) {
+    if (n == 0) return GSTEX_OK;
+    REQUIRE(dst, "null pointer argument");
+    return check_hip(gstex::fill_f32(dst, n, v, as_stream(stream)), "fill_f32");
+}
+
+int gstex_texture_sample_forward(int num_charts, int num_probs, int channels,
+                                 const int32_t* texture_dims, const float* uvs, int num_queries,
+                                 const float* texture, int num_texels, float* output,
+                                 void* stream) {
+    (void)num_charts;
+    (void)num_probs;
+    REQUIRE(num_queries >= 0 && channels >= 0 && num_texels >= 0, "negative sizes");
+    if (num_queries == 0 || channels == 0) return GSTEX_OK;
+    REQUIRE(num_texels > 0, "texture has no rows");
+    REQUIRE(texture_dims && uvs && texture && output, "null pointer argument");
+    return check_hip(gstex::sample_forward(num_queries, channels, num_texels, texture_dims, uvs,
+                                           texture, output, as_stream(stream)),
+                     "texture_sample_forward");
+}
+
+int gstex_texture_sample_backward(int num_charts, int num_probs, int channels,
+                                  const int32_t* texture_dims, const float* uvs, int num_queries,
+                                  const float* texture, int num_texels, const float* v_output,
+                                  float* v_texture, void* stream) {
+    (void)num_charts;
+    (void)num_probs;
+    (void)texture;
+    REQUIRE(num_queries >= 0 && channels >= 0 && num_texels >= 0, "negative sizes");
+    if (num_texels == 0 || channels == 0) return GSTEX_OK;
+    REQUIRE(v_texture && (num_queries == 0 || (texture_dims && uvs && v_output)),
+            "null pointer argument");
+    return check_hip(gstex::sample_backward(num_queries, channels, num_texels, texture_dims, uvs,
+                                            v_output, v_texture, as_stream(stream)),
+                     "texture_sample_backward");
+}
+
+}  // extern "C"

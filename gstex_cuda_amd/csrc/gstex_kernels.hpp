@@ -1,0 +1,122 @@
+// Host-side launchers of the gfx950 kernels (internal; the exported C ABI is
+// in include/gstex_hip.h and capi.hip).  Every launcher enqueues on the given
+// stream, allocates nothing, and returns the launch status.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+namespace gstex {
+
+// projection / records
+hipError_t launch_get_aabb_2d(int n, const float* means, const float* scales, float glob_scale,
+                              const float* quats, const float* viewmat, float fx, float fy,
+                              float cx, float cy, float* centers, float* extents,
+                              hipStream_t stream);
+hipError_t launch_project_fused(int n, const float* means, const float* scales, float glob_scale,
+                                const float* quats, const float* viewmat, float fx, float fy,
+                                float cx, float cy, int bw, int tiles_x, int tiles_y,
+                                float* centers, float* extents, float* depths, int* tiles_hit,
+                                hipStream_t stream);
+hipError_t launch_num_tiles_hit(int n, const float* centers, const float* extents, int bw,
+                                int tiles_x, int tiles_y, int* out, hipStream_t stream);
+hipError_t launch_project_points(int n, const float* points, const float* viewmat, float fx,
+                                 float fy, float cx, float cy, int clip, float* pix,
+                                 float* depths, hipStream_t stream);
+hipError_t launch_build_records(int n, const float* means, const float* scales, float glob_scale,
+                                const float* quats, const float* colors, const float* opacities,
+                                const float* uv0, const float* umap, const float* vmap,
+                                const int* texture_dims, const float* viewmat, float fx, float fy,
+                                float cx, float cy, float* records, hipStream_t stream);
+
+// SH
+hipError_t launch_sh_forward(int n, int degree, int degrees_to_use, const float* viewdirs,
+                             const float* coeffs, float* colors, hipStream_t stream);
+hipError_t launch_sh_backward(int n, int degree, int degrees_to_use, const float* viewdirs,
+                              const float* v_colors, float* v_coeffs, hipStream_t stream);
+int sh_num_bases(int degree);
+
+// fills
+hipError_t fill_f32(float* p, size_t n, float v, hipStream_t s);
+hipError_t fill_i32(int* p, size_t n, int v, hipStream_t s);
+
+// binning
+hipError_t map_intersects(int n, int num_isects, const float* centers, const float* extents,
+                          const float* depths, const int* cum, int tiles_x, int tiles_y, int bw,
+                          int64_t* keys, int* ids, hipStream_t s);
+size_t sort_isects_workspace_bytes(int n);
+hipError_t sort_isects(int n, const int64_t* keys_in, const int* ids_in, int64_t* keys_out,
+                       int* ids_out, int end_bit, void* ws, size_t ws_bytes, hipStream_t s);
+hipError_t tile_bin_edges(int m, const int64_t* keys_sorted, int num_tiles, int* bins,
+                          hipStream_t s);
+size_t bin_gaussians_workspace_bytes(int n, int m);
+hipError_t bin_gaussians(int n, int m, const float* centers, const float* extents,
+                         const float* depths, const int* counts, int tiles_x, int tiles_y, int bw,
+                         int* ids_sorted, int* tile_bins, void* ws, size_t ws_bytes,
+                         hipStream_t s);
+
+// raster
+struct RasterCall {
+    int img_w, img_h, block_width, channels, settings;
+    int num_points, num_ids, num_texels;
+    float glob_scale, fx, fy, cx, cy;
+    const int* texture_dims;
+    const int* ids_sorted;
+    const int* tile_bins;
+    const float* colors;
+    const float* opacities;
+    const float* means;
+    const float* scales;
+    const float* quats;
+    const float* uv0;
+    const float* umap;
+    const float* vmap;
+    const float* texture;
+    const float* viewmat;
+    const float* c2w;
+    const float* background;
+};
+
+size_t raster_forward_workspace_bytes(int n);
+size_t raster_backward_workspace_bytes(int n);
+hipError_t raster_forward(const RasterCall& c, float* out_img, float* out_depth, float* out_reg,
+                          float* out_texture, float* out_normal, float* final_Ts, int* final_idx,
+                          int* depth_idx, float* out_reg_s, void* ws, size_t ws_bytes,
+                          hipStream_t s);
+hipError_t raster_backward(const RasterCall& c, const float* final_Ts, const int* final_idx,
+                           const int* depth_idx, const float* final_s, const float* v_img,
+                           const float* v_depth, const float* v_reg, const float* v_alpha,
+                           const float* v_texture_out, const float* v_normal, float* v_colors,
+                           float* v_opacity, float* v_means, float* v_scales, float* v_quats,
+                           float* v_uv0, float* v_umap, float* v_vmap, float* v_texture, void* ws,
+                           size_t ws_bytes, hipStream_t s);
+
+// per-stage raster entry points (records built by launch_build_records)
+hipError_t raster_forward_stage(const RasterCall& c, const float* records, float* out_img,
+                                float* out_depth, float* out_reg, float* out_texture,
+                                float* out_normal, float* final_Ts, int* final_idx,
+                                int* depth_idx, float* out_reg_s, hipStream_t s);
+hipError_t raster_backward_stage(const RasterCall& c, const float* records,
+                                 const float* final_Ts, const int* final_idx,
+                                 const int* depth_idx, const float* final_s, const float* v_img,
+                                 const float* v_depth, const float* v_reg, const float* v_alpha,
+                                 const float* v_texture_out, const float* v_normal,
+                                 float* grad_rec, float* v_texture, hipStream_t s);
+hipError_t grad_finalize(int n, const float* grad_rec, const float* means, const float* scales,
+                         float glob_scale, const float* quats, const float* viewmat, float fx,
+                         float fy, float* v_colors, float* v_opacity, float* v_means,
+                         float* v_scales, float* v_quats, float* v_uv0, float* v_umap,
+                         float* v_vmap, hipStream_t s);
+constexpr size_t kRecordBytes = 128;
+constexpr size_t kGradRecordBytes = 128;
+
+// sampling
+hipError_t sample_forward(int q, int C, int num_texels, const int* dims, const float* uvs,
+                          const float* texture, float* out, hipStream_t s);
+hipError_t sample_backward(int q, int C, int num_texels, const int* dims, const float* uvs,
+                           const float* v_out, float* v_texture, hipStream_t s);
+
+}  // namespace gstex

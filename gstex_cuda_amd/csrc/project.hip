@@ -1,0 +1,224 @@
+// Per-Gaussian kernels: screen AABB, depth, tile counts and the splat records
+// consumed by the raster kernels.  One lane per Gaussian; these kernels stream
+// N x (40..128) bytes and are HBM-bound.
+#include "gstex_common.hpp"
+#include "gstex_kernels.hpp"
+
+namespace gstex {
+
+// Screen AABB of the four corners mean +- 3*g*s.x*ax1 +- 3*g*s.y*ax2, each
+// projected with view z clamped to >= 0.01; a mean at view z <= 0.01 gives
+// (projected mean, zero extent).  Semantics of get_aabb_2d.cu:11-89.
+__device__ __forceinline__ void aabb_one(const float* __restrict__ means,
+                                         const float* __restrict__ scales, float glob_scale,
+                                         const float* __restrict__ quats, const float* vm,
+                                         float fx, float fy, float cx, float cy, int i,
+                                         float& ccx, float& ccy, float& ex, float& ey, float& depth) {
+    const float t_near = 0.01f;
+    V3 p = v3(means[3 * i], means[3 * i + 1], means[3 * i + 2]);
+    V3 pv = xform43(vm, p);
+    depth = pv.z;
+    const bool clipped = pv.z <= t_near;
+    const float4 q = reinterpret_cast<const float4*>(quats)[i];
+    V3 a1, a2, a3;
+    quat_axes(q.x, q.y, q.z, q.w, a1, a2, a3);
+    const float ell = 3.0f * glob_scale;
+    const float e1 = ell * scales[3 * i], e2 = ell * scales[3 * i + 1];
+    const V3 d1 = v3(e1 * a1.x, e1 * a1.y, e1 * a1.z);
+    const V3 d2 = v3(e2 * a2.x, e2 * a2.y, e2 * a2.z);
+    float mxx = -INFINITY, mxy = -INFINITY, mnx = INFINITY, mny = INFINITY;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float s1 = (c & 2) ? -1.f : 1.f, s2 = (c & 1) ? -1.f : 1.f;
+        V3 corner = v3(p.x + s1 * d1.x + s2 * d2.x, p.y + s1 * d1.y + s2 * d2.y,
+                       p.z + s1 * d1.z + s2 * d2.z);
+        V3 cv = xform43(vm, corner);
+        cv.z = fmaxf(cv.z, t_near);
+        float u, v;
+        project_pix(fx, fy, cx, cy, cv, u, v);
+        mxx = fmaxf(mxx, u);
+        mxy = fmaxf(mxy, v);
+        mnx = fminf(mnx, u);
+        mny = fminf(mny, v);
+    }
+    if (clipped) {
+        project_pix(fx, fy, cx, cy, pv, ccx, ccy);
+        ex = 0.f;
+        ey = 0.f;
+    } else {
+        ccx = 0.5f * (mxx + mnx);
+        ccy = 0.5f * (mxy + mny);
+        ex = 0.5f * (mxx - mnx);
+        ey = 0.5f * (mxy - mny);
+    }
+}
+
+// Tile count of one AABB with the float32 arithmetic of get_aabb_2d.py:70-92.
+__device__ __forceinline__ int tiles_hit_one(float ccx, float ccy, float ex, float ey, int bw,
+                                             int tiles_x, int tiles_y) {
+    const float fbw = (float)bw;
+    int lox = f2i_trunc(floorf((ccx - ex) / fbw));
+    int loy = f2i_trunc(floorf((ccy - ey) / fbw));
+    int hix = f2i_trunc(floorf((ccx + ex) / fbw + 1.f));
+    int hiy = f2i_trunc(floorf((ccy + ey) / fbw + 1.f));
+    lox = min(max(lox, 0), tiles_x);
+    loy = min(max(loy, 0), tiles_y);
+    hix = min(max(hix, 0), tiles_x);
+    hiy = min(max(hiy, 0), tiles_y);
+    return (hix - lox) * (hiy - loy);
+}
+
+__global__ __launch_bounds__(256) void k_get_aabb_2d(int n, const float* __restrict__ means,
+                                                     const float* __restrict__ scales,
+                                                     float glob_scale,
+                                                     const float* __restrict__ quats,
+                                                     const float* __restrict__ viewmat, float fx,
+                                                     float fy, float cx, float cy,
+                                                     float* __restrict__ centers,
+                                                     float* __restrict__ extents) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float ccx, ccy, ex, ey, depth;
+    aabb_one(means, scales, glob_scale, quats, viewmat, fx, fy, cx, cy, i, ccx, ccy, ex, ey, depth);
+    reinterpret_cast<float2*>(centers)[i] = make_float2(ccx, ccy);
+    reinterpret_cast<float2*>(extents)[i] = make_float2(ex, ey);
+}
+
+__global__ __launch_bounds__(256) void k_project_fused(
+    int n, const float* __restrict__ means, const float* __restrict__ scales, float glob_scale,
+    const float* __restrict__ quats, const float* __restrict__ viewmat, float fx, float fy,
+    float cx, float cy, int bw, int tiles_x, int tiles_y, float* __restrict__ centers,
+    float* __restrict__ extents, float* __restrict__ depths, int* __restrict__ tiles_hit) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float ccx, ccy, ex, ey, depth;
+    aabb_one(means, scales, glob_scale, quats, viewmat, fx, fy, cx, cy, i, ccx, ccy, ex, ey, depth);
+    reinterpret_cast<float2*>(centers)[i] = make_float2(ccx, ccy);
+    reinterpret_cast<float2*>(extents)[i] = make_float2(ex, ey);
+    depths[i] = depth;
+    tiles_hit[i] = tiles_hit_one(ccx, ccy, ex, ey, bw, tiles_x, tiles_y);
+}
+
+__global__ __launch_bounds__(256) void k_num_tiles_hit(int n, const float* __restrict__ centers,
+                                                       const float* __restrict__ extents, int bw,
+                                                       int tiles_x, int tiles_y,
+                                                       int* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float2 c = reinterpret_cast<const float2*>(centers)[i];
+    const float2 e = reinterpret_cast<const float2*>(extents)[i];
+    out[i] = tiles_hit_one(c.x, c.y, e.x, e.y, bw, tiles_x, tiles_y);
+}
+
+__global__ __launch_bounds__(256) void k_project_points(int n, const float* __restrict__ points,
+                                                        const float* __restrict__ viewmat,
+                                                        float fx, float fy, float cx, float cy,
+                                                        int clip, float* __restrict__ pix,
+                                                        float* __restrict__ depths) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V3 pv = xform43(viewmat, v3(points[3 * i], points[3 * i + 1], points[3 * i + 2]));
+    if (clip) pv.z = fminf(fmaxf(pv.z, 0.01f), 1000.0f);
+    float u, v;
+    project_pix(fx, fy, cx, cy, pv, u, v);
+    reinterpret_cast<float2*>(pix)[i] = make_float2(u, v);
+    depths[i] = pv.z;
+}
+
+// Splat records: axes from the (unnormalised) quaternion exactly as the
+// raster loop of texture.cu:166-169 derives them, plus every per-Gaussian
+// field the raster needs, in one 128-byte line.
+__global__ __launch_bounds__(256) void k_build_records(
+    int n, const float* __restrict__ means, const float* __restrict__ scales, float glob_scale,
+    const float* __restrict__ quats, const float* __restrict__ colors,
+    const float* __restrict__ opacities, const float* __restrict__ uv0,
+    const float* __restrict__ umap, const float* __restrict__ vmap,
+    const int* __restrict__ texture_dims, const float* __restrict__ viewmat, float fx, float fy,
+    float cx, float cy, float* __restrict__ records) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 q = reinterpret_cast<const float4*>(quats)[i];
+    V3 a1, a2, a3;
+    quat_axes(q.x, q.y, q.z, q.w, a1, a2, a3);
+    const V3 m = v3(means[3 * i], means[3 * i + 1], means[3 * i + 2]);
+    const V3 pv = xform43(viewmat, m);
+    float pu, pvv;
+    project_pix(fx, fy, cx, cy, pv, pu, pvv);
+    float r[kRecordFloats];
+    r[R_AX1 + 0] = a1.x; r[R_AX1 + 1] = a1.y; r[R_AX1 + 2] = a1.z;
+    r[R_INV_SX] = 1.f / (scales[3 * i] * glob_scale);
+    r[R_AX2 + 0] = a2.x; r[R_AX2 + 1] = a2.y; r[R_AX2 + 2] = a2.z;
+    r[R_INV_SY] = 1.f / (scales[3 * i + 1] * glob_scale);
+    r[R_AX3 + 0] = a3.x; r[R_AX3 + 1] = a3.y; r[R_AX3 + 2] = a3.z;
+    r[R_OPAC] = opacities[i];
+    r[R_MEAN + 0] = m.x; r[R_MEAN + 1] = m.y; r[R_MEAN + 2] = m.z;
+    r[R_PIX_X] = pu;
+    r[R_RGB + 0] = colors[3 * i]; r[R_RGB + 1] = colors[3 * i + 1]; r[R_RGB + 2] = colors[3 * i + 2];
+    r[R_PIX_Y] = pvv;
+    r[R_UV0 + 0] = uv0[2 * i]; r[R_UV0 + 1] = uv0[2 * i + 1];
+    r[R_UMAP + 0] = umap[3 * i]; r[R_UMAP + 1] = umap[3 * i + 1]; r[R_UMAP + 2] = umap[3 * i + 2];
+    r[R_VMAP + 0] = vmap[3 * i]; r[R_VMAP + 1] = vmap[3 * i + 1]; r[R_VMAP + 2] = vmap[3 * i + 2];
+    r[R_TEX_H] = __int_as_float(texture_dims[3 * i]);
+    r[R_TEX_W] = __int_as_float(texture_dims[3 * i + 1]);
+    r[R_TEX_START] = __int_as_float(texture_dims[3 * i + 2]);
+    r[R_SPARE] = 0.f;
+    float4* dst = reinterpret_cast<float4*>(records + (size_t)i * kRecordFloats);
+#pragma unroll
+    for (int k = 0; k < kRecordFloats / 4; ++k)
+        dst[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
+}
+
+hipError_t launch_get_aabb_2d(int n, const float* means, const float* scales, float glob_scale,
+                              const float* quats, const float* viewmat, float fx, float fy,
+                              float cx, float cy, float* centers, float* extents,
+                              hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    k_get_aabb_2d<<<(n + 255) / 256, 256, 0, stream>>>(n, means, scales, glob_scale, quats, viewmat,
+                                                      fx, fy, cx, cy, centers, extents);
+    return hipGetLastError();
+}
+
+hipError_t launch_project_fused(int n, const float* means, const float* scales, float glob_scale,
+                                const float* quats, const float* viewmat, float fx, float fy,
+                                float cx, float cy, int bw, int tiles_x, int tiles_y,
+                                float* centers, float* extents, float* depths, int* tiles_hit,
+                                hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    k_project_fused<<<(n + 255) / 256, 256, 0, stream>>>(n, means, scales, glob_scale, quats,
+                                                        viewmat, fx, fy, cx, cy, bw, tiles_x,
+                                                        tiles_y, centers, extents, depths,
+                                                        tiles_hit);
+    return hipGetLastError();
+}
+
+hipError_t launch_num_tiles_hit(int n, const float* centers, const float* extents, int bw,
+                                int tiles_x, int tiles_y, int* out, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    k_num_tiles_hit<<<(n + 255) / 256, 256, 0, stream>>>(n, centers, extents, bw, tiles_x, tiles_y,
+                                                        out);
+    return hipGetLastError();
+}
+
+hipError_t launch_project_points(int n, const float* points, const float* viewmat, float fx,
+                                 float fy, float cx, float cy, int clip, float* pix,
+                                 float* depths, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    k_project_points<<<(n + 255) / 256, 256, 0, stream>>>(n, points, viewmat, fx, fy, cx, cy, clip,
+                                                         pix, depths);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_records(int n, const float* means, const float* scales, float glob_scale,
+                                const float* quats, const float* colors, const float* opacities,
+                                const float* uv0, const float* umap, const float* vmap,
+                                const int* texture_dims, const float* viewmat, float fx, float fy,
+                                float cx, float cy, float* records, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    k_build_records<<<(n + 255) / 256, 256, 0, stream>>>(n, means, scales, glob_scale, quats,
+                                                        colors, opacities, uv0, umap, vmap,
+                                                        texture_dims, viewmat, fx, fy, cx, cy,
+                                                        records);
+    return hipGetLastError();
+}
+
+}  // namespace gstex

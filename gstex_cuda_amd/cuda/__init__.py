@@ -1,0 +1,359 @@
+"""Drop-in for ``gstex_cuda.cuda``: the ten native entry points of the reference
+(cuda/__init__.py:14-26, bound in cuda/csrc/ext.cpp:8-25), same names, same
+positional arguments, same return tuples.
+
+Each function validates its tensors like the reference's CHECK_INPUT
+(``"<name> must be a CUDA tensor"`` / ``"<name> must be contiguous"`` as
+RuntimeError), allocates outputs with torch on the input's device, and runs
+the gfx950 kernels through the C ABI on torch's current stream.  There is no
+CPU path: without the built library every call raises.
+"""
+from __future__ import annotations
+
+from typing import Tuple
+
+import torch
+
+from .. import _lib
+from .._lib import call
+
+__all__ = [
+    "compute_sh_forward", "compute_sh_backward", "map_gaussian_to_intersects",
+    "get_tile_bin_edges", "get_aabb_2d", "texture_forward", "texture_backward",
+    "texture_sample_forward", "texture_sample_backward", "texture_edit",
+    # additions beyond the reference's ten
+    "project_gaussians", "project_points", "num_tiles_hit", "sort_intersects", "bin_gaussians",
+]
+
+
+def _check(t: torch.Tensor, name: str) -> None:
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise RuntimeError(f"{name} must be a CUDA tensor")
+    if not t.is_contiguous():
+        raise RuntimeError(f"{name} must be contiguous")
+
+
+def _dtype(t: torch.Tensor, name: str, dtype: torch.dtype) -> None:
+    if t.dtype != dtype:
+        raise RuntimeError(f"{name} must have dtype {dtype}, got {t.dtype}")
+
+
+def _ptr(t: torch.Tensor):
+    return t.data_ptr() if t.numel() > 0 else None
+
+
+def _stream(device: torch.device):
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def _workspace(nbytes: int, device: torch.device) -> Tuple[torch.Tensor, int]:
+    ws = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+    return ws, int(nbytes)
+
+
+# --------------------------------------------------------------------------- SH
+def compute_sh_forward(num_points: int, degree: int, degrees_to_use: int,
+                       viewdirs: torch.Tensor, coeffs: torch.Tensor) -> torch.Tensor:
+    """bindings.cu:18-43: colours (N, 3) from coeffs (N, D, 3)."""
+    _check(viewdirs, "viewdirs")
+    _check(coeffs, "coeffs")
+    nb = _num_bases(degree)
+    if coeffs.dim() != 3 or coeffs.size(0) != num_points or coeffs.size(1) != nb or coeffs.size(2) != 3:
+        raise RuntimeError("coeffs must have dimensions (N, D, 3)")
+    _dtype(viewdirs, "viewdirs", torch.float32)
+    _dtype(coeffs, "coeffs", torch.float32)
+    colors = torch.empty((num_points, 3), dtype=torch.float32, device=coeffs.device)
+    with torch.cuda.device(coeffs.device):
+        call("gstex_compute_sh_forward", num_points, degree, degrees_to_use, _ptr(viewdirs),
+             _ptr(coeffs), _ptr(colors), _stream(coeffs.device))
+    return colors
+
+
+def compute_sh_backward(num_points: int, degree: int, degrees_to_use: int,
+                        viewdirs: torch.Tensor, v_colors: torch.Tensor) -> torch.Tensor:
+    """bindings.cu:45-75: v_coeffs (N, D, 3)."""
+    _check(viewdirs, "viewdirs")
+    _check(v_colors, "v_colors")
+    if viewdirs.dim() != 2 or viewdirs.size(0) != num_points or viewdirs.size(1) != 3:
+        raise RuntimeError("viewdirs must have dimensions (N, 3)")
+    if v_colors.dim() != 2 or v_colors.size(0) != num_points or v_colors.size(1) != 3:
+        raise RuntimeError("v_colors must have dimensions (N, 3)")
+    nb = _num_bases(degree)
+    v_coeffs = torch.empty((num_points, nb, 3), dtype=torch.float32, device=v_colors.device)
+    with torch.cuda.device(v_colors.device):
+        call("gstex_compute_sh_backward", num_points, degree, degrees_to_use, _ptr(viewdirs),
+             _ptr(v_colors), _ptr(v_coeffs), _stream(v_colors.device))
+    return v_coeffs
+
+
+def _num_bases(degree: int) -> int:
+    return {0: 1, 1: 4, 2: 9, 3: 16}.get(int(degree), 25)
+
+
+# --------------------------------------------------------------------------- projection
+def get_aabb_2d(means, scales, glob_scale, quats, viewmat, fx, fy, cx, cy):
+    """get_aabb_2d.cu:91-126 -> (centers (N,2), extents (N,2))."""
+    for t, n in ((means, "means"), (scales, "scales"), (quats, "quats"), (viewmat, "viewmat")):
+        _check(t, n)
+        _dtype(t, n, torch.float32)
+    n = means.size(0)
+    centers = torch.empty((n, 2), dtype=torch.float32, device=means.device)
+    extents = torch.empty((n, 2), dtype=torch.float32, device=means.device)
+    with torch.cuda.device(means.device):
+        call("gstex_get_aabb_2d", n, _ptr(means), _ptr(scales), float(glob_scale), _ptr(quats),
+             _ptr(viewmat), float(fx), float(fy), float(cx), float(cy), _ptr(centers),
+             _ptr(extents), _stream(means.device))
+    return centers, extents
+
+
+def num_tiles_hit(centers, extents, img_height, img_width, block_width):
+    """get_aabb_2d.py:70-92 on the GPU -> (N,) int32."""
+    _check(centers, "centers")
+    _check(extents, "extents")
+    n = centers.size(0)
+    out = torch.empty((n,), dtype=torch.int32, device=centers.device)
+    with torch.cuda.device(centers.device):
+        call("gstex_num_tiles_hit", n, _ptr(centers), _ptr(extents), int(img_height),
+             int(img_width), int(block_width), _ptr(out), _stream(centers.device))
+    return out
+
+
+def project_points(points, viewmat, fx, fy, cx, cy, clip=False):
+    """get_aabb_2d.py:22-32 -> (pix (N,2), depths (N,))."""
+    _check(points, "points")
+    _check(viewmat, "viewmat")
+    if viewmat.numel() < 12:
+        raise RuntimeError("viewmat must hold at least three rows of four")
+    n = points.size(0)
+    pix = torch.empty((n, 2), dtype=torch.float32, device=points.device)
+    depths = torch.empty((n,), dtype=torch.float32, device=points.device)
+    with torch.cuda.device(points.device):
+        call("gstex_project_points", n, _ptr(points), _ptr(viewmat), float(fx), float(fy),
+             float(cx), float(cy), int(bool(clip)), _ptr(pix), _ptr(depths),
+             _stream(points.device))
+    return pix, depths
+
+
+def project_gaussians(means, scales, glob_scale, quats, viewmat, fx, fy, cx, cy, img_height,
+                      img_width, block_width):
+    """Fused get_aabb_2d + depth + get_num_tiles_hit_2d ->
+    (centers, extents, depths, num_tiles_hit)."""
+    for t, nm in ((means, "means"), (scales, "scales"), (quats, "quats"), (viewmat, "viewmat")):
+        _check(t, nm)
+        _dtype(t, nm, torch.float32)
+    n = means.size(0)
+    dev = means.device
+    centers = torch.empty((n, 2), dtype=torch.float32, device=dev)
+    extents = torch.empty((n, 2), dtype=torch.float32, device=dev)
+    depths = torch.empty((n,), dtype=torch.float32, device=dev)
+    tiles = torch.empty((n,), dtype=torch.int32, device=dev)
+    with torch.cuda.device(dev):
+        call("gstex_project_gaussians", n, _ptr(means), _ptr(scales), float(glob_scale),
+             _ptr(quats), _ptr(viewmat), float(fx), float(fy), float(cx), float(cy),
+             int(img_height), int(img_width), int(block_width), _ptr(centers), _ptr(extents),
+             _ptr(depths), _ptr(tiles), _stream(dev))
+    return centers, extents, depths, tiles
+
+
+# --------------------------------------------------------------------------- binning
+def map_gaussian_to_intersects(num_points, num_intersects, centers, extents, depths,
+                               cum_tiles_hit, tile_bounds, block_width, wrapped=False):
+    """bindings.cu:77-121 -> (isect_ids int64 (I,), gaussian_ids int32 (I,))."""
+    for t, nm in ((centers, "centers"), (extents, "extents"), (depths, "depths"),
+                  (cum_tiles_hit, "cum_tiles_hit")):
+        _check(t, nm)
+    _dtype(cum_tiles_hit, "cum_tiles_hit", torch.int32)
+    dev = centers.device
+    isect_ids = torch.empty((num_intersects,), dtype=torch.int64, device=dev)
+    gaussian_ids = torch.empty((num_intersects,), dtype=torch.int32, device=dev)
+    with torch.cuda.device(dev):
+        call("gstex_map_gaussian_to_intersects", int(num_points), int(num_intersects),
+             _ptr(centers), _ptr(extents), _ptr(depths), _ptr(cum_tiles_hit),
+             int(tile_bounds[0]), int(tile_bounds[1]), int(block_width), int(bool(wrapped)),
+             _ptr(isect_ids), _ptr(gaussian_ids), _stream(dev))
+    return isect_ids, gaussian_ids
+
+
+def sort_intersects(isect_ids, gaussian_ids, end_bit=64):
+    """Stable radix sort replacing torch.sort + gather (utils.py:159-160)."""
+    _check(isect_ids, "isect_ids")
+    _check(gaussian_ids, "gaussian_ids")
+    m = isect_ids.numel()
+    dev = isect_ids.device
+    keys_out = torch.empty_like(isect_ids)
+    ids_out = torch.empty_like(gaussian_ids)
+    if m == 0:
+        return keys_out, ids_out
+    with torch.cuda.device(dev):
+        ws, nb = _workspace(_lib.lib().gstex_sort_intersects_workspace_size(m), dev)
+        call("gstex_sort_intersects", m, int(end_bit), _ptr(isect_ids), _ptr(gaussian_ids),
+             _ptr(keys_out), _ptr(ids_out), _ptr(ws), nb, _stream(dev))
+    return keys_out, ids_out
+
+
+def get_tile_bin_edges(num_intersects, isect_ids_sorted, tile_bounds):
+    """bindings.cu:123-141 -> tile_bins (T, 2) int32."""
+    _check(isect_ids_sorted, "isect_ids_sorted")
+    dev = isect_ids_sorted.device
+    num_tiles = int(tile_bounds[0]) * int(tile_bounds[1])
+    tile_bins = torch.empty((num_tiles, 2), dtype=torch.int32, device=dev)
+    with torch.cuda.device(dev):
+        call("gstex_get_tile_bin_edges", int(num_intersects), _ptr(isect_ids_sorted),
+             int(tile_bounds[0]), int(tile_bounds[1]), _ptr(tile_bins), _stream(dev))
+    return tile_bins
+
+
+def bin_gaussians(num_points, num_intersects, centers, extents, depths, num_tiles_hit,
+                  tile_bounds, block_width):
+    """Depth-first binning -> (gaussian_ids_sorted (I,) int32, tile_bins (T,2) int32)."""
+    for t, nm in ((centers, "centers"), (extents, "extents"), (depths, "depths"),
+                  (num_tiles_hit, "num_tiles_hit")):
+        _check(t, nm)
+    _dtype(num_tiles_hit, "num_tiles_hit", torch.int32)
+    dev = centers.device
+    num_tiles = int(tile_bounds[0]) * int(tile_bounds[1])
+    ids = torch.empty((num_intersects,), dtype=torch.int32, device=dev)
+    bins = torch.empty((num_tiles, 2), dtype=torch.int32, device=dev)
+    with torch.cuda.device(dev):
+        ws, nb = _workspace(
+            _lib.lib().gstex_bin_gaussians_workspace_size(int(num_points), int(num_intersects)),
+            dev)
+        call("gstex_bin_gaussians", int(num_points), int(num_intersects), _ptr(centers),
+             _ptr(extents), _ptr(depths), _ptr(num_tiles_hit), int(tile_bounds[0]),
+             int(tile_bounds[1]), int(block_width), _ptr(ids), _ptr(bins), _ptr(ws), nb,
+             _stream(dev))
+    return ids, bins
+
+
+# --------------------------------------------------------------------------- raster
+_RASTER_INPUTS = ("texture_dims", "gaussian_ids_sorted", "tile_bins", "colors", "opacities",
+                  "means", "scales", "quats", "uv0", "umap", "vmap", "texture", "viewmat", "c2w",
+                  "background")
+
+
+def _raster_checks(named):
+    for name in _RASTER_INPUTS:
+        _check(named[name], name)
+    for name in ("texture_dims", "gaussian_ids_sorted", "tile_bins"):
+        _dtype(named[name], name, torch.int32)
+    for name in ("colors", "opacities", "means", "scales", "quats", "uv0", "umap", "vmap",
+                 "texture", "viewmat", "c2w", "background"):
+        _dtype(named[name], name, torch.float32)
+
+
+def texture_forward(tile_bounds, block, img_size, texture_info, texture_dims,
+                    gaussian_ids_sorted, tile_bins, colors, opacities, means, scales, glob_scale,
+                    quats, uv0, umap, vmap, texture, viewmat, c2w, fx, fy, cx, cy, settings,
+                    background):
+    """texture.cu:762-901 -> (out_img, out_depth, out_reg, out_texture, out_normal,
+    final_Ts, final_idx, depth_idx, out_reg_s)."""
+    _raster_checks(locals())
+    dev = means.device
+    W, H = int(img_size[0]), int(img_size[1])
+    C = int(texture_info[2])
+    n = means.size(0)
+    f32 = dict(dtype=torch.float32, device=dev)
+    out_img = torch.empty((H, W, colors.size(1)), **f32)
+    out_depth = torch.empty((H, W), **f32)
+    out_reg = torch.empty((H, W), **f32)
+    out_texture = torch.empty((H, W, C), **f32)
+    out_normal = torch.empty((H, W, 3), **f32)
+    final_Ts = torch.empty((H, W), **f32)
+    final_idx = torch.empty((H, W), dtype=torch.int32, device=dev)
+    depth_idx = torch.empty((H, W), dtype=torch.int32, device=dev)
+    out_reg_s = torch.empty((H, W, 3), **f32)
+    if int(block[0]) != int(block[1]):
+        raise RuntimeError("block must be square (block_width, block_width, 1)")
+    with torch.cuda.device(dev):
+        ws, nb = _workspace(_lib.lib().gstex_texture_forward_workspace_size(n), dev)
+        call("gstex_texture_forward", int(tile_bounds[0]), int(tile_bounds[1]), int(block[0]),
+             W, H, int(texture_info[0]), int(texture_info[1]), C, _ptr(texture_dims),
+             _ptr(gaussian_ids_sorted), gaussian_ids_sorted.numel(), _ptr(tile_bins),
+             _ptr(colors), _ptr(opacities), _ptr(means), _ptr(scales), n, float(glob_scale),
+             _ptr(quats), _ptr(uv0), _ptr(umap), _ptr(vmap), _ptr(texture),
+             texture.size(0) if texture.dim() > 0 else 0, _ptr(viewmat), _ptr(c2w), float(fx),
+             float(fy), float(cx), float(cy), int(settings), _ptr(background), _ptr(out_img),
+             _ptr(out_depth), _ptr(out_reg), _ptr(out_texture), _ptr(out_normal), _ptr(final_Ts),
+             _ptr(final_idx), _ptr(depth_idx), _ptr(out_reg_s), _ptr(ws), nb, _stream(dev))
+    return (out_img, out_depth, out_reg, out_texture, out_normal, final_Ts, final_idx, depth_idx,
+            out_reg_s)
+
+
+def texture_backward(img_height, img_width, block_width, texture_info, texture_dims,
+                     gaussian_ids_sorted, tile_bins, colors, opacities, means, scales, glob_scale,
+                     quats, uv0, umap, vmap, texture, viewmat, c2w, fx, fy, cx, cy, settings,
+                     background, final_Ts, final_idx, depth_idx, final_s, v_output,
+                     v_output_depth, v_output_reg, v_output_alpha, v_output_texture,
+                     v_output_normal):
+    """texture.cu:903-1053 -> (v_colors, v_opacity, v_means, v_scales, v_quats, v_uv0, v_umap,
+    v_vmap, v_texture)."""
+    named = locals()
+    _raster_checks(named)
+    for name in ("final_Ts", "final_idx", "depth_idx", "final_s", "v_output", "v_output_depth",
+                 "v_output_reg", "v_output_alpha", "v_output_texture", "v_output_normal"):
+        _check(named[name], name)
+    dev = means.device
+    n = means.size(0)
+    C = int(texture_info[2])
+    num_probs = int(texture_info[1])
+    f32 = dict(dtype=torch.float32, device=dev)
+    v_colors = torch.empty((n, colors.size(1)), **f32)
+    v_opacity = torch.empty((n, 1), **f32)
+    v_means = torch.empty((n, 3), **f32)
+    v_scales = torch.empty((n, 3), **f32)
+    v_quats = torch.empty((n, 4), **f32)
+    v_uv0 = torch.empty((n, num_probs, 2), **f32)
+    v_umap = torch.empty((n, num_probs, 3), **f32)
+    v_vmap = torch.empty((n, num_probs, 3), **f32)
+    v_texture = torch.empty((texture.size(0), C), **f32)
+    with torch.cuda.device(dev):
+        ws, nb = _workspace(_lib.lib().gstex_texture_backward_workspace_size(n), dev)
+        call("gstex_texture_backward", int(img_height), int(img_width), int(block_width),
+             int(texture_info[0]), num_probs, C, _ptr(texture_dims), _ptr(gaussian_ids_sorted),
+             gaussian_ids_sorted.numel(), _ptr(tile_bins), _ptr(colors), _ptr(opacities),
+             _ptr(means), _ptr(scales), n, float(glob_scale), _ptr(quats), _ptr(uv0), _ptr(umap),
+             _ptr(vmap), _ptr(texture), texture.size(0), _ptr(viewmat), _ptr(c2w), float(fx),
+             float(fy), float(cx), float(cy), int(settings), _ptr(background), _ptr(final_Ts),
+             _ptr(final_idx), _ptr(depth_idx), _ptr(final_s), _ptr(v_output),
+             _ptr(v_output_depth), _ptr(v_output_reg), _ptr(v_output_alpha),
+             _ptr(v_output_texture), _ptr(v_output_normal), _ptr(v_colors), _ptr(v_opacity),
+             _ptr(v_means), _ptr(v_scales), _ptr(v_quats), _ptr(v_uv0), _ptr(v_umap),
+             _ptr(v_vmap), _ptr(v_texture), _ptr(ws), nb, _stream(dev))
+    return v_colors, v_opacity, v_means, v_scales, v_quats, v_uv0, v_umap, v_vmap, v_texture
+
+
+# --------------------------------------------------------------------------- sampling
+def texture_sample_forward(texture_info, texture_dims, uvs, texture):
+    """texture_sample.cu:72-105 -> (Q, C)."""
+    for t, nm in ((texture_dims, "texture_dims"), (uvs, "uvs"), (texture, "texture")):
+        _check(t, nm)
+    dev = texture.device
+    C = int(texture_info[2])
+    q = uvs.size(0)
+    out = torch.empty((q, C), dtype=torch.float32, device=dev)
+    with torch.cuda.device(dev):
+        call("gstex_texture_sample_forward", int(texture_info[0]), int(texture_info[1]), C,
+             _ptr(texture_dims), _ptr(uvs), q, _ptr(texture), texture.size(0), _ptr(out),
+             _stream(dev))
+    return out
+
+
+def texture_sample_backward(texture_info, texture_dims, uvs, texture, v_output):
+    """Adjoint of texture_sample_forward w.r.t. the texels -> (X, C)
+    (texture_sample.cu:107-141, reading v_output)."""
+    for t, nm in ((texture_dims, "texture_dims"), (uvs, "uvs"), (texture, "texture"),
+                  (v_output, "v_output")):
+        _check(t, nm)
+    dev = texture.device
+    C = int(texture_info[2])
+    v_texture = torch.empty((texture.size(0), C), dtype=torch.float32, device=dev)
+    with torch.cuda.device(dev):
+        call("gstex_texture_sample_backward", int(texture_info[0]), int(texture_info[1]), C,
+             _ptr(texture_dims), _ptr(uvs), uvs.size(0), _ptr(texture), texture.size(0),
+             _ptr(v_output), _ptr(v_texture), _stream(dev))
+    return v_texture
+
+
+def texture_edit(*args, **kwargs):
+    """texture_edit.cu is outside this build's scope."""
+    raise NotImplementedError("texture_edit is not provided by gstex_cuda_amd")

@@ -1,0 +1,236 @@
+"""One-view forward + backward of the textured-2DGS path on the per-stage C ABI.
+
+``RenderStep`` runs what a GStex training iteration runs for one view
+(example.py:121-187,256-258) without autograd bookkeeping: SH colours,
+projection with tile counts, depth-first binning, splat records, raster
+forward, the L2 photometric loss gradient, raster backward, gradient finalize
+and the SH adjoint.  Buffers are allocated once and reused; every kernel runs
+on torch's current stream, and a kernel's duration can be read from HIP events
+recorded around it.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Dict, Optional
+
+import torch
+
+from . import _lib
+from ._lib import call
+
+
+def _p(t: Optional[torch.Tensor]):
+    return None if t is None or t.numel() == 0 else t.data_ptr()
+
+
+@dataclass
+class Camera:
+    viewmat: torch.Tensor
+    c2w: torch.Tensor
+    fx: float
+    fy: float
+    cx: float
+    cy: float
+    img_h: int
+    img_w: int
+
+
+class StageTimer:
+    """HIP-event pairs per stage, recorded on the current stream."""
+
+    def __init__(self):
+        self.events: Dict[str, list] = {}
+
+    def start(self, name: str):
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        self.events.setdefault(name, []).append([ev, None])
+
+    def stop(self, name: str):
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        self.events[name][-1][1] = ev
+
+    def mean_ms(self, name: str) -> float:
+        pairs = self.events.get(name, [])
+        if not pairs:
+            return float("nan")
+        return sum(a.elapsed_time(b) for a, b in pairs) / len(pairs)
+
+    def clear(self):
+        self.events.clear()
+
+
+class RenderStep:
+    """Forward + backward of one view with reusable device buffers.
+
+    ``params`` (device tensors, reference layouts): means (N,3), scales (N,3),
+    quats (N,4), opacities (N,1), uv0 (N,1,2), umap (N,1,3), vmap (N,1,3),
+    texture (X,C), texture_dims (N,3) int32, and either colors (N,3) or
+    sh_coeffs (N,D,3) with ``sh_degree``.
+    """
+
+    def __init__(self, params: Dict[str, torch.Tensor], cam: Camera, block_width: int = 16,
+                 settings: int = 1 << 8, background: Optional[torch.Tensor] = None,
+                 sh_degree: int = -1):
+        self.p = params
+        self.cam = cam
+        self.bw = block_width
+        self.settings = settings
+        self.sh_degree = sh_degree
+        dev = params["means"].device
+        self.dev = dev
+        n = params["means"].shape[0]
+        self.n = n
+        self.C = params["texture"].shape[1]
+        self.X = params["texture"].shape[0]
+        H, W = cam.img_h, cam.img_w
+        self.tiles = ((W + block_width - 1) // block_width, (H + block_width - 1) // block_width)
+        f32 = dict(dtype=torch.float32, device=dev)
+        i32 = dict(dtype=torch.int32, device=dev)
+        self.background = (torch.zeros(3, **f32) if background is None else background)
+        # per-Gaussian
+        self.centers = torch.empty((n, 2), **f32)
+        self.extents = torch.empty((n, 2), **f32)
+        self.depths = torch.empty((n,), **f32)
+        self.tiles_hit = torch.empty((n,), **i32)
+        self.colors = params.get("colors")
+        if sh_degree >= 0:
+            self.colors = torch.empty((n, 3), **f32)
+        self.records = torch.empty((max(_lib.lib().gstex_splat_records_size(n), 1),),
+                                   dtype=torch.uint8, device=dev)
+        self.grad_records = torch.empty((n, 32), **f32)
+        self.tile_bins = torch.empty((self.tiles[0] * self.tiles[1], 2), **i32)
+        self.ids = torch.empty((0,), **i32)
+        self.bin_ws = torch.empty((1,), dtype=torch.uint8, device=dev)
+        # per-pixel
+        self.out_img = torch.empty((H, W, 3), **f32)
+        self.out_depth = torch.empty((H, W), **f32)
+        self.out_reg = torch.empty((H, W), **f32)
+        self.out_texture = torch.empty((H, W, self.C), **f32)
+        self.out_normal = torch.empty((H, W, 3), **f32)
+        self.final_Ts = torch.empty((H, W), **f32)
+        self.final_idx = torch.empty((H, W), **i32)
+        self.depth_idx = torch.empty((H, W), **i32)
+        self.out_reg_s = torch.empty((H, W, 3), **f32)
+        self.v_img = torch.empty((H, W, 3), **f32)
+        self.v_tex = torch.empty((H, W, self.C), **f32)
+        self.v_zero1 = torch.zeros((H, W), **f32)
+        self.v_zero3 = torch.zeros((H, W, 3), **f32)
+        # gradients (reference layouts)
+        self.grads = dict(
+            v_colors=torch.empty((n, 3), **f32), v_opacity=torch.empty((n, 1), **f32),
+            v_means=torch.empty((n, 3), **f32), v_scales=torch.empty((n, 3), **f32),
+            v_quats=torch.empty((n, 4), **f32), v_uv0=torch.empty((n, 1, 2), **f32),
+            v_umap=torch.empty((n, 1, 3), **f32), v_vmap=torch.empty((n, 1, 3), **f32),
+            v_texture=torch.empty((self.X, self.C), **f32))
+        if sh_degree >= 0:
+            self.grads["v_sh_coeffs"] = torch.empty_like(params["sh_coeffs"])
+        self.num_intersects = 0
+
+    def _stream(self):
+        return torch.cuda.current_stream(self.dev).cuda_stream
+
+    def _ensure_bins(self, m: int):
+        if self.ids.numel() < m:
+            self.ids = torch.empty((int(m * 1.25) + 1024,), dtype=torch.int32, device=self.dev)
+        need = _lib.lib().gstex_bin_gaussians_workspace_size(self.n, m)
+        if self.bin_ws.numel() < need:
+            self.bin_ws = torch.empty((int(need * 1.25) + 4096,), dtype=torch.uint8,
+                                      device=self.dev)
+
+    def forward(self, timer: Optional[StageTimer] = None):
+        p, cam, s = self.p, self.cam, self._stream()
+        n = self.n
+        if self.sh_degree >= 0:
+            viewdirs = p["means"] - cam.c2w[:3, 3]
+            call("gstex_compute_sh_forward", n, self.sh_degree, self.sh_degree,
+                 _p(viewdirs), _p(p["sh_coeffs"]), _p(self.colors), s)
+            self._viewdirs = viewdirs
+        call("gstex_project_gaussians", n, _p(p["means"]), _p(p["scales"]), 1.0, _p(p["quats"]),
+             _p(cam.viewmat), cam.fx, cam.fy, cam.cx, cam.cy, cam.img_h, cam.img_w, self.bw,
+             _p(self.centers), _p(self.extents), _p(self.depths), _p(self.tiles_hit), s)
+        m = int(self.tiles_hit.sum(dtype=torch.int64).item())
+        self.num_intersects = m
+        self._ensure_bins(m)
+        if m > 0:
+            call("gstex_bin_gaussians", n, m, _p(self.centers), _p(self.extents),
+                 _p(self.depths), _p(self.tiles_hit), self.tiles[0], self.tiles[1], self.bw,
+                 _p(self.ids), _p(self.tile_bins), _p(self.bin_ws), self.bin_ws.numel(), s)
+        else:
+            self.tile_bins.zero_()
+        call("gstex_build_splat_records", n, _p(p["means"]), _p(p["scales"]), 1.0,
+             _p(p["quats"]), _p(self.colors), _p(p["opacities"]), _p(p["uv0"]), _p(p["umap"]),
+             _p(p["vmap"]), _p(p["texture_dims"]), _p(cam.viewmat), cam.fx, cam.fy, cam.cx,
+             cam.cy, _p(self.records), s)
+        if timer:
+            timer.start("raster_forward")
+        call("gstex_raster_forward", cam.img_w, cam.img_h, self.bw, self.C, self.settings,
+             _p(self.records), n, _p(self.ids), m, _p(self.tile_bins), _p(p["texture"]), self.X,
+             _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy, cam.cx, cam.cy, _p(self.background),
+             _p(self.out_img), _p(self.out_depth), _p(self.out_reg), _p(self.out_texture),
+             _p(self.out_normal), _p(self.final_Ts), _p(self.final_idx), _p(self.depth_idx),
+             _p(self.out_reg_s), s)
+        if timer:
+            timer.stop("raster_forward")
+
+    def l2_loss_grad(self, target: torch.Tensor):
+        """d/d(img, texture) of mean((out_img + out_texture - target)^2)."""
+        scale = 2.0 / target.numel()
+        torch.add(self.out_img, self.out_texture, out=self.v_img)
+        self.v_img.sub_(target).mul_(scale)
+        if self.C == 3:
+            self.v_tex.copy_(self.v_img)
+        else:
+            self.v_tex.zero_()
+            self.v_tex[..., :3].copy_(self.v_img)
+
+    def backward(self, timer: Optional[StageTimer] = None, v_depth=None, v_reg=None,
+                 v_alpha=None, v_normal=None):
+        p, cam, s, g = self.p, self.cam, self._stream(), self.grads
+        n, m = self.n, self.num_intersects
+        call("gstex_fill_f32", _p(self.grad_records), self.grad_records.numel(), 0.0, s)
+        call("gstex_fill_f32", _p(g["v_texture"]), g["v_texture"].numel(), 0.0, s)
+        v_depth = self.v_zero1 if v_depth is None else v_depth
+        v_reg = self.v_zero1 if v_reg is None else v_reg
+        v_alpha = self.v_zero1 if v_alpha is None else v_alpha
+        v_normal = self.v_zero3 if v_normal is None else v_normal
+        if timer:
+            timer.start("raster_backward")
+        call("gstex_raster_backward", cam.img_w, cam.img_h, self.bw, self.C, self.settings,
+             _p(self.records), n, _p(self.ids), m, _p(self.tile_bins), _p(p["texture"]), self.X,
+             _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy, cam.cx, cam.cy, _p(self.background),
+             _p(self.final_Ts), _p(self.final_idx), _p(self.depth_idx), _p(self.out_reg_s),
+             _p(self.v_img), _p(v_depth), _p(v_reg), _p(v_alpha), _p(self.v_tex), _p(v_normal),
+             _p(self.grad_records), _p(g["v_texture"]), s)
+        if timer:
+            timer.stop("raster_backward")
+        call("gstex_finalize_gradients", n, _p(self.grad_records), _p(p["means"]),
+             _p(p["scales"]), 1.0, _p(p["quats"]), _p(cam.viewmat), cam.fx, cam.fy,
+             _p(g["v_colors"]), _p(g["v_opacity"]), _p(g["v_means"]), _p(g["v_scales"]),
+             _p(g["v_quats"]), _p(g["v_uv0"]), _p(g["v_umap"]), _p(g["v_vmap"]), s)
+        if self.sh_degree >= 0:
+            call("gstex_compute_sh_backward", n, self.sh_degree, self.sh_degree,
+                 _p(self._viewdirs), _p(g["v_colors"]), _p(g["v_sh_coeffs"]), s)
+
+    def step(self, target: torch.Tensor, timer: Optional[StageTimer] = None):
+        """Forward, L2 loss gradient against ``target``, backward."""
+        self.forward(timer)
+        self.l2_loss_grad(target)
+        self.backward(timer)
+
+
+def scene_to_step(scene, device, **kw) -> RenderStep:
+    """RenderStep for a ``synthetic.Scene``."""
+    t = scene.to_torch(device)
+    params = dict(means=t["means"], scales=t["scales"], quats=t["quats"],
+                  opacities=t["opacities"], uv0=t["uv0"], umap=t["umap"], vmap=t["vmap"],
+                  texture=t["texture"], texture_dims=t["texture_dims"])
+    if scene.sh_coeffs is not None:
+        params["sh_coeffs"] = t["sh_coeffs"]
+    else:
+        params["colors"] = t["colors"]
+    cam = Camera(viewmat=t["viewmat"], c2w=t["c2w"], fx=scene.fx, fy=scene.fy, cx=scene.cx,
+                 cy=scene.cy, img_h=scene.img_h, img_w=scene.img_w)
+    return RenderStep(params, cam, block_width=scene.block_width, settings=scene.settings,
+                      background=t["background"], sh_degree=scene.sh_degree, **kw)

@@ -1,0 +1,213 @@
+/*
+ * gstex_hip — C ABI of the gfx950 textured 2D-Gaussian rasteriser.
+ *
+ * Every entry point takes device pointers, plain sizes and scalars, and a
+ * hipStream_t passed as void* (NULL = the legacy default stream).  Nothing
+ * allocates device memory: outputs and workspaces are caller-owned, with
+ * workspace sizes given by the *_workspace_size functions.  Each call returns
+ * 0 on success or a nonzero code; gstex_last_error() then describes the most
+ * recent failure of the calling thread.  Launches are asynchronous.
+ *
+ * Layouts are the reference's (row-major, contiguous):
+ *   means/scales (N,3) f32, quats (N,4) f32 (w,x,y,z), colors (N,3),
+ *   opacities (N,1), uv0 (N,1,2), umap/vmap (N,1,3), texture_dims (N,3) i32
+ *   {h, w, start}, texture (X,C) f32, viewmat/c2w (4,4) row-major,
+ *   images (H,W,k) f32, tile_bins (T,2) i32.
+ *
+ * Each declaration cites the reference interface it replaces
+ * (paths relative to gstex_cuda/ in the reference tree).
+ */
+#ifndef GSTEX_HIP_H
+#define GSTEX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GSTEX_OK 0
+#define GSTEX_ERR_ARG 1    /* invalid argument (shape, size, null pointer, setting) */
+#define GSTEX_ERR_HIP 2    /* a HIP runtime / launch error */
+#define GSTEX_ERR_SPACE 3  /* workspace too small */
+
+/* Description of the last failure on this thread (empty string if none). */
+const char *gstex_last_error(void);
+/* ABI version of this library. */
+int gstex_abi_version(void);
+
+/* Replaces get_aabb_2d_tensor (cuda/csrc/get_aabb_2d.cu:91-126), bound as
+ * _C.get_aabb_2d (cuda/csrc/ext.cpp:17).  centers/extents: (N,2) f32. */
+int gstex_get_aabb_2d(int num_points, const float *means, const float *scales, float glob_scale,
+                      const float *quats, const float *viewmat, float fx, float fy, float cx,
+                      float cy, float *centers, float *extents, void *stream);
+
+/* Replaces the torch function get_num_tiles_hit_2d (get_aabb_2d.py:70-92). */
+int gstex_num_tiles_hit(int num_points, const float *centers, const float *extents, int img_height,
+                        int img_width, int block_width, int32_t *num_tiles_hit, void *stream);
+
+/* Fused caller-side projection (example.py:146-152): get_aabb_2d plus the
+ * view-space depth of project_points (get_aabb_2d.py:22-32) plus
+ * get_num_tiles_hit_2d (get_aabb_2d.py:70-92), one pass over the Gaussians. */
+int gstex_project_gaussians(int num_points, const float *means, const float *scales,
+                            float glob_scale, const float *quats, const float *viewmat, float fx,
+                            float fy, float cx, float cy, int img_height, int img_width,
+                            int block_width, float *centers, float *extents, float *depths,
+                            int32_t *num_tiles_hit, void *stream);
+
+/* Replaces the torch function project_points (get_aabb_2d.py:22-32): pixel
+ * coordinates (N,2) and view-space depth (N,) of points (N,3) under the first
+ * three rows of a row-major viewmat; clip != 0 clamps view z to [0.01, 1000]
+ * first (and returns the clamped depth). */
+int gstex_project_points(int num_points, const float *points, const float *viewmat, float fx,
+                         float fy, float cx, float cy, int clip, float *pix, float *depths,
+                         void *stream);
+
+/* Replaces compute_sh_forward_tensor (cuda/csrc/bindings.cu:18-43).
+ * coeffs (N, bases(degree), 3); degrees_to_use <= degree. */
+int gstex_compute_sh_forward(int num_points, int degree, int degrees_to_use, const float *viewdirs,
+                             const float *coeffs, float *colors, void *stream);
+
+/* Replaces compute_sh_backward_tensor (cuda/csrc/bindings.cu:45-75).
+ * v_coeffs (N, bases(degree), 3) is fully written. */
+int gstex_compute_sh_backward(int num_points, int degree, int degrees_to_use,
+                              const float *viewdirs, const float *v_colors, float *v_coeffs,
+                              void *stream);
+
+/* Replaces map_gaussian_to_intersects_tensor (cuda/csrc/bindings.cu:77-121).
+ * isect_ids (I,) int64 and gaussian_ids (I,) int32 are fully written
+ * (zero where a Gaussian counts a slot but writes none).  wrapped must be 0. */
+int gstex_map_gaussian_to_intersects(int num_points, int num_intersects, const float *centers,
+                                     const float *extents, const float *depths,
+                                     const int32_t *cum_tiles_hit, int tiles_x, int tiles_y,
+                                     int block_width, int wrapped, int64_t *isect_ids,
+                                     int32_t *gaussian_ids, void *stream);
+
+/* Replaces torch.sort + torch.gather in bin_and_sort_gaussians (utils.py:159-160):
+ * stable ascending LSD radix sort of the int64 keys over bits [0, end_bit)
+ * (end_bit <= 64; keys must be non-negative when end_bit < 64), carrying the
+ * int32 ids. */
+size_t gstex_sort_intersects_workspace_size(int num_intersects);
+int gstex_sort_intersects(int num_intersects, int end_bit, const int64_t *isect_ids,
+                          const int32_t *gaussian_ids, int64_t *isect_ids_sorted,
+                          int32_t *gaussian_ids_sorted, void *workspace, size_t workspace_bytes,
+                          void *stream);
+
+/* Replaces get_tile_bin_edges_tensor (cuda/csrc/bindings.cu:123-141).
+ * tile_bins (tiles_x*tiles_y, 2) int32 is fully written. */
+int gstex_get_tile_bin_edges(int num_intersects, const int64_t *isect_ids_sorted, int tiles_x,
+                             int tiles_y, int32_t *tile_bins, void *stream);
+
+/* Replaces bin_and_sort_gaussians (utils.py:106-162) where only the sorted ids
+ * and tile bins are consumed (texture.py:207-222): identical outputs, computed
+ * by sorting the Gaussians by depth once and stable-sorting the emitted
+ * intersects by tile.  num_intersects = sum(num_tiles_hit). */
+size_t gstex_bin_gaussians_workspace_size(int num_points, int num_intersects);
+int gstex_bin_gaussians(int num_points, int num_intersects, const float *centers,
+                        const float *extents, const float *depths, const int32_t *num_tiles_hit,
+                        int tiles_x, int tiles_y, int block_width, int32_t *gaussian_ids_sorted,
+                        int32_t *tile_bins, void *workspace, size_t workspace_bytes, void *stream);
+
+/* Replaces texture_forward_tensor (cuda/csrc/texture.cu:762-901).  The nine
+ * outputs are fully written.  num_ids = length of gaussian_ids_sorted,
+ * num_texels = rows of texture, channels = texture_info[2]; num_probs must be 1. */
+size_t gstex_texture_forward_workspace_size(int num_points);
+int gstex_texture_forward(int tiles_x, int tiles_y, int block_width, int img_width, int img_height,
+                          int num_charts, int num_probs, int channels,
+                          const int32_t *texture_dims, const int32_t *gaussian_ids_sorted,
+                          int num_ids, const int32_t *tile_bins, const float *colors,
+                          const float *opacities, const float *means, const float *scales,
+                          int num_points, float glob_scale, const float *quats, const float *uv0,
+                          const float *umap, const float *vmap, const float *texture,
+                          int num_texels, const float *viewmat, const float *c2w, float fx,
+                          float fy, float cx, float cy, int settings, const float *background,
+                          float *out_img, float *out_depth, float *out_reg, float *out_texture,
+                          float *out_normal, float *final_Ts, int32_t *final_idx,
+                          int32_t *depth_idx, float *out_reg_s, void *workspace,
+                          size_t workspace_bytes, void *stream);
+
+/* Replaces texture_backward_tensor (cuda/csrc/texture.cu:903-1053).  The nine
+ * gradient outputs are fully written (v_scales[:,2] = 0). */
+size_t gstex_texture_backward_workspace_size(int num_points);
+int gstex_texture_backward(int img_height, int img_width, int block_width, int num_charts,
+                           int num_probs, int channels, const int32_t *texture_dims,
+                           const int32_t *gaussian_ids_sorted, int num_ids,
+                           const int32_t *tile_bins, const float *colors, const float *opacities,
+                           const float *means, const float *scales, int num_points,
+                           float glob_scale, const float *quats, const float *uv0,
+                           const float *umap, const float *vmap, const float *texture,
+                           int num_texels, const float *viewmat, const float *c2w, float fx,
+                           float fy, float cx, float cy, int settings, const float *background,
+                           const float *final_Ts, const int32_t *final_idx,
+                           const int32_t *depth_idx, const float *final_s, const float *v_output,
+                           const float *v_output_depth, const float *v_output_reg,
+                           const float *v_output_alpha, const float *v_output_texture,
+                           const float *v_output_normal, float *v_colors, float *v_opacity,
+                           float *v_means, float *v_scales, float *v_quats, float *v_uv0,
+                           float *v_umap, float *v_vmap, float *v_texture, void *workspace,
+                           size_t workspace_bytes, void *stream);
+
+/* ---- Per-stage render path ------------------------------------------------
+ * texture_forward_tensor / texture_backward_tensor (cuda/csrc/texture.cu:762-1053)
+ * split into their stages, so a training step builds the per-Gaussian splat
+ * records once for both passes and each kernel can be timed alone:
+ *   gstex_build_splat_records -> gstex_raster_forward -> gstex_raster_backward
+ *   -> gstex_finalize_gradients.
+ * records: gstex_splat_records_size(N) bytes, 256-byte aligned.
+ * grad_records: N x 32 float32 accumulators (the raster backward adds into them,
+ * so the caller zeroes them and v_texture first). */
+size_t gstex_splat_records_size(int num_points);
+int gstex_build_splat_records(int num_points, const float *means, const float *scales,
+                              float glob_scale, const float *quats, const float *colors,
+                              const float *opacities, const float *uv0, const float *umap,
+                              const float *vmap, const int32_t *texture_dims,
+                              const float *viewmat, float fx, float fy, float cx, float cy,
+                              void *records, void *stream);
+int gstex_raster_forward(int img_width, int img_height, int block_width, int channels,
+                         int settings, const void *records, int num_points,
+                         const int32_t *gaussian_ids_sorted, int num_ids,
+                         const int32_t *tile_bins, const float *texture, int num_texels,
+                         const float *viewmat, const float *c2w, float fx, float fy, float cx,
+                         float cy, const float *background, float *out_img, float *out_depth,
+                         float *out_reg, float *out_texture, float *out_normal, float *final_Ts,
+                         int32_t *final_idx, int32_t *depth_idx, float *out_reg_s, void *stream);
+int gstex_raster_backward(int img_width, int img_height, int block_width, int channels,
+                          int settings, const void *records, int num_points,
+                          const int32_t *gaussian_ids_sorted, int num_ids,
+                          const int32_t *tile_bins, const float *texture, int num_texels,
+                          const float *viewmat, const float *c2w, float fx, float fy, float cx,
+                          float cy, const float *background, const float *final_Ts,
+                          const int32_t *final_idx, const int32_t *depth_idx,
+                          const float *final_s, const float *v_output,
+                          const float *v_output_depth, const float *v_output_reg,
+                          const float *v_output_alpha, const float *v_output_texture,
+                          const float *v_output_normal, float *grad_records, float *v_texture,
+                          void *stream);
+int gstex_finalize_gradients(int num_points, const float *grad_records, const float *means,
+                             const float *scales, float glob_scale, const float *quats,
+                             const float *viewmat, float fx, float fy, float *v_colors,
+                             float *v_opacity, float *v_means, float *v_scales, float *v_quats,
+                             float *v_uv0, float *v_umap, float *v_vmap, void *stream);
+/* Fills n float32 values with v (a plain kernel, safe inside captured graphs). */
+int gstex_fill_f32(float *dst, size_t n, float v, void *stream);
+
+/* Replaces texture_sample_forward_tensor (cuda/csrc/texture_sample.cu:72-105). */
+int gstex_texture_sample_forward(int num_charts, int num_probs, int channels,
+                                 const int32_t *texture_dims, const float *uvs, int num_queries,
+                                 const float *texture, int num_texels, float *output,
+                                 void *stream);
+
+/* Replaces texture_sample_backward_tensor (cuda/csrc/texture_sample.cu:107-141)
+ * with the adjoint w.r.t. the texels computed from v_output (the reference
+ * kernel reads v_texture there, texture_sample.cu:58).  v_texture fully written. */
+int gstex_texture_sample_backward(int num_charts, int num_probs, int channels,
+                                  const int32_t *texture_dims, const float *uvs, int num_queries,
+                                  const float *texture, int num_texels, const float *v_output,
+                                  float *v_texture, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSTEX_HIP_H */

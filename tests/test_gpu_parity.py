@@ -1,0 +1,382 @@
+"""GPU parity: every HIP entry point, through the C ABI, against the oracle on
+the same seeded inputs.
+
+Tolerances (FP32 kernels vs the FP64 oracle unless stated):
+  projection    centres/extents within 1e-4 px + 1e-5 relative
+  integer work  tile counts, intersect keys, ids, sorted order, bin edges:
+                bit-exact given the same float inputs
+  SH, sampling  1e-5 absolute
+  raster fwd    >= 99.9% of pixels within 1e-4 of the FP64 oracle, image
+                PSNR >= 60 dB, final_idx equal on >= 99.9% of pixels (a pair
+                whose alpha sits on the 1/255 or 1e-4 thresholds may round the
+                other way in FP32)
+  raster bwd    per gradient tensor: relative L2 error <= 2e-3 and 99% of
+                entries within 2e-3 x the tensor's max magnitude
+"""
+import numpy as np
+import pytest
+import torch
+
+from gpu_helpers import (DEV, gpu_backward, gpu_forward, grad_close, log, np_, oracle_with,
+                         project_and_bin, random_grads, raster_inputs, to_dev)
+from helpers import camera_facing_scene, frac_close, load_kat, psnr, tile_bounds
+
+pytestmark = pytest.mark.gpu
+
+KAT = load_kat()
+
+
+def scene(workload=1, **kw):
+    from gstex_cuda_amd.synthetic import make_scene
+
+    return make_scene(workload, **kw)
+
+
+# ------------------------------------------------------------------ projection
+@pytest.mark.parametrize("n", [1, 1000, 70001])
+def test_aabb_matches_oracle(oracle, n):
+    from gstex_cuda_amd import cuda as _C
+
+    s = scene(2, num_points=n, seed=n)
+    log("aabb", n)
+    c, e = _C.get_aabb_2d(to_dev(s.means), to_dev(s.scales), 1.0, to_dev(s.quats),
+                          to_dev(s.viewmat), s.fx, s.fy, s.cx, s.cy)
+    c64, e64 = oracle.get_aabb_2d(s.means, s.scales, 1.0, s.quats, s.viewmat, s.fx, s.fy, s.cx,
+                                  s.cy, "f64")
+    np.testing.assert_allclose(np_(c), c64, rtol=1e-5, atol=1e-4)
+    np.testing.assert_allclose(np_(e), e64, rtol=1e-5, atol=1e-4)
+    # the fused projection returns the same boxes, the view depth and the tile counts
+    pb = project_and_bin(s)
+    assert torch.equal(pb["centers"], c) and torch.equal(pb["extents"], e)
+    np.testing.assert_allclose(np_(pb["depths"]), s.means[:, 2], rtol=1e-6)
+    want = oracle.num_tiles_hit(np_(c), np_(e), s.img_h, s.img_w, 16)
+    assert np.array_equal(np_(pb["num_tiles_hit"]), want)
+
+
+def test_project_points_and_clip():
+    from gstex_cuda_amd.get_aabb_2d import project_points
+
+    pts = np.array([[0.1, 0.2, 2.0], [0.0, 0.0, 0.001], [1.0, -1.0, 2000.0]], np.float32)
+    vm = to_dev(np.eye(4, dtype=np.float32))
+    pix, d = project_points(to_dev(pts), vm, (100.0, 100.0, 50.0, 40.0))
+    np.testing.assert_allclose(np_(d), pts[:, 2], rtol=1e-7)
+    np.testing.assert_allclose(np_(pix)[0], [100 * 0.1 / 2.0 + 50, 100 * 0.2 / 2.0 + 40], rtol=1e-6)
+    _, dc = project_points(to_dev(pts), vm[:3], (100.0, 100.0, 50.0, 40.0), clip=True)
+    np.testing.assert_allclose(np_(dc), [2.0, 0.01, 1000.0], rtol=1e-7)
+
+
+# ------------------------------------------------------------------ binning
+def _binning_case(oracle, s, bw):
+    from gstex_cuda_amd import cuda as _C
+    from gstex_cuda_amd.utils import bin_and_sort_gaussians, compute_cumulative_intersects
+
+    pb = project_and_bin(s, bw)
+    c, e, d, nt = pb["centers"], pb["extents"], pb["depths"], pb["num_tiles_hit"]
+    m, cum = compute_cumulative_intersects(nt)
+    tb = tile_bounds(s.img_w, s.img_h, bw)
+    log("binning", s.num_points, s.img_w, s.img_h, bw, "I =", m)
+    keys, ids, ks, isd, bins = bin_and_sort_gaussians(s.num_points, m, c, e, d, cum, tb, bw)
+    o_keys, o_ids = oracle.map_gaussian_to_intersects(np_(c), np_(e), np_(d), np_(cum), tb, bw)
+    assert np.array_equal(np_(keys), o_keys)
+    assert np.array_equal(np_(ids), o_ids)
+    o_ks, o_is = oracle.sort_pairs(o_keys, o_ids)
+    assert np.array_equal(np_(ks), o_ks)
+    assert np.array_equal(np_(isd), o_is)
+    o_bins = oracle.tile_bin_edges(o_ks, tb[0] * tb[1])
+    assert np.array_equal(np_(bins), o_bins)
+    # depth-first binning used by the renderer gives the same ids and bins
+    assert np.array_equal(np_(pb["ids"]), o_is)
+    assert np.array_equal(np_(pb["bins"]), o_bins)
+    return m
+
+
+@pytest.mark.parametrize("bw", [16, 8, 5])
+def test_binning_bit_exact(oracle, bw):
+    s = scene(1, num_points=3000, seed=40 + bw, img_w=200, img_h=136)
+    assert _binning_case(oracle, s, bw) > 0
+
+
+def test_binning_zero_slots_and_ties(oracle):
+    # clipped Gaussians with centres inside the image (counted, never written)
+    # and many equal depths (ties broken by Gaussian id)
+    s = scene(1, num_points=2000, seed=9)
+    s.means[:50, 2] = -1.5   # behind the camera
+    s.means[:50, :2] *= -0.1
+    s.means[100:600, 2] = 3.0  # identical depth bits
+    _binning_case(oracle, s, 16)
+
+
+def test_binning_c2_scale(oracle):
+    # full-resolution 1080p with 200k Gaussians: exact against the oracle
+    s = scene(2, num_points=200_000)
+    assert _binning_case(oracle, s, 16) > 1_000_000
+
+
+def test_sort_full_64bit_signed():
+    from gstex_cuda_amd import cuda as _C
+
+    rng = np.random.default_rng(3)
+    keys = rng.integers(-(1 << 62), 1 << 62, 100_000, dtype=np.int64)
+    keys[:1000] = keys[1000:2000]  # duplicates
+    ids = np.arange(100_000, dtype=np.int32)
+    ks, is_ = _C.sort_intersects(to_dev(keys), to_dev(ids), end_bit=64)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(np_(ks), keys[order])
+    assert np.array_equal(np_(is_), ids[order])
+
+
+def test_empty_image_and_no_intersects(oracle):
+    from gstex_cuda_amd.texture import texture_gaussians
+
+    s = scene(1, num_points=100, seed=3)
+    s.means[:, 2] = -5.0  # everything behind the camera, outside the image
+    s.means[:, :2] = 50.0
+    pb = project_and_bin(s)
+    assert pb["num_intersects"] == 0
+    r = {k: v.clone().requires_grad_(k in ("colors", "means")) for k, v in
+         raster_inputs(s).items()}
+    out = texture_gaussians((100, 1, 3), r["texture_dims"], pb["centers"], pb["extents"],
+                            pb["depths"], pb["num_tiles_hit"], r["colors"], r["opacities"],
+                            r["means"], r["scales"], 1.0, r["quats"], r["uv0"], r["umap"],
+                            r["vmap"], r["texture"], r["viewmat"], r["c2w"], s.fx, s.fy, s.cx,
+                            s.cy, s.img_h, s.img_w, 16, s.settings, r["background"])
+    img = np_(out[0])
+    assert np.allclose(img, s.background)
+    (out[0].sum() + out[4].sum()).backward()
+    assert torch.count_nonzero(r["colors"].grad) == 0
+
+
+# ------------------------------------------------------------------ SH
+@pytest.mark.parametrize("degree,used", [(0, 0), (1, 1), (2, 1), (3, 3), (4, 4), (4, 2)])
+def test_sh_matches_oracle(oracle, degree, used):
+    from gstex_cuda_amd import cuda as _C
+
+    rng = np.random.default_rng(degree * 7 + used)
+    n, nb = 5000, (degree + 1) ** 2
+    dirs = rng.standard_normal((n, 3)).astype(np.float32)
+    co = (0.3 * rng.standard_normal((n, nb, 3))).astype(np.float32)
+    vc = rng.standard_normal((n, 3)).astype(np.float32)
+    col = _C.compute_sh_forward(n, degree, used, to_dev(dirs), to_dev(co))
+    np.testing.assert_allclose(np_(col), oracle.sh_forward(degree, used, dirs, co, "f64"), atol=1e-5)
+    vco = _C.compute_sh_backward(n, degree, used, to_dev(dirs), to_dev(vc))
+    np.testing.assert_allclose(np_(vco), oracle.sh_backward(degree, used, dirs, vc, "f64"), atol=1e-5)
+
+
+def test_sh_adam_converges_like_reference_test():
+    # tests/test_sh.py of the reference: degree 4, one point, 1000 Adam steps to colour 0.5
+    from gstex_cuda_amd import sh
+
+    g = torch.Generator(device="cpu").manual_seed(0)
+    vd = torch.randn(1, 3, generator=g)
+    vd = (vd / vd.norm(dim=-1, keepdim=True)).to(DEV)
+    coeffs = torch.rand(1, 25, 3, generator=g).to(DEV).requires_grad_(True)
+    opt = torch.optim.Adam([coeffs], lr=1e-2)
+    gt = torch.full((1, 3), 0.5, device=DEV)
+    for _ in range(1000):
+        opt.zero_grad()
+        col = sh.spherical_harmonics(4, vd, coeffs)
+        loss = ((col - gt) ** 2).mean()
+        loss.backward()
+        opt.step()
+    torch.testing.assert_close(sh.spherical_harmonics(4, vd, coeffs), gt)
+
+
+# ------------------------------------------------------------------ sampling
+def test_texture_sample_reference_scenario(oracle):
+    # tests/test_sample.py of the reference, seeded: 5 charts of 2..7 x 2..8 texels,
+    # 10 channels, 100 queries in [0, 1)
+    from gstex_cuda_amd.texture_sample import texture_sample
+
+    rng = np.random.default_rng(1)
+    sz = 5
+    h = rng.integers(0, 6, sz) + 2
+    w = rng.integers(0, 7, sz) + 2
+    start = np.concatenate([[0], np.cumsum(h * w)[:-1]])
+    dims = np.stack([h, w, start], 1).astype(np.int32)
+    tex = rng.uniform(0, 1, (int((h * w).sum()), 10)).astype(np.float32)
+    uvs = rng.uniform(0, 1, (100, 2)).astype(np.float32)
+    qd = dims[rng.integers(0, sz, 100)]
+    t = to_dev(tex).requires_grad_(True)
+    out = texture_sample((sz, 1, 10), to_dev(qd), t, to_dev(uvs))
+    np.testing.assert_allclose(np_(out), oracle.texture_sample(qd, uvs, tex, "f64"), atol=1e-5)
+    v = rng.standard_normal((100, 10)).astype(np.float32)
+    out.backward(to_dev(v))
+    np.testing.assert_allclose(np_(t.grad), oracle.texture_sample_vjp(qd, uvs, v, tex.shape[0], "f64"),
+                               atol=1e-5)
+
+
+def test_texture_sample_affine_kat():
+    from gstex_cuda_amd import cuda as _C
+
+    bl = KAT["bilinear"]
+    tex = np.array(bl["texture"], np.float32)
+    dims = np.array([q["dims"] for q in bl["queries"]], np.int32)
+    uvs = np.array([q["uv"] for q in bl["queries"]], np.float32)
+    out = _C.texture_sample_forward((4, 1, 3), to_dev(dims), to_dev(uvs), to_dev(tex))
+    np.testing.assert_allclose(np_(out), [q["value"] for q in bl["queries"]], atol=2e-5)
+
+
+# ------------------------------------------------------------------ raster forward
+def _check_forward(g, o, label, min_psnr=60.0):
+    img_err = np.abs(np_(g["out_img"]) - o["out_img"])
+    p = psnr(np_(g["out_img"]), o["out_img"])
+    fc = frac_close(np_(g["out_img"]), o["out_img"], 1e-4)
+    fi = frac_close(np_(g["final_idx"]), o["final_idx"], 0)
+    log(label, "psnr %.1f" % p, "frac1e-4 %.5f" % fc, "maxerr %.2e" % img_err.max(), "idx %.5f" % fi)
+    assert p >= min_psnr and fc >= 0.999 and fi >= 0.999
+    for k in ("out_texture", "out_normal", "out_depth", "out_reg", "final_Ts"):
+        f = frac_close(np_(g[k]), o[k], 1e-4, 1e-4)
+        assert f >= 0.999, (label, k, f)
+    assert frac_close(np_(g["depth_idx"]), o["depth_idx"], 0) >= 0.999
+
+
+@pytest.mark.parametrize("case", ["c1", "nonsquare_bw16", "bw8", "bw5", "nearest", "blur",
+                                  "ndc_noflip", "viz_normal_flip"])
+def test_forward_matches_oracle(oracle, case):
+    bw, settings = 16, 1 << 8
+    if case == "c1":
+        s = scene(1)
+    else:
+        s = scene(1, num_points=1500, seed=sum(map(ord, case)), img_w=200, img_h=136)
+    if case == "bw8":
+        bw = 8
+    if case == "bw5":
+        bw = 5
+    if case == "nearest":
+        settings |= 1 << 2
+    if case == "blur":
+        settings |= 1 << 9
+    if case == "ndc_noflip":
+        settings = 1 << 10
+    if case == "viz_normal_flip":
+        settings |= 1 << 15
+    pb = project_and_bin(s, bw)
+    g, _ = gpu_forward(s, pb["ids"], pb["bins"], settings=settings, bw=bw)
+    o = oracle.texture_forward(oracle_with(oracle, s, pb["ids"], pb["bins"], settings=settings,
+                                           bw=bw), "f64")
+    _check_forward(g, o, case)
+
+
+@pytest.mark.parametrize("channels", [1, 2, 4, 7, 16, 33])
+def test_forward_channel_counts(oracle, channels):
+    s = scene(1, num_points=800, seed=channels, img_w=96, img_h=80)
+    rng = np.random.default_rng(channels)
+    s.texture = rng.uniform(0, 1, (s.texture.shape[0], channels)).astype(np.float32)
+    pb = project_and_bin(s)
+    g, _ = gpu_forward(s, pb["ids"], pb["bins"])
+    o = oracle.texture_forward(oracle_with(oracle, s, pb["ids"], pb["bins"]), "f64")
+    _check_forward(g, o, "C=%d" % channels)
+
+
+def test_forward_closed_form_single_splat():
+    k = KAT["splat1"]
+    s = camera_facing_scene([k["mean"]], [k["scale"]], [k["opacity"]], [k["color"]], k["W"],
+                            k["H"], k["f"], k["cx"], k["cy"], texel=k["texel"],
+                            background=k["background"])
+    pb = project_and_bin(s)
+    g, _ = gpu_forward(s, pb["ids"], pb["bins"])
+    np.testing.assert_allclose(np_(g["out_img"]), np.array(k["img"]), atol=2e-6)
+    np.testing.assert_allclose(1 - np_(g["final_Ts"]), np.array(k["alpha"]), atol=2e-6)
+
+
+def test_forward_invariants_c2_full_size(oracle):
+    # 1080p, 500k Gaussians: sum of weights + T = 1 everywhere, and a sample of
+    # tiles rendered exactly by the oracle
+    s = scene(2)
+    s.colors[:] = 1.0
+    pb = project_and_bin(s)
+    log("c2 intersects", pb["num_intersects"])
+    g, _ = gpu_forward(s, pb["ids"], pb["bins"])
+    T = np_(g["final_Ts"])
+    np.testing.assert_allclose(np_(g["out_img"]), np.repeat((1 - T)[..., None], 3, -1), atol=2e-5)
+    assert np.all((T > 1e-4) & (T <= 1.0))
+    assert np.all(np_(g["out_reg"]) >= -1e-6)
+    tiles = tile_bounds(s.img_w, s.img_h, 16)
+    nt = tiles[0] * tiles[1]
+    counts = np.diff(np_(pb["bins"]), axis=1)[:, 0]
+    rng = np.random.default_rng(0)
+    pick = np.concatenate([rng.choice(nt, 12, replace=False), np.argsort(counts)[-4:]])
+    mask = np.zeros(nt, np.uint8)
+    mask[pick] = 1
+    sc = oracle_with(oracle, s, pb["ids"], pb["bins"])
+    sc.tile_mask = mask
+    o = oracle.texture_forward(sc, "f64")
+    sel = np.zeros((s.img_h, s.img_w), bool)
+    for t in pick:
+        ty, tx = divmod(int(t), tiles[0])
+        sel[ty * 16:(ty + 1) * 16, tx * 16:(tx + 1) * 16] = True
+    fc = frac_close(np_(g["out_texture"])[sel], o["out_texture"][sel], 1e-4)
+    log("c2 tile sample frac", fc, "busiest tile entries", int(counts.max()))
+    assert fc >= 0.999
+
+
+# ------------------------------------------------------------------ raster backward
+@pytest.mark.parametrize("case", ["c1", "bw8", "no_uv_grad", "ndc", "blur", "nearest", "C7"])
+def test_backward_matches_oracle(oracle, case):
+    bw, settings = 16, 1 << 8
+    s = scene(1, num_points=600 if case != "c1" else 1000, seed=len(case) * 13,
+              img_w=128 if case != "c1" else 256, img_h=96 if case != "c1" else 256)
+    if case == "bw8":
+        bw = 8
+    if case == "no_uv_grad":
+        settings = 0
+    if case == "ndc":
+        settings |= 1 << 10
+    if case == "blur":
+        settings |= 1 << 9
+    if case == "nearest":
+        settings |= 1 << 2
+    if case == "C7":
+        s.texture = np.random.default_rng(7).uniform(0, 1, (s.texture.shape[0], 7)).astype(np.float32)
+    pb = project_and_bin(s, bw)
+    g, _ = gpu_forward(s, pb["ids"], pb["bins"], settings=settings, bw=bw)
+    sc = oracle_with(oracle, s, pb["ids"], pb["bins"], settings=settings, bw=bw)
+    o = oracle.texture_forward(sc, "f64")
+    rng = np.random.default_rng(5)
+    grads = random_grads(rng, s.img_h, s.img_w, s.texture.shape[1], 1e-3)
+    gg = gpu_backward(s, pb["ids"], pb["bins"], g, grads, settings=settings, bw=bw)
+    og = oracle.texture_backward(sc, o, grads["img"], grads["depth"], grads["reg"], grads["alpha"],
+                                 grads["tex"], grads["normal"], "f64")
+    bad = []
+    for k in og:
+        ok, l2, frac = grad_close(np_(gg[k]), og[k])
+        log(case, k, "relL2 %.2e" % l2, "frac %.4f" % frac)
+        if not ok:
+            bad.append((k, l2, frac))
+    assert not bad, bad
+
+
+def test_autograd_end_to_end(oracle):
+    from gstex_cuda_amd.texture import texture_gaussians
+
+    s = scene(1, num_points=700, seed=77, img_w=160, img_h=112)
+    pb = project_and_bin(s)
+    r = raster_inputs(s)
+    leaves = {k: r[k].clone().requires_grad_(True) for k in
+              ("colors", "opacities", "means", "scales", "quats", "uv0", "umap", "vmap", "texture")}
+    r.update(leaves)
+    out = texture_gaussians(s.texture_info, r["texture_dims"], pb["centers"], pb["extents"],
+                            pb["depths"], pb["num_tiles_hit"], r["colors"], r["opacities"],
+                            r["means"], r["scales"], 1.0, r["quats"], r["uv0"], r["umap"],
+                            r["vmap"], r["texture"], r["viewmat"], r["c2w"], s.fx, s.fy, s.cx,
+                            s.cy, s.img_h, s.img_w, 16, 1 << 8, r["background"])
+    target = to_dev(s.target)
+    loss = ((out[0] + out[4] - target) ** 2).mean() + 100 * out[2].mean() + 0.05 * (
+        out[5][..., 0] ** 2 + out[5][..., 1] ** 2 + (1 - out[5][..., 2]) ** 2).mean()
+    loss.backward()
+    sc = oracle_with(oracle, s, pb["ids"], pb["bins"])
+    o = oracle.texture_forward(sc, "f64")
+    P = s.img_h * s.img_w
+    diff = (o["out_img"] + o["out_texture"] - s.target) * 2 / (P * 3)
+    vn = np.zeros((s.img_h, s.img_w, 3))
+    vn[..., 0] = 0.05 * 2 * o["out_normal"][..., 0] / P
+    vn[..., 1] = 0.05 * 2 * o["out_normal"][..., 1] / P
+    vn[..., 2] = -0.05 * 2 * (1 - o["out_normal"][..., 2]) / P
+    z = np.zeros((s.img_h, s.img_w))
+    og = oracle.texture_backward(sc, o, diff, z, np.full_like(z, 100.0 / P), z, diff, vn, "f64")
+    pairs = dict(colors="v_colors", opacities="v_opacity", means="v_means", scales="v_scales",
+                 quats="v_quats", uv0="v_uv0", umap="v_umap", vmap="v_vmap", texture="v_texture")
+    for k, gk in pairs.items():
+        ok, l2, frac = grad_close(np_(leaves[k].grad), og[gk].reshape(leaves[k].shape))
+        log("autograd", k, "relL2 %.2e" % l2, "frac %.4f" % frac)
+        assert ok, (k, l2, frac)
