@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--num-points", type=int, default=None)
     ap.add_argument("--cpu-baseline-seconds", type=float, default=15.0,
                     help="target CPU seconds for the oracle sample (0 disables)")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="after the timed region, run one device-to-device copy of this many MiB "
+                         "(a kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -120,6 +123,14 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     value = args.steps / elapsed
+    if args.copy_calibration_mb > 0:
+        nbytes_copy = args.copy_calibration_mb << 20
+        src = torch.ones(nbytes_copy // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     I, N = step.num_intersects, step.n
     T = step.tiles[0] * step.tiles[1]

@@ -60,7 +60,7 @@ _SIGNATURES = {
                                   P, P, P, P,  # final_Ts, final_idx, depth_idx, final_s
                                   P, P, P, P, P, P,  # six output gradients
                                   P, P, P]),  # grad_records, v_texture, stream
-    "gstex_finalize_gradients": (I, [I, P, P, P, F, P, P, F, F, P, P, P, P, P, P, P, P, P]),
+    "gstex_finalize_gradients": (I, [I, P, P, P, P, P, P, F, F, P, P, P, P, P, P, P, P, P]),
     "gstex_fill_f32": (I, [P, Z, F, P]),
     "gstex_texture_sample_forward": (I, [I, I, I, P, P, I, P, I, P, P]),
     "gstex_texture_sample_backward": (I, [I, I, I, P, P, I, P, I, P, P, P]),

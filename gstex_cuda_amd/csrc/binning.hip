@@ -169,6 +169,83 @@ __global__ __launch_bounds__(1024) void k_scan_single(uint32_t* __restrict__ a, 
     if (tid == 1023 && total) *total = sums[1023];
 }
 
+// Device-wide exclusive scan in three passes: per-tile sums, a scan of the
+// tile sums, then each tile scans itself from its offset.  A tile is 4096
+// values, 16 contiguous per lane.
+constexpr int kScanThreads = 256;
+constexpr int kScanPer = 16;
+constexpr int kScanTile = kScanThreads * kScanPer;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds_waves,
+                                                         uint32_t& block_total) {
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    uint32_t inc = v;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t t = __shfl_up(inc, off);
+        if (lane >= off) inc += t;
+    }
+    if (lane == kWave - 1) lds_waves[wave] = inc;
+    __syncthreads();
+    uint32_t wave_off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / kWave; ++w) {
+        const uint32_t s = lds_waves[w];
+        if (w < wave) wave_off += s;
+        tot += s;
+    }
+    block_total = tot;
+    return wave_off + inc - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_tile_sums(const uint32_t* __restrict__ a,
+                                                                 int n,
+                                                                 uint32_t* __restrict__ sums) {
+    __shared__ uint32_t lds[kScanThreads / kWave];
+    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanPer;
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanPer; ++j)
+        if (base + j < (size_t)n) s += a[base + j];
+    uint32_t tot;
+    block_exclusive_scan(s, lds, tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_tile_apply(uint32_t* __restrict__ a, int n,
+                                                                  const uint32_t* __restrict__ offs) {
+    __shared__ uint32_t lds[kScanThreads / kWave];
+    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanPer;
+    uint32_t v[kScanPer];
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanPer; ++j) {
+        v[j] = base + j < (size_t)n ? a[base + j] : 0u;
+        s += v[j];
+    }
+    uint32_t tot;
+    uint32_t run = offs[blockIdx.x] + block_exclusive_scan(s, lds, tot);
+#pragma unroll
+    for (int j = 0; j < kScanPer; ++j) {
+        if (base + j < (size_t)n) a[base + j] = run;
+        run += v[j];
+    }
+}
+
+static size_t scan_partials_count(int n) { return (size_t)(n + kScanTile - 1) / kScanTile + 1; }
+
+// In-place exclusive scan of n uint32 values; writes the total to *total if
+// given.  partials holds scan_partials_count(n) values.
+static hipError_t scan_exclusive(uint32_t* a, int n, uint32_t* total, uint32_t* partials,
+                                 hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int tiles = (n + kScanTile - 1) / kScanTile;
+    k_scan_tile_sums<<<tiles, kScanThreads, 0, s>>>(a, n, partials);
+    k_scan_single<<<1, 1024, 0, s>>>(partials, tiles, total);
+    k_scan_tile_apply<<<tiles, kScanThreads, 0, s>>>(a, n, partials);
+    return hipGetLastError();
+}
+
 // Stable scatter: each round ranks 256 items (one per lane) with wave ballots,
 // combines the four waves' per-digit counts in LDS, and writes each item to
 // base[digit] + rank.  Item order inside a block follows the input order.
@@ -232,7 +309,8 @@ template <typename K>
 size_t radix_workspace_bytes(int n) {
     const size_t nb = (size_t)(n + kSortTile - 1) / kSortTile;
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    return up(sizeof(K) * (size_t)n) + up(sizeof(uint32_t) * (size_t)n) + up(sizeof(uint32_t) * 256 * nb) + 256;
+    return up(sizeof(K) * (size_t)n) + up(sizeof(uint32_t) * (size_t)n) +
+           up(sizeof(uint32_t) * 256 * nb) + up(sizeof(uint32_t) * scan_partials_count((int)(256 * nb))) + 256;
 }
 
 // Stable LSD sort of (key, value) pairs over key bits [0, end_bit).  With all 64
@@ -252,6 +330,8 @@ hipError_t radix_sort_pairs(int n, const K* keys_in, const uint32_t* vals_in, K*
     uint32_t* tv = (uint32_t*)p;
     p += up(sizeof(uint32_t) * (size_t)n);
     uint32_t* hist = (uint32_t*)p;
+    p += up(sizeof(uint32_t) * 256 * (size_t)nblocks);
+    uint32_t* partials = (uint32_t*)p;
     const int passes = (end_bit + 7) / 8;
     if (passes == 0) {
         hipError_t e = hipMemcpyAsync(keys_out, keys_in, sizeof(K) * (size_t)n,
@@ -271,7 +351,8 @@ hipError_t radix_sort_pairs(int n, const K* keys_in, const uint32_t* vals_in, K*
         K* dk = to_out ? keys_out : tk;
         uint32_t* dv = to_out ? vals_out : tv;
         k_radix_hist<K><<<nblocks, kSortThreads, 0, s>>>(n, sk, flip, shift, bits, hist, nblocks);
-        k_scan_single<<<1, 1024, 0, s>>>(hist, (1 << bits) * nblocks, nullptr);
+        hipError_t es = scan_exclusive(hist, (1 << bits) * nblocks, nullptr, partials, s);
+        if (es != hipSuccess) return es;
         k_radix_scatter<K><<<nblocks, kSortThreads, 0, s>>>(n, sk, sv, dk, dv, flip, shift, bits,
                                                              hist, nblocks);
         hipError_t e = hipGetLastError();
@@ -409,7 +490,7 @@ static BinLayout bin_layout(int n, int m) {
         up(4 * (size_t)n) * 2,  // depth keys in/out
         up(4 * (size_t)n) * 2,  // order in/out
         up(4 * (size_t)n),      // written
-        up(4 * (size_t)n + 4),  // written gathered / offsets (+ total)
+        up(4 * (size_t)n + 4) + up(4 * scan_partials_count(n)),  // offsets (+ total), partials
         up(4 * (size_t)m) * 2,  // tile keys in/out
         up(4 * (size_t)m),      // ids in (pre-sort)
         up(std::max(radix_workspace_bytes<uint32_t>(n), radix_workspace_bytes<uint32_t>(m))),
@@ -458,7 +539,9 @@ hipError_t bin_gaussians(int n, int m, const float* centers, const float* extent
     e = radix_sort_pairs<uint32_t>(n, dkey, ord, dkey2, ord2, 32, rws, rws_bytes, s);
     if (e != hipSuccess) return e;
     k_gather_written<<<gn, 256, 0, s>>>(n, ord2, written, offs);
-    k_scan_single<<<1, 1024, 0, s>>>(offs, n, total);
+    if ((e = scan_exclusive(offs, n, total, (uint32_t*)(b + L.off[3] + up(4 * (size_t)n + 4)),
+                            s)) != hipSuccess)
+        return e;
     k_zero_slots<<<gm, 256, 0, s>>>(m, total, tkey, tid_in);
     k_bin_emit<<<gn, 256, 0, s>>>(n, m, ord2, offs, total, centers, extents, tiles_x, tiles_y, bw,
                                   written, tkey, tid_in);

@@ -422,19 +422,19 @@ int gstex_raster_backward(int img_width, int img_height, int block_width, int ch
                      "raster_backward");
 }
 
-int gstex_finalize_gradients(int num_points, const float* grad_records, const float* means,
-                             const float* scales, float glob_scale, const float* quats,
-                             const float* viewmat, float fx, float fy, float* v_colors,
+int gstex_finalize_gradients(int num_points, const float* grad_records, const void* records,
+                             const float* quats, const float* scales, const float* viewmat,
+                             const float* c2w, float fx, float fy, float* v_colors,
                              float* v_opacity, float* v_means, float* v_scales, float* v_quats,
                              float* v_uv0, float* v_umap, float* v_vmap, void* stream) {
     REQUIRE(num_points >= 0, "num_points must be >= 0");
     if (num_points == 0) return GSTEX_OK;
-    REQUIRE(grad_records && means && scales && quats && viewmat && v_colors && v_opacity &&
-                v_means && v_scales && v_quats && v_uv0 && v_umap && v_vmap,
+    REQUIRE(grad_records && records && scales && quats && viewmat && c2w && v_colors &&
+                v_opacity && v_means && v_scales && v_quats && v_uv0 && v_umap && v_vmap,
             "null pointer argument");
-    return check_hip(gstex::grad_finalize(num_points, grad_records, means, scales, glob_scale,
-                                          quats, viewmat, fx, fy, v_colors, v_opacity, v_means,
-                                          v_scales, v_quats, v_uv0, v_umap, v_vmap,
+    return check_hip(gstex::grad_finalize(num_points, grad_records, (const float*)records, quats,
+                                          scales, viewmat, c2w, fx, fy, v_colors, v_opacity,
+                                          v_means, v_scales, v_quats, v_uv0, v_umap, v_vmap,
                                           as_stream(stream)),
                      "finalize_gradients");
 }

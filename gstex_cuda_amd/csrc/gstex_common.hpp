@@ -38,25 +38,36 @@ enum RecordSlot : int {
     kRecordFloats = 32
 };
 
-// Per-Gaussian gradient accumulator written by the backward raster kernel
-// with one wave-reduced atomic per slot, then mapped to the reference's
-// gradient tensors by the finalize kernel.
+// Per-Gaussian scalars the backward raster kernel sums over pixels (one
+// wave-reduced atomic per slot); the finalize kernel maps them to the
+// reference's gradient tensors.  With vl1, vl2 the adjoints of the in-plane
+// coordinates l1, l2, vu, vv those of the texture coordinates, vnum / vden
+// those of the ray-plane numerator / denominator:
 enum GradSlot : int {
-    G_RGB = 0,      // 0..2
-    G_OPAC = 3,
-    G_MEAN = 4,     // 4..6 direct d/d mean
-    G_AX1 = 7,      // 7..9
-    G_AX2 = 10,     // 10..12
-    G_AX3 = 13,     // 13..15
-    G_INV_SX = 16,
-    G_INV_SY = 17,
-    G_UV0 = 18,     // 18..19
-    G_UMAP = 20,    // 20..22
-    G_VMAP = 23,    // 23..25
-    G_PIX = 26,     // 26..27 d/d projected mean (blur term)
-    kGradUsed = 28,
+    G_RGB = 0,    // 0..2   sum vis * dL/dimg
+    G_OPAC = 3,   //        sum G * dL/dalpha
+    G_P11 = 4,    //        sum vl1 * l1
+    G_P12 = 5,    //        sum vl1 * l2
+    G_P21 = 6,    //        sum vl2 * l1
+    G_P22 = 7,    //        sum vl2 * l2
+    G_U1 = 8,     //        sum vu * l1
+    G_U2 = 9,     //        sum vu * l2
+    G_V1 = 10,    //        sum vv * l1
+    G_V2 = 11,    //        sum vv * l2
+    G_VL1 = 12,   //        sum vl1
+    G_VL2 = 13,   //        sum vl2
+    G_VU = 14,    //        sum vu
+    G_VV = 15,    //        sum vv
+    G_VNUM = 16,  //        sum vnum
+    G_AX3X = 17,  // 17..19 sum vden * ray + vis * dL/dnormal
+    G_EXTRA = 20, // 20..31 texel gradients carried through the same reduction
+    G_PIX = 30,   // 30..31 d/d projected mean (blur term)
     kGradFloats = 32
 };
+
+// Per-wave LDS accumulator for one Gaussian's texel gradients (h * w * C floats);
+// covers a 16 x 16 chart with up to 4 channels.
+constexpr int kTexAccFloats = 1024;
 
 // settings bits, texture.cu:56-65 / 381-387
 constexpr int kSetNearest = 1 << 2;

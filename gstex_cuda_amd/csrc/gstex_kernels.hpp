@@ -105,8 +105,8 @@ hipError_t raster_backward_stage(const RasterCall& c, const float* records,
                                  const float* v_depth, const float* v_reg, const float* v_alpha,
                                  const float* v_texture_out, const float* v_normal,
                                  float* grad_rec, float* v_texture, hipStream_t s);
-hipError_t grad_finalize(int n, const float* grad_rec, const float* means, const float* scales,
-                         float glob_scale, const float* quats, const float* viewmat, float fx,
+hipError_t grad_finalize(int n, const float* grad_rec, const float* records, const float* quats,
+                         const float* scales, const float* viewmat, const float* c2w, float fx,
                          float fy, float* v_colors, float* v_opacity, float* v_means,
                          float* v_scales, float* v_quats, float* v_uv0, float* v_umap,
                          float* v_vmap, hipStream_t s);

@@ -1,0 +1,448 @@
+// Back-to-front adjoint of the tile rasteriser (texture.cu:331-760 semantics)
+// and the per-Gaussian gradient finalize.
+//
+// Per (wave, Gaussian) the lanes' contributions are summed with one butterfly
+// reduce-scatter over 32 values, whose first rounds are DPP lane permutes (no
+// LDS traffic); lane L then holds the wave total of one value and issues its
+// atomic.  The 32 values are 20 per-Gaussian scalars from which the finalize
+// kernel rebuilds every parameter gradient, plus up to 12 texel-gradient
+// values.  Using delta = l1*ax1 + l2*ax2 (the ray hits the splat plane), each
+// outer product sum_pixels v * delta becomes ax1 * sum(v*l1) + ax2 * sum(v*l2).
+#include "raster_common.hpp"
+
+namespace gstex {
+
+// ------------------------------------------------------------------ wave reduction
+constexpr int kDppQuadXor1 = 0xB1;      // quad_perm [1,0,3,2]
+constexpr int kDppQuadXor2 = 0x4E;      // quad_perm [2,3,0,1]
+constexpr int kDppRowMirror = 0x140;    // lane i <-> 15 - i within 16
+constexpr int kDppRowHalfMirror = 0x141; // lane i <-> 7 - i within 8
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
+__device__ __forceinline__ float swizzle_xor16(float v) {
+    // ds_swizzle bit mode: and 0x1f, or 0, xor 0x10 (within each 32-lane half)
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));
+}
+
+// One halving round: partners exchange the half of the values they give up.
+template <int CTRL, int NV, int H>
+__device__ __forceinline__ void halve(float (&v)[NV], bool upper) {
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+        const float lo = v[k] + dpp<CTRL>(v[k]);
+        const float hi = v[k + H] + dpp<CTRL>(v[k + H]);
+        v[k] = upper ? hi : lo;
+    }
+}
+
+// Reduce-scatter NV (32 or 16) per-lane values over the wave.  The partner
+// masks 15, 7, 2, 1 (DPP), 16 (swizzle) and 32 (bpermute) span all six lane
+// bits, so every result is a sum over all 64 lanes.  Lane L ends with value
+//   NV = 32: 16*b3 + 8*b2 + 4*b1 + 2*b0 + b4      (b_i = bit i of L)
+//   NV = 16:  8*b3 + 4*b2 + 2*b1 + b0
+template <int NV>
+__device__ __forceinline__ float wave_reduce_scatter(float (&v)[NV], int lane) {
+    static_assert(NV == 32 || NV == 16, "32 or 16 values");
+    halve<kDppRowMirror, NV, NV / 2>(v, lane & 8);
+    halve<kDppRowHalfMirror, NV, NV / 4>(v, lane & 4);
+    halve<kDppQuadXor2, NV, NV / 8>(v, lane & 2);
+    halve<kDppQuadXor1, NV, NV / 16>(v, lane & 1);
+    float r;
+    if constexpr (NV == 32) {
+        const float lo = v[0] + swizzle_xor16(v[0]);
+        const float hi = v[1] + swizzle_xor16(v[1]);
+        r = (lane & 16) ? hi : lo;
+    } else {
+        r = v[0] + swizzle_xor16(v[0]);
+    }
+    return r + __shfl_xor(r, 32);
+}
+
+__device__ __forceinline__ int reduce_slot32(int lane) {
+    return ((lane >> 3) & 1) * 16 + ((lane >> 2) & 1) * 8 + ((lane >> 1) & 1) * 4 +
+           (lane & 1) * 2 + ((lane >> 4) & 1);
+}
+
+__device__ __forceinline__ int reduce_slot16(int lane) {
+    return ((lane >> 3) & 1) * 8 + ((lane >> 2) & 1) * 4 + ((lane >> 1) & 1) * 2 + (lane & 1);
+}
+
+// ------------------------------------------------------------------ backward
+template <int CM>
+__global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) {
+    const int tile = blockIdx.x;
+    const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
+    const int lane = threadIdx.x & (kWave - 1);
+    int lx, ly;
+    bool in_tile;
+    lane_pixel(a.bw, threadIdx.x, lx, ly, in_tile);
+    const int j = tx * a.bw + lx, i = ty * a.bw + ly;
+    const bool inside = in_tile && i < a.img_h && j < a.img_w;
+    const PixelRay pr = pixel_ray(a.c2w, a.viewmat, a.fx, a.fy, a.cx, a.cy, j, i);
+    const bool use_blur = a.settings & kSetBlur, use_ndc = a.settings & kSetNdc;
+    const bool bilinear = !(a.settings & kSetNearest), prop_uv = a.settings & kSetUvGrad;
+    const int C = a.channels;
+    const int pid = inside ? i * a.img_w + j : 0;
+
+    float T = 1.f, S0 = 0.f, S1 = 0.f, S2 = 0.f, vdep = 0.f, vreg = 0.f, vT = 0.f;
+    V3 vo = v3(0.f, 0.f, 0.f), vn = v3(0.f, 0.f, 0.f);
+    int dfin = -1, bfin = -1;
+    float vtex[CM];
+#pragma unroll
+    for (int c = 0; c < CM; ++c) vtex[c] = 0.f;
+    if (inside) {
+        T = b.final_Ts[pid];
+        S0 = b.final_s[3 * pid];
+        S1 = b.final_s[3 * pid + 1];
+        S2 = b.final_s[3 * pid + 2];
+        dfin = b.depth_idx[pid];
+        bfin = b.final_idx[pid];
+        vo = v3(b.v_img[3 * pid], b.v_img[3 * pid + 1], b.v_img[3 * pid + 2]);
+        vn = v3(b.v_normal[3 * pid], b.v_normal[3 * pid + 1], b.v_normal[3 * pid + 2]);
+        vdep = b.v_depth[pid];
+        vreg = b.v_reg[pid];
+        vT = a.background[0] * vo.x + a.background[1] * vo.y + a.background[2] * vo.z -
+             b.v_alpha[pid];
+#pragma unroll
+        for (int c = 0; c < CM; ++c)
+            if (c < C) vtex[c] = b.v_texture_out[(size_t)pid * C + c];
+    }
+    int r0, r1;
+    tile_range(a, tile, r0, r1);
+    // Entries past the wave's last contributing index are dead for every lane.
+    int wave_last = bfin;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) wave_last = max(wave_last, __shfl_xor(wave_last, off));
+    wave_last = __builtin_amdgcn_readfirstlane(wave_last);
+    const int e_hi = min(r1 - 1, wave_last);
+    const float tn = 0.01f, tf = 1000.0f;
+
+    // Wave-private texel-gradient accumulator (one chart at a time), kept zero
+    // between Gaussians by the flush below.
+    __shared__ float tex_acc[256 / kWave][kTexAccFloats];
+    float* acc = tex_acc[threadIdx.x / kWave];
+    for (int k = lane; k < kTexAccFloats; k += kWave) acc[k] = 0.f;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+
+    for (int e = e_hi; e >= r0; --e) {
+        const int g = splat_id(a, e);
+        const Splat s = load_splat(a.records + (size_t)g * kRecordFloats);
+        bool valid = inside && e <= bfin;
+        Pair p;
+        if (valid) {
+            p = pair_eval(s, pr, use_blur);
+            valid = !p.skip;
+        }
+        const uint64_t vmask = __ballot(valid);
+        if (vmask == 0) continue;
+
+        // Transmittance before this Gaussian and the texel taps.
+        float wt[4] = {0.f, 0.f, 0.f, 0.f}, fu = 0.f, fv = 0.f, vis = 0.f;
+        int row[4] = {0, 0, 0, 0};
+        if (valid) {
+            T *= 1.f / (1.f - p.alpha);
+            vis = p.alpha * T;
+            float u, v;
+            splat_uv(s, p.delta, u, v);
+            chart_taps(s, u, v, bilinear, a.num_texels, wt, row, fu, fv);
+        }
+        // How the texel gradients reach HBM (wave-uniform choice):
+        //   single  1x1 chart, C <= 4: C values in the reduction
+        //   quad32  every valid lane samples the same texel quad, C <= 3, no
+        //           blur: 4 taps x C values in the reduction
+        //   quad16  same texel quad otherwise (C <= 4): a second 16-value reduction
+        //   lds     chart of <= kTexAccFloats values: LDS accumulation + flush
+        //   direct  one atomic per tap
+        const int first = __ffsll((unsigned long long)vmask) - 1;
+        const int hw = s.h * s.w;
+        const bool tex_single = hw == 1 && C >= 1 && C <= 4 && s.start >= 0 && s.start < a.num_texels;
+        const int row0 = __builtin_amdgcn_readlane(row[0], first);
+        const int row1 = __builtin_amdgcn_readlane(row[1], first);
+        const int row2 = __builtin_amdgcn_readlane(row[2], first);
+        const int row3 = __builtin_amdgcn_readlane(row[3], first);
+        const bool same_quad = !tex_single && C >= 1 && C <= 4 && !__any(valid && row[0] != row0);
+        const bool quad32 = same_quad && C <= 3 && !use_blur;
+        const bool quad16 = same_quad && !quad32;
+        const bool tex_lds = !tex_single && !same_quad && C > 0 && hw * C <= kTexAccFloats &&
+                             s.start >= 0 && s.start + hw <= a.num_texels;
+
+        float gr[kGradFloats];
+#pragma unroll
+        for (int k = 0; k < kGradFloats; ++k) gr[k] = 0.f;
+        float tq[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) tq[k] = 0.f;
+        if (valid) {
+            gr[G_RGB + 0] = vis * vo.x;
+            gr[G_RGB + 1] = vis * vo.y;
+            gr[G_RGB + 2] = vis * vo.z;
+            float vvis = dot3(s.rgb, vo) + dot3(s.a3, vn);
+            float vu = 0.f, vv = 0.f;
+            const float wsum = (wt[0] + wt[1]) + (wt[2] + wt[3]);
+#pragma unroll
+            for (int c = 0; c < CM; ++c) {
+                if (c < C) {
+                    const float c0 = a.texture[(size_t)row[0] * C + c];
+                    const float c1 = a.texture[(size_t)row[1] * C + c];
+                    const float c2 = a.texture[(size_t)row[2] * C + c];
+                    const float c3 = a.texture[(size_t)row[3] * C + c];
+                    const float val = wt[0] * c0 + wt[1] * c1 + wt[2] * c2 + wt[3] * c3;
+                    const float vval = vis * vtex[c];
+                    if (tex_single) {
+                        if (c < 4) gr[G_EXTRA + c] = wsum * vval;
+                    } else if (quad32) {
+                        if (c < 3) {
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) gr[G_EXTRA + 3 * k + c] = wt[k] * vval;
+                        }
+                    } else if (quad16) {
+                        if (c < 4) {
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) tq[4 * k + c] = wt[k] * vval;
+                        }
+                    } else if (tex_lds) {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const float add = wt[k] * vval;
+                            if (add != 0.f) atomicAdd(acc + (row[k] - s.start) * C + c, add);
+                        }
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const float add = wt[k] * vval;
+                            if (add != 0.f)
+                                unsafeAtomicAdd(b.v_texture + (size_t)row[k] * C + c, add);
+                        }
+                    }
+                    if (bilinear && prop_uv) {
+                        // d val / d(u, v) through the bilinear weights
+                        // (texture_helpers.cuh:262-299), tap differences first so
+                        // coincident taps give exactly zero
+                        vu += (float)s.h * (vval * ((1.f - fv) * (c2 - c0) + fv * (c3 - c1)));
+                        vv += (float)s.w * (vval * ((1.f - fu) * (c1 - c0) + fu * (c3 - c2)));
+                    }
+                    vvis += val * vtex[c];
+                }
+            }
+
+            float valpha = T * vvis - T * vT;
+            float vTcur = p.alpha * vvis + (1.f - p.alpha) * vT;
+            const float t_view = p.t * pr.view_depth;
+            const float dval = use_ndc ? (tf * t_view - tf * tn) / ((tf - tn) * t_view) : p.t;
+            const float vd_val = 2.f * (vis * dval * S0 - vis * S1) * vreg;
+            const float vw = (dval * dval * S0 - 2.f * dval * S1 + S2) * vreg;
+            valpha += vw * T;
+            vTcur += vw * p.alpha;
+            vT = vTcur;
+
+            float vt = use_ndc ? 0.f : vd_val;
+            float vtview = (e == dfin && dfin != -1) ? vdep : 0.f;
+            if (use_ndc) vtview += (tf * tn) / ((tf - tn) * t_view * t_view) * vd_val;
+            vt += pr.view_depth * vtview;
+
+            gr[G_OPAC] = (p.blur_sel ? p.Gb : p.G) * valpha;
+            const float vsigma = p.blur_sel ? 0.f : -s.opac * p.G * valpha;
+            const float vl1 = vsigma * s.is1 * s.is1 * p.l1;
+            const float vl2 = vsigma * s.is2 * s.is2 * p.l2;
+            const V3 vdelta = v3(vl1 * s.a1.x + vl2 * s.a2.x + s.umap.x * vu + s.vmap.x * vv,
+                                 vl1 * s.a1.y + vl2 * s.a2.y + s.umap.y * vu + s.vmap.y * vv,
+                                 vl1 * s.a1.z + vl2 * s.a2.z + s.umap.z * vu + s.vmap.z * vv);
+            vt += dot3(vdelta, pr.r);
+            const float vnum = vt / p.den;
+            const float vden = -vnum * p.t;
+            gr[G_P11] = vl1 * p.l1;
+            gr[G_P12] = vl1 * p.l2;
+            gr[G_P21] = vl2 * p.l1;
+            gr[G_P22] = vl2 * p.l2;
+            gr[G_U1] = vu * p.l1;
+            gr[G_U2] = vu * p.l2;
+            gr[G_V1] = vv * p.l1;
+            gr[G_V2] = vv * p.l2;
+            gr[G_VL1] = vl1;
+            gr[G_VL2] = vl2;
+            gr[G_VU] = vu;
+            gr[G_VV] = vv;
+            gr[G_VNUM] = vnum;
+            gr[G_AX3X + 0] = vden * pr.r.x + vis * vn.x;
+            gr[G_AX3X + 1] = vden * pr.r.y + vis * vn.y;
+            gr[G_AX3X + 2] = vden * pr.r.z + vis * vn.z;
+            if (use_blur) {
+                const float vsb = p.blur_sel ? -s.opac * p.Gb * valpha : 0.f;
+                gr[G_PIX + 0] = 2.0f * vsb * (s.pix_x - pr.px);
+                gr[G_PIX + 1] = 2.0f * vsb * (s.pix_y - pr.py);
+            }
+        }
+        const float tot = wave_reduce_scatter<kGradFloats>(gr, lane);
+        const int slot = reduce_slot32(lane);
+        if (lane < 32 && tot != 0.f) {
+            if (slot < G_EXTRA || (use_blur && slot >= G_PIX)) {
+                unsafeAtomicAdd(b.grad_rec + (size_t)g * kGradFloats + slot, tot);
+            } else if (tex_single) {
+                if (slot - G_EXTRA < C)
+                    unsafeAtomicAdd(b.v_texture + (size_t)s.start * C + (slot - G_EXTRA), tot);
+            } else if (quad32) {
+                const int t = slot - G_EXTRA, k = t / 3, c = t - 3 * k;
+                const int rr = k == 0 ? row0 : k == 1 ? row1 : k == 2 ? row2 : row3;
+                if (c < C) unsafeAtomicAdd(b.v_texture + (size_t)rr * C + c, tot);
+            }
+        }
+        if (quad16) {
+            const float tt = wave_reduce_scatter<16>(tq, lane);
+            const int t = reduce_slot16(lane), k = t >> 2, c = t & 3;
+            const int rr = k == 0 ? row0 : k == 1 ? row1 : k == 2 ? row2 : row3;
+            if (lane < 16 && c < C && tt != 0.f)
+                unsafeAtomicAdd(b.v_texture + (size_t)rr * C + c, tt);
+        } else if (tex_lds) {
+            // every lane's LDS adds precede these reads in the wave's LDS queue
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            const int n_acc = hw * C;
+            for (int k = lane; k < n_acc; k += kWave) {
+                const float v = acc[k];
+                if (v != 0.f) {
+                    unsafeAtomicAdd(b.v_texture + (size_t)s.start * C + k, v);
+                    acc[k] = 0.f;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        }
+    }
+}
+
+// Rebuild the reference gradient tensors from the reduced scalars and the
+// splat record (axes, mean, uv maps, inverse scales).
+__global__ __launch_bounds__(256) void k_grad_finalize(
+    int n, const float* __restrict__ grad_rec, const float* __restrict__ records,
+    const float* __restrict__ quats, const float* __restrict__ scales,
+    const float* __restrict__ viewmat, const float* __restrict__ c2w, float fx, float fy,
+    float* __restrict__ v_colors, float* __restrict__ v_opacity, float* __restrict__ v_means,
+    float* __restrict__ v_scales, float* __restrict__ v_quats, float* __restrict__ v_uv0,
+    float* __restrict__ v_umap, float* __restrict__ v_vmap) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    float q[kGradFloats];
+    const float4* src = reinterpret_cast<const float4*>(grad_rec + (size_t)g * kGradFloats);
+#pragma unroll
+    for (int k = 0; k < kGradFloats / 4; ++k) {
+        const float4 t = src[k];
+        q[4 * k] = t.x; q[4 * k + 1] = t.y; q[4 * k + 2] = t.z; q[4 * k + 3] = t.w;
+    }
+    const Splat s = load_splat(records + (size_t)g * kRecordFloats);
+    const V3 o = v3(c2w[3], c2w[7], c2w[11]);
+    const V3 mo = sub3(s.m, o);
+    v_colors[3 * g] = q[G_RGB];
+    v_colors[3 * g + 1] = q[G_RGB + 1];
+    v_colors[3 * g + 2] = q[G_RGB + 2];
+    v_opacity[g] = q[G_OPAC];
+    const V3 ga1 = v3(s.a1.x * q[G_P11] + s.a2.x * q[G_P12], s.a1.y * q[G_P11] + s.a2.y * q[G_P12],
+                      s.a1.z * q[G_P11] + s.a2.z * q[G_P12]);
+    const V3 ga2 = v3(s.a1.x * q[G_P21] + s.a2.x * q[G_P22], s.a1.y * q[G_P21] + s.a2.y * q[G_P22],
+                      s.a1.z * q[G_P21] + s.a2.z * q[G_P22]);
+    const V3 ga3 = v3(mo.x * q[G_VNUM] + q[G_AX3X], mo.y * q[G_VNUM] + q[G_AX3X + 1],
+                      mo.z * q[G_VNUM] + q[G_AX3X + 2]);
+    V3 vm = v3(-(s.a1.x * q[G_VL1] + s.a2.x * q[G_VL2] + s.umap.x * q[G_VU] + s.vmap.x * q[G_VV]) +
+                   s.a3.x * q[G_VNUM],
+               -(s.a1.y * q[G_VL1] + s.a2.y * q[G_VL2] + s.umap.y * q[G_VU] + s.vmap.y * q[G_VV]) +
+                   s.a3.y * q[G_VNUM],
+               -(s.a1.z * q[G_VL1] + s.a2.z * q[G_VL2] + s.umap.z * q[G_VU] + s.vmap.z * q[G_VV]) +
+                   s.a3.z * q[G_VNUM]);
+    if (q[G_PIX] != 0.f || q[G_PIX + 1] != 0.f) {
+        // blur term: d projected mean / d mean (helpers.cuh:114-121,155-164)
+        const V3 pv = xform43(viewmat, s.m);
+        const float rw = 1.f / (pv.z + 1e-6f);
+        const float vpx = fx * q[G_PIX], vpy = fy * q[G_PIX + 1];
+        const V3 vview = v3(vpx * rw, vpy * rw, -(vpx * pv.x + vpy * pv.y) * rw * rw);
+        const V3 vmx = xform43_rot_t(viewmat, vview);
+        vm = v3(vm.x + vmx.x, vm.y + vmx.y, vm.z + vmx.z);
+    }
+    v_means[3 * g] = vm.x;
+    v_means[3 * g + 1] = vm.y;
+    v_means[3 * g + 2] = vm.z;
+    const float4 qt = reinterpret_cast<const float4*>(quats)[g];
+    float vq[4];
+    quat_axes_vjp(qt.x, qt.y, qt.z, qt.w, ga1, ga2, ga3, vq);
+    reinterpret_cast<float4*>(v_quats)[g] = make_float4(vq[0], vq[1], vq[2], vq[3]);
+    // sigma = 0.5 (l1^2 is1^2 + ...) with is1 = 1 / (s_x g): d/ds_x = -(sum vl1 l1) / s_x
+    v_scales[3 * g] = -q[G_P11] / scales[3 * g];
+    v_scales[3 * g + 1] = -q[G_P22] / scales[3 * g + 1];
+    v_scales[3 * g + 2] = 0.f;
+    v_uv0[2 * g] = q[G_VU];
+    v_uv0[2 * g + 1] = q[G_VV];
+    v_umap[3 * g] = s.a1.x * q[G_U1] + s.a2.x * q[G_U2];
+    v_umap[3 * g + 1] = s.a1.y * q[G_U1] + s.a2.y * q[G_U2];
+    v_umap[3 * g + 2] = s.a1.z * q[G_U1] + s.a2.z * q[G_U2];
+    v_vmap[3 * g] = s.a1.x * q[G_V1] + s.a2.x * q[G_V2];
+    v_vmap[3 * g + 1] = s.a1.y * q[G_V1] + s.a2.y * q[G_V2];
+    v_vmap[3 * g + 2] = s.a1.z * q[G_V1] + s.a2.z * q[G_V2];
+}
+
+// ------------------------------------------------------------------ launchers
+template <int CM>
+static hipError_t launch_bwd_cm(const RasterArgs& a, const BwdIn& b, hipStream_t s) {
+    const int tiles = a.tiles_x * a.tiles_y;
+    k_raster_backward<CM><<<tiles, raster_block_threads(a.bw), 0, s>>>(a, b);
+    return hipGetLastError();
+}
+
+size_t raster_backward_workspace_bytes(int n) {
+    return 2 * (size_t)n * kRecordFloats * sizeof(float) + 256;
+}
+
+hipError_t raster_backward_stage(const RasterCall& c, const float* records,
+                                 const float* final_Ts, const int* final_idx,
+                                 const int* depth_idx, const float* final_s, const float* v_img,
+                                 const float* v_depth, const float* v_reg, const float* v_alpha,
+                                 const float* v_texture_out, const float* v_normal,
+                                 float* grad_rec, float* v_texture, hipStream_t s) {
+    const RasterArgs a = make_raster_args(c, records);
+    const BwdIn b{final_Ts, final_idx, depth_idx, final_s, v_img, v_depth, v_reg, v_alpha,
+                  v_texture_out, v_normal, grad_rec, v_texture};
+    if (a.tiles_x * a.tiles_y <= 0) return hipSuccess;
+    if (c.channels <= 1) return launch_bwd_cm<1>(a, b, s);
+    if (c.channels <= 3) return launch_bwd_cm<3>(a, b, s);
+    if (c.channels <= 4) return launch_bwd_cm<4>(a, b, s);
+    if (c.channels <= 8) return launch_bwd_cm<8>(a, b, s);
+    if (c.channels <= 16) return launch_bwd_cm<16>(a, b, s);
+    return launch_bwd_cm<64>(a, b, s);
+}
+
+hipError_t grad_finalize(int n, const float* grad_rec, const float* records, const float* quats,
+                         const float* scales, const float* viewmat, const float* c2w, float fx,
+                         float fy, float* v_colors, float* v_opacity, float* v_means,
+                         float* v_scales, float* v_quats, float* v_uv0, float* v_umap,
+                         float* v_vmap, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_grad_finalize<<<(n + 255) / 256, 256, 0, s>>>(n, grad_rec, records, quats, scales, viewmat,
+                                                   c2w, fx, fy, v_colors, v_opacity, v_means,
+                                                   v_scales, v_quats, v_uv0, v_umap, v_vmap);
+    return hipGetLastError();
+}
+
+hipError_t raster_backward(const RasterCall& c, const float* final_Ts, const int* final_idx,
+                           const int* depth_idx, const float* final_s, const float* v_img,
+                           const float* v_depth, const float* v_reg, const float* v_alpha,
+                           const float* v_texture_out, const float* v_normal, float* v_colors,
+                           float* v_opacity, float* v_means, float* v_scales, float* v_quats,
+                           float* v_uv0, float* v_umap, float* v_vmap, float* v_texture, void* ws,
+                           size_t ws_bytes, hipStream_t s) {
+    if (ws_bytes < raster_backward_workspace_bytes(c.num_points)) return hipErrorInvalidValue;
+    float* records = (float*)ws;
+    float* grad_rec = records + (size_t)c.num_points * kRecordFloats;
+    hipError_t e = build_raster_records(c, records, s);
+    if (e != hipSuccess) return e;
+    e = fill_f32(grad_rec, (size_t)c.num_points * kGradFloats, 0.f, s);
+    if (e != hipSuccess) return e;
+    e = fill_f32(v_texture, (size_t)c.num_texels * (size_t)c.channels, 0.f, s);
+    if (e != hipSuccess) return e;
+    e = raster_backward_stage(c, records, final_Ts, final_idx, depth_idx, final_s, v_img, v_depth,
+                              v_reg, v_alpha, v_texture_out, v_normal, grad_rec, v_texture, s);
+    if (e != hipSuccess) return e;
+    return grad_finalize(c.num_points, grad_rec, records, c.quats, c.scales, c.viewmat, c.c2w,
+                         c.fx, c.fy, v_colors, v_opacity, v_means, v_scales, v_quats, v_uv0,
+                         v_umap, v_vmap, s);
+}
+
+}  // namespace gstex

@@ -1,0 +1,235 @@
+// Device-side pieces shared by the raster forward and backward kernels:
+// launch arguments, the per-pixel camera ray, the splat record view, the
+// ray/splat intersection with its alpha, and the bilinear chart taps.
+#pragma once
+
+#include "gstex_common.hpp"
+#include "gstex_kernels.hpp"
+
+namespace gstex {
+
+struct RasterArgs {
+    int img_w, img_h, bw, tiles_x, tiles_y, channels, settings;
+    int num_points, num_ids, num_texels;
+    float fx, fy, cx, cy;
+    const float* viewmat;
+    const float* c2w;
+    const float* background;
+    const int* ids_sorted;
+    const int* tile_bins;
+    const float* records;
+    const float* texture;
+};
+
+struct FwdOut {
+    float* out_img;
+    float* out_depth;
+    float* out_reg;
+    float* out_texture;
+    float* out_normal;
+    float* final_Ts;
+    int* final_idx;
+    int* depth_idx;
+    float* out_reg_s;
+};
+
+struct BwdIn {
+    const float* final_Ts;
+    const int* final_idx;
+    const int* depth_idx;
+    const float* final_s;
+    const float* v_img;
+    const float* v_depth;
+    const float* v_reg;
+    const float* v_alpha;
+    const float* v_texture_out;
+    const float* v_normal;
+    float* grad_rec;   // (N, 32) accumulators
+    float* v_texture;  // (X, C)
+};
+
+__host__ __device__ inline int raster_block_threads(int bw) {
+    return bw == 16 ? 256 : ((bw * bw + kWave - 1) / kWave) * kWave;
+}
+
+// Pixel of this lane inside its tile.  bw = 16 uses 8x8 quadrants per wave.
+__device__ __forceinline__ void lane_pixel(int bw, int tid, int& lx, int& ly, bool& in_tile) {
+    if (bw == 16) {
+        const int w = tid >> 6, l = tid & 63;
+        lx = (w & 1) * 8 + (l & 7);
+        ly = (w >> 1) * 8 + (l >> 3);
+        in_tile = true;
+    } else {
+        lx = tid % bw;
+        ly = tid / bw;
+        in_tile = tid < bw * bw;
+    }
+}
+
+struct PixelRay {
+    V3 o, r;
+    float px, py, view_depth;
+};
+
+// Camera ray through the pixel centre (texture_helpers.cuh:336-352).
+__device__ __forceinline__ PixelRay pixel_ray(const float* c2w, const float* vm, float fx,
+                                              float fy, float cx, float cy, int j, int i) {
+    PixelRay pr;
+    pr.px = (float)j + 0.5f;
+    pr.py = (float)i + 0.5f;
+    pr.o = v3(c2w[3], c2w[7], c2w[11]);
+    const float u = (pr.px - cx) / fx, v = (pr.py - cy) / fy;
+    V3 r = xform43(c2w, v3(u, v, 1.f));
+    r = sub3(r, pr.o);
+    const float nrm = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
+    pr.r = v3(r.x / nrm, r.y / nrm, r.z / nrm);
+    pr.view_depth = vm[8] * pr.r.x + vm[9] * pr.r.y + vm[10] * pr.r.z;
+    return pr;
+}
+
+struct Splat {
+    V3 a1, a2, a3, m, rgb, umap, vmap;
+    float is1, is2, opac, pix_x, pix_y, u0, v0;
+    int h, w, start;
+};
+
+__device__ __forceinline__ Splat load_splat(const float* __restrict__ rec) {
+    Splat s;
+    const float4* q = reinterpret_cast<const float4*>(rec);
+    const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5], q6 = q[6],
+                 q7 = q[7];
+    s.a1 = v3(q0.x, q0.y, q0.z);
+    s.is1 = q0.w;
+    s.a2 = v3(q1.x, q1.y, q1.z);
+    s.is2 = q1.w;
+    s.a3 = v3(q2.x, q2.y, q2.z);
+    s.opac = q2.w;
+    s.m = v3(q3.x, q3.y, q3.z);
+    s.pix_x = q3.w;
+    s.rgb = v3(q4.x, q4.y, q4.z);
+    s.pix_y = q4.w;
+    s.u0 = q5.x;
+    s.v0 = q5.y;
+    s.umap = v3(q5.z, q5.w, q6.x);
+    s.vmap = v3(q6.y, q6.z, q6.w);
+    s.h = __float_as_int(q7.x);
+    s.w = __float_as_int(q7.y);
+    s.start = __float_as_int(q7.z);
+    return s;
+}
+
+struct Pair {
+    V3 delta;
+    float den, t, l1, l2, G, Gb, alpha;
+    bool blur_sel, skip;
+};
+
+// Ray/splat intersection and alpha for one (pixel, Gaussian) pair.
+__device__ __forceinline__ Pair pair_eval(const Splat& s, const PixelRay& pr, bool use_blur) {
+    Pair p;
+    float den = dot3(s.a3, pr.r);
+    const float eps = 1e-6f;
+    if (0.f <= den && den < eps) den = eps;
+    else if (-eps < den && den <= 0.f) den = -eps;
+    p.den = den;
+    p.t = dot3(s.a3, sub3(s.m, pr.o)) / den;
+    const V3 pos = v3(pr.o.x + p.t * pr.r.x, pr.o.y + p.t * pr.r.y, pr.o.z + p.t * pr.r.z);
+    p.delta = sub3(pos, s.m);
+    p.l1 = dot3(p.delta, s.a1);
+    p.l2 = dot3(p.delta, s.a2);
+    const float sigma = 0.5f * (s.is1 * s.is1 * p.l1 * p.l1 + s.is2 * s.is2 * p.l2 * p.l2);
+    p.G = __expf(-sigma);
+    p.Gb = 0.f;
+    p.blur_sel = false;
+    if (use_blur) {
+        const float dx = s.pix_x - pr.px, dy = s.pix_y - pr.py;
+        const float sigma_blur = 0.5f * 2.0f * (dx * dx + dy * dy);
+        p.Gb = __expf(-sigma_blur);
+        p.blur_sel = sigma_blur < sigma;
+    }
+    p.alpha = fminf(0.99f, s.opac * (p.blur_sel ? p.Gb : p.G));
+    p.skip = p.t < 0.01f || p.t > 1000.0f || p.alpha < 1.f / 255.f;
+    return p;
+}
+
+// Bilinear (or nearest) taps of one chart at clamped (u, v): weights and the
+// texel rows (start + i*w + j), clamped into [0, X) for memory safety.
+__device__ __forceinline__ void chart_taps(const Splat& s, float u, float v, bool bilinear,
+                                           int num_texels, float wt[4], int row[4], float& fu,
+                                           float& fv) {
+    const float tu = (float)s.h * u, tv = (float)s.w * v;
+    int iu = f2i_trunc(tu), iv = f2i_trunc(tv);
+    fu = tu - (float)iu;
+    fv = tv - (float)iv;
+    const int ju = max(min(iu + 1, s.h - 1), 0), jv = max(min(iv + 1, s.w - 1), 0);
+    iu = max(min(iu, s.h - 1), 0);
+    iv = max(min(iv, s.w - 1), 0);
+    const float wii = (1.f - fu) * (1.f - fv), wij = (1.f - fu) * fv, wji = fu * (1.f - fv),
+                wjj = fu * fv;
+    if (bilinear) {
+        wt[0] = wii; wt[1] = wij; wt[2] = wji; wt[3] = wjj;
+    } else {
+        wt[0] = wt[1] = wt[2] = wt[3] = 0.f;
+        if (wii >= wij && wii >= wji && wii >= wjj) wt[0] = 1.f;
+        else if (wij >= wii && wij >= wji && wij >= wjj) wt[1] = 1.f;
+        else if (wji >= wii && wji >= wij && wji >= wjj) wt[2] = 1.f;
+        else wt[3] = 1.f;
+    }
+    const int lim = num_texels - 1;
+    row[0] = min(max(s.start + iu * s.w + iv, 0), lim);
+    row[1] = min(max(s.start + iu * s.w + jv, 0), lim);
+    row[2] = min(max(s.start + ju * s.w + iv, 0), lim);
+    row[3] = min(max(s.start + ju * s.w + jv, 0), lim);
+}
+
+__device__ __forceinline__ void splat_uv(const Splat& s, V3 delta, float& u, float& v) {
+    u = s.u0 + dot3(s.umap, delta);
+    v = s.v0 + dot3(s.vmap, delta);
+    u = fminf(fmaxf(u, 0.f), 1.f);
+    v = fminf(fmaxf(v, 0.f), 1.f);
+}
+
+__device__ __forceinline__ void tile_range(const RasterArgs& a, int tile, int& r0, int& r1) {
+    const int2 rg = reinterpret_cast<const int2*>(a.tile_bins)[tile];
+    r0 = min(max(rg.x, 0), a.num_ids);
+    r1 = min(max(rg.y, r0), a.num_ids);
+}
+
+__device__ __forceinline__ int splat_id(const RasterArgs& a, int e) {
+    const int g = __builtin_amdgcn_readfirstlane(a.ids_sorted[e]);
+    return min(max(g, 0), a.num_points - 1);
+}
+
+inline RasterArgs make_raster_args(const RasterCall& c, const float* records) {
+    RasterArgs a;
+    a.img_w = c.img_w;
+    a.img_h = c.img_h;
+    a.bw = c.block_width;
+    a.tiles_x = (c.img_w + c.block_width - 1) / c.block_width;
+    a.tiles_y = (c.img_h + c.block_width - 1) / c.block_width;
+    a.channels = c.channels;
+    a.settings = c.settings;
+    a.num_points = c.num_points;
+    a.num_ids = c.num_ids;
+    a.num_texels = c.num_texels;
+    a.fx = c.fx;
+    a.fy = c.fy;
+    a.cx = c.cx;
+    a.cy = c.cy;
+    a.viewmat = c.viewmat;
+    a.c2w = c.c2w;
+    a.background = c.background;
+    a.ids_sorted = c.ids_sorted;
+    a.tile_bins = c.tile_bins;
+    a.records = records;
+    a.texture = c.texture;
+    return a;
+}
+
+inline hipError_t build_raster_records(const RasterCall& c, float* records, hipStream_t s) {
+    return launch_build_records(c.num_points, c.means, c.scales, c.glob_scale, c.quats, c.colors,
+                                c.opacities, c.uv0, c.umap, c.vmap, c.texture_dims, c.viewmat,
+                                c.fx, c.fy, c.cx, c.cy, records, s);
+}
+
+}  // namespace gstex

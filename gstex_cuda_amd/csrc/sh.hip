@@ -1,133 +1,165 @@
 // Real spherical harmonics up to degree 4: colours from per-Gaussian
 // coefficients and the adjoint w.r.t. the coefficients.  Basis constants and
 // band grouping follow sh.cuh:6-210 (the reference's sign convention).
+//
+// Kernels are instantiated per evaluated degree so every basis value and
+// coefficient index is a compile-time constant (no scratch); the coefficient
+// stride (bases of the stored degree) stays a runtime value.
 #include "gstex_common.hpp"
 #include "gstex_kernels.hpp"
 
 namespace gstex {
 
-__constant__ float kShC2[5] = {1.0925484305920792f, -1.0925484305920792f, 0.31539156525252005f,
-                               -1.0925484305920792f, 0.5462742152960396f};
-__constant__ float kShC3[7] = {-0.5900435899266435f, 2.890611442640554f, -0.4570457994644658f,
-                               0.3731763325901154f,  -0.4570457994644658f, 1.445305721320277f,
-                               -0.5900435899266435f};
-__constant__ float kShC4[9] = {2.5033429417967046f,  -1.7701307697799304f, 0.9461746957575601f,
-                               -0.6690465435572892f, 0.10578554691520431f, -0.6690465435572892f,
-                               0.47308734787878004f, -1.7701307697799304f, 0.6258357354491761f};
 constexpr float kShC0 = 0.28209479177387814f;
 constexpr float kShC1 = 0.4886025119029199f;
+constexpr float kShC2_0 = 1.0925484305920792f, kShC2_1 = -1.0925484305920792f,
+                kShC2_2 = 0.31539156525252005f, kShC2_3 = -1.0925484305920792f,
+                kShC2_4 = 0.5462742152960396f;
+constexpr float kShC3_0 = -0.5900435899266435f, kShC3_1 = 2.890611442640554f,
+                kShC3_2 = -0.4570457994644658f, kShC3_3 = 0.3731763325901154f,
+                kShC3_4 = -0.4570457994644658f, kShC3_5 = 1.445305721320277f,
+                kShC3_6 = -0.5900435899266435f;
+constexpr float kShC4_0 = 2.5033429417967046f, kShC4_1 = -1.7701307697799304f,
+                kShC4_2 = 0.9461746957575601f, kShC4_3 = -0.6690465435572892f,
+                kShC4_4 = 0.10578554691520431f, kShC4_5 = -0.6690465435572892f,
+                kShC4_6 = 0.47308734787878004f, kShC4_7 = -1.7701307697799304f,
+                kShC4_8 = 0.6258357354491761f;
 
 __host__ __device__ inline int num_sh_bases(int degree) {
     return degree <= 0 ? 1 : degree == 1 ? 4 : degree == 2 ? 9 : degree == 3 ? 16 : 25;
 }
 
-// Basis values b[1..24] (b[0] = C0) for the normalised direction.
-__device__ __forceinline__ void sh_basis(int degree, float dx, float dy, float dz, float* b) {
+// Basis values 1..24 at the normalised direction for bands up to D.
+template <int D>
+__device__ __forceinline__ void sh_basis(float dx, float dy, float dz, float (&b)[25], float& x,
+                                         float& y, float& z) {
     const float norm = sqrtf(dx * dx + dy * dy + dz * dz);
-    const float x = dx / norm, y = dy / norm, z = dz / norm;
+    x = dx / norm;
+    y = dy / norm;
+    z = dz / norm;
     const float xx = x * x, xy = x * y, xz = x * z, yy = y * y, yz = y * z, zz = z * z;
+    b[0] = kShC0;
     b[1] = -kShC1 * y;
     b[2] = kShC1 * z;
     b[3] = -kShC1 * x;
-    if (degree < 2) return;
-    b[4] = kShC2[0] * xy;
-    b[5] = kShC2[1] * yz;
-    b[6] = kShC2[2] * (2.f * zz - xx - yy);
-    b[7] = kShC2[3] * xz;
-    b[8] = kShC2[4] * (xx - yy);
-    if (degree < 3) return;
-    b[9] = kShC3[0] * y * (3.f * xx - yy);
-    b[10] = kShC3[1] * xy * z;
-    b[11] = kShC3[2] * y * (4.f * zz - xx - yy);
-    b[12] = kShC3[3] * z * (2.f * zz - 3.f * xx - 3.f * yy);
-    b[13] = kShC3[4] * x * (4.f * zz - xx - yy);
-    b[14] = kShC3[5] * z * (xx - yy);
-    b[15] = kShC3[6] * x * (xx - 3.f * yy);
-    if (degree < 4) return;
-    b[16] = kShC4[0] * xy * (xx - yy);
-    b[17] = kShC4[1] * yz * (3.f * xx - yy);
-    b[18] = kShC4[2] * xy * (7.f * zz - 1.f);
-    b[19] = kShC4[3] * yz * (7.f * zz - 3.f);
-    b[20] = kShC4[4] * (zz * (35.f * zz - 30.f) + 3.f);
-    b[21] = kShC4[5] * xz * (7.f * zz - 3.f);
-    b[22] = kShC4[6] * (xx - yy) * (7.f * zz - 1.f);
-    b[23] = kShC4[7] * xz * (xx - 3.f * yy);
-    b[24] = kShC4[8] * (xx * (xx - 3.f * yy) - yy * (3.f * xx - yy));
+    if constexpr (D >= 2) {
+        b[4] = kShC2_0 * xy;
+        b[5] = kShC2_1 * yz;
+        b[6] = kShC2_2 * (2.f * zz - xx - yy);
+        b[7] = kShC2_3 * xz;
+        b[8] = kShC2_4 * (xx - yy);
+    }
+    if constexpr (D >= 3) {
+        b[9] = kShC3_0 * y * (3.f * xx - yy);
+        b[10] = kShC3_1 * xy * z;
+        b[11] = kShC3_2 * y * (4.f * zz - xx - yy);
+        b[12] = kShC3_3 * z * (2.f * zz - 3.f * xx - 3.f * yy);
+        b[13] = kShC3_4 * x * (4.f * zz - xx - yy);
+        b[14] = kShC3_5 * z * (xx - yy);
+        b[15] = kShC3_6 * x * (xx - 3.f * yy);
+    }
+    if constexpr (D >= 4) {
+        b[16] = kShC4_0 * xy * (xx - yy);
+        b[17] = kShC4_1 * yz * (3.f * xx - yy);
+        b[18] = kShC4_2 * xy * (7.f * zz - 1.f);
+        b[19] = kShC4_3 * yz * (7.f * zz - 3.f);
+        b[20] = kShC4_4 * (zz * (35.f * zz - 30.f) + 3.f);
+        b[21] = kShC4_5 * xz * (7.f * zz - 3.f);
+        b[22] = kShC4_6 * (xx - yy) * (7.f * zz - 1.f);
+        b[23] = kShC4_7 * xz * (xx - 3.f * yy);
+        b[24] = kShC4_8 * (xx * (xx - 3.f * yy) - yy * (3.f * xx - yy));
+    }
 }
 
-// One lane per Gaussian; coefficients are (N, bases(degree), 3).
-__global__ __launch_bounds__(256) void k_sh_forward(int n, int degree, int degrees_to_use,
-                                                    const float* __restrict__ viewdirs,
+template <int LO, int HI>
+__device__ __forceinline__ float band_sum(const float (&b)[25], const float* c, int ch) {
+    float s = b[LO] * c[LO * 3 + ch];
+#pragma unroll
+    for (int k = LO + 1; k < HI; ++k) s += b[k] * c[k * 3 + ch];
+    return s;
+}
+
+// One lane per Gaussian; coefficients are (N, nb, 3), evaluated up to degree D.
+template <int D>
+__global__ __launch_bounds__(256) void k_sh_forward(int n, int nb, const float* __restrict__ viewdirs,
                                                     const float* __restrict__ coeffs,
                                                     float* __restrict__ colors) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int nb = num_sh_bases(degree);
     const float* c = coeffs + (size_t)i * nb * 3;
     float acc[3];
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) acc[ch] = kShC0 * c[ch];
-    if (degrees_to_use >= 1) {
-        const float dx = viewdirs[3 * i], dy = viewdirs[3 * i + 1], dz = viewdirs[3 * i + 2];
-        float b[25];
-        sh_basis(degrees_to_use, dx, dy, dz, b);
-        const float norm = sqrtf(dx * dx + dy * dy + dz * dz);
-        const float x = dx / norm, y = dy / norm, z = dz / norm;
+    if constexpr (D >= 1) {
+        float b[25], x, y, z;
+        sh_basis<D>(viewdirs[3 * i], viewdirs[3 * i + 1], viewdirs[3 * i + 2], b, x, y, z);
 #pragma unroll
-        for (int ch = 0; ch < 3; ++ch)
+        for (int ch = 0; ch < 3; ++ch) {
             acc[ch] += kShC1 * (-y * c[3 + ch] + z * c[6 + ch] - x * c[9 + ch]);
-        const int lo[3] = {4, 9, 16}, hi[3] = {9, 16, 25};
-        for (int band = 0; band < 3; ++band) {
-            if (degrees_to_use < band + 2) break;
-#pragma unroll
-            for (int ch = 0; ch < 3; ++ch) {
-                float s = b[lo[band]] * c[lo[band] * 3 + ch];
-                for (int k = lo[band] + 1; k < hi[band]; ++k) s += b[k] * c[k * 3 + ch];
-                acc[ch] += s;
-            }
+            if constexpr (D >= 2) acc[ch] += band_sum<4, 9>(b, c, ch);
+            if constexpr (D >= 3) acc[ch] += band_sum<9, 16>(b, c, ch);
+            if constexpr (D >= 4) acc[ch] += band_sum<16, 25>(b, c, ch);
         }
     }
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) colors[3 * i + ch] = acc[ch];
 }
 
-// v_coeffs[k][c] = basis_k * v_colors[c] for k < bases(degrees_to_use), zero
-// above (sh.cuh:234-253; the output is fully written here, so no memset).
-__global__ __launch_bounds__(256) void k_sh_backward(int n, int degree, int degrees_to_use,
-                                                     const float* __restrict__ viewdirs,
+// v_coeffs[k][c] = basis_k * v_colors[c] for k < bases(D), zero above
+// (sh.cuh:234-253); the output is fully written, so no memset.
+template <int D>
+__global__ __launch_bounds__(256) void k_sh_backward(int n, int nb, const float* __restrict__ viewdirs,
                                                      const float* __restrict__ v_colors,
                                                      float* __restrict__ v_coeffs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int nb = num_sh_bases(degree);
-    const int used = num_sh_bases(degrees_to_use) < nb ? num_sh_bases(degrees_to_use) : nb;
+    constexpr int used = D <= 0 ? 1 : D == 1 ? 4 : D == 2 ? 9 : D == 3 ? 16 : 25;
     float b[25];
     b[0] = kShC0;
-    if (degrees_to_use >= 1)
-        sh_basis(degrees_to_use, viewdirs[3 * i], viewdirs[3 * i + 1], viewdirs[3 * i + 2], b);
+    if constexpr (D >= 1) {
+        float x, y, z;
+        sh_basis<D>(viewdirs[3 * i], viewdirs[3 * i + 1], viewdirs[3 * i + 2], b, x, y, z);
+    }
     const float g0 = v_colors[3 * i], g1 = v_colors[3 * i + 1], g2 = v_colors[3 * i + 2];
     float* out = v_coeffs + (size_t)i * nb * 3;
-    for (int k = 0; k < nb; ++k) {
-        const float bk = k < used ? b[k] : 0.f;
-        out[3 * k] = bk * g0;
-        out[3 * k + 1] = bk * g1;
-        out[3 * k + 2] = bk * g2;
+#pragma unroll
+    for (int k = 0; k < used; ++k) {
+        out[3 * k] = b[k] * g0;
+        out[3 * k + 1] = b[k] * g1;
+        out[3 * k + 2] = b[k] * g2;
+    }
+    for (int k = used; k < nb; ++k) {
+        out[3 * k] = 0.f;
+        out[3 * k + 1] = 0.f;
+        out[3 * k + 2] = 0.f;
     }
 }
 
 hipError_t launch_sh_forward(int n, int degree, int degrees_to_use, const float* viewdirs,
                              const float* coeffs, float* colors, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    k_sh_forward<<<(n + 255) / 256, 256, 0, stream>>>(n, degree, degrees_to_use, viewdirs, coeffs,
-                                                     colors);
+    const int nb = num_sh_bases(degree), g = (n + 255) / 256;
+    switch (degrees_to_use) {
+        case 0: k_sh_forward<0><<<g, 256, 0, stream>>>(n, nb, viewdirs, coeffs, colors); break;
+        case 1: k_sh_forward<1><<<g, 256, 0, stream>>>(n, nb, viewdirs, coeffs, colors); break;
+        case 2: k_sh_forward<2><<<g, 256, 0, stream>>>(n, nb, viewdirs, coeffs, colors); break;
+        case 3: k_sh_forward<3><<<g, 256, 0, stream>>>(n, nb, viewdirs, coeffs, colors); break;
+        default: k_sh_forward<4><<<g, 256, 0, stream>>>(n, nb, viewdirs, coeffs, colors); break;
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_sh_backward(int n, int degree, int degrees_to_use, const float* viewdirs,
                               const float* v_colors, float* v_coeffs, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    k_sh_backward<<<(n + 255) / 256, 256, 0, stream>>>(n, degree, degrees_to_use, viewdirs,
-                                                      v_colors, v_coeffs);
+    const int nb = num_sh_bases(degree), g = (n + 255) / 256;
+    switch (degrees_to_use) {
+        case 0: k_sh_backward<0><<<g, 256, 0, stream>>>(n, nb, viewdirs, v_colors, v_coeffs); break;
+        case 1: k_sh_backward<1><<<g, 256, 0, stream>>>(n, nb, viewdirs, v_colors, v_coeffs); break;
+        case 2: k_sh_backward<2><<<g, 256, 0, stream>>>(n, nb, viewdirs, v_colors, v_coeffs); break;
+        case 3: k_sh_backward<3><<<g, 256, 0, stream>>>(n, nb, viewdirs, v_colors, v_coeffs); break;
+        default: k_sh_backward<4><<<g, 256, 0, stream>>>(n, nb, viewdirs, v_colors, v_coeffs); break;
+    }
     return hipGetLastError();
 }
 

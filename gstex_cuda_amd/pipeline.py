@@ -205,8 +205,8 @@ class RenderStep:
              _p(self.grad_records), _p(g["v_texture"]), s)
         if timer:
             timer.stop("raster_backward")
-        call("gstex_finalize_gradients", n, _p(self.grad_records), _p(p["means"]),
-             _p(p["scales"]), 1.0, _p(p["quats"]), _p(cam.viewmat), cam.fx, cam.fy,
+        call("gstex_finalize_gradients", n, _p(self.grad_records), _p(self.records),
+             _p(p["quats"]), _p(p["scales"]), _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy,
              _p(g["v_colors"]), _p(g["v_opacity"]), _p(g["v_means"]), _p(g["v_scales"]),
              _p(g["v_quats"]), _p(g["v_uv0"]), _p(g["v_umap"]), _p(g["v_vmap"]), s)
         if self.sh_degree >= 0:

@@ -184,9 +184,9 @@ int gstex_raster_backward(int img_width, int img_height, int block_width, int ch
                           const float *v_output_alpha, const float *v_output_texture,
                           const float *v_output_normal, float *grad_records, float *v_texture,
                           void *stream);
-int gstex_finalize_gradients(int num_points, const float *grad_records, const float *means,
-                             const float *scales, float glob_scale, const float *quats,
-                             const float *viewmat, float fx, float fy, float *v_colors,
+int gstex_finalize_gradients(int num_points, const float *grad_records, const void *records,
+                             const float *quats, const float *scales, const float *viewmat,
+                             const float *c2w, float fx, float fy, float *v_colors,
                              float *v_opacity, float *v_means, float *v_scales, float *v_quats,
                              float *v_uv0, float *v_umap, float *v_vmap, void *stream);
 /* Fills n float32 values with v (a plain kernel, safe inside captured graphs). */

@@ -346,6 +346,32 @@ def test_backward_matches_oracle(oracle, case):
     assert not bad, bad
 
 
+@pytest.mark.parametrize("side,channels", [(1, 3), (1, 5), (16, 7), (16, 3), (0, 3)])
+def test_backward_texel_gradient_routes(oracle, side, channels):
+    """Texel gradients leave the backward by three routes chosen per Gaussian:
+    1x1 charts with <= 4 channels ride the wave reduction, charts of up to 1024
+    values go through the LDS accumulator, larger ones atomically per tap.
+    side 0 = mixed 1..16 grids sized by projected area."""
+    from gstex_cuda_amd.synthetic import SceneSpec, make_scene
+
+    spec = SceneSpec(num_points=700, img_w=128, img_h=96, focal=96.0, z_range=(2.0, 8.0),
+                     texel_side=max(side, 1), texel_budget=60_000 if side == 0 else 0,
+                     channels=channels)
+    s = make_scene(seed=side * 10 + channels, spec=spec)
+    pb = project_and_bin(s)
+    g, _ = gpu_forward(s, pb["ids"], pb["bins"])
+    sc = oracle_with(oracle, s, pb["ids"], pb["bins"])
+    o = oracle.texture_forward(sc, "f64")
+    grads = random_grads(np.random.default_rng(side), s.img_h, s.img_w, channels, 1e-3)
+    gg = gpu_backward(s, pb["ids"], pb["bins"], g, grads)
+    og = oracle.texture_backward(sc, o, grads["img"], grads["depth"], grads["reg"], grads["alpha"],
+                                 grads["tex"], grads["normal"], "f64")
+    for k in ("v_texture", "v_uv0", "v_umap", "v_means", "v_colors"):
+        ok, l2, frac = grad_close(np_(gg[k]), og[k])
+        log("texel route", side, channels, k, "relL2 %.2e" % l2, "frac %.4f" % frac)
+        assert ok, (k, l2, frac)
+
+
 def test_autograd_end_to_end(oracle):
     from gstex_cuda_amd.texture import texture_gaussians
 
