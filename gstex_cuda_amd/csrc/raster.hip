@@ -43,7 +43,7 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
     for (int e = r0; e < r1; ++e) {
         if (!__any(!done)) break;
         const int g = splat_id(a, e);
-        const Splat s = load_splat(a.records + (size_t)g * kRecordFloats);
+        const Splat s = splat_record(a, g);
         if (done) continue;
         const Pair p = pair_eval(s, pr, use_blur);
         const float next_T = T * (1.f - p.alpha);

@@ -39,36 +39,24 @@ __device__ __forceinline__ void halve(float (&v)[NV], bool upper) {
     }
 }
 
-// Reduce-scatter NV (32 or 16) per-lane values over the wave.  The partner
-// masks 15, 7, 2, 1 (DPP), 16 (swizzle) and 32 (bpermute) span all six lane
-// bits, so every result is a sum over all 64 lanes.  Lane L ends with value
-//   NV = 32: 16*b3 + 8*b2 + 4*b1 + 2*b0 + b4      (b_i = bit i of L)
-//   NV = 16:  8*b3 + 4*b2 + 2*b1 + b0
-template <int NV>
-__device__ __forceinline__ float wave_reduce_scatter(float (&v)[NV], int lane) {
-    static_assert(NV == 32 || NV == 16, "32 or 16 values");
-    halve<kDppRowMirror, NV, NV / 2>(v, lane & 8);
-    halve<kDppRowHalfMirror, NV, NV / 4>(v, lane & 4);
-    halve<kDppQuadXor2, NV, NV / 8>(v, lane & 2);
-    halve<kDppQuadXor1, NV, NV / 16>(v, lane & 1);
-    float r;
-    if constexpr (NV == 32) {
-        const float lo = v[0] + swizzle_xor16(v[0]);
-        const float hi = v[1] + swizzle_xor16(v[1]);
-        r = (lane & 16) ? hi : lo;
-    } else {
-        r = v[0] + swizzle_xor16(v[0]);
-    }
+// Reduce-scatter 32 per-lane values over the wave.  The partner masks 15, 7,
+// 2, 1 (DPP), 16 (swizzle) and 32 (bpermute) span all six lane bits, so every
+// result is a sum over all 64 lanes.  Lane L ends with value
+//   16*b3 + 8*b2 + 4*b1 + 2*b0 + b4      (b_i = bit i of L)
+__device__ __forceinline__ float wave_reduce_scatter(float (&v)[32], int lane) {
+    halve<kDppRowMirror, 32, 16>(v, lane & 8);
+    halve<kDppRowHalfMirror, 32, 8>(v, lane & 4);
+    halve<kDppQuadXor2, 32, 4>(v, lane & 2);
+    halve<kDppQuadXor1, 32, 2>(v, lane & 1);
+    const float lo = v[0] + swizzle_xor16(v[0]);
+    const float hi = v[1] + swizzle_xor16(v[1]);
+    const float r = (lane & 16) ? hi : lo;
     return r + __shfl_xor(r, 32);
 }
 
 __device__ __forceinline__ int reduce_slot32(int lane) {
     return ((lane >> 3) & 1) * 16 + ((lane >> 2) & 1) * 8 + ((lane >> 1) & 1) * 4 +
            (lane & 1) * 2 + ((lane >> 4) & 1);
-}
-
-__device__ __forceinline__ int reduce_slot16(int lane) {
-    return ((lane >> 3) & 1) * 8 + ((lane >> 2) & 1) * 4 + ((lane >> 1) & 1) * 2 + (lane & 1);
 }
 
 // ------------------------------------------------------------------ backward
@@ -130,7 +118,7 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
 
     for (int e = e_hi; e >= r0; --e) {
         const int g = splat_id(a, e);
-        const Splat s = load_splat(a.records + (size_t)g * kRecordFloats);
+        const Splat s = splat_record(a, g);
         bool valid = inside && e <= bfin;
         Pair p;
         if (valid) {
@@ -154,7 +142,6 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
         //   single  1x1 chart, C <= 4: C values in the reduction
         //   quad32  every valid lane samples the same texel quad, C <= 3, no
         //           blur: 4 taps x C values in the reduction
-        //   quad16  same texel quad otherwise (C <= 4): a second 16-value reduction
         //   lds     chart of <= kTexAccFloats values: LDS accumulation + flush
         //   direct  one atomic per tap
         const int first = __ffsll((unsigned long long)vmask) - 1;
@@ -164,18 +151,14 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
         const int row1 = __builtin_amdgcn_readlane(row[1], first);
         const int row2 = __builtin_amdgcn_readlane(row[2], first);
         const int row3 = __builtin_amdgcn_readlane(row[3], first);
-        const bool same_quad = !tex_single && C >= 1 && C <= 4 && !__any(valid && row[0] != row0);
-        const bool quad32 = same_quad && C <= 3 && !use_blur;
-        const bool quad16 = same_quad && !quad32;
-        const bool tex_lds = !tex_single && !same_quad && C > 0 && hw * C <= kTexAccFloats &&
+        const bool quad32 = !tex_single && C >= 1 && C <= 3 && !use_blur &&
+                            !__any(valid && row[0] != row0);
+        const bool tex_lds = !tex_single && !quad32 && C > 0 && hw * C <= kTexAccFloats &&
                              s.start >= 0 && s.start + hw <= a.num_texels;
 
         float gr[kGradFloats];
 #pragma unroll
         for (int k = 0; k < kGradFloats; ++k) gr[k] = 0.f;
-        float tq[16];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) tq[k] = 0.f;
         if (valid) {
             gr[G_RGB + 0] = vis * vo.x;
             gr[G_RGB + 1] = vis * vo.y;
@@ -198,11 +181,6 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
                         if (c < 3) {
 #pragma unroll
                             for (int k = 0; k < 4; ++k) gr[G_EXTRA + 3 * k + c] = wt[k] * vval;
-                        }
-                    } else if (quad16) {
-                        if (c < 4) {
-#pragma unroll
-                            for (int k = 0; k < 4; ++k) tq[4 * k + c] = wt[k] * vval;
                         }
                     } else if (tex_lds) {
 #pragma unroll
@@ -276,7 +254,7 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
                 gr[G_PIX + 1] = 2.0f * vsb * (s.pix_y - pr.py);
             }
         }
-        const float tot = wave_reduce_scatter<kGradFloats>(gr, lane);
+        const float tot = wave_reduce_scatter(gr, lane);
         const int slot = reduce_slot32(lane);
         if (lane < 32 && tot != 0.f) {
             if (slot < G_EXTRA || (use_blur && slot >= G_PIX)) {
@@ -290,13 +268,7 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
                 if (c < C) unsafeAtomicAdd(b.v_texture + (size_t)rr * C + c, tot);
             }
         }
-        if (quad16) {
-            const float tt = wave_reduce_scatter<16>(tq, lane);
-            const int t = reduce_slot16(lane), k = t >> 2, c = t & 3;
-            const int rr = k == 0 ? row0 : k == 1 ? row1 : k == 2 ? row2 : row3;
-            if (lane < 16 && c < C && tt != 0.f)
-                unsafeAtomicAdd(b.v_texture + (size_t)rr * C + c, tt);
-        } else if (tex_lds) {
+        if (tex_lds) {
             // every lane's LDS adds precede these reads in the wave's LDS queue
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             const int n_acc = hw * C;

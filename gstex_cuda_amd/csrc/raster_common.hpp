@@ -93,28 +93,39 @@ struct Splat {
     int h, w, start;
 };
 
-__device__ __forceinline__ Splat load_splat(const float* __restrict__ rec) {
+// Records and ids are read-only while a raster kernel runs.  Reading them
+// through the constant address space lets a wave-uniform address become a
+// scalar (SMEM) load even in kernels that issue atomics, which otherwise
+// forces 128-byte records into vector registers.
+#define GSTEX_CONST __attribute__((address_space(4)))
+
+template <typename T>
+__device__ __forceinline__ const GSTEX_CONST T* as_const_space(const T* p) {
+    return (const GSTEX_CONST T*)p;
+}
+
+// Adjacent float reads of one record merge into wide loads (SMEM dwordx8/x16
+// for a uniform constant-space address, dwordx4 otherwise).
+template <typename P>
+__device__ __forceinline__ Splat load_splat(P q) {
     Splat s;
-    const float4* q = reinterpret_cast<const float4*>(rec);
-    const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5], q6 = q[6],
-                 q7 = q[7];
-    s.a1 = v3(q0.x, q0.y, q0.z);
-    s.is1 = q0.w;
-    s.a2 = v3(q1.x, q1.y, q1.z);
-    s.is2 = q1.w;
-    s.a3 = v3(q2.x, q2.y, q2.z);
-    s.opac = q2.w;
-    s.m = v3(q3.x, q3.y, q3.z);
-    s.pix_x = q3.w;
-    s.rgb = v3(q4.x, q4.y, q4.z);
-    s.pix_y = q4.w;
-    s.u0 = q5.x;
-    s.v0 = q5.y;
-    s.umap = v3(q5.z, q5.w, q6.x);
-    s.vmap = v3(q6.y, q6.z, q6.w);
-    s.h = __float_as_int(q7.x);
-    s.w = __float_as_int(q7.y);
-    s.start = __float_as_int(q7.z);
+    s.a1 = v3(q[R_AX1], q[R_AX1 + 1], q[R_AX1 + 2]);
+    s.is1 = q[R_INV_SX];
+    s.a2 = v3(q[R_AX2], q[R_AX2 + 1], q[R_AX2 + 2]);
+    s.is2 = q[R_INV_SY];
+    s.a3 = v3(q[R_AX3], q[R_AX3 + 1], q[R_AX3 + 2]);
+    s.opac = q[R_OPAC];
+    s.m = v3(q[R_MEAN], q[R_MEAN + 1], q[R_MEAN + 2]);
+    s.pix_x = q[R_PIX_X];
+    s.rgb = v3(q[R_RGB], q[R_RGB + 1], q[R_RGB + 2]);
+    s.pix_y = q[R_PIX_Y];
+    s.u0 = q[R_UV0];
+    s.v0 = q[R_UV0 + 1];
+    s.umap = v3(q[R_UMAP], q[R_UMAP + 1], q[R_UMAP + 2]);
+    s.vmap = v3(q[R_VMAP], q[R_VMAP + 1], q[R_VMAP + 2]);
+    s.h = __float_as_int(q[R_TEX_H]);
+    s.w = __float_as_int(q[R_TEX_W]);
+    s.start = __float_as_int(q[R_TEX_START]);
     return s;
 }
 
@@ -196,8 +207,13 @@ __device__ __forceinline__ void tile_range(const RasterArgs& a, int tile, int& r
 }
 
 __device__ __forceinline__ int splat_id(const RasterArgs& a, int e) {
-    const int g = __builtin_amdgcn_readfirstlane(a.ids_sorted[e]);
+    const int g = __builtin_amdgcn_readfirstlane(as_const_space(a.ids_sorted)[e]);
     return min(max(g, 0), a.num_points - 1);
+}
+
+// Record of Gaussian g (wave-uniform) as scalar loads.
+__device__ __forceinline__ Splat splat_record(const RasterArgs& a, int g) {
+    return load_splat(as_const_space(a.records) + (size_t)g * kRecordFloats);
 }
 
 inline RasterArgs make_raster_args(const RasterCall& c, const float* records) {
