@@ -54,12 +54,14 @@ _SIGNATURES = {
                                    P, Z, P]),
     "gstex_splat_records_size": (Z, [I]),
     "gstex_build_splat_records": (I, [I, P, P, F, P, P, P, P, P, P, P, P, F, F, F, F, P, P]),
+    "gstex_raster_wave_masks_size": (Z, [I, I, I]),
     "gstex_raster_forward": (I, [I, I, I, I, I, P, I, P, I, P, P, I, P, P, F, F, F, F, P,
-                                 P, P, P, P, P, P, P, P, P, P]),
+                                 P, P, P, P, P, P, P, P, P,  # nine outputs
+                                 P, Z, P]),  # wave_masks, bytes, stream
     "gstex_raster_backward": (I, [I, I, I, I, I, P, I, P, I, P, P, I, P, P, F, F, F, F, P,
                                   P, P, P, P,  # final_Ts, final_idx, depth_idx, final_s
                                   P, P, P, P, P, P,  # six output gradients
-                                  P, P, P]),  # grad_records, v_texture, stream
+                                  P, P, P, Z, P]),  # grad_records, v_texture, masks, bytes, stream
     "gstex_finalize_gradients": (I, [I, P, P, P, P, P, P, F, F, P, P, P, P, P, P, P, P, P]),
     "gstex_fill_f32": (I, [P, Z, F, P]),
     "gstex_texture_sample_forward": (I, [I, I, I, P, P, I, P, I, P, P]),

@@ -347,11 +347,18 @@ static gstex::RasterCall stage_call(int img_width, int img_height, int block_wid
                                     int settings, int num_points, const int32_t* ids, int num_ids,
                                     const int32_t* tile_bins, const float* texture, int num_texels,
                                     const float* viewmat, const float* c2w, float fx, float fy,
-                                    float cx, float cy, const float* background) {
+                                    float cx, float cy, const float* background,
+                                    uint64_t* wave_masks) {
     return gstex::RasterCall{img_width, img_height, block_width, channels, settings, num_points,
                              num_ids, num_texels, 1.f, fx, fy, cx, cy, nullptr, ids, tile_bins,
                              nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                             nullptr, texture, viewmat, c2w, background};
+                             nullptr, texture, viewmat, c2w, background, wave_masks};
+}
+
+size_t gstex_raster_wave_masks_size(int num_ids, int tiles_x, int tiles_y) {
+    if (num_ids < 0 || tiles_x <= 0 || tiles_y <= 0) return 0;
+    return gstex::kMaxTileWaves * gstex::raster_wave_mask_words(num_ids, tiles_x * tiles_y) *
+           sizeof(uint64_t);
 }
 
 int gstex_raster_forward(int img_width, int img_height, int block_width, int channels,
@@ -361,7 +368,8 @@ int gstex_raster_forward(int img_width, int img_height, int block_width, int cha
                          const float* viewmat, const float* c2w, float fx, float fy, float cx,
                          float cy, const float* background, float* out_img, float* out_depth,
                          float* out_reg, float* out_texture, float* out_normal, float* final_Ts,
-                         int32_t* final_idx, int32_t* depth_idx, float* out_reg_s, void* stream) {
+                         int32_t* final_idx, int32_t* depth_idx, float* out_reg_s,
+                         uint64_t* wave_masks, size_t wave_masks_bytes, void* stream) {
     const int tx = img_width > 0 && block_width > 0 ? tiles_of(img_width, block_width) : 0;
     const int ty = img_height > 0 && block_width > 0 ? tiles_of(img_height, block_width) : 0;
     int rc = check_raster_common(tx, ty, block_width, img_width, img_height, 1, channels,
@@ -374,10 +382,12 @@ int gstex_raster_forward(int img_width, int img_height, int block_width, int cha
                 out_reg_s && (channels == 0 || out_texture),
             "null output pointer");
     REQUIRE(num_ids == 0 || num_points > 0, "gaussian_ids_sorted is non-empty but num_points is 0");
+    REQUIRE(!wave_masks || wave_masks_bytes >= gstex_raster_wave_masks_size(num_ids, tx, ty),
+            "wave_masks_bytes is below gstex_raster_wave_masks_size");
     const gstex::RasterCall c =
         stage_call(img_width, img_height, block_width, channels, settings, num_points,
                    gaussian_ids_sorted, num_ids, tile_bins, texture, num_texels, viewmat, c2w, fx,
-                   fy, cx, cy, background);
+                   fy, cx, cy, background, wave_masks);
     return check_hip(gstex::raster_forward_stage(c, (const float*)records, out_img, out_depth,
                                                  out_reg, out_texture, out_normal, final_Ts,
                                                  final_idx, depth_idx, out_reg_s,
@@ -396,6 +406,7 @@ int gstex_raster_backward(int img_width, int img_height, int block_width, int ch
                           const float* v_output_depth, const float* v_output_reg,
                           const float* v_output_alpha, const float* v_output_texture,
                           const float* v_output_normal, float* grad_records, float* v_texture,
+                          const uint64_t* wave_masks, size_t wave_masks_bytes,
                           void* stream) {
     const int tx = img_width > 0 && block_width > 0 ? tiles_of(img_width, block_width) : 0;
     const int ty = img_height > 0 && block_width > 0 ? tiles_of(img_height, block_width) : 0;
@@ -410,10 +421,12 @@ int gstex_raster_backward(int img_width, int img_height, int block_width, int ch
                 v_output_reg && v_output_alpha && v_output_normal,
             "null pointer argument");
     REQUIRE(num_ids == 0 || num_points > 0, "gaussian_ids_sorted is non-empty but num_points is 0");
+    REQUIRE(!wave_masks || wave_masks_bytes >= gstex_raster_wave_masks_size(num_ids, tx, ty),
+            "wave_masks_bytes is below gstex_raster_wave_masks_size");
     const gstex::RasterCall c =
         stage_call(img_width, img_height, block_width, channels, settings, num_points,
                    gaussian_ids_sorted, num_ids, tile_bins, texture, num_texels, viewmat, c2w, fx,
-                   fy, cx, cy, background);
+                   fy, cx, cy, background, const_cast<uint64_t*>(wave_masks));
     return check_hip(gstex::raster_backward_stage(c, (const float*)records, final_Ts, final_idx,
                                                   depth_idx, final_s, v_output, v_output_depth,
                                                   v_output_reg, v_output_alpha, v_output_texture,

@@ -78,7 +78,20 @@ struct RasterCall {
     const float* viewmat;
     const float* c2w;
     const float* background;
+    // Optional per-wave record of which list entries the forward composited
+    // (see raster_wave_mask_words); nullptr disables it.
+    uint64_t* wave_masks;
 };
+
+// Wave masks: for every tile wave (at most kMaxTileWaves per tile) one bit per
+// entry of the tile's list, set when the forward composited that entry for at
+// least one of the wave's pixels.  The backward visits only set bits.  Plane
+// w holds wave w of every tile; tile t's entries [r0, r1) occupy words
+// (r0 >> 6) + t .. ((r1 - 1) >> 6) + t, so tiles never share a word.
+constexpr int kMaxTileWaves = 4;
+inline size_t raster_wave_mask_words(int num_ids, int num_tiles) {
+    return (size_t)(num_ids >> 6) + (size_t)num_tiles + 1;
+}
 
 size_t raster_forward_workspace_bytes(int n);
 size_t raster_backward_workspace_bytes(int n);

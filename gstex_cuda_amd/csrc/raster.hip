@@ -40,56 +40,90 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
 #pragma unroll
     for (int c = 0; c < CM; ++c) tex[c] = 0.f;
 
-    for (int e = r0; e < r1; ++e) {
+    // Composited-entry bits of this wave, stored one 64-entry word at a time
+    // for the backward (every word the loop passes is written, zeros included).
+    const int lane = threadIdx.x & (kWave - 1);
+    uint64_t* const mplane =
+        a.wave_masks ? a.wave_masks + (size_t)(threadIdx.x / kWave) * a.mask_plane : nullptr;
+    const int mword0 = (r0 >> 6) + tile, mentry0 = r0 & ~63;
+    uint64_t mbits = 0;
+
+    int e = r0;
+    for (; e < r1; ++e) {
         if (!__any(!done)) break;
         const int g = splat_id(a, e);
         const Splat s = splat_record(a, g);
-        if (done) continue;
-        const Pair p = pair_eval(s, pr, use_blur);
-        const float next_T = T * (1.f - p.alpha);
-        if (next_T <= 1e-4f) done = true;
-        if (p.skip || done) continue;
-        const float vis = p.alpha * T;
-        rgb0 += s.rgb.x * vis;
-        rgb1 += s.rgb.y * vis;
-        rgb2 += s.rgb.z * vis;
-        V3 hn = s.a3;
-        if (flip && dot3(pr.r, s.a3) > 0.f) hn = v3(-hn.x, -hn.y, -hn.z);
-        n0 += vis * hn.x;
-        n1 += vis * hn.y;
-        n2 += vis * hn.z;
-        const float t_view = p.t * pr.view_depth;
-        float u, v;
-        splat_uv(s, p.delta, u, v);
-        float wt[4], fu, fv;
-        int row[4];
-        chart_taps(s, u, v, bilinear, a.num_texels, wt, row, fu, fv);
+        bool comp = false;
+        if (!done) {
+            const Pair p = pair_eval(s, pr, use_blur);
+            const float next_T = T * (1.f - p.alpha);
+            if (next_T <= 1e-4f) done = true;
+            comp = !(p.skip || done);
+            if (comp) {
+                const float vis = p.alpha * T;
+                rgb0 += s.rgb.x * vis;
+                rgb1 += s.rgb.y * vis;
+                rgb2 += s.rgb.z * vis;
+                V3 hn = s.a3;
+                if (flip && dot3(pr.r, s.a3) > 0.f) hn = v3(-hn.x, -hn.y, -hn.z);
+                n0 += vis * hn.x;
+                n1 += vis * hn.y;
+                n2 += vis * hn.z;
+                const float t_view = p.t * pr.view_depth;
+                float u, v;
+                splat_uv(s, p.delta, u, v);
+                float wt[4], fu, fv;
+                int row[4];
+                chart_taps(s, u, v, bilinear, a.num_texels, wt, row, fu, fv);
+                // texels in chunks of up to 4 channels, all taps loaded before use
+                constexpr int CK = CM < 4 ? CM : 4;
 #pragma unroll
-        for (int c = 0; c < CM; ++c) {
-            if (c < C) {
-                const float val = wt[0] * a.texture[(size_t)row[0] * C + c] +
-                                  wt[1] * a.texture[(size_t)row[1] * C + c] +
-                                  wt[2] * a.texture[(size_t)row[2] * C + c] +
-                                  wt[3] * a.texture[(size_t)row[3] * C + c];
-                tex[c] += vis * val;
+                for (int c0 = 0; c0 < CM; c0 += CK) {
+                    if (c0 < C) {
+                        float tv[4][CK];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+#pragma unroll
+                            for (int cc = 0; cc < CK; ++cc)
+                                tv[k][cc] = c0 + cc < C
+                                                ? a.texture[(size_t)row[k] * C + c0 + cc]
+                                                : 0.f;
+#pragma unroll
+                        for (int cc = 0; cc < CK; ++cc) {
+                            const float val = wt[0] * tv[0][cc] + wt[1] * tv[1][cc] +
+                                              wt[2] * tv[2][cc] + wt[3] * tv[3][cc];
+                            tex[c0 + cc] += vis * val;
+                        }
+                    }
+                }
+                if (T > 0.5f) {
+                    pdepth = t_view;
+                    pdi = e;
+                }
+                float dval = p.t;
+                if (use_ndc) {
+                    const float tn = 0.01f, tf = 1000.0f;
+                    dval = (tf * t_view - tf * tn) / ((tf - tn) * t_view);
+                }
+                preg += vis * (dval * dval * s0 + s2 - 2.f * dval * s1);
+                s0 += vis;
+                s1 += vis * dval;
+                s2 += vis * dval * dval;
+                T = next_T;
+                cur_idx = e;
             }
         }
-        if (T > 0.5f) {
-            pdepth = t_view;
-            pdi = e;
+        if (mplane) {
+            if (__any(comp)) mbits |= 1ull << (e & 63);
+            if ((e & 63) == 63) {
+                if (lane == 0) mplane[mword0 + ((e - mentry0) >> 6)] = mbits;
+                mbits = 0;
+            }
         }
-        float dval = p.t;
-        if (use_ndc) {
-            const float tn = 0.01f, tf = 1000.0f;
-            dval = (tf * t_view - tf * tn) / ((tf - tn) * t_view);
-        }
-        preg += vis * (dval * dval * s0 + s2 - 2.f * dval * s1);
-        s0 += vis;
-        s1 += vis * dval;
-        s2 += vis * dval * dval;
-        T = next_T;
-        cur_idx = e;
     }
+    // the partial word holding the last visited entry
+    if (mplane && e > r0 && (e & 63) != 0 && lane == 0)
+        mplane[mword0 + ((e - 1 - mentry0) >> 6)] = mbits;
     if (!inside) return;
     const int pid = i * a.img_w + j;
     const float bg0 = a.background[0], bg1 = a.background[1], bg2 = a.background[2];

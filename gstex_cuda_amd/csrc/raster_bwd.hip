@@ -54,6 +54,32 @@ __device__ __forceinline__ float wave_reduce_scatter(float (&v)[32], int lane) {
     return r + __shfl_xor(r, 32);
 }
 
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
+}
+
+// Wave-wide min / max of an int: the four DPP rounds cover each row of 16
+// lanes (partner masks 15, 7, 2, 1), the swizzle pairs the rows of each half,
+// and lanes 0 and 32 hold the two half results.
+__device__ __forceinline__ int wave_min(int v) {
+    v = min(v, dpp_i<kDppRowMirror>(v));
+    v = min(v, dpp_i<kDppRowHalfMirror>(v));
+    v = min(v, dpp_i<kDppQuadXor2>(v));
+    v = min(v, dpp_i<kDppQuadXor1>(v));
+    v = min(v, __builtin_amdgcn_ds_swizzle(v, 0x401F));
+    return min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 32));
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+    v = max(v, dpp_i<kDppRowMirror>(v));
+    v = max(v, dpp_i<kDppRowHalfMirror>(v));
+    v = max(v, dpp_i<kDppQuadXor2>(v));
+    v = max(v, dpp_i<kDppQuadXor1>(v));
+    v = max(v, __builtin_amdgcn_ds_swizzle(v, 0x401F));
+    return max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 32));
+}
+
 __device__ __forceinline__ int reduce_slot32(int lane) {
     return ((lane >> 3) & 1) * 16 + ((lane >> 2) & 1) * 8 + ((lane >> 1) & 1) * 4 +
            (lane & 1) * 2 + ((lane >> 4) & 1);
@@ -116,9 +142,8 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
     for (int k = lane; k < kTexAccFloats; k += kWave) acc[k] = 0.f;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 
-    for (int e = e_hi; e >= r0; --e) {
-        const int g = splat_id(a, e);
-        const Splat s = splat_record(a, g);
+    // One list entry (wave-uniform index e, Gaussian g with record s).
+    auto entry = [&](const int e, const int g, const Splat& s) {
         bool valid = inside && e <= bfin;
         Pair p;
         if (valid) {
@@ -126,7 +151,7 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
             valid = !p.skip;
         }
         const uint64_t vmask = __ballot(valid);
-        if (vmask == 0) continue;
+        if (vmask == 0) return;
 
         // Transmittance before this Gaussian and the texel taps.
         float wt[4] = {0.f, 0.f, 0.f, 0.f}, fu = 0.f, fv = 0.f, vis = 0.f;
@@ -166,13 +191,22 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
             float vvis = dot3(s.rgb, vo) + dot3(s.a3, vn);
             float vu = 0.f, vv = 0.f;
             const float wsum = (wt[0] + wt[1]) + (wt[2] + wt[3]);
+            // texels in chunks of up to 4 channels, all taps loaded before use
+            constexpr int CK = CM < 4 ? CM : 4;
 #pragma unroll
-            for (int c = 0; c < CM; ++c) {
-                if (c < C) {
-                    const float c0 = a.texture[(size_t)row[0] * C + c];
-                    const float c1 = a.texture[(size_t)row[1] * C + c];
-                    const float c2 = a.texture[(size_t)row[2] * C + c];
-                    const float c3 = a.texture[(size_t)row[3] * C + c];
+            for (int cb = 0; cb < CM; cb += CK) {
+                if (cb >= C) break;
+                float tv[4][CK];
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+#pragma unroll
+                    for (int cc = 0; cc < CK; ++cc)
+                        tv[k][cc] = cb + cc < C ? a.texture[(size_t)row[k] * C + cb + cc] : 0.f;
+#pragma unroll
+                for (int cc = 0; cc < CK; ++cc) {
+                    const int c = cb + cc;
+                    if (c >= C) break;
+                    const float c0 = tv[0][cc], c1 = tv[1][cc], c2 = tv[2][cc], c3 = tv[3][cc];
                     const float val = wt[0] * c0 + wt[1] * c1 + wt[2] * c2 + wt[3] * c3;
                     const float vval = vis * vtex[c];
                     if (tex_single) {
@@ -269,10 +303,18 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
             }
         }
         if (tex_lds) {
+            // Flush only the texel rows some valid lane touched.
+            int rlo = 0x7fffffff, rhi = -1;
+            if (valid) {
+                rlo = min(min(row[0], row[1]), min(row[2], row[3]));
+                rhi = max(max(row[0], row[1]), max(row[2], row[3]));
+            }
+            rlo = wave_min(rlo);
+            rhi = wave_max(rhi);
             // every lane's LDS adds precede these reads in the wave's LDS queue
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            const int n_acc = hw * C;
-            for (int k = lane; k < n_acc; k += kWave) {
+            const int n_acc = (rhi + 1 - s.start) * C;
+            for (int k = (rlo - s.start) * C + lane; k < n_acc; k += kWave) {
                 const float v = acc[k];
                 if (v != 0.f) {
                     unsafeAtomicAdd(b.v_texture + (size_t)s.start * C + k, v);
@@ -281,6 +323,40 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         }
+    };
+
+    // Entry order: back to front over [r0, e_hi].  With wave masks only the
+    // entries the forward composited for this wave are visited, read one
+    // 64-entry word at a time.  The next entry's id (and mask word) is loaded
+    // before the current entry's record.
+    const bool use_masks = a.wave_masks != nullptr;
+    const GSTEX_CONST uint64_t* mplane =
+        use_masks ? as_const_space(a.wave_masks) + (size_t)(threadIdx.x / kWave) * a.mask_plane
+                  : nullptr;
+    const int mword0 = (r0 >> 6) + tile, mentry0 = r0 & ~63;
+    int wi = e_hi >= r0 ? ((e_hi - mentry0) >> 6) + 1 : 0;  // next word to read, plus one
+    uint64_t m = 0;
+    auto next_entry = [&](int e_prev) -> int {
+        if (!use_masks) return e_prev - 1 >= r0 ? e_prev - 1 : -1;
+        while (m == 0) {
+            if (--wi < 0) return -1;
+            const int lo = mentry0 + 64 * wi;
+            m = mplane[mword0 + wi];
+            if (e_hi - lo < 63) m &= (2ull << (e_hi - lo)) - 1;
+            if (r0 > lo) m &= ~((1ull << (r0 - lo)) - 1);
+        }
+        const int bit = 63 - __builtin_clzll(m);
+        m &= ~(1ull << bit);
+        return mentry0 + 64 * wi + bit;
+    };
+    int e = next_entry(e_hi + 1);
+    int g = e >= 0 ? splat_id(a, e) : 0;
+    while (e >= 0) {
+        const int e_next = next_entry(e);
+        const int g_next = e_next >= 0 ? splat_id(a, e_next) : 0;
+        entry(e, g, splat_record(a, g));
+        e = e_next;
+        g = g_next;
     }
 }
 

@@ -19,6 +19,8 @@ struct RasterArgs {
     const int* tile_bins;
     const float* records;
     const float* texture;
+    uint64_t* wave_masks;  // kMaxTileWaves planes of mask_plane words, or nullptr
+    int mask_plane;
 };
 
 struct FwdOut {
@@ -239,6 +241,8 @@ inline RasterArgs make_raster_args(const RasterCall& c, const float* records) {
     a.tile_bins = c.tile_bins;
     a.records = records;
     a.texture = c.texture;
+    a.wave_masks = c.wave_masks;
+    a.mask_plane = (int)raster_wave_mask_words(c.num_ids, a.tiles_x * a.tiles_y);
     return a;
 }
 

@@ -23,6 +23,7 @@ __all__ = [
     "texture_sample_forward", "texture_sample_backward", "texture_edit",
     # additions beyond the reference's ten
     "project_gaussians", "project_points", "num_tiles_hit", "sort_intersects", "bin_gaussians",
+    "build_splat_records", "raster_forward", "raster_backward", "finalize_gradients",
 ]
 
 
@@ -320,6 +321,108 @@ def texture_backward(img_height, img_width, block_width, texture_info, texture_d
              _ptr(v_means), _ptr(v_scales), _ptr(v_quats), _ptr(v_uv0), _ptr(v_umap),
              _ptr(v_vmap), _ptr(v_texture), _ptr(ws), nb, _stream(dev))
     return v_colors, v_opacity, v_means, v_scales, v_quats, v_uv0, v_umap, v_vmap, v_texture
+
+
+# --------------------------------------------------------------------------- per-stage raster
+# The training path splits texture_forward / texture_backward into stages so
+# that splat records are built once per view and the backward visits only the
+# entries the forward composited (wave masks).  Same semantics as the two
+# reference-shaped ops above.
+def build_splat_records(means, scales, glob_scale, quats, colors, opacities, uv0, umap, vmap,
+                        texture_dims, viewmat, fx, fy, cx, cy):
+    """128-byte per-Gaussian records (uint8 tensor) for raster_forward/backward."""
+    named = locals()
+    for name in ("means", "scales", "quats", "colors", "opacities", "uv0", "umap", "vmap",
+                 "texture_dims", "viewmat"):
+        _check(named[name], name)
+    dev = means.device
+    n = means.size(0)
+    records = torch.empty((max(int(_lib.lib().gstex_splat_records_size(n)), 1),),
+                          dtype=torch.uint8, device=dev)
+    with torch.cuda.device(dev):
+        call("gstex_build_splat_records", n, _ptr(means), _ptr(scales), float(glob_scale),
+             _ptr(quats), _ptr(colors), _ptr(opacities), _ptr(uv0), _ptr(umap), _ptr(vmap),
+             _ptr(texture_dims), _ptr(viewmat), float(fx), float(fy), float(cx), float(cy),
+             _ptr(records), _stream(dev))
+    return records
+
+
+def raster_forward(records, num_points, img_width, img_height, block_width, channels, settings,
+                   gaussian_ids_sorted, tile_bins, texture, viewmat, c2w, fx, fy, cx, cy,
+                   background, wave_masks=True):
+    """Forward compositing from splat records -> (the nine texture_forward outputs,
+    wave_masks or None)."""
+    for name, t in (("gaussian_ids_sorted", gaussian_ids_sorted), ("tile_bins", tile_bins),
+                    ("texture", texture), ("viewmat", viewmat), ("c2w", c2w),
+                    ("background", background)):
+        _check(t, name)
+    dev = records.device
+    W, H, C = int(img_width), int(img_height), int(channels)
+    tx, ty = (W + block_width - 1) // block_width, (H + block_width - 1) // block_width
+    f32 = dict(dtype=torch.float32, device=dev)
+    outs = (torch.empty((H, W, 3), **f32), torch.empty((H, W), **f32),
+            torch.empty((H, W), **f32), torch.empty((H, W, C), **f32),
+            torch.empty((H, W, 3), **f32), torch.empty((H, W), **f32),
+            torch.empty((H, W), dtype=torch.int32, device=dev),
+            torch.empty((H, W), dtype=torch.int32, device=dev), torch.empty((H, W, 3), **f32))
+    m = gaussian_ids_sorted.numel()
+    masks = None
+    if wave_masks:
+        nb = int(_lib.lib().gstex_raster_wave_masks_size(m, tx, ty))
+        masks = torch.empty((max(nb, 8),), dtype=torch.uint8, device=dev)
+    with torch.cuda.device(dev):
+        call("gstex_raster_forward", W, H, int(block_width), C, int(settings), _ptr(records),
+             int(num_points), _ptr(gaussian_ids_sorted), m, _ptr(tile_bins), _ptr(texture),
+             texture.size(0), _ptr(viewmat), _ptr(c2w), float(fx), float(fy), float(cx),
+             float(cy), _ptr(background), *[_ptr(o) for o in outs],
+             None if masks is None else _ptr(masks), 0 if masks is None else masks.numel(),
+             _stream(dev))
+    return outs, masks
+
+
+def raster_backward(records, num_points, img_width, img_height, block_width, channels, settings,
+                    gaussian_ids_sorted, tile_bins, texture, viewmat, c2w, fx, fy, cx, cy,
+                    background, final_Ts, final_idx, depth_idx, final_s, v_output,
+                    v_output_depth, v_output_reg, v_output_alpha, v_output_texture,
+                    v_output_normal, wave_masks=None):
+    """Adjoint of raster_forward -> (grad_records (N, 32), v_texture (X, C))."""
+    named = locals()
+    for name in ("final_Ts", "final_idx", "depth_idx", "final_s", "v_output", "v_output_depth",
+                 "v_output_reg", "v_output_alpha", "v_output_texture", "v_output_normal"):
+        _check(named[name], name)
+    dev = records.device
+    n, C = int(num_points), int(channels)
+    grad_records = torch.zeros((n, 32), dtype=torch.float32, device=dev)
+    v_texture = torch.zeros((texture.size(0), C), dtype=torch.float32, device=dev)
+    with torch.cuda.device(dev):
+        call("gstex_raster_backward", int(img_width), int(img_height), int(block_width), C,
+             int(settings), _ptr(records), n, _ptr(gaussian_ids_sorted),
+             gaussian_ids_sorted.numel(), _ptr(tile_bins), _ptr(texture), texture.size(0),
+             _ptr(viewmat), _ptr(c2w), float(fx), float(fy), float(cx), float(cy),
+             _ptr(background), _ptr(final_Ts), _ptr(final_idx), _ptr(depth_idx), _ptr(final_s),
+             _ptr(v_output), _ptr(v_output_depth), _ptr(v_output_reg), _ptr(v_output_alpha),
+             _ptr(v_output_texture), _ptr(v_output_normal), _ptr(grad_records),
+             _ptr(v_texture), None if wave_masks is None else _ptr(wave_masks),
+             0 if wave_masks is None else wave_masks.numel(), _stream(dev))
+    return grad_records, v_texture
+
+
+def finalize_gradients(grad_records, records, quats, scales, viewmat, c2w, fx, fy,
+                       num_probs=1):
+    """Reference gradient tensors from the backward's per-Gaussian sums ->
+    (v_colors, v_opacity, v_means, v_scales, v_quats, v_uv0, v_umap, v_vmap)."""
+    dev = grad_records.device
+    n = grad_records.size(0)
+    f32 = dict(dtype=torch.float32, device=dev)
+    outs = (torch.empty((n, 3), **f32), torch.empty((n, 1), **f32), torch.empty((n, 3), **f32),
+            torch.empty((n, 3), **f32), torch.empty((n, 4), **f32),
+            torch.empty((n, num_probs, 2), **f32), torch.empty((n, num_probs, 3), **f32),
+            torch.empty((n, num_probs, 3), **f32))
+    with torch.cuda.device(dev):
+        call("gstex_finalize_gradients", n, _ptr(grad_records), _ptr(records), _ptr(quats),
+             _ptr(scales), _ptr(viewmat), _ptr(c2w), float(fx), float(fy),
+             *[_ptr(o) for o in outs], _stream(dev))
+    return outs
 
 
 # --------------------------------------------------------------------------- sampling

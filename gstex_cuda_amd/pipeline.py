@@ -72,7 +72,7 @@ class RenderStep:
 
     def __init__(self, params: Dict[str, torch.Tensor], cam: Camera, block_width: int = 16,
                  settings: int = 1 << 8, background: Optional[torch.Tensor] = None,
-                 sh_degree: int = -1):
+                 sh_degree: int = -1, use_wave_masks: bool = True):
         self.p = params
         self.cam = cam
         self.bw = block_width
@@ -103,6 +103,8 @@ class RenderStep:
         self.tile_bins = torch.empty((self.tiles[0] * self.tiles[1], 2), **i32)
         self.ids = torch.empty((0,), **i32)
         self.bin_ws = torch.empty((1,), dtype=torch.uint8, device=dev)
+        self.wave_masks = torch.empty((1,), dtype=torch.uint8, device=dev)
+        self.use_wave_masks = use_wave_masks
         # per-pixel
         self.out_img = torch.empty((H, W, 3), **f32)
         self.out_depth = torch.empty((H, W), **f32)
@@ -138,6 +140,15 @@ class RenderStep:
         if self.bin_ws.numel() < need:
             self.bin_ws = torch.empty((int(need * 1.25) + 4096,), dtype=torch.uint8,
                                       device=self.dev)
+        need = _lib.lib().gstex_raster_wave_masks_size(m, self.tiles[0], self.tiles[1])
+        if self.wave_masks.numel() < need:
+            self.wave_masks = torch.empty((int(need * 1.25) + 4096,), dtype=torch.uint8,
+                                          device=self.dev)
+
+    def _masks(self):
+        if not self.use_wave_masks:
+            return None, 0
+        return _p(self.wave_masks), self.wave_masks.numel()
 
     def forward(self, timer: Optional[StageTimer] = None):
         p, cam, s = self.p, self.cam, self._stream()
@@ -170,7 +181,7 @@ class RenderStep:
              _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy, cam.cx, cam.cy, _p(self.background),
              _p(self.out_img), _p(self.out_depth), _p(self.out_reg), _p(self.out_texture),
              _p(self.out_normal), _p(self.final_Ts), _p(self.final_idx), _p(self.depth_idx),
-             _p(self.out_reg_s), s)
+             _p(self.out_reg_s), *self._masks(), s)
         if timer:
             timer.stop("raster_forward")
 
@@ -202,7 +213,7 @@ class RenderStep:
              _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy, cam.cx, cam.cy, _p(self.background),
              _p(self.final_Ts), _p(self.final_idx), _p(self.depth_idx), _p(self.out_reg_s),
              _p(self.v_img), _p(v_depth), _p(v_reg), _p(v_alpha), _p(self.v_tex), _p(v_normal),
-             _p(self.grad_records), _p(g["v_texture"]), s)
+             _p(self.grad_records), _p(g["v_texture"]), *self._masks(), s)
         if timer:
             timer.stop("raster_backward")
         call("gstex_finalize_gradients", n, _p(self.grad_records), _p(self.records),

@@ -77,12 +77,22 @@ class _TextureGaussians(Function):
             gaussian_ids_sorted, tile_bins = _C.bin_gaussians(
                 num_points, num_intersects, centers, extents, depths,
                 num_tiles_hit.to(torch.int32).contiguous(), tile_bounds, block_width)
-        (out_img, out_depth, out_reg, out_texture, out_normal, final_Ts, final_idx, depth_idx,
-         out_reg_s) = _C.texture_forward(
-            tile_bounds, (block_width, block_width, 1), (img_width, img_height, 1), texture_info,
-            texture_dims, gaussian_ids_sorted, tile_bins, colors, opacity, means, scales,
-            glob_scale, quats, uv0, umap, vmap, texture, viewmat, c2w, fx, fy, cx, cy, settings,
-            background)
+        # Splat records are built once and kept for the backward; the wave
+        # masks let the backward skip entries no pixel of a wave composited.
+        if int(texture_info[1]) != 1:
+            raise RuntimeError("texture_info[1] (num_probs) must be 1")
+        _C._raster_checks(dict(texture_dims=texture_dims, gaussian_ids_sorted=gaussian_ids_sorted,
+                               tile_bins=tile_bins, colors=colors, opacities=opacity,
+                               means=means, scales=scales, quats=quats, uv0=uv0, umap=umap,
+                               vmap=vmap, texture=texture, viewmat=viewmat, c2w=c2w,
+                               background=background))
+        records = _C.build_splat_records(means, scales, glob_scale, quats, colors, opacity, uv0,
+                                         umap, vmap, texture_dims, viewmat, fx, fy, cx, cy)
+        C = int(texture_info[2])
+        ((out_img, out_depth, out_reg, out_texture, out_normal, final_Ts, final_idx, depth_idx,
+          out_reg_s), wave_masks) = _C.raster_forward(
+            records, num_points, img_width, img_height, block_width, C, settings,
+            gaussian_ids_sorted, tile_bins, texture, viewmat, c2w, fx, fy, cx, cy, background)
         ctx.img_width = img_width
         ctx.img_height = img_height
         ctx.texture_info = texture_info
@@ -91,17 +101,17 @@ class _TextureGaussians(Function):
         ctx.num_intersects = num_intersects
         ctx.block_width = block_width
         ctx.settings = settings
-        ctx.save_for_backward(texture_dims, gaussian_ids_sorted, tile_bins, colors, opacity, means,
-                              scales, quats, uv0, umap, vmap, texture, viewmat, c2w, background,
-                              final_Ts, final_idx, depth_idx, out_reg_s)
+        ctx.save_for_backward(gaussian_ids_sorted, tile_bins, records, wave_masks, scales, quats,
+                              uv0, umap, vmap, texture, viewmat, c2w, background, final_Ts,
+                              final_idx, depth_idx, out_reg_s)
         out_alpha = 1 - final_Ts
         return out_img, out_depth, out_reg, out_alpha, out_texture, out_normal
 
     @staticmethod
     def backward(ctx, v_out_img, v_out_depth, v_out_reg, v_out_alpha, v_out_texture,
                  v_out_normal):
-        (texture_dims, gaussian_ids_sorted, tile_bins, colors, opacity, means, scales, quats, uv0,
-         umap, vmap, texture, viewmat, c2w, background, final_Ts, final_idx, depth_idx,
+        (gaussian_ids_sorted, tile_bins, records, wave_masks, scales, quats, uv0, umap, vmap,
+         texture, viewmat, c2w, background, final_Ts, final_idx, depth_idx,
          out_reg_s) = ctx.saved_tensors
 
         def grad_or_zeros(g, like):
@@ -115,13 +125,15 @@ class _TextureGaussians(Function):
         C = int(ctx.texture_info[2])
         v_out_texture = grad_or_zeros(
             v_out_texture, final_Ts.unsqueeze(-1).expand(-1, -1, C))
-        (v_colors, v_opacity, v_means, v_scales, v_quats, v_uv0, v_umap, v_vmap,
-         v_texture) = _C.texture_backward(
-            ctx.img_height, ctx.img_width, ctx.block_width, ctx.texture_info, texture_dims,
-            gaussian_ids_sorted, tile_bins, colors, opacity, means, scales, ctx.glob_scale, quats,
-            uv0, umap, vmap, texture, viewmat, c2w, ctx.fx, ctx.fy, ctx.cx, ctx.cy, ctx.settings,
+        n = scales.size(0)
+        grad_records, v_texture = _C.raster_backward(
+            records, n, ctx.img_width, ctx.img_height, ctx.block_width, C, ctx.settings,
+            gaussian_ids_sorted, tile_bins, texture, viewmat, c2w, ctx.fx, ctx.fy, ctx.cx, ctx.cy,
             background, final_Ts, final_idx, depth_idx, out_reg_s, v_out_img, v_out_depth,
-            v_out_reg, v_out_alpha, v_out_texture, v_out_normal)
+            v_out_reg, v_out_alpha, v_out_texture, v_out_normal, wave_masks)
+        (v_colors, v_opacity, v_means, v_scales, v_quats, v_uv0, v_umap,
+         v_vmap) = _C.finalize_gradients(grad_records, records, quats, scales, viewmat, c2w,
+                                         ctx.fx, ctx.fy)
         return (None, None, None, None, None, None, v_colors, v_opacity, v_means, v_scales, None,
                 v_quats, v_uv0.view_as(uv0), v_umap.view_as(umap), v_vmap.view_as(vmap),
                 v_texture, None, None, None, None, None, None, None, None, None, None, None)

@@ -164,6 +164,14 @@ int gstex_build_splat_records(int num_points, const float *means, const float *s
                               const float *vmap, const int32_t *texture_dims,
                               const float *viewmat, float fx, float fy, float cx, float cy,
                               void *records, void *stream);
+/* Wave masks (optional, for the per-stage forward/backward pair): one bit per
+ * (tile wave, list entry), written by gstex_raster_forward and read by
+ * gstex_raster_backward, which then visits only the entries the forward
+ * composited for that wave.  Size in bytes for a binning with num_ids entries;
+ * the forward writes every word the backward reads, so no clearing is needed.
+ * Pass the same buffer, ids and tile_bins to both calls, or NULL to both;
+ * wave_masks_bytes is the buffer's size (checked against the size above). */
+size_t gstex_raster_wave_masks_size(int num_ids, int tiles_x, int tiles_y);
 int gstex_raster_forward(int img_width, int img_height, int block_width, int channels,
                          int settings, const void *records, int num_points,
                          const int32_t *gaussian_ids_sorted, int num_ids,
@@ -171,7 +179,8 @@ int gstex_raster_forward(int img_width, int img_height, int block_width, int cha
                          const float *viewmat, const float *c2w, float fx, float fy, float cx,
                          float cy, const float *background, float *out_img, float *out_depth,
                          float *out_reg, float *out_texture, float *out_normal, float *final_Ts,
-                         int32_t *final_idx, int32_t *depth_idx, float *out_reg_s, void *stream);
+                         int32_t *final_idx, int32_t *depth_idx, float *out_reg_s,
+                         uint64_t *wave_masks, size_t wave_masks_bytes, void *stream);
 int gstex_raster_backward(int img_width, int img_height, int block_width, int channels,
                           int settings, const void *records, int num_points,
                           const int32_t *gaussian_ids_sorted, int num_ids,
@@ -183,6 +192,7 @@ int gstex_raster_backward(int img_width, int img_height, int block_width, int ch
                           const float *v_output_depth, const float *v_output_reg,
                           const float *v_output_alpha, const float *v_output_texture,
                           const float *v_output_normal, float *grad_records, float *v_texture,
+                          const uint64_t *wave_masks, size_t wave_masks_bytes,
                           void *stream);
 int gstex_finalize_gradients(int num_points, const float *grad_records, const void *records,
                              const float *quats, const float *scales, const float *viewmat,
