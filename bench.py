@@ -8,7 +8,9 @@ backward, gradient finalize, SH adjoint; inputs are resident in HBM before the
 timed region.  Prints one JSON line.
 
 Roofline: the slower of the two raster kernels, timed live with HIP events on
-the stream it runs on, against its algorithmic bytes per launch:
+the stream it runs on, against its algorithmic bytes per launch (``traffic`` is
+the measured HBM bytes per launch of that kernel, read from the committed
+rocprofv3 FETCH_SIZE / WRITE_SIZE summary profiles/hbm_traffic_c<workload>.md):
   forward   I*(4+128) + T*8 + P*(56 + 4C) + X*C*4
   backward  I*(4+128) + T*8 + P*(60 + 4C) + 2*X*C*4 + N*128
 (I intersects, T tiles, P pixels, X texels, C channels, N Gaussians).
@@ -53,6 +55,20 @@ def raster_bytes(kind, I, T, P, X, C, N):
     if kind == "raster_forward":
         return I * (4 + 128) + T * 8 + P * (56 + 4 * C) + X * C * 4
     return I * (4 + 128) + T * 8 + P * (60 + 4 * C) + 2 * X * C * 4 + N * 128
+
+
+def measured_traffic(workload: int, kernel: str):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary for this workload (copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_c{workload}.md")
+    if not os.path.exists(path):
+        return None, None
+    key = f"k_{kernel}_traffic_bytes_per_launch:"
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith(key):
+                return int(line.split(":", 1)[1]), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def cpu_baseline(scene, step, target_seconds):
@@ -140,9 +156,11 @@ def main():
     dom = max(kern, key=kern.get)
     nbytes = raster_bytes(dom, I, T, P, X, C, N)
     achieved = nbytes / (kern[dom] * 1e-3) / 1e9
+    traffic, traffic_src = (measured_traffic(args.workload, dom) if args.num_points is None
+                            else (None, None))
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None, "algorithmic_bytes": int(nbytes),
+            "traffic": traffic, "traffic_source": traffic_src, "algorithmic_bytes": int(nbytes),
             "kernel_ms": {k: round(v, 4) for k, v in kern.items()}}
     cpu = None
     if args.cpu_baseline_seconds > 0:

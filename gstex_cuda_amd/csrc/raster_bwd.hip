@@ -17,6 +17,7 @@ constexpr int kDppQuadXor1 = 0xB1;      // quad_perm [1,0,3,2]
 constexpr int kDppQuadXor2 = 0x4E;      // quad_perm [2,3,0,1]
 constexpr int kDppRowMirror = 0x140;    // lane i <-> 15 - i within 16
 constexpr int kDppRowHalfMirror = 0x141; // lane i <-> 7 - i within 8
+constexpr int kDppRowRor8 = 0x128;      // lane i <- lane (i + 8) mod 16 within 16 (= i ^ 8)
 
 template <int CTRL>
 __device__ __forceinline__ float dpp(float v) {
@@ -217,10 +218,12 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
                             for (int k = 0; k < 4; ++k) gr[G_EXTRA + 3 * k + c] = wt[k] * vval;
                         }
                     } else if (tex_lds) {
+                        if constexpr (CM > 4) {
 #pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            const float add = wt[k] * vval;
-                            if (add != 0.f) atomicAdd(acc + (row[k] - s.start) * C + c, add);
+                            for (int k = 0; k < 4; ++k) {
+                                const float add = wt[k] * vval;
+                                if (add != 0.f) atomicAdd(acc + (row[k] - s.start) * C + c, add);
+                            }
                         }
                     } else {
 #pragma unroll
@@ -286,6 +289,38 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
                 const float vsb = p.blur_sel ? -s.opac * p.Gb * valpha : 0.f;
                 gr[G_PIX + 0] = 2.0f * vsb * (s.pix_x - pr.px);
                 gr[G_PIX + 1] = 2.0f * vsb * (s.pix_y - pr.py);
+            }
+        }
+        if constexpr (CM <= 4) {
+            if (tex_lds) {
+                // Lanes that sample the same texel quad (equal first tap row)
+                // are merged pairwise before the LDS adds, so the adds of an
+                // 8x8 pixel block land on few distinct lanes: horizontal pairs
+                // (lane bit 0), vertical pairs (bit 3), then bit 1.  On a match
+                // the lower lane takes the sum and the upper lane drops out.
+                const int key = valid ? row[0] : -1;
+                const bool m1 = dpp_i<kDppQuadXor1>(key) == key;
+                const bool m8 = dpp_i<kDppRowRor8>(key) == key;
+                const bool m2 = dpp_i<kDppQuadXor2>(key) == key;
+                const bool lo1 = !(lane & 1), lo8 = !(lane & 8), lo2 = !(lane & 2);
+#pragma unroll
+                for (int c = 0; c < CM; ++c) {
+                    if (c >= C) break;
+                    float q[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        q[k] = wt[k] * (vis * vtex[c]);
+                        const float t1 = q[k] + dpp<kDppQuadXor1>(q[k]);
+                        q[k] = m1 ? (lo1 ? t1 : 0.f) : q[k];
+                        const float t8 = q[k] + dpp<kDppRowRor8>(q[k]);
+                        q[k] = m8 ? (lo8 ? t8 : 0.f) : q[k];
+                        const float t2 = q[k] + dpp<kDppQuadXor2>(q[k]);
+                        q[k] = m2 ? (lo2 ? t2 : 0.f) : q[k];
+                    }
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        if (q[k] != 0.f) atomicAdd(acc + (row[k] - s.start) * C + c, q[k]);
+                }
             }
         }
         const float tot = wave_reduce_scatter(gr, lane);
