@@ -29,6 +29,7 @@ _SIGNATURES = {
     "gstex_project_gaussians": (I, [I, P, P, F, P, P, F, F, F, F, I, I, I, P, P, P, P, P]),
     "gstex_project_points": (I, [I, P, P, F, F, F, F, I, P, P, P]),
     "gstex_compute_sh_forward": (I, [I, I, I, P, P, P, P]),
+    "gstex_compute_sh_forward_bf16": (I, [I, I, I, P, P, P, P]),
     "gstex_compute_sh_backward": (I, [I, I, I, P, P, P, P]),
     "gstex_map_gaussian_to_intersects": (I, [I, I, P, P, P, P, I, I, I, I, P, P, P]),
     "gstex_sort_intersects_workspace_size": (Z, [I]),
@@ -53,12 +54,13 @@ _SIGNATURES = {
                                    P, P, P, P, P, P, P, P, P,  # nine parameter gradients
                                    P, Z, P]),
     "gstex_splat_records_size": (Z, [I]),
-    "gstex_build_splat_records": (I, [I, P, P, F, P, P, P, P, P, P, P, P, F, F, F, F, P, P]),
+    "gstex_build_splat_records": (I, [I, P, P, F, P, P, P, P, P, P, P, P, P, F, F, F, F, P, P]),
+    "gstex_set_raster_culling": (I, [I]),
     "gstex_raster_wave_masks_size": (Z, [I, I, I]),
-    "gstex_raster_forward": (I, [I, I, I, I, I, P, I, P, I, P, P, I, P, P, F, F, F, F, P,
+    "gstex_raster_forward": (I, [I, I, I, I, I, P, I, P, I, P, P, I, I, P, P, F, F, F, F, P,
                                  P, P, P, P, P, P, P, P, P,  # nine outputs
                                  P, Z, P]),  # wave_masks, bytes, stream
-    "gstex_raster_backward": (I, [I, I, I, I, I, P, I, P, I, P, P, I, P, P, F, F, F, F, P,
+    "gstex_raster_backward": (I, [I, I, I, I, I, P, I, P, I, P, P, I, I, P, P, F, F, F, F, P,
                                   P, P, P, P,  # final_Ts, final_idx, depth_idx, final_s
                                   P, P, P, P, P, P,  # six output gradients
                                   P, P, P, Z, P]),  # grad_records, v_texture, masks, bytes, stream

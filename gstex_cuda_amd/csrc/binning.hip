@@ -232,6 +232,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tile_apply(uint32_t* __re
     }
 }
 
+// Up to this many values one 1024-lane workgroup scans faster than three launches.
+constexpr int kScanSingleMax = 1 << 16;
 static size_t scan_partials_count(int n) { return (size_t)(n + kScanTile - 1) / kScanTile + 1; }
 
 // In-place exclusive scan of n uint32 values; writes the total to *total if
@@ -239,6 +241,10 @@ static size_t scan_partials_count(int n) { return (size_t)(n + kScanTile - 1) / 
 static hipError_t scan_exclusive(uint32_t* a, int n, uint32_t* total, uint32_t* partials,
                                  hipStream_t s) {
     if (n <= 0) return hipSuccess;
+    if (n <= kScanSingleMax) {
+        k_scan_single<<<1, 1024, 0, s>>>(a, n, total);
+        return hipGetLastError();
+    }
     const int tiles = (n + kScanTile - 1) / kScanTile;
     k_scan_tile_sums<<<tiles, kScanThreads, 0, s>>>(a, n, partials);
     k_scan_single<<<1, 1024, 0, s>>>(partials, tiles, total);
@@ -246,30 +252,66 @@ static hipError_t scan_exclusive(uint32_t* a, int n, uint32_t* total, uint32_t* 
     return hipGetLastError();
 }
 
-// Stable scatter: each round ranks 256 items (one per lane) with wave ballots,
-// combines the four waves' per-digit counts in LDS, and writes each item to
-// base[digit] + rank.  Item order inside a block follows the input order.
+// Stable scatter of one block tile (kSortTile items) in two phases.  Ranking:
+// each round ranks 256 items (one per lane) with wave ballots and combines the
+// four waves' per-digit counts in LDS, placing every item at its stable
+// position in an LDS copy of the tile sorted by digit.  Write-out: lanes take
+// consecutive LDS positions, so each digit's run goes to consecutive global
+// addresses (hist[digit][block] onward) in coalesced stores.
 template <typename K>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     int n, const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     K* __restrict__ keys_out, uint32_t* __restrict__ vals_out, K flip, int shift, int bits,
     const uint32_t* __restrict__ hist, int nblocks) {
-    __shared__ uint32_t base_cnt[256];
+    __shared__ uint32_t gbase[256];   // global start of each digit's run for this block
+    __shared__ uint32_t lstart[256];  // start of each digit's run in the LDS tile
+    __shared__ uint32_t run[256];     // next free LDS slot per digit
     __shared__ uint32_t wave_cnt[kSortThreads / kWave][256];
+    __shared__ K skeys[kSortTile];
+    __shared__ uint32_t svals[kSortTile];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const uint32_t mask = (1u << bits) - 1u;
     const int ndig = 1 << bits;
-    if (tid < ndig) base_cnt[tid] = hist[(size_t)tid * nblocks + blockIdx.x];
-    const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     const size_t base = (size_t)blockIdx.x * kSortTile;
+    const int count = (int)min((size_t)kSortTile, (size_t)n - base);
+
+    // local digit histogram -> exclusive prefix (one digit per lane)
+    run[tid] = 0;
+    __syncthreads();
     for (int r = 0; r < kSortItems; ++r) {
-        const size_t idx = base + (size_t)r * kSortThreads + tid;
-        const bool valid = idx < (size_t)n;
+        const int li = r * kSortThreads + tid;
+        if (li < count) atomicAdd(&run[digit_of(keys_in[base + li] ^ flip, shift, mask)], 1u);
+    }
+    __syncthreads();
+    {
+        uint32_t v = run[tid];
+        // inclusive scan over the 256 digits: within each wave, then across waves
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const uint32_t t = __shfl_up(v, off);
+            if (lane >= off) v += t;
+        }
+        wave_cnt[0][tid] = v;
+        __syncthreads();
+        uint32_t add = 0;
+        for (int w = 0; w < wave; ++w) add += wave_cnt[0][w * kWave + kWave - 1];
+        const uint32_t excl = v + add - run[tid];
+        __syncthreads();
+        lstart[tid] = excl;
+        run[tid] = excl;
+        gbase[tid] = tid < ndig ? hist[(size_t)tid * nblocks + blockIdx.x] : 0u;
+    }
+    __syncthreads();
+
+    const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    for (int r = 0; r < kSortItems; ++r) {
+        const int li = r * kSortThreads + tid;
+        const bool valid = li < count;
         K key = 0;
         uint32_t val = 0, d = 0;
         if (valid) {
-            key = keys_in[idx];
-            val = vals_in[idx];
+            key = keys_in[base + li];
+            val = vals_in[base + li];
             d = digit_of(key ^ flip, shift, mask);
         }
         uint64_t peers = __ballot(valid);
@@ -285,23 +327,31 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
         __syncthreads();
         if (valid && rank == 0) wave_cnt[wave][d] = tot;
         __syncthreads();
-        uint32_t pos = 0;
         if (valid) {
-            pos = base_cnt[d] + rank;
+            uint32_t pos = run[d] + rank;
             for (int w = 0; w < wave; ++w) pos += wave_cnt[w][d];
+            skeys[pos] = key;
+            svals[pos] = val;
         }
         __syncthreads();
-        if (tid < ndig) {
+        {
             uint32_t add = 0;
 #pragma unroll
             for (int w = 0; w < kSortThreads / kWave; ++w) add += wave_cnt[w][tid];
-            base_cnt[tid] += add;
-        }
-        if (valid) {
-            keys_out[pos] = key;
-            vals_out[pos] = val;
+            run[tid] += add;
         }
         __syncthreads();
+    }
+
+    for (int r = 0; r < kSortItems; ++r) {
+        const int p = r * kSortThreads + tid;
+        if (p < count) {
+            const K key = skeys[p];
+            const uint32_t d = digit_of(key ^ flip, shift, mask);
+            const uint32_t gp = gbase[d] + (uint32_t)p - lstart[d];
+            keys_out[gp] = key;
+            vals_out[gp] = svals[p];
+        }
     }
 }
 
@@ -430,33 +480,62 @@ __global__ __launch_bounds__(256) void k_gather_written(int n, const uint32_t* _
     out[p] = (uint32_t)written[order[p]];
 }
 
-// Emit (tile, id) for every depth-ordered Gaussian after the zero slots.
-// zero slots occupy [0, m - total_written) and read as (tile 0, id 0).
+// Emit (tile, id) for every depth-ordered Gaussian after the zero slots, which
+// occupy [0, m - total_written) and read as (tile 0, id 0).  A workgroup takes
+// 256 consecutive Gaussians and spreads their outputs evenly over its lanes:
+// output k of the group belongs to the Gaussian found by a binary search over
+// the group's output offsets (in LDS), and consecutive lanes write consecutive
+// slots.  A Gaussian's tiles go row by row over its rectangle, capped at its
+// count.
 __global__ __launch_bounds__(256) void k_bin_emit(
     int n, int m, const uint32_t* __restrict__ order, const uint32_t* __restrict__ offsets,
     const uint32_t* __restrict__ total_written, const float* __restrict__ centers,
     const float* __restrict__ extents, int tiles_x, int tiles_y, int bw, const int* __restrict__ written,
     uint32_t* __restrict__ tiles_out, uint32_t* __restrict__ ids_out) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const uint32_t g = order[p];
-    const int w = written[g];
-    if (w <= 0) return;
+    __shared__ uint32_t s_off[257];
+    __shared__ uint32_t s_g[256];
+    __shared__ int s_x0[256], s_y0[256], s_wd[256];
+    const int tid = threadIdx.x;
+    const int p0 = blockIdx.x * 256, p = p0 + tid;
+    const int cnt = min(256, n - p0);
     const uint32_t tw = *total_written;
     const uint32_t zbase = tw <= (uint32_t)m ? (uint32_t)m - tw : 0u;
-    const float2 c = reinterpret_cast<const float2*>(centers)[g];
-    const float2 e = reinterpret_cast<const float2*>(extents)[g];
-    int x0, y0, x1, y1;
-    tile_rect(c.x, c.y, e.x, e.y, bw, tiles_x, tiles_y, x0, y0, x1, y1);
-    uint32_t dst = zbase + offsets[p];
-    int k = 0;
-    // dst < m holds when num_intersects = sum(num_tiles_hit); the bound keeps an
-    // inconsistent caller-supplied count from writing past the outputs
-    for (int ty = y0; ty < y1 && k < w; ++ty)
-        for (int tx = x0; tx < x1 && k < w && dst < (uint32_t)m; ++tx, ++k, ++dst) {
-            tiles_out[dst] = (uint32_t)(ty * tiles_x + tx);
-            ids_out[dst] = g;
+    if (tid < cnt) {
+        const uint32_t g = order[p];
+        const int w = max(written[g], 0);
+        const float2 c = reinterpret_cast<const float2*>(centers)[g];
+        const float2 e = reinterpret_cast<const float2*>(extents)[g];
+        int x0, y0, x1, y1;
+        tile_rect(c.x, c.y, e.x, e.y, bw, tiles_x, tiles_y, x0, y0, x1, y1);
+        s_off[tid] = offsets[p];
+        s_g[tid] = g;
+        s_x0[tid] = x0;
+        s_y0[tid] = y0;
+        s_wd[tid] = max(x1 - x0, 1);
+        if (tid == cnt - 1) s_off[cnt] = offsets[p] + (uint32_t)w;
+    }
+    __syncthreads();
+    const uint32_t o0 = s_off[0], total = s_off[cnt] - o0;
+    for (uint32_t k = tid; k < total; k += 256) {
+        // last Gaussian whose first output is <= o0 + k
+        int lo = 0, hi = cnt - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] - o0 <= k) lo = mid;
+            else hi = mid - 1;
         }
+        const uint32_t li = o0 + k - s_off[lo];
+        const int wd = s_wd[lo];
+        const int ty = s_y0[lo] + (int)(li / (uint32_t)wd);
+        const int tx = s_x0[lo] + (int)(li % (uint32_t)wd);
+        // dst < m holds when num_intersects = sum(num_tiles_hit); the bound keeps an
+        // inconsistent caller-supplied count from writing past the outputs
+        const uint32_t dst = zbase + o0 + k;
+        if (dst < (uint32_t)m) {
+            tiles_out[dst] = (uint32_t)(ty * tiles_x + tx);
+            ids_out[dst] = s_g[lo];
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_zero_slots(int m, const uint32_t* __restrict__ total_written,

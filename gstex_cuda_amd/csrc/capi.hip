@@ -131,6 +131,20 @@ int gstex_compute_sh_forward(int num_points, int degree, int degrees_to_use, con
                      "compute_sh_forward");
 }
 
+int gstex_compute_sh_forward_bf16(int num_points, int degree, int degrees_to_use,
+                                  const float* viewdirs, const uint16_t* coeffs, float* colors,
+                                  void* stream) {
+    REQUIRE(num_points >= 0, "num_points must be >= 0");
+    REQUIRE(degree >= 0 && degree <= 4, "SH degree must be in [0, 4], got %d", degree);
+    REQUIRE(degrees_to_use >= 0 && degrees_to_use <= degree,
+            "degrees_to_use must be in [0, degree], got %d", degrees_to_use);
+    if (num_points == 0) return GSTEX_OK;
+    REQUIRE(viewdirs && coeffs && colors, "null pointer argument");
+    return check_hip(gstex::launch_sh_forward_bf16(num_points, degree, degrees_to_use, viewdirs,
+                                                   coeffs, colors, as_stream(stream)),
+                     "compute_sh_forward_bf16");
+}
+
 int gstex_compute_sh_backward(int num_points, int degree, int degrees_to_use,
                               const float* viewdirs, const float* v_colors, float* v_coeffs,
                               void* stream) {
@@ -320,39 +334,42 @@ int gstex_texture_backward(int img_height, int img_width, int block_width, int n
                      "texture_backward");
 }
 
+int gstex_set_raster_culling(int enabled) { return gstex::set_raster_culling(enabled != 0) ? 1 : 0; }
+
 size_t gstex_splat_records_size(int num_points) {
-    return num_points > 0 ? (size_t)num_points * gstex::kRecordBytes : 0;
+    return gstex::splat_records_bytes(num_points);
 }
 
 int gstex_build_splat_records(int num_points, const float* means, const float* scales,
                               float glob_scale, const float* quats, const float* colors,
                               const float* opacities, const float* uv0, const float* umap,
                               const float* vmap, const int32_t* texture_dims,
-                              const float* viewmat, float fx, float fy, float cx, float cy,
-                              void* records, void* stream) {
+                              const float* viewmat, const float* c2w, float fx, float fy,
+                              float cx, float cy, void* records, void* stream) {
     REQUIRE(num_points >= 0, "num_points must be >= 0");
     if (num_points == 0) return GSTEX_OK;
     REQUIRE(means && scales && quats && colors && opacities && uv0 && umap && vmap &&
-                texture_dims && viewmat && records,
+                texture_dims && viewmat && c2w && records,
             "null pointer argument");
     REQUIRE(((uintptr_t)records & 15) == 0, "records must be 16-byte aligned");
     return check_hip(gstex::launch_build_records(num_points, means, scales, glob_scale, quats,
                                                  colors, opacities, uv0, umap, vmap, texture_dims,
-                                                 viewmat, fx, fy, cx, cy, (float*)records,
+                                                 viewmat, c2w, fx, fy, cx, cy, (float*)records,
                                                  as_stream(stream)),
                      "build_splat_records");
 }
 
 static gstex::RasterCall stage_call(int img_width, int img_height, int block_width, int channels,
                                     int settings, int num_points, const int32_t* ids, int num_ids,
-                                    const int32_t* tile_bins, const float* texture, int num_texels,
-                                    const float* viewmat, const float* c2w, float fx, float fy,
-                                    float cx, float cy, const float* background,
-                                    uint64_t* wave_masks) {
+                                    const int32_t* tile_bins, const void* texture,
+                                    int num_texels, int texture_format, const float* viewmat,
+                                    const float* c2w, float fx, float fy, float cx, float cy,
+                                    const float* background, uint64_t* wave_masks) {
     return gstex::RasterCall{img_width, img_height, block_width, channels, settings, num_points,
                              num_ids, num_texels, 1.f, fx, fy, cx, cy, nullptr, ids, tile_bins,
                              nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                             nullptr, texture, viewmat, c2w, background, wave_masks};
+                             nullptr, texture, viewmat, c2w, background, wave_masks,
+                             texture_format == GSTEX_TEXTURE_BF16};
 }
 
 size_t gstex_raster_wave_masks_size(int num_ids, int tiles_x, int tiles_y) {
@@ -364,7 +381,8 @@ size_t gstex_raster_wave_masks_size(int num_ids, int tiles_x, int tiles_y) {
 int gstex_raster_forward(int img_width, int img_height, int block_width, int channels,
                          int settings, const void* records, int num_points,
                          const int32_t* gaussian_ids_sorted, int num_ids,
-                         const int32_t* tile_bins, const float* texture, int num_texels,
+                         const int32_t* tile_bins, const void* texture, int num_texels,
+                         int texture_format,
                          const float* viewmat, const float* c2w, float fx, float fy, float cx,
                          float cy, const float* background, float* out_img, float* out_depth,
                          float* out_reg, float* out_texture, float* out_normal, float* final_Ts,
@@ -384,9 +402,13 @@ int gstex_raster_forward(int img_width, int img_height, int block_width, int cha
     REQUIRE(num_ids == 0 || num_points > 0, "gaussian_ids_sorted is non-empty but num_points is 0");
     REQUIRE(!wave_masks || wave_masks_bytes >= gstex_raster_wave_masks_size(num_ids, tx, ty),
             "wave_masks_bytes is below gstex_raster_wave_masks_size");
+    REQUIRE(texture_format == GSTEX_TEXTURE_F32 || texture_format == GSTEX_TEXTURE_BF16,
+            "texture_format must be GSTEX_TEXTURE_F32 (0) or GSTEX_TEXTURE_BF16 (1), got %d",
+            texture_format);
     const gstex::RasterCall c =
         stage_call(img_width, img_height, block_width, channels, settings, num_points,
-                   gaussian_ids_sorted, num_ids, tile_bins, texture, num_texels, viewmat, c2w, fx,
+                   gaussian_ids_sorted, num_ids, tile_bins, texture, num_texels, texture_format,
+                   viewmat, c2w, fx,
                    fy, cx, cy, background, wave_masks);
     return check_hip(gstex::raster_forward_stage(c, (const float*)records, out_img, out_depth,
                                                  out_reg, out_texture, out_normal, final_Ts,
@@ -398,7 +420,8 @@ int gstex_raster_forward(int img_width, int img_height, int block_width, int cha
 int gstex_raster_backward(int img_width, int img_height, int block_width, int channels,
                           int settings, const void* records, int num_points,
                           const int32_t* gaussian_ids_sorted, int num_ids,
-                          const int32_t* tile_bins, const float* texture, int num_texels,
+                          const int32_t* tile_bins, const void* texture, int num_texels,
+                          int texture_format,
                           const float* viewmat, const float* c2w, float fx, float fy, float cx,
                           float cy, const float* background, const float* final_Ts,
                           const int32_t* final_idx, const int32_t* depth_idx,
@@ -423,9 +446,13 @@ int gstex_raster_backward(int img_width, int img_height, int block_width, int ch
     REQUIRE(num_ids == 0 || num_points > 0, "gaussian_ids_sorted is non-empty but num_points is 0");
     REQUIRE(!wave_masks || wave_masks_bytes >= gstex_raster_wave_masks_size(num_ids, tx, ty),
             "wave_masks_bytes is below gstex_raster_wave_masks_size");
+    REQUIRE(texture_format == GSTEX_TEXTURE_F32 || texture_format == GSTEX_TEXTURE_BF16,
+            "texture_format must be GSTEX_TEXTURE_F32 (0) or GSTEX_TEXTURE_BF16 (1), got %d",
+            texture_format);
     const gstex::RasterCall c =
         stage_call(img_width, img_height, block_width, channels, settings, num_points,
-                   gaussian_ids_sorted, num_ids, tile_bins, texture, num_texels, viewmat, c2w, fx,
+                   gaussian_ids_sorted, num_ids, tile_bins, texture, num_texels, texture_format,
+                   viewmat, c2w, fx,
                    fy, cx, cy, background, const_cast<uint64_t*>(wave_masks));
     return check_hip(gstex::raster_backward_stage(c, (const float*)records, final_Ts, final_idx,
                                                   depth_idx, final_s, v_output, v_output_depth,

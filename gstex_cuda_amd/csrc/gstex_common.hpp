@@ -128,6 +128,13 @@ __device__ __forceinline__ void quat_axes_vjp(float w, float x, float y, float z
                     w * (g1.y - g2.x));
 }
 
+// Two int16 values (clamped) in one 32-bit word: lo | hi << 16.
+__host__ __device__ inline unsigned pack_i16(int lo, int hi) {
+    lo = lo < -32768 ? -32768 : (lo > 32767 ? 32767 : lo);
+    hi = hi < -32768 ? -32768 : (hi > 32767 ? 32767 : hi);
+    return (unsigned)(lo & 0xffff) | ((unsigned)(hi & 0xffff) << 16);
+}
+
 // Float -> int32 truncation (v_cvt_i32_f32: toward zero, saturating, NaN -> 0).
 __device__ __forceinline__ int f2i_trunc(float x) { return (int)x; }
 

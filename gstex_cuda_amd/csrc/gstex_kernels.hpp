@@ -26,15 +26,21 @@ hipError_t launch_num_tiles_hit(int n, const float* centers, const float* extent
 hipError_t launch_project_points(int n, const float* points, const float* viewmat, float fx,
                                  float fy, float cx, float cy, int clip, float* pix,
                                  float* depths, hipStream_t stream);
+// Splat records: n records of kRecordFloats floats, followed by n cull boxes
+// (uint2: int16 pixel-index ranges x0 | x1 << 16, y0 | y1 << 16).
 hipError_t launch_build_records(int n, const float* means, const float* scales, float glob_scale,
                                 const float* quats, const float* colors, const float* opacities,
                                 const float* uv0, const float* umap, const float* vmap,
-                                const int* texture_dims, const float* viewmat, float fx, float fy,
-                                float cx, float cy, float* records, hipStream_t stream);
+                                const int* texture_dims, const float* viewmat, const float* c2w,
+                                float fx, float fy, float cx, float cy, float* records,
+                                hipStream_t stream);
 
 // SH
 hipError_t launch_sh_forward(int n, int degree, int degrees_to_use, const float* viewdirs,
                              const float* coeffs, float* colors, hipStream_t stream);
+hipError_t launch_sh_forward_bf16(int n, int degree, int degrees_to_use, const float* viewdirs,
+                                  const unsigned short* coeffs, float* colors,
+                                  hipStream_t stream);
 hipError_t launch_sh_backward(int n, int degree, int degrees_to_use, const float* viewdirs,
                               const float* v_colors, float* v_coeffs, hipStream_t stream);
 int sh_num_bases(int degree);
@@ -74,13 +80,14 @@ struct RasterCall {
     const float* uv0;
     const float* umap;
     const float* vmap;
-    const float* texture;
+    const void* texture;  // (X, C) texels: float32, or bfloat16 when texture_bf16
     const float* viewmat;
     const float* c2w;
     const float* background;
     // Optional per-wave record of which list entries the forward composited
     // (see raster_wave_mask_words); nullptr disables it.
     uint64_t* wave_masks;
+    int texture_bf16;
 };
 
 // Wave masks: for every tile wave (at most kMaxTileWaves per tile) one bit per
@@ -93,6 +100,9 @@ inline size_t raster_wave_mask_words(int num_ids, int num_tiles) {
     return (size_t)(num_ids >> 6) + (size_t)num_tiles + 1;
 }
 
+// Process-wide switch for the per-wave culling by screen boxes (default on).
+bool raster_culling_enabled();
+bool set_raster_culling(bool on);
 size_t raster_forward_workspace_bytes(int n);
 size_t raster_backward_workspace_bytes(int n);
 hipError_t raster_forward(const RasterCall& c, float* out_img, float* out_depth, float* out_reg,
@@ -124,6 +134,16 @@ hipError_t grad_finalize(int n, const float* grad_rec, const float* records, con
                          float* v_scales, float* v_quats, float* v_uv0, float* v_umap,
                          float* v_vmap, hipStream_t s);
 constexpr size_t kRecordBytes = 128;
+constexpr size_t kCullBoxBytes = 8;
+inline size_t splat_records_bytes(int n) {
+    return n > 0 ? (size_t)n * (kRecordBytes + kCullBoxBytes) : 0;
+}
+__host__ __device__ inline uint2* record_boxes(float* records, int n) {
+    return reinterpret_cast<uint2*>(records + (size_t)n * (kRecordBytes / sizeof(float)));
+}
+__host__ __device__ inline const uint2* record_boxes(const float* records, int n) {
+    return reinterpret_cast<const uint2*>(records + (size_t)n * (kRecordBytes / sizeof(float)));
+}
 constexpr size_t kGradRecordBytes = 128;
 
 // sampling

@@ -128,13 +128,87 @@ __global__ __launch_bounds__(256) void k_project_points(int n, const float* __re
 // Splat records: axes from the (unnormalised) quaternion exactly as the
 // raster loop of texture.cu:166-169 derives them, plus every per-Gaussian
 // field the raster needs, in one 128-byte line.
+// Screen box (pixel index ranges, int16 pairs) outside of which the Gaussian
+// cannot pass the raster's alpha test, so a wave whose pixels all lie outside
+// may skip it.  alpha = min(0.99, opac * exp(-sigma)) >= 1/255 needs
+// sigma <= ln(255 opac): on the splat plane that is the ellipse
+// (l1/s1)^2 + (l2/s2)^2 <= 2 ln(255 opac), inside the rectangle
+// m +- r s1 ax1 +- r s2 ax2.  A pixel whose ray hits plane point P is the
+// projection of P in the ray model of the raster (c2w, fx, fy, cx, cy), so the
+// box of the four projected corners holds every such pixel when all corners
+// lie in front of the camera.  The blur alpha (settings bit 9) adds a disc of
+// radius sqrt(ln(255 opac)) pixels around the projected mean.  The radius is
+// padded by 2% and the box by 2 pixels.  No box (the full range) is given when
+// the bound does not hold: non-orthonormal axes, a corner behind or at the
+// camera, the camera within the plane's clamp distance, a singular c2w, or a
+// non-finite input; an empty box when opac < 1/255 (never visible).
+__device__ uint2 cull_box(V3 m, V3 a1, V3 a2, V3 a3, float s1, float s2, float opac,
+                          const float* c2w, float fx, float fy, float cx, float cy, float pu,
+                          float pv) {
+    const uint2 all = make_uint2(pack_i16(-32768, 32767), pack_i16(-32768, 32767));
+    const uint2 none = make_uint2(pack_i16(32767, -32768), pack_i16(32767, -32768));
+    // non-finite or zero scales / opacity give NaN alphas (0.99 after the clamp)
+    if (!(fabsf(opac) < 3e38f) || !(fabsf(s1) < 3e38f) || !(fabsf(s2) < 3e38f) ||
+        !(fabsf(1.f / s1) < 3e38f) || !(fabsf(1.f / s2) < 3e38f))
+        return all;
+    if (opac < 1.f / 255.f) return none;
+    const float o11 = dot3(a1, a1) - 1.f, o22 = dot3(a2, a2) - 1.f, o33 = dot3(a3, a3) - 1.f;
+    const float o12 = dot3(a1, a2), o13 = dot3(a1, a3), o23 = dot3(a2, a3);
+    const float tol = 1e-4f;
+    if (!(fabsf(o11) < tol && fabsf(o22) < tol && fabsf(o33) < tol && fabsf(o12) < tol &&
+          fabsf(o13) < tol && fabsf(o23) < tol))
+        return all;
+    const V3 o = v3(c2w[3], c2w[7], c2w[11]);
+    // ray/plane denominators are clamped at 1e-6 and t is kept only up to 1000,
+    // so a plane passing within 1e-3 of the camera is left unbounded
+    if (!(fabsf(dot3(a3, sub3(m, o))) > 2e-3f)) return all;
+    // inverse of the c2w rotation block (adjugate / determinant)
+    const float r00 = c2w[0], r01 = c2w[1], r02 = c2w[2];
+    const float r10 = c2w[4], r11 = c2w[5], r12 = c2w[6];
+    const float r20 = c2w[8], r21 = c2w[9], r22 = c2w[10];
+    const float c00 = r11 * r22 - r12 * r21, c01 = r02 * r21 - r01 * r22, c02 = r01 * r12 - r02 * r11;
+    const float c10 = r12 * r20 - r10 * r22, c11 = r00 * r22 - r02 * r20, c12 = r02 * r10 - r00 * r12;
+    const float c20 = r10 * r21 - r11 * r20, c21 = r01 * r20 - r00 * r21, c22 = r00 * r11 - r01 * r10;
+    const float det = r00 * c00 + r01 * c10 + r02 * c20;
+    if (!(fabsf(det) > 1e-12f)) return all;
+    const float id = 1.f / det;
+    const float r = 1.02f * sqrtf(2.f * logf(255.f * opac)) + 1e-3f;
+    float umin = 3e38f, umax = -3e38f, vmin = 3e38f, vmax = -3e38f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float e1 = (k & 1) ? r * s1 : -r * s1, e2 = (k & 2) ? r * s2 : -r * s2;
+        const V3 d = v3(m.x + e1 * a1.x + e2 * a2.x - o.x, m.y + e1 * a1.y + e2 * a2.y - o.y,
+                        m.z + e1 * a1.z + e2 * a2.z - o.z);
+        const float qx = id * (c00 * d.x + c01 * d.y + c02 * d.z);
+        const float qy = id * (c10 * d.x + c11 * d.y + c12 * d.z);
+        const float qz = id * (c20 * d.x + c21 * d.y + c22 * d.z);
+        if (!(qz > 1e-6f)) return all;
+        const float u = fx * qx / qz + cx - 0.5f, v = fy * qy / qz + cy - 0.5f;
+        umin = fminf(umin, u); umax = fmaxf(umax, u);
+        vmin = fminf(vmin, v); vmax = fmaxf(vmax, v);
+    }
+    // blur disc around the projected mean (pixel index coordinates)
+    const float rb = sqrtf(logf(255.f * opac)) + 0.05f;
+    umin = fminf(umin, pu - 0.5f - rb); umax = fmaxf(umax, pu - 0.5f + rb);
+    vmin = fminf(vmin, pv - 0.5f - rb); vmax = fmaxf(vmax, pv - 0.5f + rb);
+    if (!(umin > -30000.f && umax < 30000.f && vmin > -30000.f && vmax < 30000.f)) {
+        // clamp far boxes (still conservative)
+        umin = fmaxf(umin, -30000.f); vmin = fmaxf(vmin, -30000.f);
+        umax = fminf(umax, 30000.f); vmax = fminf(vmax, 30000.f);
+        if (!(umin <= umax && vmin <= vmax)) return all;
+    }
+    return make_uint2(pack_i16((int)floorf(umin) - 2, (int)ceilf(umax) + 2),
+                      pack_i16((int)floorf(vmin) - 2, (int)ceilf(vmax) + 2));
+}
+
 __global__ __launch_bounds__(256) void k_build_records(
     int n, const float* __restrict__ means, const float* __restrict__ scales, float glob_scale,
     const float* __restrict__ quats, const float* __restrict__ colors,
     const float* __restrict__ opacities, const float* __restrict__ uv0,
     const float* __restrict__ umap, const float* __restrict__ vmap,
-    const int* __restrict__ texture_dims, const float* __restrict__ viewmat, float fx, float fy,
-    float cx, float cy, float* __restrict__ records) {
+    const int* __restrict__ texture_dims, const float* __restrict__ viewmat,
+    const float* __restrict__ c2w, float fx, float fy, float cx, float cy,
+    float* __restrict__ records, uint2* __restrict__ boxes) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 q = reinterpret_cast<const float4*>(quats)[i];
@@ -166,6 +240,8 @@ __global__ __launch_bounds__(256) void k_build_records(
 #pragma unroll
     for (int k = 0; k < kRecordFloats / 4; ++k)
         dst[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
+    boxes[i] = cull_box(m, a1, a2, a3, scales[3 * i] * glob_scale, scales[3 * i + 1] * glob_scale,
+                        r[R_OPAC], c2w, fx, fy, cx, cy, pu, pvv);
 }
 
 hipError_t launch_get_aabb_2d(int n, const float* means, const float* scales, float glob_scale,
@@ -211,13 +287,13 @@ hipError_t launch_project_points(int n, const float* points, const float* viewma
 hipError_t launch_build_records(int n, const float* means, const float* scales, float glob_scale,
                                 const float* quats, const float* colors, const float* opacities,
                                 const float* uv0, const float* umap, const float* vmap,
-                                const int* texture_dims, const float* viewmat, float fx, float fy,
-                                float cx, float cy, float* records, hipStream_t stream) {
+                                const int* texture_dims, const float* viewmat, const float* c2w,
+                                float fx, float fy, float cx, float cy, float* records,
+                                hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    k_build_records<<<(n + 255) / 256, 256, 0, stream>>>(n, means, scales, glob_scale, quats,
-                                                        colors, opacities, uv0, umap, vmap,
-                                                        texture_dims, viewmat, fx, fy, cx, cy,
-                                                        records);
+    k_build_records<<<(n + 255) / 256, 256, 0, stream>>>(
+        n, means, scales, glob_scale, quats, colors, opacities, uv0, umap, vmap, texture_dims,
+        viewmat, c2w, fx, fy, cx, cy, records, record_boxes(records, n));
     return hipGetLastError();
 }
 

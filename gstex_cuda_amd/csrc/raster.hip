@@ -15,7 +15,9 @@
 
 namespace gstex {
 
-template <int CM>
+// GENERAL = false compiles out the blur (settings bit 9) and NDC distortion
+// (bit 10) code for the common settings; TB selects bfloat16 texels.
+template <int CM, bool GENERAL, bool TB>
 __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) {
     const int tile = blockIdx.x;
     const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
@@ -25,7 +27,8 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
     const int j = tx * a.bw + lx, i = ty * a.bw + ly;
     const bool inside = in_tile && i < a.img_h && j < a.img_w;
     const PixelRay pr = pixel_ray(a.c2w, a.viewmat, a.fx, a.fy, a.cx, a.cy, j, i);
-    const bool use_blur = a.settings & kSetBlur, use_ndc = a.settings & kSetNdc;
+    const bool use_blur = GENERAL && (a.settings & kSetBlur);
+    const bool use_ndc = GENERAL && (a.settings & kSetNdc);
     const bool flip = a.settings & kSetNormalFlip, bilinear = !(a.settings & kSetNearest);
     const int C = a.channels;
 
@@ -40,54 +43,70 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
 #pragma unroll
     for (int c = 0; c < CM; ++c) tex[c] = 0.f;
 
-    // Composited-entry bits of this wave, stored one 64-entry word at a time
-    // for the backward (every word the loop passes is written, zeros included).
+    // The list is walked in 64-entry windows aligned to multiples of 64: one
+    // coalesced read of the window's ids, a ballot of the entries whose screen
+    // box meets this wave's pixels, then the candidates front to back.  The
+    // composited entries of each window form one word of the wave masks
+    // (every window the walk reaches is written, zeros included).
     const int lane = threadIdx.x & (kWave - 1);
     uint64_t* const mplane =
-        a.wave_masks ? a.wave_masks + (size_t)(threadIdx.x / kWave) * a.mask_plane : nullptr;
+        a.wave_masks ? a.wave_masks + (size_t)wave_index() * a.mask_plane : nullptr;
     const int mword0 = (r0 >> 6) + tile, mentry0 = r0 & ~63;
-    uint64_t mbits = 0;
+    const WaveRect rect = wave_rect(inside, j, i);
 
-    int e = r0;
-    for (; e < r1; ++e) {
+    // A culled entry has alpha < 1/255 at every pixel of the wave, so it can
+    // still end a pixel (T (1 - alpha) <= 1e-4) only once T <= 1e-4 / (1 - 1/255);
+    // from the first live pixel at that level on, every entry is evaluated.
+    constexpr float kNearDoneT = 1.0041e-4f;
+    bool cull = a.boxes != nullptr;
+    for (int base = mentry0; base < r1; base += 64) {
         if (!__any(!done)) break;
-        const int g = splat_id(a, e);
-        const Splat s = splat_record(a, g);
-        bool comp = false;
-        if (!done) {
-            const Pair p = pair_eval(s, pr, use_blur);
-            const float next_T = T * (1.f - p.alpha);
-            if (next_T <= 1e-4f) done = true;
-            comp = !(p.skip || done);
-            if (comp) {
-                const float vis = p.alpha * T;
-                rgb0 += s.rgb.x * vis;
-                rgb1 += s.rgb.y * vis;
-                rgb2 += s.rgb.z * vis;
-                V3 hn = s.a3;
-                if (flip && dot3(pr.r, s.a3) > 0.f) hn = v3(-hn.x, -hn.y, -hn.z);
-                n0 += vis * hn.x;
-                n1 += vis * hn.y;
-                n2 += vis * hn.z;
-                const float t_view = p.t * pr.view_depth;
-                float u, v;
-                splat_uv(s, p.delta, u, v);
-                float wt[4], fu, fv;
-                int row[4];
-                chart_taps(s, u, v, bilinear, a.num_texels, wt, row, fu, fv);
-                // texels in chunks of up to 4 channels, all taps loaded before use
-                constexpr int CK = CM < 4 ? CM : 4;
+        if (cull && __any(!done && T <= kNearDoneT)) cull = false;
+        const Window w = window_candidates(a, base, r0, r1 - 1, rect, lane, cull);
+        uint64_t cand = w.cand;
+        uint64_t mbits = 0;
+        while (cand != 0) {
+            if (!__any(!done)) break;
+            const int bit = __ffsll((unsigned long long)cand) - 1;
+            cand &= cand - 1;
+            const int e = base + bit;
+            const int g = __builtin_amdgcn_readlane(w.gid, bit);
+            const Splat s = splat_record(a, g);
+            bool comp = false;
+            if (!done) {
+                const Pair p = pair_eval(s, pr, use_blur);
+                const float next_T = T * (1.f - p.alpha);
+                if (next_T <= 1e-4f) done = true;
+                comp = !(p.skip || done);
+                if (comp) {
+                    const float vis = p.alpha * T;
+                    rgb0 += s.rgb.x * vis;
+                    rgb1 += s.rgb.y * vis;
+                    rgb2 += s.rgb.z * vis;
+                    V3 hn = s.a3;
+                    if (flip && dot3(pr.r, s.a3) > 0.f) hn = v3(-hn.x, -hn.y, -hn.z);
+                    n0 += vis * hn.x;
+                    n1 += vis * hn.y;
+                    n2 += vis * hn.z;
+                    const float t_view = p.t * pr.view_depth;
+                    float u, v;
+                    splat_uv(s, p.delta, u, v);
+                    float wt[4], fu, fv;
+                    int row[4];
+                    chart_taps(s, u, v, bilinear, a.num_texels, wt, row, fu, fv);
+                    // texels in chunks of up to 4 channels, all taps loaded before use
+                    constexpr int CK = CM < 4 ? CM : 4;
 #pragma unroll
-                for (int c0 = 0; c0 < CM; c0 += CK) {
-                    if (c0 < C) {
+                    for (int c0 = 0; c0 < CM; c0 += CK) {
+                        if (c0 >= C) break;
                         float tv[4][CK];
 #pragma unroll
-                        for (int k = 0; k < 4; ++k)
+                        for (int k = 0; k < 4; ++k) {
+                            const int tk = row[k] * C + c0;
 #pragma unroll
                             for (int cc = 0; cc < CK; ++cc)
-                                tv[k][cc] = c0 + cc < C
-                                                ? a.texture[(size_t)row[k] * C + c0 + cc]
-                                                : 0.f;
+                                tv[k][cc] = c0 + cc < C ? texel<TB>(a.texture, tk + cc) : 0.f;
+                        }
 #pragma unroll
                         for (int cc = 0; cc < CK; ++cc) {
                             const float val = wt[0] * tv[0][cc] + wt[1] * tv[1][cc] +
@@ -95,35 +114,31 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
                             tex[c0 + cc] += vis * val;
                         }
                     }
+                    if (T > 0.5f) {
+                        pdepth = t_view;
+                        pdi = e;
+                    }
+                    float dval = p.t;
+                    if (use_ndc) {
+                        const float tn = 0.01f, tf = 1000.0f;
+                        dval = (tf * t_view - tf * tn) / ((tf - tn) * t_view);
+                    }
+                    preg += vis * (dval * dval * s0 + s2 - 2.f * dval * s1);
+                    s0 += vis;
+                    s1 += vis * dval;
+                    s2 += vis * dval * dval;
+                    T = next_T;
+                    cur_idx = e;
                 }
-                if (T > 0.5f) {
-                    pdepth = t_view;
-                    pdi = e;
-                }
-                float dval = p.t;
-                if (use_ndc) {
-                    const float tn = 0.01f, tf = 1000.0f;
-                    dval = (tf * t_view - tf * tn) / ((tf - tn) * t_view);
-                }
-                preg += vis * (dval * dval * s0 + s2 - 2.f * dval * s1);
-                s0 += vis;
-                s1 += vis * dval;
-                s2 += vis * dval * dval;
-                T = next_T;
-                cur_idx = e;
+            }
+            if (mplane && __any(comp)) mbits |= 1ull << bit;
+            if (cull && __any(!done && T <= kNearDoneT)) {
+                cull = false;
+                cand = bit == 63 ? 0ull : w.in_range & ~((2ull << bit) - 1);
             }
         }
-        if (mplane) {
-            if (__any(comp)) mbits |= 1ull << (e & 63);
-            if ((e & 63) == 63) {
-                if (lane == 0) mplane[mword0 + ((e - mentry0) >> 6)] = mbits;
-                mbits = 0;
-            }
-        }
+        if (mplane && lane == 0) mplane[mword0 + ((base - mentry0) >> 6)] = mbits;
     }
-    // the partial word holding the last visited entry
-    if (mplane && e > r0 && (e & 63) != 0 && lane == 0)
-        mplane[mword0 + ((e - 1 - mentry0) >> 6)] = mbits;
     if (!inside) return;
     const int pid = i * a.img_w + j;
     const float bg0 = a.background[0], bg1 = a.background[1], bg2 = a.background[2];
@@ -150,12 +165,28 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
 template <int CM>
 static hipError_t launch_fwd_cm(const RasterArgs& a, const FwdOut& o, hipStream_t s) {
     const int tiles = a.tiles_x * a.tiles_y;
-    k_raster_forward<CM><<<tiles, raster_block_threads(a.bw), 0, s>>>(a, o);
+    const bool general = a.settings & (kSetBlur | kSetNdc);
+    const dim3 grid(tiles), block(raster_block_threads(a.bw));
+    if (a.texture_bf16) {
+        if (general) k_raster_forward<CM, true, true><<<grid, block, 0, s>>>(a, o);
+        else k_raster_forward<CM, false, true><<<grid, block, 0, s>>>(a, o);
+    } else {
+        if (general) k_raster_forward<CM, true, false><<<grid, block, 0, s>>>(a, o);
+        else k_raster_forward<CM, false, false><<<grid, block, 0, s>>>(a, o);
+    }
     return hipGetLastError();
 }
 
+static bool g_raster_culling = true;
+bool raster_culling_enabled() { return g_raster_culling; }
+bool set_raster_culling(bool on) {
+    const bool prev = g_raster_culling;
+    g_raster_culling = on;
+    return prev;
+}
+
 size_t raster_forward_workspace_bytes(int n) {
-    return (size_t)n * kRecordFloats * sizeof(float) + 256;
+    return splat_records_bytes(n) + 256;
 }
 
 hipError_t raster_forward_stage(const RasterCall& c, const float* records, float* out_img,

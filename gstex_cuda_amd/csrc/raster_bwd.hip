@@ -13,22 +13,6 @@
 namespace gstex {
 
 // ------------------------------------------------------------------ wave reduction
-constexpr int kDppQuadXor1 = 0xB1;      // quad_perm [1,0,3,2]
-constexpr int kDppQuadXor2 = 0x4E;      // quad_perm [2,3,0,1]
-constexpr int kDppRowMirror = 0x140;    // lane i <-> 15 - i within 16
-constexpr int kDppRowHalfMirror = 0x141; // lane i <-> 7 - i within 8
-constexpr int kDppRowRor8 = 0x128;      // lane i <- lane (i + 8) mod 16 within 16 (= i ^ 8)
-
-template <int CTRL>
-__device__ __forceinline__ float dpp(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
-}
-
-__device__ __forceinline__ float swizzle_xor16(float v) {
-    // ds_swizzle bit mode: and 0x1f, or 0, xor 0x10 (within each 32-lane half)
-    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));
-}
-
 // One halving round: partners exchange the half of the values they give up.
 template <int CTRL, int NV, int H>
 __device__ __forceinline__ void halve(float (&v)[NV], bool upper) {
@@ -55,39 +39,15 @@ __device__ __forceinline__ float wave_reduce_scatter(float (&v)[32], int lane) {
     return r + __shfl_xor(r, 32);
 }
 
-template <int CTRL>
-__device__ __forceinline__ int dpp_i(int v) {
-    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
-}
-
-// Wave-wide min / max of an int: the four DPP rounds cover each row of 16
-// lanes (partner masks 15, 7, 2, 1), the swizzle pairs the rows of each half,
-// and lanes 0 and 32 hold the two half results.
-__device__ __forceinline__ int wave_min(int v) {
-    v = min(v, dpp_i<kDppRowMirror>(v));
-    v = min(v, dpp_i<kDppRowHalfMirror>(v));
-    v = min(v, dpp_i<kDppQuadXor2>(v));
-    v = min(v, dpp_i<kDppQuadXor1>(v));
-    v = min(v, __builtin_amdgcn_ds_swizzle(v, 0x401F));
-    return min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 32));
-}
-
-__device__ __forceinline__ int wave_max(int v) {
-    v = max(v, dpp_i<kDppRowMirror>(v));
-    v = max(v, dpp_i<kDppRowHalfMirror>(v));
-    v = max(v, dpp_i<kDppQuadXor2>(v));
-    v = max(v, dpp_i<kDppQuadXor1>(v));
-    v = max(v, __builtin_amdgcn_ds_swizzle(v, 0x401F));
-    return max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 32));
-}
-
 __device__ __forceinline__ int reduce_slot32(int lane) {
     return ((lane >> 3) & 1) * 16 + ((lane >> 2) & 1) * 8 + ((lane >> 1) & 1) * 4 +
            (lane & 1) * 2 + ((lane >> 4) & 1);
 }
 
 // ------------------------------------------------------------------ backward
-template <int CM>
+// GENERAL = false compiles out the blur (settings bit 9) and NDC distortion
+// (bit 10) code for the common settings; TB selects bfloat16 texels.
+template <int CM, bool GENERAL, bool TB>
 __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) {
     const int tile = blockIdx.x;
     const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
@@ -98,7 +58,8 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
     const int j = tx * a.bw + lx, i = ty * a.bw + ly;
     const bool inside = in_tile && i < a.img_h && j < a.img_w;
     const PixelRay pr = pixel_ray(a.c2w, a.viewmat, a.fx, a.fy, a.cx, a.cy, j, i);
-    const bool use_blur = a.settings & kSetBlur, use_ndc = a.settings & kSetNdc;
+    const bool use_blur = GENERAL && (a.settings & kSetBlur);
+    const bool use_ndc = GENERAL && (a.settings & kSetNdc);
     const bool bilinear = !(a.settings & kSetNearest), prop_uv = a.settings & kSetUvGrad;
     const int C = a.channels;
     const int pid = inside ? i * a.img_w + j : 0;
@@ -139,7 +100,7 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
     // Wave-private texel-gradient accumulator (one chart at a time), kept zero
     // between Gaussians by the flush below.
     __shared__ float tex_acc[256 / kWave][kTexAccFloats];
-    float* acc = tex_acc[threadIdx.x / kWave];
+    float* acc = tex_acc[wave_index()];
     for (int k = lane; k < kTexAccFloats; k += kWave) acc[k] = 0.f;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 
@@ -199,10 +160,12 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
                 if (cb >= C) break;
                 float tv[4][CK];
 #pragma unroll
-                for (int k = 0; k < 4; ++k)
+                for (int k = 0; k < 4; ++k) {
+                    const int tk = row[k] * C + cb;
 #pragma unroll
                     for (int cc = 0; cc < CK; ++cc)
-                        tv[k][cc] = cb + cc < C ? a.texture[(size_t)row[k] * C + cb + cc] : 0.f;
+                        tv[k][cc] = cb + cc < C ? texel<TB>(a.texture, tk + cc) : 0.f;
+                }
 #pragma unroll
                 for (int cc = 0; cc < CK; ++cc) {
                     const int c = cb + cc;
@@ -267,7 +230,7 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
                                  vl1 * s.a1.y + vl2 * s.a2.y + s.umap.y * vu + s.vmap.y * vv,
                                  vl1 * s.a1.z + vl2 * s.a2.z + s.umap.z * vu + s.vmap.z * vv);
             vt += dot3(vdelta, pr.r);
-            const float vnum = vt / p.den;
+            const float vnum = vt * __builtin_amdgcn_rcpf(p.den);
             const float vden = -vnum * p.t;
             gr[G_P11] = vl1 * p.l1;
             gr[G_P12] = vl1 * p.l2;
@@ -299,27 +262,32 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
                 // (lane bit 0), vertical pairs (bit 3), then bit 1.  On a match
                 // the lower lane takes the sum and the upper lane drops out.
                 const int key = valid ? row[0] : -1;
+                // per round: keep = 0 for the upper lane of a matched pair,
+                // take = 1 for the lower one; q <- keep * q + take * partner(q)
                 const bool m1 = dpp_i<kDppQuadXor1>(key) == key;
                 const bool m8 = dpp_i<kDppRowRor8>(key) == key;
                 const bool m2 = dpp_i<kDppQuadXor2>(key) == key;
-                const bool lo1 = !(lane & 1), lo8 = !(lane & 8), lo2 = !(lane & 2);
+                const float keep1 = m1 && (lane & 1) ? 0.f : 1.f, take1 = m1 && !(lane & 1);
+                const float keep8 = m8 && (lane & 8) ? 0.f : 1.f, take8 = m8 && !(lane & 8);
+                const float keep2 = m2 && (lane & 2) ? 0.f : 1.f, take2 = m2 && !(lane & 2);
+                int lofs[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) lofs[k] = (row[k] - s.start) * C;
 #pragma unroll
                 for (int c = 0; c < CM; ++c) {
                     if (c >= C) break;
+                    const float vv_c = vis * vtex[c];
                     float q[4];
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
-                        q[k] = wt[k] * (vis * vtex[c]);
-                        const float t1 = q[k] + dpp<kDppQuadXor1>(q[k]);
-                        q[k] = m1 ? (lo1 ? t1 : 0.f) : q[k];
-                        const float t8 = q[k] + dpp<kDppRowRor8>(q[k]);
-                        q[k] = m8 ? (lo8 ? t8 : 0.f) : q[k];
-                        const float t2 = q[k] + dpp<kDppQuadXor2>(q[k]);
-                        q[k] = m2 ? (lo2 ? t2 : 0.f) : q[k];
+                        q[k] = wt[k] * vv_c;
+                        q[k] = fmaf(dpp<kDppQuadXor1>(q[k]), take1, q[k] * keep1);
+                        q[k] = fmaf(dpp<kDppRowRor8>(q[k]), take8, q[k] * keep8);
+                        q[k] = fmaf(dpp<kDppQuadXor2>(q[k]), take2, q[k] * keep2);
                     }
 #pragma unroll
                     for (int k = 0; k < 4; ++k)
-                        if (q[k] != 0.f) atomicAdd(acc + (row[k] - s.start) * C + c, q[k]);
+                        if (q[k] != 0.f) atomicAdd(acc + lofs[k] + c, q[k]);
                 }
             }
         }
@@ -360,38 +328,27 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
         }
     };
 
-    // Entry order: back to front over [r0, e_hi].  With wave masks only the
-    // entries the forward composited for this wave are visited, read one
-    // 64-entry word at a time.  The next entry's id (and mask word) is loaded
-    // before the current entry's record.
+    // Entry order: back to front over [r0, e_hi], in 64-entry windows aligned
+    // to multiples of 64 (one coalesced id read per window).  With wave masks
+    // only the entries the forward composited for this wave are visited;
+    // without them, the entries whose screen box meets the wave's pixels (a
+    // culled entry has alpha < 1/255 at all of them, so it adds nothing).
     const bool use_masks = a.wave_masks != nullptr;
     const GSTEX_CONST uint64_t* mplane =
-        use_masks ? as_const_space(a.wave_masks) + (size_t)(threadIdx.x / kWave) * a.mask_plane
+        use_masks ? as_const_space(a.wave_masks) + (size_t)wave_index() * a.mask_plane
                   : nullptr;
     const int mword0 = (r0 >> 6) + tile, mentry0 = r0 & ~63;
-    int wi = e_hi >= r0 ? ((e_hi - mentry0) >> 6) + 1 : 0;  // next word to read, plus one
-    uint64_t m = 0;
-    auto next_entry = [&](int e_prev) -> int {
-        if (!use_masks) return e_prev - 1 >= r0 ? e_prev - 1 : -1;
-        while (m == 0) {
-            if (--wi < 0) return -1;
-            const int lo = mentry0 + 64 * wi;
-            m = mplane[mword0 + wi];
-            if (e_hi - lo < 63) m &= (2ull << (e_hi - lo)) - 1;
-            if (r0 > lo) m &= ~((1ull << (r0 - lo)) - 1);
+    const WaveRect rect = wave_rect(inside, j, i);
+    for (int base = e_hi >= r0 ? (e_hi & ~63) : mentry0 - 64; base >= mentry0; base -= 64) {
+        const Window w = window_candidates(a, base, r0, e_hi, rect, lane,
+                                           !use_masks && a.boxes != nullptr);
+        uint64_t m = use_masks ? (mplane[mword0 + ((base - mentry0) >> 6)] & w.in_range) : w.cand;
+        while (m != 0) {
+            const int bit = 63 - __builtin_clzll(m);
+            m &= ~(1ull << bit);
+            const int g = __builtin_amdgcn_readlane(w.gid, bit);
+            entry(base + bit, g, splat_record(a, g));
         }
-        const int bit = 63 - __builtin_clzll(m);
-        m &= ~(1ull << bit);
-        return mentry0 + 64 * wi + bit;
-    };
-    int e = next_entry(e_hi + 1);
-    int g = e >= 0 ? splat_id(a, e) : 0;
-    while (e >= 0) {
-        const int e_next = next_entry(e);
-        const int g_next = e_next >= 0 ? splat_id(a, e_next) : 0;
-        entry(e, g, splat_record(a, g));
-        e = e_next;
-        g = g_next;
     }
 }
 
@@ -466,12 +423,20 @@ __global__ __launch_bounds__(256) void k_grad_finalize(
 template <int CM>
 static hipError_t launch_bwd_cm(const RasterArgs& a, const BwdIn& b, hipStream_t s) {
     const int tiles = a.tiles_x * a.tiles_y;
-    k_raster_backward<CM><<<tiles, raster_block_threads(a.bw), 0, s>>>(a, b);
+    const bool general = a.settings & (kSetBlur | kSetNdc);
+    const dim3 grid(tiles), block(raster_block_threads(a.bw));
+    if (a.texture_bf16) {
+        if (general) k_raster_backward<CM, true, true><<<grid, block, 0, s>>>(a, b);
+        else k_raster_backward<CM, false, true><<<grid, block, 0, s>>>(a, b);
+    } else {
+        if (general) k_raster_backward<CM, true, false><<<grid, block, 0, s>>>(a, b);
+        else k_raster_backward<CM, false, false><<<grid, block, 0, s>>>(a, b);
+    }
     return hipGetLastError();
 }
 
 size_t raster_backward_workspace_bytes(int n) {
-    return 2 * (size_t)n * kRecordFloats * sizeof(float) + 256;
+    return splat_records_bytes(n) + (size_t)n * kGradRecordBytes + 512;
 }
 
 hipError_t raster_backward_stage(const RasterCall& c, const float* records,
@@ -513,7 +478,9 @@ hipError_t raster_backward(const RasterCall& c, const float* final_Ts, const int
                            size_t ws_bytes, hipStream_t s) {
     if (ws_bytes < raster_backward_workspace_bytes(c.num_points)) return hipErrorInvalidValue;
     float* records = (float*)ws;
-    float* grad_rec = records + (size_t)c.num_points * kRecordFloats;
+    // gradient records after the splat records + cull boxes, 256-byte aligned
+    const size_t rec_bytes = (splat_records_bytes(c.num_points) + 255) & ~(size_t)255;
+    float* grad_rec = (float*)((char*)ws + rec_bytes);
     hipError_t e = build_raster_records(c, records, s);
     if (e != hipSuccess) return e;
     e = fill_f32(grad_rec, (size_t)c.num_points * kGradFloats, 0.f, s);

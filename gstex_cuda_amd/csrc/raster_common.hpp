@@ -18,10 +18,24 @@ struct RasterArgs {
     const int* ids_sorted;
     const int* tile_bins;
     const float* records;
-    const float* texture;
+    const void* texture;   // float32 or bfloat16 texels (kernel template parameter)
     uint64_t* wave_masks;  // kMaxTileWaves planes of mask_plane words, or nullptr
     int mask_plane;
+    const uint2* boxes;    // per-Gaussian screen boxes (culling), or nullptr
+    bool texture_bf16;
 };
+
+// Texel value idx of a float32 (TB = false) or bfloat16 (TB = true) texture;
+// bfloat16 widens exactly to float32.
+template <bool TB>
+__device__ __forceinline__ float texel(const void* tex, int idx) {
+    if constexpr (TB) {
+        const unsigned short h = static_cast<const unsigned short*>(tex)[idx];
+        return __uint_as_float((unsigned)h << 16);
+    } else {
+        return static_cast<const float*>(tex)[idx];
+    }
+}
 
 struct FwdOut {
     float* out_img;
@@ -202,6 +216,56 @@ __device__ __forceinline__ void splat_uv(const Splat& s, V3 delta, float& u, flo
     v = fminf(fmaxf(v, 0.f), 1.f);
 }
 
+// ------------------------------------------------------------------ wave primitives
+constexpr int kDppQuadXor1 = 0xB1;      // quad_perm [1,0,3,2]
+constexpr int kDppQuadXor2 = 0x4E;      // quad_perm [2,3,0,1]
+constexpr int kDppRowMirror = 0x140;    // lane i <-> 15 - i within 16
+constexpr int kDppRowHalfMirror = 0x141; // lane i <-> 7 - i within 8
+constexpr int kDppRowRor8 = 0x128;      // lane i <- lane (i + 8) mod 16 within 16 (= i ^ 8)
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
+__device__ __forceinline__ float swizzle_xor16(float v) {
+    // ds_swizzle bit mode: and 0x1f, or 0, xor 0x10 (within each 32-lane half)
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));
+}
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
+}
+
+// Wave-wide min / max of an int: the four DPP rounds cover each row of 16
+// lanes (partner masks 15, 7, 2, 1), the swizzle pairs the rows of each half,
+// and lanes 0 and 32 hold the two half results.
+__device__ __forceinline__ int wave_min(int v) {
+    v = min(v, dpp_i<kDppRowMirror>(v));
+    v = min(v, dpp_i<kDppRowHalfMirror>(v));
+    v = min(v, dpp_i<kDppQuadXor2>(v));
+    v = min(v, dpp_i<kDppQuadXor1>(v));
+    v = min(v, __builtin_amdgcn_ds_swizzle(v, 0x401F));
+    return min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 32));
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+    v = max(v, dpp_i<kDppRowMirror>(v));
+    v = max(v, dpp_i<kDppRowHalfMirror>(v));
+    v = max(v, dpp_i<kDppQuadXor2>(v));
+    v = max(v, dpp_i<kDppQuadXor1>(v));
+    v = max(v, __builtin_amdgcn_ds_swizzle(v, 0x401F));
+    return max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 32));
+}
+
+// Index of this wave within the workgroup, as a wave-uniform (SGPR) value:
+// divergence analysis treats any function of threadIdx.x as per-lane, which
+// would turn loads addressed by it into vector loads.
+__device__ __forceinline__ int wave_index() {
+    return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+}
+
 __device__ __forceinline__ void tile_range(const RasterArgs& a, int tile, int& r0, int& r1) {
     const int2 rg = reinterpret_cast<const int2*>(a.tile_bins)[tile];
     r0 = min(max(rg.x, 0), a.num_ids);
@@ -216,6 +280,54 @@ __device__ __forceinline__ int splat_id(const RasterArgs& a, int e) {
 // Record of Gaussian g (wave-uniform) as scalar loads.
 __device__ __forceinline__ Splat splat_record(const RasterArgs& a, int g) {
     return load_splat(as_const_space(a.records) + (size_t)g * kRecordFloats);
+}
+
+// Pixel-index rectangle [x0, x1] x [y0, y1] covered by this wave's in-image
+// pixels (wave-uniform; empty when it has none).
+struct WaveRect {
+    int x0, x1, y0, y1;
+};
+
+__device__ __forceinline__ WaveRect wave_rect(bool inside, int j, int i) {
+    WaveRect r;
+    r.x0 = wave_min(inside ? j : 0x7fffffff);
+    r.x1 = wave_max(inside ? j : -0x7fffffff);
+    r.y0 = wave_min(inside ? i : 0x7fffffff);
+    r.y1 = wave_max(inside ? i : -0x7fffffff);
+    return r;
+}
+
+// Does a screen box (int16 pairs lo | hi << 16 for x and y) meet the rectangle?
+__device__ __forceinline__ bool box_meets(uint2 bx, const WaveRect& r) {
+    const int x0 = (int)(short)(bx.x & 0xffff), x1 = (int)(short)(bx.x >> 16);
+    const int y0 = (int)(short)(bx.y & 0xffff), y1 = (int)(short)(bx.y >> 16);
+    return x1 >= r.x0 && x0 <= r.x1 && y1 >= r.y0 && y0 <= r.y1;
+}
+
+// One 64-entry window of a tile list (entries base .. base + 63, base a
+// multiple of 64): this lane's Gaussian id for entry base + lane (0 outside
+// [lo, hi]), the wave mask of the entries in [lo, hi], and of those the ones
+// whose screen box meets the wave rectangle (all of them without boxes or
+// with cull = false).
+struct Window {
+    uint64_t in_range, cand;
+    int gid;
+};
+
+__device__ __forceinline__ Window window_candidates(const RasterArgs& a, int base, int lo, int hi,
+                                                   const WaveRect& rect, int lane, bool cull) {
+    Window w;
+    const int e = base + lane;
+    const bool in_range = e >= lo && e <= hi;
+    bool cand = in_range;
+    w.gid = 0;
+    if (in_range) {
+        w.gid = min(max(a.ids_sorted[e], 0), a.num_points - 1);
+        if (cull) cand = box_meets(a.boxes[w.gid], rect);
+    }
+    w.in_range = __ballot(in_range);
+    w.cand = __ballot(cand);
+    return w;
 }
 
 inline RasterArgs make_raster_args(const RasterCall& c, const float* records) {
@@ -241,15 +353,21 @@ inline RasterArgs make_raster_args(const RasterCall& c, const float* records) {
     a.tile_bins = c.tile_bins;
     a.records = records;
     a.texture = c.texture;
+    a.texture_bf16 = c.texture_bf16 != 0;
     a.wave_masks = c.wave_masks;
     a.mask_plane = (int)raster_wave_mask_words(c.num_ids, a.tiles_x * a.tiles_y);
+    // boxes hold int16 pixel indices, so culling is off for larger images
+    a.boxes = raster_culling_enabled() && c.num_points > 0 && c.img_w < 32000 &&
+                      c.img_h < 32000
+                  ? record_boxes(records, c.num_points)
+                  : nullptr;
     return a;
 }
 
 inline hipError_t build_raster_records(const RasterCall& c, float* records, hipStream_t s) {
     return launch_build_records(c.num_points, c.means, c.scales, c.glob_scale, c.quats, c.colors,
                                 c.opacities, c.uv0, c.umap, c.vmap, c.texture_dims, c.viewmat,
-                                c.fx, c.fy, c.cx, c.cy, records, s);
+                                c.c2w, c.fx, c.fy, c.cx, c.cy, records, s);
 }
 
 }  // namespace gstex

@@ -71,31 +71,38 @@ __device__ __forceinline__ void sh_basis(float dx, float dy, float dz, float (&b
     }
 }
 
-template <int LO, int HI>
-__device__ __forceinline__ float band_sum(const float (&b)[25], const float* c, int ch) {
-    float s = b[LO] * c[LO * 3 + ch];
+// Coefficient k of a float32 or bfloat16 (uint16 bits, widened exactly) array.
+__device__ __forceinline__ float coef(const float* c, int k) { return c[k]; }
+__device__ __forceinline__ float coef(const unsigned short* c, int k) {
+    return __uint_as_float((unsigned)c[k] << 16);
+}
+
+template <int LO, int HI, typename CT>
+__device__ __forceinline__ float band_sum(const float (&b)[25], const CT* c, int ch) {
+    float s = b[LO] * coef(c, LO * 3 + ch);
 #pragma unroll
-    for (int k = LO + 1; k < HI; ++k) s += b[k] * c[k * 3 + ch];
+    for (int k = LO + 1; k < HI; ++k) s += b[k] * coef(c, k * 3 + ch);
     return s;
 }
 
-// One lane per Gaussian; coefficients are (N, nb, 3), evaluated up to degree D.
-template <int D>
+// One lane per Gaussian; coefficients are (N, nb, 3) float32 or bfloat16,
+// evaluated up to degree D in float32.
+template <int D, typename CT>
 __global__ __launch_bounds__(256) void k_sh_forward(int n, int nb, const float* __restrict__ viewdirs,
-                                                    const float* __restrict__ coeffs,
+                                                    const CT* __restrict__ coeffs,
                                                     float* __restrict__ colors) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float* c = coeffs + (size_t)i * nb * 3;
+    const CT* c = coeffs + (size_t)i * nb * 3;
     float acc[3];
 #pragma unroll
-    for (int ch = 0; ch < 3; ++ch) acc[ch] = kShC0 * c[ch];
+    for (int ch = 0; ch < 3; ++ch) acc[ch] = kShC0 * coef(c, ch);
     if constexpr (D >= 1) {
         float b[25], x, y, z;
         sh_basis<D>(viewdirs[3 * i], viewdirs[3 * i + 1], viewdirs[3 * i + 2], b, x, y, z);
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
-            acc[ch] += kShC1 * (-y * c[3 + ch] + z * c[6 + ch] - x * c[9 + ch]);
+            acc[ch] += kShC1 * (-y * coef(c, 3 + ch) + z * coef(c, 6 + ch) - x * coef(c, 9 + ch));
             if constexpr (D >= 2) acc[ch] += band_sum<4, 9>(b, c, ch);
             if constexpr (D >= 3) acc[ch] += band_sum<9, 16>(b, c, ch);
             if constexpr (D >= 4) acc[ch] += band_sum<16, 25>(b, c, ch);
@@ -135,18 +142,31 @@ __global__ __launch_bounds__(256) void k_sh_backward(int n, int nb, const float*
     }
 }
 
-hipError_t launch_sh_forward(int n, int degree, int degrees_to_use, const float* viewdirs,
-                             const float* coeffs, float* colors, hipStream_t stream) {
+template <typename CT>
+static hipError_t sh_forward_t(int n, int degree, int degrees_to_use, const float* viewdirs,
+                               const CT* coeffs, float* colors, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     const int nb = num_sh_bases(degree), g = (n + 255) / 256;
     switch (degrees_to_use) {
-        case 0: k_sh_forward<0><<<g, 256, 0, stream>>>(n, nb, viewdirs, coeffs, colors); break;
-        case 1: k_sh_forward<1><<<g, 256, 0, stream>>>(n, nb, viewdirs, coeffs, colors); break;
-        case 2: k_sh_forward<2><<<g, 256, 0, stream>>>(n, nb, viewdirs, coeffs, colors); break;
-        case 3: k_sh_forward<3><<<g, 256, 0, stream>>>(n, nb, viewdirs, coeffs, colors); break;
-        default: k_sh_forward<4><<<g, 256, 0, stream>>>(n, nb, viewdirs, coeffs, colors); break;
+        case 0: k_sh_forward<0, CT><<<g, 256, 0, stream>>>(n, nb, viewdirs, coeffs, colors); break;
+        case 1: k_sh_forward<1, CT><<<g, 256, 0, stream>>>(n, nb, viewdirs, coeffs, colors); break;
+        case 2: k_sh_forward<2, CT><<<g, 256, 0, stream>>>(n, nb, viewdirs, coeffs, colors); break;
+        case 3: k_sh_forward<3, CT><<<g, 256, 0, stream>>>(n, nb, viewdirs, coeffs, colors); break;
+        default: k_sh_forward<4, CT><<<g, 256, 0, stream>>>(n, nb, viewdirs, coeffs, colors); break;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_sh_forward(int n, int degree, int degrees_to_use, const float* viewdirs,
+                             const float* coeffs, float* colors, hipStream_t stream) {
+    return sh_forward_t<float>(n, degree, degrees_to_use, viewdirs, coeffs, colors, stream);
+}
+
+hipError_t launch_sh_forward_bf16(int n, int degree, int degrees_to_use, const float* viewdirs,
+                                  const unsigned short* coeffs, float* colors,
+                                  hipStream_t stream) {
+    return sh_forward_t<unsigned short>(n, degree, degrees_to_use, viewdirs, coeffs, colors,
+                                        stream);
 }
 
 hipError_t launch_sh_backward(int n, int degree, int degrees_to_use, const float* viewdirs,

@@ -24,6 +24,7 @@ __all__ = [
     # additions beyond the reference's ten
     "project_gaussians", "project_points", "num_tiles_hit", "sort_intersects", "bin_gaussians",
     "build_splat_records", "raster_forward", "raster_backward", "finalize_gradients",
+    "set_raster_culling",
 ]
 
 
@@ -62,11 +63,14 @@ def compute_sh_forward(num_points: int, degree: int, degrees_to_use: int,
     if coeffs.dim() != 3 or coeffs.size(0) != num_points or coeffs.size(1) != nb or coeffs.size(2) != 3:
         raise RuntimeError("coeffs must have dimensions (N, D, 3)")
     _dtype(viewdirs, "viewdirs", torch.float32)
-    _dtype(coeffs, "coeffs", torch.float32)
+    if coeffs.dtype not in (torch.float32, torch.bfloat16):
+        raise RuntimeError(f"coeffs must have dtype float32 or bfloat16, got {coeffs.dtype}")
     colors = torch.empty((num_points, 3), dtype=torch.float32, device=coeffs.device)
+    fn = ("gstex_compute_sh_forward_bf16" if coeffs.dtype == torch.bfloat16
+          else "gstex_compute_sh_forward")
     with torch.cuda.device(coeffs.device):
-        call("gstex_compute_sh_forward", num_points, degree, degrees_to_use, _ptr(viewdirs),
-             _ptr(coeffs), _ptr(colors), _stream(coeffs.device))
+        call(fn, num_points, degree, degrees_to_use, _ptr(viewdirs), _ptr(coeffs), _ptr(colors),
+             _stream(coeffs.device))
     return colors
 
 
@@ -232,14 +236,18 @@ _RASTER_INPUTS = ("texture_dims", "gaussian_ids_sorted", "tile_bins", "colors", 
                   "background")
 
 
-def _raster_checks(named):
+def _raster_checks(named, bf16_texture_ok=False):
     for name in _RASTER_INPUTS:
         _check(named[name], name)
     for name in ("texture_dims", "gaussian_ids_sorted", "tile_bins"):
         _dtype(named[name], name, torch.int32)
     for name in ("colors", "opacities", "means", "scales", "quats", "uv0", "umap", "vmap",
-                 "texture", "viewmat", "c2w", "background"):
+                 "viewmat", "c2w", "background"):
         _dtype(named[name], name, torch.float32)
+    if bf16_texture_ok:
+        _texture_format(named["texture"])
+    else:
+        _dtype(named["texture"], "texture", torch.float32)
 
 
 def texture_forward(tile_bounds, block, img_size, texture_info, texture_dims,
@@ -328,12 +336,21 @@ def texture_backward(img_height, img_width, block_width, texture_info, texture_d
 # that splat records are built once per view and the backward visits only the
 # entries the forward composited (wave masks).  Same semantics as the two
 # reference-shaped ops above.
+def _texture_format(texture: torch.Tensor) -> int:
+    if texture.dtype == torch.bfloat16:
+        return 1
+    if texture.dtype == torch.float32:
+        return 0
+    raise RuntimeError(f"texture must have dtype float32 or bfloat16, got {texture.dtype}")
+
+
 def build_splat_records(means, scales, glob_scale, quats, colors, opacities, uv0, umap, vmap,
-                        texture_dims, viewmat, fx, fy, cx, cy):
-    """128-byte per-Gaussian records (uint8 tensor) for raster_forward/backward."""
+                        texture_dims, viewmat, c2w, fx, fy, cx, cy):
+    """Per-Gaussian splat records + screen boxes (uint8 tensor) for
+    raster_forward/backward."""
     named = locals()
     for name in ("means", "scales", "quats", "colors", "opacities", "uv0", "umap", "vmap",
-                 "texture_dims", "viewmat"):
+                 "texture_dims", "viewmat", "c2w"):
         _check(named[name], name)
     dev = means.device
     n = means.size(0)
@@ -342,9 +359,15 @@ def build_splat_records(means, scales, glob_scale, quats, colors, opacities, uv0
     with torch.cuda.device(dev):
         call("gstex_build_splat_records", n, _ptr(means), _ptr(scales), float(glob_scale),
              _ptr(quats), _ptr(colors), _ptr(opacities), _ptr(uv0), _ptr(umap), _ptr(vmap),
-             _ptr(texture_dims), _ptr(viewmat), float(fx), float(fy), float(cx), float(cy),
-             _ptr(records), _stream(dev))
+             _ptr(texture_dims), _ptr(viewmat), _ptr(c2w), float(fx), float(fy), float(cx),
+             float(cy), _ptr(records), _stream(dev))
     return records
+
+
+def set_raster_culling(enabled: bool) -> bool:
+    """Turn the raster kernels' per-wave screen-box culling on or off (results are
+    identical either way); returns the previous setting."""
+    return bool(_lib.lib().gstex_set_raster_culling(int(bool(enabled))))
 
 
 def raster_forward(records, num_points, img_width, img_height, block_width, channels, settings,
@@ -373,8 +396,8 @@ def raster_forward(records, num_points, img_width, img_height, block_width, chan
     with torch.cuda.device(dev):
         call("gstex_raster_forward", W, H, int(block_width), C, int(settings), _ptr(records),
              int(num_points), _ptr(gaussian_ids_sorted), m, _ptr(tile_bins), _ptr(texture),
-             texture.size(0), _ptr(viewmat), _ptr(c2w), float(fx), float(fy), float(cx),
-             float(cy), _ptr(background), *[_ptr(o) for o in outs],
+             texture.size(0), _texture_format(texture), _ptr(viewmat), _ptr(c2w), float(fx),
+             float(fy), float(cx), float(cy), _ptr(background), *[_ptr(o) for o in outs],
              None if masks is None else _ptr(masks), 0 if masks is None else masks.numel(),
              _stream(dev))
     return outs, masks
@@ -398,8 +421,8 @@ def raster_backward(records, num_points, img_width, img_height, block_width, cha
         call("gstex_raster_backward", int(img_width), int(img_height), int(block_width), C,
              int(settings), _ptr(records), n, _ptr(gaussian_ids_sorted),
              gaussian_ids_sorted.numel(), _ptr(tile_bins), _ptr(texture), texture.size(0),
-             _ptr(viewmat), _ptr(c2w), float(fx), float(fy), float(cx), float(cy),
-             _ptr(background), _ptr(final_Ts), _ptr(final_idx), _ptr(depth_idx), _ptr(final_s),
+             _texture_format(texture), _ptr(viewmat), _ptr(c2w), float(fx), float(fy),
+             float(cx), float(cy), _ptr(background), _ptr(final_Ts), _ptr(final_idx), _ptr(depth_idx), _ptr(final_s),
              _ptr(v_output), _ptr(v_output_depth), _ptr(v_output_reg), _ptr(v_output_alpha),
              _ptr(v_output_texture), _ptr(v_output_normal), _ptr(grad_records),
              _ptr(v_texture), None if wave_masks is None else _ptr(wave_masks),

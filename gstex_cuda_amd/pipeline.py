@@ -66,8 +66,8 @@ class RenderStep:
 
     ``params`` (device tensors, reference layouts): means (N,3), scales (N,3),
     quats (N,4), opacities (N,1), uv0 (N,1,2), umap (N,1,3), vmap (N,1,3),
-    texture (X,C), texture_dims (N,3) int32, and either colors (N,3) or
-    sh_coeffs (N,D,3) with ``sh_degree``.
+    texture (X,C) float32 or bfloat16, texture_dims (N,3) int32, and either
+    colors (N,3) or sh_coeffs (N,D,3) float32 or bfloat16 with ``sh_degree``.
     """
 
     def __init__(self, params: Dict[str, torch.Tensor], cam: Camera, block_width: int = 16,
@@ -84,6 +84,9 @@ class RenderStep:
         self.n = n
         self.C = params["texture"].shape[1]
         self.X = params["texture"].shape[0]
+        # bfloat16 texels / SH coefficients are read as stored; all arithmetic
+        # and every gradient (v_texture, v_sh_coeffs) is float32
+        self.texture_format = 1 if params["texture"].dtype == torch.bfloat16 else 0
         H, W = cam.img_h, cam.img_w
         self.tiles = ((W + block_width - 1) // block_width, (H + block_width - 1) // block_width)
         f32 = dict(dtype=torch.float32, device=dev)
@@ -127,7 +130,7 @@ class RenderStep:
             v_umap=torch.empty((n, 1, 3), **f32), v_vmap=torch.empty((n, 1, 3), **f32),
             v_texture=torch.empty((self.X, self.C), **f32))
         if sh_degree >= 0:
-            self.grads["v_sh_coeffs"] = torch.empty_like(params["sh_coeffs"])
+            self.grads["v_sh_coeffs"] = torch.empty(params["sh_coeffs"].shape, **f32)
         self.num_intersects = 0
 
     def _stream(self):
@@ -155,8 +158,10 @@ class RenderStep:
         n = self.n
         if self.sh_degree >= 0:
             viewdirs = p["means"] - cam.c2w[:3, 3]
-            call("gstex_compute_sh_forward", n, self.sh_degree, self.sh_degree,
-                 _p(viewdirs), _p(p["sh_coeffs"]), _p(self.colors), s)
+            fn = ("gstex_compute_sh_forward_bf16" if p["sh_coeffs"].dtype == torch.bfloat16
+                  else "gstex_compute_sh_forward")
+            call(fn, n, self.sh_degree, self.sh_degree, _p(viewdirs), _p(p["sh_coeffs"]),
+                 _p(self.colors), s)
             self._viewdirs = viewdirs
         call("gstex_project_gaussians", n, _p(p["means"]), _p(p["scales"]), 1.0, _p(p["quats"]),
              _p(cam.viewmat), cam.fx, cam.fy, cam.cx, cam.cy, cam.img_h, cam.img_w, self.bw,
@@ -172,13 +177,13 @@ class RenderStep:
             self.tile_bins.zero_()
         call("gstex_build_splat_records", n, _p(p["means"]), _p(p["scales"]), 1.0,
              _p(p["quats"]), _p(self.colors), _p(p["opacities"]), _p(p["uv0"]), _p(p["umap"]),
-             _p(p["vmap"]), _p(p["texture_dims"]), _p(cam.viewmat), cam.fx, cam.fy, cam.cx,
-             cam.cy, _p(self.records), s)
+             _p(p["vmap"]), _p(p["texture_dims"]), _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy,
+             cam.cx, cam.cy, _p(self.records), s)
         if timer:
             timer.start("raster_forward")
         call("gstex_raster_forward", cam.img_w, cam.img_h, self.bw, self.C, self.settings,
              _p(self.records), n, _p(self.ids), m, _p(self.tile_bins), _p(p["texture"]), self.X,
-             _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy, cam.cx, cam.cy, _p(self.background),
+             self.texture_format, _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy, cam.cx, cam.cy, _p(self.background),
              _p(self.out_img), _p(self.out_depth), _p(self.out_reg), _p(self.out_texture),
              _p(self.out_normal), _p(self.final_Ts), _p(self.final_idx), _p(self.depth_idx),
              _p(self.out_reg_s), *self._masks(), s)
@@ -210,7 +215,7 @@ class RenderStep:
             timer.start("raster_backward")
         call("gstex_raster_backward", cam.img_w, cam.img_h, self.bw, self.C, self.settings,
              _p(self.records), n, _p(self.ids), m, _p(self.tile_bins), _p(p["texture"]), self.X,
-             _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy, cam.cx, cam.cy, _p(self.background),
+             self.texture_format, _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy, cam.cx, cam.cy, _p(self.background),
              _p(self.final_Ts), _p(self.final_idx), _p(self.depth_idx), _p(self.out_reg_s),
              _p(self.v_img), _p(v_depth), _p(v_reg), _p(v_alpha), _p(self.v_tex), _p(v_normal),
              _p(self.grad_records), _p(g["v_texture"]), *self._masks(), s)

@@ -81,13 +81,14 @@ class _TextureGaussians(Function):
         # masks let the backward skip entries no pixel of a wave composited.
         if int(texture_info[1]) != 1:
             raise RuntimeError("texture_info[1] (num_probs) must be 1")
+        # texels may be stored as bfloat16 (float32 arithmetic and gradients)
         _C._raster_checks(dict(texture_dims=texture_dims, gaussian_ids_sorted=gaussian_ids_sorted,
                                tile_bins=tile_bins, colors=colors, opacities=opacity,
                                means=means, scales=scales, quats=quats, uv0=uv0, umap=umap,
                                vmap=vmap, texture=texture, viewmat=viewmat, c2w=c2w,
-                               background=background))
+                               background=background), bf16_texture_ok=True)
         records = _C.build_splat_records(means, scales, glob_scale, quats, colors, opacity, uv0,
-                                         umap, vmap, texture_dims, viewmat, fx, fy, cx, cy)
+                                         umap, vmap, texture_dims, viewmat, c2w, fx, fy, cx, cy)
         C = int(texture_info[2])
         ((out_img, out_depth, out_reg, out_texture, out_normal, final_Ts, final_idx, depth_idx,
           out_reg_s), wave_masks) = _C.raster_forward(

@@ -32,6 +32,11 @@ extern "C" {
 #define GSTEX_ERR_HIP 2    /* a HIP runtime / launch error */
 #define GSTEX_ERR_SPACE 3  /* workspace too small */
 
+/* Texel storage formats of the per-stage raster entry points (bfloat16 texels
+ * are widened exactly to float32; compositing and gradients stay float32). */
+#define GSTEX_TEXTURE_F32 0
+#define GSTEX_TEXTURE_BF16 1
+
 /* Description of the last failure on this thread (empty string if none). */
 const char *gstex_last_error(void);
 /* ABI version of this library. */
@@ -68,6 +73,13 @@ int gstex_project_points(int num_points, const float *points, const float *viewm
  * coeffs (N, bases(degree), 3); degrees_to_use <= degree. */
 int gstex_compute_sh_forward(int num_points, int degree, int degrees_to_use, const float *viewdirs,
                              const float *coeffs, float *colors, void *stream);
+
+/* compute_sh_forward with bfloat16 coefficients (N, D, 3), widened exactly to
+ * float32; the arithmetic and the colours are float32.  No reference
+ * counterpart: bfloat16 storage of SH features. */
+int gstex_compute_sh_forward_bf16(int num_points, int degree, int degrees_to_use,
+                                  const float *viewdirs, const uint16_t *coeffs, float *colors,
+                                  void *stream);
 
 /* Replaces compute_sh_backward_tensor (cuda/csrc/bindings.cu:45-75).
  * v_coeffs (N, bases(degree), 3) is fully written. */
@@ -154,7 +166,10 @@ int gstex_texture_backward(int img_height, int img_width, int block_width, int n
  * records once for both passes and each kernel can be timed alone:
  *   gstex_build_splat_records -> gstex_raster_forward -> gstex_raster_backward
  *   -> gstex_finalize_gradients.
- * records: gstex_splat_records_size(N) bytes, 256-byte aligned.
+ * records: gstex_splat_records_size(N) bytes, 256-byte aligned: N 128-byte
+ * splat records followed by N 8-byte screen boxes, outside of which a Gaussian
+ * cannot pass the alpha test (the raster kernels skip it for waves whose pixels
+ * all lie outside; gstex_set_raster_culling(0) turns that off).
  * grad_records: N x 32 float32 accumulators (the raster backward adds into them,
  * so the caller zeroes them and v_texture first). */
 size_t gstex_splat_records_size(int num_points);
@@ -162,8 +177,12 @@ int gstex_build_splat_records(int num_points, const float *means, const float *s
                               float glob_scale, const float *quats, const float *colors,
                               const float *opacities, const float *uv0, const float *umap,
                               const float *vmap, const int32_t *texture_dims,
-                              const float *viewmat, float fx, float fy, float cx, float cy,
-                              void *records, void *stream);
+                              const float *viewmat, const float *c2w, float fx, float fy,
+                              float cx, float cy, void *records, void *stream);
+/* Process-wide switch for the per-wave culling by screen boxes (1 = on, the
+ * default; 0 = every list entry is evaluated).  Results are identical either
+ * way; returns the previous setting. */
+int gstex_set_raster_culling(int enabled);
 /* Wave masks (optional, for the per-stage forward/backward pair): one bit per
  * (tile wave, list entry), written by gstex_raster_forward and read by
  * gstex_raster_backward, which then visits only the entries the forward
@@ -175,7 +194,8 @@ size_t gstex_raster_wave_masks_size(int num_ids, int tiles_x, int tiles_y);
 int gstex_raster_forward(int img_width, int img_height, int block_width, int channels,
                          int settings, const void *records, int num_points,
                          const int32_t *gaussian_ids_sorted, int num_ids,
-                         const int32_t *tile_bins, const float *texture, int num_texels,
+                         const int32_t *tile_bins, const void *texture, int num_texels,
+                         int texture_format,
                          const float *viewmat, const float *c2w, float fx, float fy, float cx,
                          float cy, const float *background, float *out_img, float *out_depth,
                          float *out_reg, float *out_texture, float *out_normal, float *final_Ts,
@@ -184,7 +204,8 @@ int gstex_raster_forward(int img_width, int img_height, int block_width, int cha
 int gstex_raster_backward(int img_width, int img_height, int block_width, int channels,
                           int settings, const void *records, int num_points,
                           const int32_t *gaussian_ids_sorted, int num_ids,
-                          const int32_t *tile_bins, const float *texture, int num_texels,
+                          const int32_t *tile_bins, const void *texture, int num_texels,
+                          int texture_format,
                           const float *viewmat, const float *c2w, float fx, float fy, float cx,
                           float cy, const float *background, const float *final_Ts,
                           const int32_t *final_idx, const int32_t *depth_idx,
