@@ -15,6 +15,11 @@ rocprofv3 FETCH_SIZE / WRITE_SIZE summary profiles/hbm_traffic_c<workload>.md):
   backward  I*(4+128) + T*8 + P*(60 + 4C) + 2*X*C*4 + N*128
 (I intersects, T tiles, P pixels, X texels, C channels, N Gaussians).
 
+variants.bf16_batched_graph: views/s of an 8-view batch of the same scene
+(seeded cameras around the original, same resolution) with bfloat16 texels and
+SH coefficients, the whole batch captured as one graph and replayed; vs_plain
+is its views/s over the plain single-view iter/s.
+
 cpu_baseline: the FP32 C oracle (single thread, the reference's semantics) on a
 bounded sample of whole tiles of the same frame, forward + backward, scaled to
 the full frame by the sampled share of intersects.
@@ -45,6 +50,9 @@ def parse():
     ap.add_argument("--num-points", type=int, default=None)
     ap.add_argument("--cpu-baseline-seconds", type=float, default=15.0,
                     help="target CPU seconds for the oracle sample (0 disables)")
+    ap.add_argument("--batched-views", type=int, default=8,
+                    help="also time a batch of this many views with bfloat16 texels/SH, "
+                         "captured as one graph (0 disables)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the timed region, run one device-to-device copy of this many MiB "
                          "(a kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE)")
@@ -69,6 +77,38 @@ def measured_traffic(workload: int, kernel: str):
             if line.startswith(key):
                 return int(line.split(":", 1)[1]), os.path.relpath(path, ROOT)
     return None, None
+
+
+def batched_variant(scene, dev, num_views, steps, warmup):
+    """Views/s of a multi-view batch of the same scene (seeded cameras around the
+    original one, same resolution), bfloat16 texels and SH coefficients, the
+    whole batch (SH, projection, binning, raster forward, loss, raster
+    backward, finalize, SH adjoint per view, gradient sum) captured as one
+    graph and replayed."""
+    import torch
+
+    from gstex_cuda_amd import synthetic
+    from gstex_cuda_amd.pipeline import scene_to_multiview
+
+    views = synthetic.make_views(scene, num_views)
+    mv = scene_to_multiview(scene, views, dev, bf16=True)
+    mv.capture()
+    for _ in range(warmup):
+        mv.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        mv.replay()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ok = mv.capacity_ok()
+    ms = 1e3 * elapsed / steps
+    out = {"views": num_views, "views_per_s": round(num_views * steps / elapsed, 3),
+           "ms_per_batch": round(ms, 4), "dtype": "bf16 storage, f32 compute",
+           "graph": True, "capacity_ok": bool(ok)}
+    del mv
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(scene, step, target_seconds):
@@ -162,6 +202,12 @@ def main():
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": traffic, "traffic_source": traffic_src, "algorithmic_bytes": int(nbytes),
             "kernel_ms": {k: round(v, 4) for k, v in kern.items()}}
+    variants = {}
+    if args.batched_views > 0:
+        b = batched_variant(scene, dev, args.batched_views, max(args.steps // 4, 3),
+                            max(args.warmup // 2, 1))
+        b["vs_plain"] = round(b["views_per_s"] / value, 3)
+        variants["bf16_batched_graph"] = b
     cpu = None
     if args.cpu_baseline_seconds > 0:
         cpu = cpu_baseline(scene, step, args.cpu_baseline_seconds)
@@ -174,7 +220,7 @@ def main():
                                    f"FP32 plain path",
                        "num_gaussians": N, "resolution": f"{scene.img_w}x{scene.img_h}",
                        "texels": X, "channels": C, "intersects": I, "block_width": 16},
-            "roofline": roof, "cpu_baseline": cpu}
+            "roofline": roof, "cpu_baseline": cpu, "variants": variants}
     print(json.dumps(line), flush=True)
 
 

@@ -37,6 +37,7 @@ _SIGNATURES = {
     "gstex_get_tile_bin_edges": (I, [I, P, I, I, P, P]),
     "gstex_bin_gaussians_workspace_size": (Z, [I, I]),
     "gstex_bin_gaussians": (I, [I, I, P, P, P, P, I, I, I, P, P, P, Z, P]),
+    "gstex_bin_gaussians_capped": (I, [I, I, P, P, P, P, I, I, I, P, P, P, P, Z, P]),
     "gstex_texture_forward_workspace_size": (Z, [I]),
     "gstex_texture_forward": (I, [I, I, I, I, I, I, I, I,  # tiles, bw, img, texture_info
                                   P, P, I, P,  # dims, ids, num_ids, bins
@@ -56,6 +57,7 @@ _SIGNATURES = {
     "gstex_splat_records_size": (Z, [I]),
     "gstex_build_splat_records": (I, [I, P, P, F, P, P, P, P, P, P, P, P, P, F, F, F, F, P, P]),
     "gstex_set_raster_culling": (I, [I]),
+    "gstex_set_precise_edge_on": (I, [I]),
     "gstex_raster_wave_masks_size": (Z, [I, I, I]),
     "gstex_raster_forward": (I, [I, I, I, I, I, P, I, P, I, P, P, I, I, P, P, F, F, F, F, P,
                                  P, P, P, P, P, P, P, P, P,  # nine outputs
@@ -66,6 +68,12 @@ _SIGNATURES = {
                                   P, P, P, Z, P]),  # grad_records, v_texture, masks, bytes, stream
     "gstex_finalize_gradients": (I, [I, P, P, P, P, P, P, F, F, P, P, P, P, P, P, P, P, P]),
     "gstex_fill_f32": (I, [P, Z, F, P]),
+    "gstex_texture_edit_workspace_size": (Z, [I]),
+    "gstex_texture_edit": (I, [I, I, I, I, I, I, I, I, I,  # tiles, bw, img, texture_info, X
+                               P, P, P, P, P, P, I, P,  # dims, edits, ids, num_ids, bins
+                               P, P, P, I, F, P, P, P, P,  # opac, means, scales, N, g, quats, uv
+                               P, P, F, F, F, F, I, P,  # viewmat, c2w, intrinsics, settings, bg
+                               P, P, Z, P]),  # out, workspace, bytes, stream
     "gstex_texture_sample_forward": (I, [I, I, I, P, P, I, P, I, P, P]),
     "gstex_texture_sample_backward": (I, [I, I, I, P, P, I, P, I, P, P, P]),
 }

@@ -1,0 +1,108 @@
+"""Measured choice of raster launch options for a scene shape.
+
+Launch options are timed on the caller's own scene and the fastest
+combination is kept:
+
+* ``use_wave_masks`` -- the backward visits only entries the forward composited;
+* ``culling`` -- per-wave screen-box culling of list entries;
+* ``block_width`` -- tile side (only the caller's by default).
+
+The first two never change results (up to float summation order).  The tile
+side does, slightly, as in the reference: a pixel sees every Gaussian whose
++-3 sigma AABB meets its tile, and alpha >= 1/255 reaches past 3 sigma for
+opacities above 0.35, so pixels just outside an AABB are lit or not depending
+on the tiling.  Pass ``block_widths=(16, 8)`` to let the tuner choose it.
+
+A trial is the HIP-event duration of one whole step (SH, projection, binning,
+raster forward, loss gradient, raster backward, finalize) after a warm-up step.  Results are cached per
+(N, width, height, channels, settings, texel format) in memory and, when
+``cache_path`` is given, in a JSON file.
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import os
+from dataclasses import asdict, dataclass
+from typing import Dict, Iterable, Optional
+
+import torch
+
+from . import cuda as _C
+from .pipeline import Camera, RenderStep, StageTimer
+
+
+@dataclass(frozen=True)
+class RasterConfig:
+    block_width: int = 16
+    use_wave_masks: bool = True
+    culling: bool = True
+
+
+_CACHE: Dict[str, dict] = {}
+
+
+def _key(params, cam: Camera, settings: int) -> str:
+    tex = params["texture"]
+    return (f"n{params['means'].shape[0]}_w{cam.img_w}_h{cam.img_h}_c{tex.shape[1]}"
+            f"_s{settings}_{str(tex.dtype).split('.')[-1]}")
+
+
+def _time_config(params, cam, target, cfg: RasterConfig, settings, background, sh_degree,
+                 trials: int) -> float:
+    prev = _C.set_raster_culling(cfg.culling)
+    try:
+        step = RenderStep(params, cam, block_width=cfg.block_width, settings=settings,
+                          background=background, sh_degree=sh_degree,
+                          use_wave_masks=cfg.use_wave_masks)
+        step.step(target)
+        torch.cuda.synchronize()
+        timer = StageTimer()
+        for _ in range(trials):
+            timer.start("all")
+            step.step(target)
+            timer.stop("all")
+        torch.cuda.synchronize()
+        return timer.mean_ms("all")
+    finally:
+        _C.set_raster_culling(prev)
+
+
+def tune(params, cam: Camera, target: torch.Tensor, settings: int = 1 << 8,
+         background: Optional[torch.Tensor] = None, sh_degree: int = -1,
+         block_widths: Iterable[int] = (16,), trials: int = 3,
+         cache_path: Optional[str] = None) -> RasterConfig:
+    """Time every candidate configuration on this scene and return the fastest."""
+    key = _key(params, cam, settings)
+    if cache_path and os.path.exists(cache_path) and not _CACHE:
+        with open(cache_path) as fh:
+            _CACHE.update(json.load(fh))
+    if key in _CACHE:
+        return RasterConfig(**_CACHE[key]["best"])
+    results = []
+    for bw, masks, cull in itertools.product(block_widths, (True, False), (True, False)):
+        cfg = RasterConfig(bw, masks, cull)
+        ms = _time_config(params, cam, target, cfg, settings, background, sh_degree, trials)
+        results.append((ms, cfg))
+    results.sort(key=lambda r: r[0])
+    best = results[0][1]
+    _CACHE[key] = {"best": asdict(best),
+                   "timings_ms": {f"bw{c.block_width}_masks{int(c.use_wave_masks)}"
+                                  f"_cull{int(c.culling)}": round(ms, 4) for ms, c in results}}
+    if cache_path:
+        with open(cache_path, "w") as fh:
+            json.dump(_CACHE, fh, indent=1)
+    return best
+
+
+def tuned_step(params, cam: Camera, target: torch.Tensor, **kw) -> RenderStep:
+    """A RenderStep built with the tuned configuration (culling switched process-wide)."""
+    cfg = tune(params, cam, target, settings=kw.get("settings", 1 << 8),
+               background=kw.get("background"), sh_degree=kw.get("sh_degree", -1))
+    _C.set_raster_culling(cfg.culling)
+    return RenderStep(params, cam, block_width=cfg.block_width,
+                      use_wave_masks=cfg.use_wave_masks, **kw)
+
+
+def last_timings() -> Dict[str, dict]:
+    return dict(_CACHE)

@@ -488,8 +488,9 @@ __global__ __launch_bounds__(256) void k_gather_written(int n, const uint32_t* _
 // slots.  A Gaussian's tiles go row by row over its rectangle, capped at its
 // count.
 __global__ __launch_bounds__(256) void k_bin_emit(
-    int n, int m, const uint32_t* __restrict__ order, const uint32_t* __restrict__ offsets,
-    const uint32_t* __restrict__ total_written, const float* __restrict__ centers,
+    int n, int cap, const uint32_t* __restrict__ m_total, const uint32_t* __restrict__ order,
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ total_written,
+    const float* __restrict__ centers,
     const float* __restrict__ extents, int tiles_x, int tiles_y, int bw, const int* __restrict__ written,
     uint32_t* __restrict__ tiles_out, uint32_t* __restrict__ ids_out) {
     __shared__ uint32_t s_off[257];
@@ -498,8 +499,9 @@ __global__ __launch_bounds__(256) void k_bin_emit(
     const int tid = threadIdx.x;
     const int p0 = blockIdx.x * 256, p = p0 + tid;
     const int cnt = min(256, n - p0);
-    const uint32_t tw = *total_written;
-    const uint32_t zbase = tw <= (uint32_t)m ? (uint32_t)m - tw : 0u;
+    const uint32_t tw = *total_written, m = *m_total;
+    const uint32_t zbase = tw <= m ? m - tw : 0u;
+    const uint32_t limit = min(m, (uint32_t)cap);
     if (tid < cnt) {
         const uint32_t g = order[p];
         const int w = max(written[g], 0);
@@ -528,25 +530,44 @@ __global__ __launch_bounds__(256) void k_bin_emit(
         const int wd = s_wd[lo];
         const int ty = s_y0[lo] + (int)(li / (uint32_t)wd);
         const int tx = s_x0[lo] + (int)(li % (uint32_t)wd);
-        // dst < m holds when num_intersects = sum(num_tiles_hit); the bound keeps an
-        // inconsistent caller-supplied count from writing past the outputs
+        // entries past the capacity are dropped (the caller sees m > cap)
         const uint32_t dst = zbase + o0 + k;
-        if (dst < (uint32_t)m) {
+        if (dst < limit) {
             tiles_out[dst] = (uint32_t)(ty * tiles_x + tx);
             ids_out[dst] = s_g[lo];
         }
     }
 }
 
-__global__ __launch_bounds__(256) void k_zero_slots(int m, const uint32_t* __restrict__ total_written,
-                                                    uint32_t* __restrict__ tiles_out,
+// Total intersect count sum(max(counts, 0)) into *m_total (zeroed beforehand by
+// a fill kernel): per-lane partial sums, a wave reduction, one atomic per wave.
+__global__ __launch_bounds__(256) void k_count_total(int n, const int* __restrict__ counts,
+                                                     uint32_t* __restrict__ m_total) {
+    uint32_t acc = 0;
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x)
+        acc += (uint32_t)max(counts[g], 0);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(m_total, acc);
+}
+
+// Slots [0, cap) of the (tile, id) list before the tile sort: the zero slots
+// (tile 0, id 0) first, then the emitted entries (k_bin_emit), and past the
+// intersect total m (if m < cap) sentinel entries with tile = num_tiles that
+// sort last and belong to no bin.
+__global__ __launch_bounds__(256) void k_zero_slots(int cap, const uint32_t* __restrict__ m_total,
+                                                    const uint32_t* __restrict__ total_written,
+                                                    int num_tiles, uint32_t* __restrict__ tiles_out,
                                                     uint32_t* __restrict__ ids_out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    const uint32_t tw = *total_written;
-    const uint32_t z = tw <= (uint32_t)m ? (uint32_t)m - tw : 0u;
+    if (i >= cap) return;
+    const uint32_t m = *m_total, tw = *total_written;
+    const uint32_t z = tw <= m ? m - tw : 0u;
     if ((uint32_t)i < z) {
         tiles_out[i] = 0u;
+        ids_out[i] = 0u;
+    } else if ((uint32_t)i >= m) {
+        tiles_out[i] = (uint32_t)num_tiles;
         ids_out[i] = 0u;
     }
 }
@@ -569,7 +590,7 @@ static BinLayout bin_layout(int n, int m) {
         up(4 * (size_t)n) * 2,  // depth keys in/out
         up(4 * (size_t)n) * 2,  // order in/out
         up(4 * (size_t)n),      // written
-        up(4 * (size_t)n + 4) + up(4 * scan_partials_count(n)),  // offsets (+ total), partials
+        up(4 * (size_t)n + 8) + up(4 * scan_partials_count(n)),  // offsets (+ totals), partials
         up(4 * (size_t)m) * 2,  // tile keys in/out
         up(4 * (size_t)m),      // ids in (pre-sort)
         up(std::max(radix_workspace_bytes<uint32_t>(n), radix_workspace_bytes<uint32_t>(m))),
@@ -586,14 +607,27 @@ static BinLayout bin_layout(int n, int m) {
 
 size_t bin_gaussians_workspace_bytes(int n, int m) { return bin_layout(n, m).total; }
 
-hipError_t bin_gaussians(int n, int m, const float* centers, const float* extents,
+hipError_t bin_gaussians(int n, int cap, const float* centers, const float* extents,
                          const float* depths, const int* counts, int tiles_x, int tiles_y, int bw,
-                         int* ids_sorted, int* tile_bins, void* ws, size_t ws_bytes,
-                         hipStream_t s) {
+                         int* ids_sorted, int* tile_bins, unsigned* m_out, void* ws,
+                         size_t ws_bytes, hipStream_t s) {
     const int num_tiles = tiles_x * tiles_y;
     hipError_t e = fill<int>(tile_bins, (size_t)num_tiles * 2, 0, s);
-    if (e != hipSuccess || m <= 0 || n <= 0) return e;
-    BinLayout L = bin_layout(n, m);
+    if (e != hipSuccess || n <= 0) return e;
+    if (cap <= 0) {
+        if (m_out) {
+            // the total is still reported, so the caller can size a capacity
+            BinLayout L = bin_layout(n, 1);
+            if (ws_bytes < L.total) return hipErrorInvalidValue;
+            uint32_t* mt = (uint32_t*)((char*)ws + L.off[3]) + n + 1;
+            if ((e = fill<uint32_t>(mt, 1, 0u, s)) != hipSuccess) return e;
+            k_count_total<<<std::min((n + 255) / 256, 1024), 256, 0, s>>>(n, counts, mt);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+            return hipMemcpyAsync(m_out, mt, sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+        }
+        return hipSuccess;
+    }
+    BinLayout L = bin_layout(n, cap);
     if (ws_bytes < L.total) return hipErrorInvalidValue;
     char* b = (char*)ws;
     auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
@@ -603,33 +637,39 @@ hipError_t bin_gaussians(int n, int m, const float* centers, const float* extent
     uint32_t* ord2 = (uint32_t*)(b + L.off[1] + up(4 * (size_t)n));
     int* written = (int*)(b + L.off[2]);
     uint32_t* offs = (uint32_t*)(b + L.off[3]);
-    uint32_t* total = offs + n;
+    uint32_t* total = offs + n;       // entries written by Gaussians
+    uint32_t* m_total = offs + n + 1; // intersect total (written + zero slots)
     uint32_t* tkey = (uint32_t*)(b + L.off[4]);
-    uint32_t* tkey2 = (uint32_t*)(b + L.off[4] + up(4 * (size_t)m));
+    uint32_t* tkey2 = (uint32_t*)(b + L.off[4] + up(4 * (size_t)cap));
     uint32_t* tid_in = (uint32_t*)(b + L.off[5]);
     void* rws = (void*)(b + L.off[6]);
     const size_t rws_bytes = up(std::max(radix_workspace_bytes<uint32_t>(n),
-                                         radix_workspace_bytes<uint32_t>(m)));
-    const int gn = (n + 255) / 256, gm = (m + 255) / 256;
+                                         radix_workspace_bytes<uint32_t>(cap)));
+    const int gn = (n + 255) / 256, gm = (cap + 255) / 256;
 
+    if ((e = fill<uint32_t>(m_total, 1, 0u, s)) != hipSuccess) return e;
+    k_count_total<<<std::min(gn, 1024), 256, 0, s>>>(n, counts, m_total);
     k_bin_prepare<<<gn, 256, 0, s>>>(n, centers, extents, depths, counts, tiles_x, tiles_y, bw,
                                      dkey, ord, written);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     e = radix_sort_pairs<uint32_t>(n, dkey, ord, dkey2, ord2, 32, rws, rws_bytes, s);
     if (e != hipSuccess) return e;
     k_gather_written<<<gn, 256, 0, s>>>(n, ord2, written, offs);
-    if ((e = scan_exclusive(offs, n, total, (uint32_t*)(b + L.off[3] + up(4 * (size_t)n + 4)),
+    if ((e = scan_exclusive(offs, n, total, (uint32_t*)(b + L.off[3] + up(4 * (size_t)n + 8)),
                             s)) != hipSuccess)
         return e;
-    k_zero_slots<<<gm, 256, 0, s>>>(m, total, tkey, tid_in);
-    k_bin_emit<<<gn, 256, 0, s>>>(n, m, ord2, offs, total, centers, extents, tiles_x, tiles_y, bw,
-                                  written, tkey, tid_in);
+    k_zero_slots<<<gm, 256, 0, s>>>(cap, m_total, total, num_tiles, tkey, tid_in);
+    k_bin_emit<<<gn, 256, 0, s>>>(n, cap, m_total, ord2, offs, total, centers, extents, tiles_x,
+                                  tiles_y, bw, written, tkey, tid_in);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    e = radix_sort_pairs<uint32_t>(m, tkey, tid_in, tkey2, (uint32_t*)ids_sorted,
-                                   bits_for(num_tiles), rws, rws_bytes, s);
+    // the sentinel tile (num_tiles) needs one more key value
+    e = radix_sort_pairs<uint32_t>(cap, tkey, tid_in, tkey2, (uint32_t*)ids_sorted,
+                                   bits_for(num_tiles + 1), rws, rws_bytes, s);
     if (e != hipSuccess) return e;
-    k_bin_edges32<<<gm, 256, 0, s>>>(m, tkey2, num_tiles, tile_bins);
-    return hipGetLastError();
+    k_bin_edges32<<<gm, 256, 0, s>>>(cap, tkey2, num_tiles, tile_bins);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (m_out) return hipMemcpyAsync(m_out, m_total, sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+    return hipSuccess;
 }
 
 }  // namespace gstex

@@ -230,9 +230,33 @@ int gstex_bin_gaussians(int num_points, int num_intersects, const float* centers
     }
     return check_hip(gstex::bin_gaussians(num_points, num_intersects, centers, extents, depths,
                                           num_tiles_hit, tiles_x, tiles_y, block_width,
-                                          gaussian_ids_sorted, tile_bins, workspace,
+                                          gaussian_ids_sorted, tile_bins, nullptr, workspace,
                                           workspace_bytes, as_stream(stream)),
                      "bin_gaussians");
+}
+
+int gstex_bin_gaussians_capped(int num_points, int capacity, const float* centers,
+                               const float* extents, const float* depths,
+                               const int32_t* num_tiles_hit, int tiles_x, int tiles_y,
+                               int block_width, int32_t* gaussian_ids_sorted, int32_t* tile_bins,
+                               uint32_t* num_intersects_out, void* workspace,
+                               size_t workspace_bytes, void* stream) {
+    REQUIRE(num_points >= 0 && capacity >= 0, "negative sizes");
+    REQUIRE(block_width > 0 && tiles_x >= 0 && tiles_y >= 0, "invalid tile bounds");
+    if (tiles_x * tiles_y == 0) return GSTEX_OK;
+    REQUIRE(tile_bins && num_intersects_out, "null pointer argument");
+    if (num_points > 0) {
+        REQUIRE(centers && extents && depths && num_tiles_hit && workspace &&
+                    (capacity == 0 || gaussian_ids_sorted),
+                "null pointer argument");
+        const size_t need = gstex::bin_gaussians_workspace_bytes(num_points, capacity > 0 ? capacity : 1);
+        REQUIRE(workspace_bytes >= need, "workspace too small: %zu < %zu", workspace_bytes, need);
+    }
+    return check_hip(gstex::bin_gaussians(num_points, capacity, centers, extents, depths,
+                                          num_tiles_hit, tiles_x, tiles_y, block_width,
+                                          gaussian_ids_sorted, tile_bins, num_intersects_out,
+                                          workspace, workspace_bytes, as_stream(stream)),
+                     "bin_gaussians_capped");
 }
 
 size_t gstex_texture_forward_workspace_size(int num_points) {
@@ -335,6 +359,56 @@ int gstex_texture_backward(int img_height, int img_width, int block_width, int n
 }
 
 int gstex_set_raster_culling(int enabled) { return gstex::set_raster_culling(enabled != 0) ? 1 : 0; }
+
+int gstex_set_precise_edge_on(int enabled) {
+    return gstex::set_raster_precise_edge(enabled != 0) ? 1 : 0;
+}
+
+size_t gstex_texture_edit_workspace_size(int num_points) {
+    return gstex::texture_edit_workspace_bytes(num_points > 0 ? num_points : 0);
+}
+
+int gstex_texture_edit(int tiles_x, int tiles_y, int block_width, int img_width, int img_height,
+                       int num_charts, int num_probs, int channels, int texture_total_size,
+                       const int32_t* texture_dims, const float* updated_img,
+                       const float* updated_alpha, const float* depth_lower,
+                       const float* depth_upper, const int32_t* gaussian_ids_sorted, int num_ids,
+                       const int32_t* tile_bins, const float* opacities, const float* means,
+                       const float* scales, int num_points, float glob_scale, const float* quats,
+                       const float* uv0, const float* umap, const float* vmap,
+                       const float* viewmat, const float* c2w, float fx, float fy, float cx,
+                       float cy, int settings, const float* background, float* updated_texture,
+                       void* workspace, size_t workspace_bytes, void* stream) {
+    (void)num_charts;
+    (void)background;
+    REQUIRE(channels >= 5,
+            "texture_edit writes 5 channels (r*a, g*a, b*a, a, weight): texture_info[2] must be "
+            ">= 5, got %d",
+            channels);
+    REQUIRE(texture_total_size > 0, "texture_total_size must be positive, got %d",
+            texture_total_size);
+    int rc = check_raster_common(tiles_x, tiles_y, block_width, img_width, img_height, num_probs,
+                                 channels, num_points, num_ids, texture_total_size, 0);
+    if (rc) return rc;
+    REQUIRE(updated_texture && updated_img && updated_alpha && depth_lower && depth_upper &&
+                tile_bins && viewmat && c2w,
+            "null pointer argument");
+    REQUIRE(num_ids == 0 || (num_points > 0 && gaussian_ids_sorted), "null pointer argument");
+    if (num_points > 0)
+        REQUIRE(texture_dims && opacities && means && scales && quats && uv0 && umap && vmap,
+                "null pointer argument");
+    REQUIRE(workspace_bytes >= gstex::texture_edit_workspace_bytes(num_points),
+            "workspace too small: %zu < %zu", workspace_bytes,
+            gstex::texture_edit_workspace_bytes(num_points));
+    gstex::RasterCall c{img_width, img_height, block_width, channels, settings, num_points, num_ids,
+                        texture_total_size, glob_scale, fx, fy, cx, cy, texture_dims,
+                        gaussian_ids_sorted, tile_bins, nullptr, opacities, means, scales, quats,
+                        uv0, umap, vmap, nullptr, viewmat, c2w, nullptr};
+    return check_hip(gstex::texture_edit(c, updated_img, updated_alpha, depth_lower, depth_upper,
+                                         updated_texture, workspace, workspace_bytes,
+                                         as_stream(stream)),
+                     "texture_edit");
+}
 
 size_t gstex_splat_records_size(int num_points) {
     return gstex::splat_records_bytes(num_points);

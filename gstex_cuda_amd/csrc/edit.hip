@@ -1,0 +1,120 @@
+// Texture edit (texture_edit.cu:11-227 semantics): splat a per-pixel edit
+// canvas back onto the Gaussians' texel charts.
+//
+// For every pair the forward would composite (same alpha, skip and stop
+// rules; blur from settings bit 0 as in the reference edit kernel) whose view
+// depth lies in the pixel's [depth_lower, depth_upper], the bilinear weights
+// of the pair's texel quad (always bilinear, replicate padding) times
+// (r*a, g*a, b*a, a, 1) are added to channels 0..4 of the output, a being the
+// pixel's edit alpha.  The walk is the forward's: quadrant waves, 64-entry
+// windows with screen-box culling, SMEM records; lanes on the same texel quad
+// are merged with DPP before the float atomics.
+#include "raster_common.hpp"
+
+namespace gstex {
+
+struct EditIn {
+    const float* upd_img;      // (H, W, 3)
+    const float* upd_alpha;    // (H, W)
+    const float* depth_lower;  // (H, W)
+    const float* depth_upper;  // (H, W)
+    float* out;                // (X, C), C >= 5
+};
+
+template <bool BLUR>
+__global__ __launch_bounds__(256) void k_texture_edit(RasterArgs a, EditIn ed) {
+    const int tile = blockIdx.x;
+    const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
+    const int lane = threadIdx.x & (kWave - 1);
+    int lx, ly;
+    bool in_tile;
+    lane_pixel(a.bw, threadIdx.x, lx, ly, in_tile);
+    const int j = tx * a.bw + lx, i = ty * a.bw + ly;
+    const bool inside = in_tile && i < a.img_h && j < a.img_w;
+    const PixelRay pr = pixel_ray(a.c2w, a.viewmat, a.fx, a.fy, a.cx, a.cy, j, i);
+    const int C = a.channels;
+    float val[5] = {0.f, 0.f, 0.f, 0.f, 1.f};
+    float lo = 0.f, hi = -1.f;
+    if (inside) {
+        const int pid = i * a.img_w + j;
+        const float al = ed.upd_alpha[pid];
+        val[0] = ed.upd_img[3 * pid] * al;
+        val[1] = ed.upd_img[3 * pid + 1] * al;
+        val[2] = ed.upd_img[3 * pid + 2] * al;
+        val[3] = al;
+        lo = ed.depth_lower[pid];
+        hi = ed.depth_upper[pid];
+    }
+    int r0, r1;
+    tile_range(a, tile, r0, r1);
+    bool done = !inside;
+    float T = 1.f;
+    const WaveRect rect = wave_rect(inside, j, i);
+    constexpr float kNearDoneT = 1.0041e-4f;  // see k_raster_forward
+    bool cull = a.boxes != nullptr;
+    for (int base = r0 & ~63; base < r1; base += 64) {
+        if (!__any(!done)) break;
+        if (cull && __any(!done && T <= kNearDoneT)) cull = false;
+        const Window w = window_candidates(a, base, r0, r1 - 1, rect, lane, cull);
+        uint64_t cand = w.cand;
+        while (cand != 0) {
+            if (!__any(!done)) break;
+            const int bit = __ffsll((unsigned long long)cand) - 1;
+            cand &= cand - 1;
+            const int g = __builtin_amdgcn_readlane(w.gid, bit);
+            const Splat s = splat_record(a, g);
+            bool hit = false;
+            float wt[4] = {0.f, 0.f, 0.f, 0.f};
+            int row[4] = {0, 0, 0, 0};
+            if (!done) {
+                const Pair p = pair_eval(s, pr, BLUR, a.precise_edge);
+                const float next_T = T * (1.f - p.alpha);
+                if (next_T <= 1e-4f) done = true;
+                if (!(p.skip || done)) {
+                    const float t_view = p.t * pr.view_depth;
+                    if (t_view >= lo && t_view <= hi) {
+                        hit = true;
+                        float u, v, fu, fv;
+                        splat_uv(s, p.delta, u, v);
+                        chart_taps(s, u, v, true, a.num_texels, wt, row, fu, fv);
+                    }
+                    T = next_T;
+                }
+            }
+            if (!__any(hit)) continue;
+            const QuadMerge qm = quad_merge_setup(hit ? row[0] : -1, lane);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                float q[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) q[t] = quad_merge(wt[t] * val[k], qm);
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    if (q[t] != 0.f) unsafeAtomicAdd(ed.out + (size_t)row[t] * C + k, q[t]);
+            }
+        }
+    }
+}
+
+size_t texture_edit_workspace_bytes(int n) { return splat_records_bytes(n) + 256; }
+
+hipError_t texture_edit(const RasterCall& c, const float* upd_img, const float* upd_alpha,
+                        const float* depth_lower, const float* depth_upper, float* out,
+                        void* ws, size_t ws_bytes, hipStream_t s) {
+    if (ws_bytes < texture_edit_workspace_bytes(c.num_points)) return hipErrorInvalidValue;
+    hipError_t e = fill_f32(out, (size_t)c.num_texels * (size_t)c.channels, 0.f, s);
+    if (e != hipSuccess) return e;
+    float* records = (float*)ws;
+    if ((e = build_raster_records(c, records, s)) != hipSuccess) return e;
+    const RasterArgs a = make_raster_args(c, records);
+    const int tiles = a.tiles_x * a.tiles_y;
+    if (tiles <= 0) return hipSuccess;
+    const EditIn ed{upd_img, upd_alpha, depth_lower, depth_upper, out};
+    if (c.settings & 1)
+        k_texture_edit<true><<<tiles, raster_block_threads(a.bw), 0, s>>>(a, ed);
+    else
+        k_texture_edit<false><<<tiles, raster_block_threads(a.bw), 0, s>>>(a, ed);
+    return hipGetLastError();
+}
+
+}  // namespace gstex

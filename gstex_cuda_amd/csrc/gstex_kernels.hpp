@@ -59,10 +59,14 @@ hipError_t sort_isects(int n, const int64_t* keys_in, const int* ids_in, int64_t
 hipError_t tile_bin_edges(int m, const int64_t* keys_sorted, int num_tiles, int* bins,
                           hipStream_t s);
 size_t bin_gaussians_workspace_bytes(int n, int m);
-hipError_t bin_gaussians(int n, int m, const float* centers, const float* extents,
+// Depth-first binning into a list of capacity cap >= the intersect total m =
+// sum(max(counts, 0)), which is computed on the device (and copied to *m_out
+// when given).  Slots past m hold sentinel entries outside every bin; entries
+// past cap are dropped (m > cap on return tells the caller to grow the list).
+hipError_t bin_gaussians(int n, int cap, const float* centers, const float* extents,
                          const float* depths, const int* counts, int tiles_x, int tiles_y, int bw,
-                         int* ids_sorted, int* tile_bins, void* ws, size_t ws_bytes,
-                         hipStream_t s);
+                         int* ids_sorted, int* tile_bins, unsigned* m_out, void* ws,
+                         size_t ws_bytes, hipStream_t s);
 
 // raster
 struct RasterCall {
@@ -103,6 +107,9 @@ inline size_t raster_wave_mask_words(int num_ids, int num_tiles) {
 // Process-wide switch for the per-wave culling by screen boxes (default on).
 bool raster_culling_enabled();
 bool set_raster_culling(bool on);
+// Process-wide switch: float64 ray/plane hits for near-edge-on splats (default off).
+bool raster_precise_edge_enabled();
+bool set_raster_precise_edge(bool on);
 size_t raster_forward_workspace_bytes(int n);
 size_t raster_backward_workspace_bytes(int n);
 hipError_t raster_forward(const RasterCall& c, float* out_img, float* out_depth, float* out_reg,
@@ -145,6 +152,12 @@ __host__ __device__ inline const uint2* record_boxes(const float* records, int n
     return reinterpret_cast<const uint2*>(records + (size_t)n * (kRecordBytes / sizeof(float)));
 }
 constexpr size_t kGradRecordBytes = 128;
+
+// texture edit (texture_edit.cu semantics; out is X x C with C >= 5, zeroed here)
+size_t texture_edit_workspace_bytes(int n);
+hipError_t texture_edit(const RasterCall& c, const float* upd_img, const float* upd_alpha,
+                        const float* depth_lower, const float* depth_upper, float* out,
+                        void* ws, size_t ws_bytes, hipStream_t s);
 
 // sampling
 hipError_t sample_forward(int q, int C, int num_texels, const int* dims, const float* uvs,

@@ -227,7 +227,10 @@ __global__ __launch_bounds__(256) void k_build_records(
     r[R_OPAC] = opacities[i];
     r[R_MEAN + 0] = m.x; r[R_MEAN + 1] = m.y; r[R_MEAN + 2] = m.z;
     r[R_PIX_X] = pu;
-    r[R_RGB + 0] = colors[3 * i]; r[R_RGB + 1] = colors[3 * i + 1]; r[R_RGB + 2] = colors[3 * i + 2];
+    // colours may be absent (texture_edit does not read them)
+    r[R_RGB + 0] = colors ? colors[3 * i] : 0.f;
+    r[R_RGB + 1] = colors ? colors[3 * i + 1] : 0.f;
+    r[R_RGB + 2] = colors ? colors[3 * i + 2] : 0.f;
     r[R_PIX_Y] = pvv;
     r[R_UV0 + 0] = uv0[2 * i]; r[R_UV0 + 1] = uv0[2 * i + 1];
     r[R_UMAP + 0] = umap[3 * i]; r[R_UMAP + 1] = umap[3 * i + 1]; r[R_UMAP + 2] = umap[3 * i + 2];

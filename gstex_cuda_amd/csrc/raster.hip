@@ -74,7 +74,7 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
             const Splat s = splat_record(a, g);
             bool comp = false;
             if (!done) {
-                const Pair p = pair_eval(s, pr, use_blur);
+                const Pair p = pair_eval(s, pr, use_blur, GENERAL && a.precise_edge);
                 const float next_T = T * (1.f - p.alpha);
                 if (next_T <= 1e-4f) done = true;
                 comp = !(p.skip || done);
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
 template <int CM>
 static hipError_t launch_fwd_cm(const RasterArgs& a, const FwdOut& o, hipStream_t s) {
     const int tiles = a.tiles_x * a.tiles_y;
-    const bool general = a.settings & (kSetBlur | kSetNdc);
+    const bool general = (a.settings & (kSetBlur | kSetNdc)) || a.precise_edge;
     const dim3 grid(tiles), block(raster_block_threads(a.bw));
     if (a.texture_bf16) {
         if (general) k_raster_forward<CM, true, true><<<grid, block, 0, s>>>(a, o);
@@ -182,6 +182,14 @@ bool raster_culling_enabled() { return g_raster_culling; }
 bool set_raster_culling(bool on) {
     const bool prev = g_raster_culling;
     g_raster_culling = on;
+    return prev;
+}
+
+static bool g_precise_edge = false;
+bool raster_precise_edge_enabled() { return g_precise_edge; }
+bool set_raster_precise_edge(bool on) {
+    const bool prev = g_precise_edge;
+    g_precise_edge = on;
     return prev;
 }
 

@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
         bool valid = inside && e <= bfin;
         Pair p;
         if (valid) {
-            p = pair_eval(s, pr, use_blur);
+            p = pair_eval(s, pr, use_blur, GENERAL && a.precise_edge);
             valid = !p.skip;
         }
         const uint64_t vmask = __ballot(valid);
@@ -256,20 +256,8 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
         }
         if constexpr (CM <= 4) {
             if (tex_lds) {
-                // Lanes that sample the same texel quad (equal first tap row)
-                // are merged pairwise before the LDS adds, so the adds of an
-                // 8x8 pixel block land on few distinct lanes: horizontal pairs
-                // (lane bit 0), vertical pairs (bit 3), then bit 1.  On a match
-                // the lower lane takes the sum and the upper lane drops out.
-                const int key = valid ? row[0] : -1;
-                // per round: keep = 0 for the upper lane of a matched pair,
-                // take = 1 for the lower one; q <- keep * q + take * partner(q)
-                const bool m1 = dpp_i<kDppQuadXor1>(key) == key;
-                const bool m8 = dpp_i<kDppRowRor8>(key) == key;
-                const bool m2 = dpp_i<kDppQuadXor2>(key) == key;
-                const float keep1 = m1 && (lane & 1) ? 0.f : 1.f, take1 = m1 && !(lane & 1);
-                const float keep8 = m8 && (lane & 8) ? 0.f : 1.f, take8 = m8 && !(lane & 8);
-                const float keep2 = m2 && (lane & 2) ? 0.f : 1.f, take2 = m2 && !(lane & 2);
+                // merge lanes on the same texel quad before the LDS adds
+                const QuadMerge qm = quad_merge_setup(valid ? row[0] : -1, lane);
                 int lofs[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) lofs[k] = (row[k] - s.start) * C;
@@ -279,12 +267,7 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
                     const float vv_c = vis * vtex[c];
                     float q[4];
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        q[k] = wt[k] * vv_c;
-                        q[k] = fmaf(dpp<kDppQuadXor1>(q[k]), take1, q[k] * keep1);
-                        q[k] = fmaf(dpp<kDppRowRor8>(q[k]), take8, q[k] * keep8);
-                        q[k] = fmaf(dpp<kDppQuadXor2>(q[k]), take2, q[k] * keep2);
-                    }
+                    for (int k = 0; k < 4; ++k) q[k] = quad_merge(wt[k] * vv_c, qm);
 #pragma unroll
                     for (int k = 0; k < 4; ++k)
                         if (q[k] != 0.f) atomicAdd(acc + lofs[k] + c, q[k]);
@@ -423,7 +406,7 @@ __global__ __launch_bounds__(256) void k_grad_finalize(
 template <int CM>
 static hipError_t launch_bwd_cm(const RasterArgs& a, const BwdIn& b, hipStream_t s) {
     const int tiles = a.tiles_x * a.tiles_y;
-    const bool general = a.settings & (kSetBlur | kSetNdc);
+    const bool general = (a.settings & (kSetBlur | kSetNdc)) || a.precise_edge;
     const dim3 grid(tiles), block(raster_block_threads(a.bw));
     if (a.texture_bf16) {
         if (general) k_raster_backward<CM, true, true><<<grid, block, 0, s>>>(a, b);

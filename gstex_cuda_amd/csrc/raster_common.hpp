@@ -23,7 +23,11 @@ struct RasterArgs {
     int mask_plane;
     const uint2* boxes;    // per-Gaussian screen boxes (culling), or nullptr
     bool texture_bf16;
+    bool precise_edge;     // float64 plane hits for near-edge-on splats
 };
+
+// |n . ray| below which a splat counts as near-edge-on for precise_edge.
+constexpr float kEdgeOnDen = 0.05f;
 
 // Texel value idx of a float32 (TB = false) or bfloat16 (TB = true) texture;
 // bfloat16 widens exactly to float32.
@@ -151,19 +155,39 @@ struct Pair {
     bool blur_sel, skip;
 };
 
-// Ray/splat intersection and alpha for one (pixel, Gaussian) pair.
-__device__ __forceinline__ Pair pair_eval(const Splat& s, const PixelRay& pr, bool use_blur) {
+// Ray/splat intersection and alpha for one (pixel, Gaussian) pair.  With
+// precise_edge, a near-edge-on splat (|n . ray| < 0.05) has its plane hit
+// (t, delta, l1, l2) solved in float64: there t = n.(m - o) / n.r and
+// delta = o + t r - m cancel badly in float32.
+__device__ __forceinline__ Pair pair_eval(const Splat& s, const PixelRay& pr, bool use_blur,
+                                          bool precise_edge = false) {
     Pair p;
     float den = dot3(s.a3, pr.r);
     const float eps = 1e-6f;
     if (0.f <= den && den < eps) den = eps;
     else if (-eps < den && den <= 0.f) den = -eps;
     p.den = den;
-    p.t = dot3(s.a3, sub3(s.m, pr.o)) / den;
-    const V3 pos = v3(pr.o.x + p.t * pr.r.x, pr.o.y + p.t * pr.r.y, pr.o.z + p.t * pr.r.z);
-    p.delta = sub3(pos, s.m);
-    p.l1 = dot3(p.delta, s.a1);
-    p.l2 = dot3(p.delta, s.a2);
+    if (precise_edge && fabsf(den) < kEdgeOnDen) {
+        const double ax = s.a3.x, ay = s.a3.y, az = s.a3.z;
+        const double rx = pr.r.x, ry = pr.r.y, rz = pr.r.z;
+        double dd = ax * rx + ay * ry + az * rz;
+        if (0.0 <= dd && dd < 1e-6) dd = 1e-6;
+        else if (-1e-6 < dd && dd <= 0.0) dd = -1e-6;
+        const double mx = (double)s.m.x - pr.o.x, my = (double)s.m.y - pr.o.y,
+                     mz = (double)s.m.z - pr.o.z;
+        const double t = (ax * mx + ay * my + az * mz) / dd;
+        const double dx = t * rx - mx, dy = t * ry - my, dz = t * rz - mz;
+        p.t = (float)t;
+        p.delta = v3((float)dx, (float)dy, (float)dz);
+        p.l1 = (float)(dx * s.a1.x + dy * s.a1.y + dz * s.a1.z);
+        p.l2 = (float)(dx * s.a2.x + dy * s.a2.y + dz * s.a2.z);
+    } else {
+        p.t = dot3(s.a3, sub3(s.m, pr.o)) / den;
+        const V3 pos = v3(pr.o.x + p.t * pr.r.x, pr.o.y + p.t * pr.r.y, pr.o.z + p.t * pr.r.z);
+        p.delta = sub3(pos, s.m);
+        p.l1 = dot3(p.delta, s.a1);
+        p.l2 = dot3(p.delta, s.a2);
+    }
     const float sigma = 0.5f * (s.is1 * s.is1 * p.l1 * p.l1 + s.is2 * s.is2 * p.l2 * p.l2);
     p.G = __expf(-sigma);
     p.Gb = 0.f;
@@ -259,6 +283,37 @@ __device__ __forceinline__ int wave_max(int v) {
     return max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 32));
 }
 
+// Pairwise merge of lanes that sample the same texel quad (equal key, the
+// first tap row; -1 for lanes with nothing to add): over lane bits 0, 3 and 1
+// (horizontal, vertical, horizontal neighbours in an 8x8 quadrant), on a match
+// the lower lane takes the pair's sum and the upper lane drops to zero, so the
+// atomics of a pixel block land on few distinct lanes.  Per round
+// q <- keep * q + take * partner(q).  Call with all lanes active.
+struct QuadMerge {
+    float keep1, take1, keep8, take8, keep2, take2;
+};
+
+__device__ __forceinline__ QuadMerge quad_merge_setup(int key, int lane) {
+    const bool m1 = dpp_i<kDppQuadXor1>(key) == key;
+    const bool m8 = dpp_i<kDppRowRor8>(key) == key;
+    const bool m2 = dpp_i<kDppQuadXor2>(key) == key;
+    QuadMerge m;
+    m.keep1 = m1 && (lane & 1) ? 0.f : 1.f;
+    m.take1 = m1 && !(lane & 1) ? 1.f : 0.f;
+    m.keep8 = m8 && (lane & 8) ? 0.f : 1.f;
+    m.take8 = m8 && !(lane & 8) ? 1.f : 0.f;
+    m.keep2 = m2 && (lane & 2) ? 0.f : 1.f;
+    m.take2 = m2 && !(lane & 2) ? 1.f : 0.f;
+    return m;
+}
+
+__device__ __forceinline__ float quad_merge(float q, const QuadMerge& m) {
+    q = fmaf(dpp<kDppQuadXor1>(q), m.take1, q * m.keep1);
+    q = fmaf(dpp<kDppRowRor8>(q), m.take8, q * m.keep8);
+    q = fmaf(dpp<kDppQuadXor2>(q), m.take2, q * m.keep2);
+    return q;
+}
+
 // Index of this wave within the workgroup, as a wave-uniform (SGPR) value:
 // divergence analysis treats any function of threadIdx.x as per-lane, which
 // would turn loads addressed by it into vector loads.
@@ -277,9 +332,25 @@ __device__ __forceinline__ int splat_id(const RasterArgs& a, int e) {
     return min(max(g, 0), a.num_points - 1);
 }
 
-// Record of Gaussian g (wave-uniform) as scalar loads.
+// Ties values to this program point in SGPRs (an empty asm with "s" operands),
+// so the scalar loads producing them are issued together here instead of
+// being sunk, one dependent batch at a time, into the branches that use them.
+__device__ __forceinline__ void pin_s(float a, float b, float c, float d) {
+    asm volatile("" ::"s"(a), "s"(b), "s"(c), "s"(d));
+}
+
+// Record of Gaussian g (wave-uniform) as scalar loads, all in flight at once.
 __device__ __forceinline__ Splat splat_record(const RasterArgs& a, int g) {
-    return load_splat(as_const_space(a.records) + (size_t)g * kRecordFloats);
+    const Splat s = load_splat(as_const_space(a.records) + (size_t)g * kRecordFloats);
+    pin_s(s.a1.x, s.a1.y, s.a1.z, s.is1);
+    pin_s(s.a2.x, s.a2.y, s.a2.z, s.is2);
+    pin_s(s.a3.x, s.a3.y, s.a3.z, s.opac);
+    pin_s(s.m.x, s.m.y, s.m.z, s.rgb.x);
+    pin_s(s.rgb.y, s.rgb.z, s.u0, s.v0);
+    pin_s(s.umap.x, s.umap.y, s.umap.z, s.vmap.x);
+    pin_s(s.vmap.y, s.vmap.z, __int_as_float(s.h), __int_as_float(s.w));
+    pin_s(__int_as_float(s.start), s.pix_x, s.pix_y, 0.f);
+    return s;
 }
 
 // Pixel-index rectangle [x0, x1] x [y0, y1] covered by this wave's in-image
@@ -301,7 +372,7 @@ __device__ __forceinline__ WaveRect wave_rect(bool inside, int j, int i) {
 __device__ __forceinline__ bool box_meets(uint2 bx, const WaveRect& r) {
     const int x0 = (int)(short)(bx.x & 0xffff), x1 = (int)(short)(bx.x >> 16);
     const int y0 = (int)(short)(bx.y & 0xffff), y1 = (int)(short)(bx.y >> 16);
-    return x1 >= r.x0 && x0 <= r.x1 && y1 >= r.y0 && y0 <= r.y1;
+    return (x1 >= r.x0) & (x0 <= r.x1) & (y1 >= r.y0) & (y0 <= r.y1);
 }
 
 // One 64-entry window of a tile list (entries base .. base + 63, base a
@@ -323,7 +394,10 @@ __device__ __forceinline__ Window window_candidates(const RasterArgs& a, int bas
     w.gid = 0;
     if (in_range) {
         w.gid = min(max(a.ids_sorted[e], 0), a.num_points - 1);
-        if (cull) cand = box_meets(a.boxes[w.gid], rect);
+        if (cull) {
+            const uint2 bx = a.boxes[w.gid];  // one 8-byte load
+            cand = box_meets(bx, rect);
+        }
     }
     w.in_range = __ballot(in_range);
     w.cand = __ballot(cand);
@@ -354,6 +428,7 @@ inline RasterArgs make_raster_args(const RasterCall& c, const float* records) {
     a.records = records;
     a.texture = c.texture;
     a.texture_bf16 = c.texture_bf16 != 0;
+    a.precise_edge = raster_precise_edge_enabled();
     a.wave_masks = c.wave_masks;
     a.mask_plane = (int)raster_wave_mask_words(c.num_ids, a.tiles_x * a.tiles_y);
     // boxes hold int16 pixel indices, so culling is off for larger images

@@ -24,7 +24,7 @@ __all__ = [
     # additions beyond the reference's ten
     "project_gaussians", "project_points", "num_tiles_hit", "sort_intersects", "bin_gaussians",
     "build_splat_records", "raster_forward", "raster_backward", "finalize_gradients",
-    "set_raster_culling",
+    "set_raster_culling", "set_precise_edge_on",
 ]
 
 
@@ -364,6 +364,12 @@ def build_splat_records(means, scales, glob_scale, quats, colors, opacities, uv0
     return records
 
 
+def set_precise_edge_on(enabled: bool) -> bool:
+    """Solve near-edge-on ray/plane hits (|n . ray| < 0.05) in float64 (default off:
+    the reference's float32 arithmetic); returns the previous setting."""
+    return bool(_lib.lib().gstex_set_precise_edge_on(int(bool(enabled))))
+
+
 def set_raster_culling(enabled: bool) -> bool:
     """Turn the raster kernels' per-wave screen-box culling on or off (results are
     identical either way); returns the previous setting."""
@@ -480,6 +486,38 @@ def texture_sample_backward(texture_info, texture_dims, uvs, texture, v_output):
     return v_texture
 
 
-def texture_edit(*args, **kwargs):
-    """texture_edit.cu is outside this build's scope."""
-    raise NotImplementedError("texture_edit is not provided by gstex_cuda_amd")
+def texture_edit(tile_bounds, block, img_size, texture_info, texture_total_size, texture_dims,
+                 updated_img, updated_alpha, depth_lower, depth_upper, gaussian_ids_sorted,
+                 tile_bins, opacities, means, scales, glob_scale, quats, uv0, umap, vmap, viewmat,
+                 c2w, fx, fy, cx, cy, settings, background):
+    """texture_edit.cu:229-346 -> updated_texture (texture_total_size, C): channels
+    0..4 hold the edit (r*a, g*a, b*a, a, weight) splatted onto the texel charts."""
+    named = locals()
+    for name in ("texture_dims", "updated_img", "updated_alpha", "depth_lower", "depth_upper",
+                 "gaussian_ids_sorted", "tile_bins", "opacities", "means", "scales", "quats",
+                 "uv0", "umap", "vmap", "viewmat", "c2w", "background"):
+        _check(named[name], name)
+    for name in ("texture_dims", "gaussian_ids_sorted", "tile_bins"):
+        _dtype(named[name], name, torch.int32)
+    for name in ("updated_img", "updated_alpha", "depth_lower", "depth_upper", "opacities",
+                 "means", "scales", "quats", "uv0", "umap", "vmap", "viewmat", "c2w"):
+        _dtype(named[name], name, torch.float32)
+    if int(block[0]) != int(block[1]):
+        raise RuntimeError("block must be square (block_width, block_width, 1)")
+    dev = means.device
+    W, H = int(img_size[0]), int(img_size[1])
+    C = int(texture_info[2])
+    X = int(texture_total_size)
+    n = means.size(0)
+    out = torch.empty((X, C), dtype=torch.float32, device=dev)
+    with torch.cuda.device(dev):
+        ws, nb = _workspace(_lib.lib().gstex_texture_edit_workspace_size(n), dev)
+        call("gstex_texture_edit", int(tile_bounds[0]), int(tile_bounds[1]), int(block[0]), W, H,
+             int(texture_info[0]), int(texture_info[1]), C, X, _ptr(texture_dims),
+             _ptr(updated_img), _ptr(updated_alpha), _ptr(depth_lower), _ptr(depth_upper),
+             _ptr(gaussian_ids_sorted), gaussian_ids_sorted.numel(), _ptr(tile_bins),
+             _ptr(opacities), _ptr(means), _ptr(scales), n, float(glob_scale), _ptr(quats),
+             _ptr(uv0), _ptr(umap), _ptr(vmap), _ptr(viewmat), _ptr(c2w), float(fx), float(fy),
+             float(cx), float(cy), int(settings), _ptr(background), _ptr(out), _ptr(ws), nb,
+             _stream(dev))
+    return out

@@ -72,7 +72,8 @@ class RenderStep:
 
     def __init__(self, params: Dict[str, torch.Tensor], cam: Camera, block_width: int = 16,
                  settings: int = 1 << 8, background: Optional[torch.Tensor] = None,
-                 sh_degree: int = -1, use_wave_masks: bool = True):
+                 sh_degree: int = -1, use_wave_masks: bool = True,
+                 capacity: Optional[int] = None):
         self.p = params
         self.cam = cam
         self.bw = block_width
@@ -131,7 +132,16 @@ class RenderStep:
             v_texture=torch.empty((self.X, self.C), **f32))
         if sh_degree >= 0:
             self.grads["v_sh_coeffs"] = torch.empty(params["sh_coeffs"].shape, **f32)
+            self._viewdirs = torch.empty((n, 3), **f32)
         self.num_intersects = 0
+        # Capacity mode: the intersect list has a fixed capacity and its length
+        # stays on the device, so a step makes no host synchronisation and can
+        # be captured in a graph (see capture()).
+        self.cap = capacity
+        self.m_dev = torch.zeros((1,), **i32)
+        self.graph = None
+        if capacity is not None:
+            self._ensure_bins(int(capacity))
 
     def _stream(self):
         return torch.cuda.current_stream(self.dev).cuda_stream
@@ -157,24 +167,31 @@ class RenderStep:
         p, cam, s = self.p, self.cam, self._stream()
         n = self.n
         if self.sh_degree >= 0:
-            viewdirs = p["means"] - cam.c2w[:3, 3]
+            torch.sub(p["means"], cam.c2w[:3, 3], out=self._viewdirs)
             fn = ("gstex_compute_sh_forward_bf16" if p["sh_coeffs"].dtype == torch.bfloat16
                   else "gstex_compute_sh_forward")
-            call(fn, n, self.sh_degree, self.sh_degree, _p(viewdirs), _p(p["sh_coeffs"]),
+            call(fn, n, self.sh_degree, self.sh_degree, _p(self._viewdirs), _p(p["sh_coeffs"]),
                  _p(self.colors), s)
-            self._viewdirs = viewdirs
         call("gstex_project_gaussians", n, _p(p["means"]), _p(p["scales"]), 1.0, _p(p["quats"]),
              _p(cam.viewmat), cam.fx, cam.fy, cam.cx, cam.cy, cam.img_h, cam.img_w, self.bw,
              _p(self.centers), _p(self.extents), _p(self.depths), _p(self.tiles_hit), s)
-        m = int(self.tiles_hit.sum(dtype=torch.int64).item())
-        self.num_intersects = m
-        self._ensure_bins(m)
-        if m > 0:
-            call("gstex_bin_gaussians", n, m, _p(self.centers), _p(self.extents),
+        if self.cap is not None:
+            m = int(self.cap)
+            call("gstex_bin_gaussians_capped", n, m, _p(self.centers), _p(self.extents),
                  _p(self.depths), _p(self.tiles_hit), self.tiles[0], self.tiles[1], self.bw,
-                 _p(self.ids), _p(self.tile_bins), _p(self.bin_ws), self.bin_ws.numel(), s)
+                 _p(self.ids), _p(self.tile_bins), _p(self.m_dev), _p(self.bin_ws),
+                 self.bin_ws.numel(), s)
         else:
-            self.tile_bins.zero_()
+            m = int(self.tiles_hit.sum(dtype=torch.int64).item())
+            self.num_intersects = m
+            self._ensure_bins(m)
+            if m > 0:
+                call("gstex_bin_gaussians", n, m, _p(self.centers), _p(self.extents),
+                     _p(self.depths), _p(self.tiles_hit), self.tiles[0], self.tiles[1], self.bw,
+                     _p(self.ids), _p(self.tile_bins), _p(self.bin_ws), self.bin_ws.numel(), s)
+            else:
+                self.tile_bins.zero_()
+        self._list_len = m
         call("gstex_build_splat_records", n, _p(p["means"]), _p(p["scales"]), 1.0,
              _p(p["quats"]), _p(self.colors), _p(p["opacities"]), _p(p["uv0"]), _p(p["umap"]),
              _p(p["vmap"]), _p(p["texture_dims"]), _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy,
@@ -183,7 +200,8 @@ class RenderStep:
             timer.start("raster_forward")
         call("gstex_raster_forward", cam.img_w, cam.img_h, self.bw, self.C, self.settings,
              _p(self.records), n, _p(self.ids), m, _p(self.tile_bins), _p(p["texture"]), self.X,
-             self.texture_format, _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy, cam.cx, cam.cy, _p(self.background),
+             self.texture_format, _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy, cam.cx, cam.cy,
+             _p(self.background),
              _p(self.out_img), _p(self.out_depth), _p(self.out_reg), _p(self.out_texture),
              _p(self.out_normal), _p(self.final_Ts), _p(self.final_idx), _p(self.depth_idx),
              _p(self.out_reg_s), *self._masks(), s)
@@ -204,7 +222,7 @@ class RenderStep:
     def backward(self, timer: Optional[StageTimer] = None, v_depth=None, v_reg=None,
                  v_alpha=None, v_normal=None):
         p, cam, s, g = self.p, self.cam, self._stream(), self.grads
-        n, m = self.n, self.num_intersects
+        n, m = self.n, self._list_len
         call("gstex_fill_f32", _p(self.grad_records), self.grad_records.numel(), 0.0, s)
         call("gstex_fill_f32", _p(g["v_texture"]), g["v_texture"].numel(), 0.0, s)
         v_depth = self.v_zero1 if v_depth is None else v_depth
@@ -215,7 +233,8 @@ class RenderStep:
             timer.start("raster_backward")
         call("gstex_raster_backward", cam.img_w, cam.img_h, self.bw, self.C, self.settings,
              _p(self.records), n, _p(self.ids), m, _p(self.tile_bins), _p(p["texture"]), self.X,
-             self.texture_format, _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy, cam.cx, cam.cy, _p(self.background),
+             self.texture_format, _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy, cam.cx, cam.cy,
+             _p(self.background),
              _p(self.final_Ts), _p(self.final_idx), _p(self.depth_idx), _p(self.out_reg_s),
              _p(self.v_img), _p(v_depth), _p(v_reg), _p(v_alpha), _p(self.v_tex), _p(v_normal),
              _p(self.grad_records), _p(g["v_texture"]), *self._masks(), s)
@@ -235,6 +254,36 @@ class RenderStep:
         self.l2_loss_grad(target)
         self.backward(timer)
 
+    def intersect_total(self) -> int:
+        """Intersect count of the last step (a host synchronisation)."""
+        if self.cap is None:
+            return self.num_intersects
+        self.num_intersects = int(self.m_dev.item())
+        return self.num_intersects
+
+    def capacity_ok(self) -> bool:
+        """True when the last step's intersects fitted the capacity (synchronises)."""
+        return self.cap is None or self.intersect_total() <= self.cap
+
+    def capture(self, target: torch.Tensor):
+        """Capture one step against ``target`` (a tensor that stays in place) as a
+        graph; ``replay()`` then reruns it with no host work.  Needs capacity mode."""
+        if self.cap is None:
+            raise RuntimeError("graph capture needs a RenderStep with a capacity")
+        side = torch.cuda.Stream(self.dev)
+        side.wait_stream(torch.cuda.current_stream(self.dev))
+        with torch.cuda.stream(side):
+            for _ in range(2):
+                self.step(target)
+        torch.cuda.current_stream(self.dev).wait_stream(side)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.step(target)
+        return self.graph
+
+    def replay(self):
+        self.graph.replay()
+
 
 def scene_to_step(scene, device, **kw) -> RenderStep:
     """RenderStep for a ``synthetic.Scene``."""
@@ -250,3 +299,92 @@ def scene_to_step(scene, device, **kw) -> RenderStep:
                  cy=scene.cy, img_h=scene.img_h, img_w=scene.img_w)
     return RenderStep(params, cam, block_width=scene.block_width, settings=scene.settings,
                       background=t["background"], sh_degree=scene.sh_degree, **kw)
+
+
+class MultiViewStep:
+    """Forward + backward of several views of one scene, gradients summed.
+
+    Each view is a capacity-mode ``RenderStep`` over the shared parameter
+    tensors (texels / SH coefficients may be bfloat16).  ``step()`` renders every
+    view, applies the L2 loss against its target and backpropagates; the
+    per-view gradients are summed into ``grads``.  ``capture()`` records the
+    whole batch as one graph, so ``replay()`` costs one launch from the host.
+    The capacities come from one eager pass per view (``capacity_factor`` of
+    the measured intersect count); ``replay_checked()`` verifies them.
+    """
+
+    def __init__(self, params: Dict[str, torch.Tensor], cams, targets, block_width: int = 16,
+                 settings: int = 1 << 8, background: Optional[torch.Tensor] = None,
+                 sh_degree: int = -1, capacity_factor: float = 1.1,
+                 use_wave_masks: bool = True):
+        self.targets = list(targets)
+        kw = dict(block_width=block_width, settings=settings, background=background,
+                  sh_degree=sh_degree, use_wave_masks=use_wave_masks)
+        self.views = []
+        for cam, tgt in zip(cams, self.targets):
+            probe = RenderStep(params, cam, **kw)
+            probe.forward()
+            cap = int(probe.num_intersects * capacity_factor) + 4096
+            del probe
+            self.views.append(RenderStep(params, cam, capacity=cap, **kw))
+        self.grads = {k: torch.zeros_like(v) for k, v in self.views[0].grads.items()}
+        self.graph = None
+
+    def step(self):
+        for v, tgt in zip(self.views, self.targets):
+            v.step(tgt)
+        for k, acc in self.grads.items():
+            acc.copy_(self.views[0].grads[k])
+            for v in self.views[1:]:
+                acc.add_(v.grads[k])
+
+    def capacity_ok(self) -> bool:
+        return all(v.capacity_ok() for v in self.views)
+
+    def capture(self):
+        dev = self.views[0].dev
+        side = torch.cuda.Stream(dev)
+        side.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(side):
+            for _ in range(2):
+                self.step()
+        torch.cuda.current_stream(dev).wait_stream(side)
+        if not self.capacity_ok():
+            raise RuntimeError("an intersect list overflowed its capacity; raise capacity_factor")
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.step()
+        return self.graph
+
+    def replay(self):
+        self.graph.replay()
+
+    def replay_checked(self):
+        """Replay and confirm every view's intersects fitted (synchronises)."""
+        self.graph.replay()
+        if not self.capacity_ok():
+            raise RuntimeError("an intersect list overflowed its capacity; recapture larger")
+
+
+def scene_to_multiview(scene, views, device, bf16: bool = False, **kw) -> MultiViewStep:
+    """A MultiViewStep for a ``synthetic.Scene`` and ``synthetic.make_views`` cameras;
+    ``bf16`` stores texels and SH coefficients as bfloat16."""
+    t = scene.to_torch(device)
+    params = dict(means=t["means"], scales=t["scales"], quats=t["quats"],
+                  opacities=t["opacities"], uv0=t["uv0"], umap=t["umap"], vmap=t["vmap"],
+                  texture=t["texture"], texture_dims=t["texture_dims"])
+    if bf16:
+        params["texture"] = params["texture"].to(torch.bfloat16)
+    if scene.sh_coeffs is not None:
+        params["sh_coeffs"] = t["sh_coeffs"].to(torch.bfloat16) if bf16 else t["sh_coeffs"]
+    else:
+        params["colors"] = t["colors"]
+    cams, targets = [], []
+    for v in views:
+        cams.append(Camera(viewmat=torch.from_numpy(v.viewmat).to(device),
+                           c2w=torch.from_numpy(v.c2w).to(device), fx=v.fx, fy=v.fy, cx=v.cx,
+                           cy=v.cy, img_h=v.img_h, img_w=v.img_w))
+        targets.append(torch.from_numpy(v.target).to(device))
+    return MultiViewStep(params, cams, targets, block_width=scene.block_width,
+                         settings=scene.settings, background=t["background"],
+                         sh_degree=scene.sh_degree, **kw)

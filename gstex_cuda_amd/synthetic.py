@@ -186,3 +186,48 @@ def make_scene(workload: int = 1, seed: Optional[int] = None, num_points: Option
                  target=target, fx=float(focal), fy=float(focal), cx=W / 2.0, cy=H / 2.0,
                  img_h=H, img_w=W, block_width=sp.block_width, settings=sp.settings,
                  sh_coeffs=sh_coeffs, sh_degree=sp.sh_degree)
+
+
+@dataclass
+class View:
+    """One camera of a multi-view batch (row-major 4x4 matrices, float32)."""
+    viewmat: np.ndarray
+    c2w: np.ndarray
+    fx: float
+    fy: float
+    cx: float
+    cy: float
+    img_h: int
+    img_w: int
+    target: np.ndarray
+
+
+def make_views(scene: Scene, num_views: int, seed: int = 7, max_angle_deg: float = 6.0,
+               max_shift: float = 0.15,
+               resolutions: Optional[list] = None) -> list:
+    """Seeded cameras around the scene's camera: a random rotation of up to
+    ``max_angle_deg`` about a random axis and a shift of up to ``max_shift`` in
+    each coordinate, optionally with per-view resolutions (W, H) drawn from
+    ``resolutions`` (focal scaled with the height).  Targets are U[0,1]."""
+    rng = np.random.default_rng(seed)
+    views = []
+    for _ in range(num_views):
+        axis = rng.standard_normal(3)
+        axis /= np.linalg.norm(axis)
+        ang = np.deg2rad(rng.uniform(-max_angle_deg, max_angle_deg))
+        K = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+        R = np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * (K @ K)  # camera-to-world rotation
+        t = rng.uniform(-max_shift, max_shift, 3)
+        c2w = np.eye(4)
+        c2w[:3, :3] = R
+        c2w[:3, 3] = t
+        viewmat = np.linalg.inv(c2w)
+        W, H = scene.img_w, scene.img_h
+        if resolutions:
+            W, H = resolutions[int(rng.integers(0, len(resolutions)))]
+        f = scene.fx * H / scene.img_h
+        views.append(View(viewmat=viewmat.astype(np.float32), c2w=c2w.astype(np.float32),
+                          fx=float(f), fy=float(f), cx=W / 2.0, cy=H / 2.0, img_h=int(H),
+                          img_w=int(W),
+                          target=rng.uniform(0.0, 1.0, (H, W, 3)).astype(np.float32)))
+    return views

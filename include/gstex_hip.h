@@ -120,6 +120,19 @@ int gstex_bin_gaussians(int num_points, int num_intersects, const float *centers
                         const float *extents, const float *depths, const int32_t *num_tiles_hit,
                         int tiles_x, int tiles_y, int block_width, int32_t *gaussian_ids_sorted,
                         int32_t *tile_bins, void *workspace, size_t workspace_bytes, void *stream);
+/* Like gstex_bin_gaussians, with no host-side intersect count (no host sync,
+ * so a caller can capture the step in a graph): the list has room for
+ * `capacity` entries, the intersect total sum(num_tiles_hit) is computed on
+ * the device and written to *num_intersects_out (device memory), slots past it
+ * lie outside every bin, and entries past the capacity are dropped -- a total
+ * above `capacity` on return means the binning must be redone with a larger
+ * list.  Workspace: gstex_bin_gaussians_workspace_size(num_points, capacity). */
+int gstex_bin_gaussians_capped(int num_points, int capacity, const float *centers,
+                               const float *extents, const float *depths,
+                               const int32_t *num_tiles_hit, int tiles_x, int tiles_y,
+                               int block_width, int32_t *gaussian_ids_sorted, int32_t *tile_bins,
+                               uint32_t *num_intersects_out, void *workspace,
+                               size_t workspace_bytes, void *stream);
 
 /* Replaces texture_forward_tensor (cuda/csrc/texture.cu:762-901).  The nine
  * outputs are fully written.  num_ids = length of gaussian_ids_sorted,
@@ -183,6 +196,12 @@ int gstex_build_splat_records(int num_points, const float *means, const float *s
  * default; 0 = every list entry is evaluated).  Results are identical either
  * way; returns the previous setting. */
 int gstex_set_raster_culling(int enabled);
+/* Process-wide switch (default 0): when 1, the raster and edit kernels solve
+ * the ray/plane hit of near-edge-on splats (|n . ray| < 0.05) in float64.
+ * This departs from the reference's float32 arithmetic (to the float64
+ * oracle's benefit) for those pairs only; returns the previous setting.  No
+ * reference counterpart. */
+int gstex_set_precise_edge_on(int enabled);
 /* Wave masks (optional, for the per-stage forward/backward pair): one bit per
  * (tile wave, list entry), written by gstex_raster_forward and read by
  * gstex_raster_backward, which then visits only the entries the forward
@@ -222,6 +241,28 @@ int gstex_finalize_gradients(int num_points, const float *grad_records, const vo
                              float *v_uv0, float *v_umap, float *v_vmap, void *stream);
 /* Fills n float32 values with v (a plain kernel, safe inside captured graphs). */
 int gstex_fill_f32(float *dst, size_t n, float v, void *stream);
+
+/* Replaces texture_edit_tensor (cuda/csrc/texture_edit.cu:229-346), bound as
+ * gstex_cuda.cuda.texture_edit.  updated_texture (texture_total_size x C,
+ * C = texture_info[2] >= 5) is zeroed, then channels 0..4 receive the edit
+ * (r*a, g*a, b*a, a, weight) of every composited pair whose view depth lies in
+ * the pixel's [depth_lower, depth_upper].  updated_img (H,W,3), updated_alpha,
+ * depth_lower, depth_upper (H,W).  settings bit 0 is the edit kernel's blur
+ * switch (texture_edit.cu:51).  The reference writes past the output when
+ * C < 5; this entry point rejects that.  background is unused, as in the
+ * reference. */
+size_t gstex_texture_edit_workspace_size(int num_points);
+int gstex_texture_edit(int tiles_x, int tiles_y, int block_width, int img_width, int img_height,
+                       int num_charts, int num_probs, int channels, int texture_total_size,
+                       const int32_t *texture_dims, const float *updated_img,
+                       const float *updated_alpha, const float *depth_lower,
+                       const float *depth_upper, const int32_t *gaussian_ids_sorted, int num_ids,
+                       const int32_t *tile_bins, const float *opacities, const float *means,
+                       const float *scales, int num_points, float glob_scale, const float *quats,
+                       const float *uv0, const float *umap, const float *vmap,
+                       const float *viewmat, const float *c2w, float fx, float fy, float cx,
+                       float cy, int settings, const float *background, float *updated_texture,
+                       void *workspace, size_t workspace_bytes, void *stream);
 
 /* Replaces texture_sample_forward_tensor (cuda/csrc/texture_sample.cu:72-105). */
 int gstex_texture_sample_forward(int num_charts, int num_probs, int channels,

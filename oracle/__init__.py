@@ -281,6 +281,26 @@ def texture_forward(scene: Scene, precision="f32"):
     return out
 
 
+def texture_edit(scene: Scene, updated_img, updated_alpha, depth_lower, depth_upper,
+                 precision="f32"):
+    """texture_edit.cu:11-227 -> (X, C) accumulated edit channels (r*a, g*a, b*a, a, 1).
+
+    ``scene.texture`` only supplies the shape (X, C), C >= 5; settings bit 0 is
+    the blur switch of the edit kernel.
+    """
+    real, sfx = _real(precision)
+    st = scene._struct(precision)
+    X, ch = scene.texture.shape
+    if ch < 5:
+        raise ValueError("texture_edit writes 5 channels; the texture needs C >= 5")
+    out = np.zeros((X, ch), real)
+    ui, ua, dl, du = (_in(x, precision) for x in (updated_img, updated_alpha, depth_lower,
+                                                  depth_upper))
+    getattr(lib(), "orc_texture_edit" + sfx)(C.byref(st), _p(ui), _p(ua), _p(dl), _p(du),
+                                             _p(out))
+    return out
+
+
 def texture_backward(scene: Scene, fwd: dict, v_img, v_depth, v_reg, v_alpha, v_texture_out,
                      v_normal, precision="f32"):
     """texture.cu:331-760 -> dict of the nine reference gradients.

@@ -625,4 +625,67 @@ void SFX(orc_texture_backward)(const orc_scene *S, const REAL *final_Ts, const i
         }
 }
 
+/* texture_edit.cu:11-227: for every pair the forward would composite (same
+ * alpha, skip and stop rules, blur from settings bit 0) whose view depth
+ * t * view_depth lies in [depth_lower, depth_upper] of the pixel, add the
+ * bilinear weights (texture_helpers.cuh:162-250, always bilinear, replicate
+ * padding) times (r*a, g*a, b*a, a, 1) -- a the pixel's edit alpha -- into
+ * channels 0..4 of out (X x C texels, C >= 5, zero-initialised by the caller).
+ * The pixel's T advances by every composited pair, in or out of range. */
+void SFX(orc_texture_edit)(const orc_scene *S, const IN *upd_img, const IN *upd_alpha,
+                           const IN *depth_lower, const IN *depth_upper, REAL *out) {
+    const int W = S->img_w, H = S->img_h, bw = S->block_width, C = S->channels;
+    const int tiles_x = (W + bw - 1) / bw, tiles_y = (H + bw - 1) / bw;
+    const int use_blur = S->settings & 1;
+    for (int ty = 0; ty < tiles_y; ++ty)
+        for (int tx = 0; tx < tiles_x; ++tx) {
+            int tile = ty * tiles_x + tx;
+            if (S->tile_mask && !S->tile_mask[tile]) continue;
+            int rx = S->tile_bins[2 * tile], ry = S->tile_bins[2 * tile + 1];
+            for (int ly = 0; ly < bw; ++ly)
+                for (int lx = 0; lx < bw; ++lx) {
+                    int i = ty * bw + ly, j = tx * bw + lx;
+                    if (i >= H || j >= W) continue;
+                    int pid = i * W + j;
+                    REAL px = (REAL)((float)j + 0.5), py = (REAL)((float)i + 0.5);
+                    REAL o[3], r[3];
+                    SFX(get_ray)(SF(c2w), (REAL)S->fx, (REAL)S->fy, (REAL)S->cx, (REAL)S->cy, px, py, o, r);
+                    REAL view_depth = (REAL)SF(viewmat)[8] * r[0] + (REAL)SF(viewmat)[9] * r[1] +
+                                      (REAL)SF(viewmat)[10] * r[2];
+                    REAL a = (REAL)upd_alpha[pid];
+                    REAL val[5] = {(REAL)upd_img[3 * pid] * a, (REAL)upd_img[3 * pid + 1] * a,
+                                   (REAL)upd_img[3 * pid + 2] * a, a, (REAL)1};
+                    REAL lo = (REAL)depth_lower[pid], hi = (REAL)depth_upper[pid];
+                    int done = 0;
+                    REAL T = 1;
+                    for (int e = rx; e < ry && !done; ++e) {
+                        int g = S->ids_sorted[e];
+                        SFX(pair_t) P;
+                        SFX(pair_geom)(S, g, o, r, px, py, use_blur, 0, &P);
+                        int skip = SFX(pair_skip)(&P);
+                        REAL next_T = T * ((REAL)1 - P.alpha);
+                        if (next_T <= (REAL)1e-4f) done = 1;
+                        if (skip || done) continue;
+                        REAL t_view = P.t * view_depth;
+                        if (t_view >= lo && t_view <= hi) {
+                            REAL u = (REAL)SF(uv0)[2 * g] + ((REAL)SF(umap)[3 * g] * P.delta[0] + (REAL)SF(umap)[3 * g + 1] * P.delta[1] + (REAL)SF(umap)[3 * g + 2] * P.delta[2]);
+                            REAL v = (REAL)SF(uv0)[2 * g + 1] + ((REAL)SF(vmap)[3 * g] * P.delta[0] + (REAL)SF(vmap)[3 * g + 1] * P.delta[1] + (REAL)SF(vmap)[3 * g + 2] * P.delta[2]);
+                            if (u <= 0) u = 0;
+                            if (u >= 1) u = 1;
+                            if (v <= 0) v = 0;
+                            if (v >= 1) v = 1;
+                            const int *d = S->texture_dims + 3 * g;
+                            for (int k = 0; k < 5; ++k) {
+                                REAL wt[4];
+                                long idx[4];
+                                SFX(chart_taps)(d[0], d[1], d[2], C, k, u, v, 1, wt, idx);
+                                for (int q = 0; q < 4; ++q) out[idx[q]] += wt[q] * val[k];
+                            }
+                        }
+                        T = next_T;
+                    }
+                }
+        }
+}
+
 #undef SF

@@ -23,12 +23,12 @@ def declared_symbols():
 
 def test_header_declares_the_reference_entry_points():
     syms = set(declared_symbols())
-    # one replacement per native entry point of the reference (ext.cpp:8-25),
-    # texture_edit excepted
+    # one replacement per native entry point of the reference (ext.cpp:8-25)
     for name in ("gstex_compute_sh_forward", "gstex_compute_sh_backward",
                  "gstex_map_gaussian_to_intersects", "gstex_get_tile_bin_edges",
                  "gstex_texture_forward", "gstex_texture_backward", "gstex_get_aabb_2d",
-                 "gstex_texture_sample_forward", "gstex_texture_sample_backward"):
+                 "gstex_texture_sample_forward", "gstex_texture_sample_backward",
+                 "gstex_texture_edit"):
         assert name in syms
 
 

@@ -350,3 +350,66 @@ def test_backward_matches_central_differences(oracle, seed, settings):
         scale = max(np.abs(n_).max(), 1e-8)
         err = np.abs(a - n_) / scale
         assert np.mean(err < 1e-4) >= 0.98 and np.max(err) < 5e-2, (p, float(err.max()))
+
+
+# ---------------------------------------------------------------- texture_edit
+def _edit_scene(s):
+    s.texture = np.zeros((s.texture.shape[0], 5), np.float32)
+    return s
+
+
+def test_texture_edit_single_splat_counts_pixels(oracle):
+    """One camera-facing splat with a 1x1 chart: the weight channel is the number
+    of pixels whose alpha passes 1/255 (closed form: the ellipse
+    (x/s)^2 + (y/s)^2 <= 2 ln(255 opac) on the plane), and the colour channels
+    are sums of r*a, g*a, b*a over those pixels."""
+    W = H = 48
+    f, z, sc, op = 48.0, 3.0, 0.2, 0.8
+    s = _edit_scene(camera_facing_scene([[0.0, 0.0, z]], [[sc, sc, 0.0]], [op], [[1, 1, 1]], W,
+                                        H, f, W / 2, H / 2))
+    rng = np.random.default_rng(1)
+    img = rng.random((H, W, 3)).astype(np.float32)
+    al = rng.random((H, W)).astype(np.float32)
+    lo = np.zeros((H, W), np.float32)
+    hi = np.full((H, W), 1e9, np.float32)
+    out = oracle.texture_edit(oracle_scene(oracle, s), img, al, lo, hi, "f64")
+    # ray through pixel (j, i) hits the plane z at (x, y) = z * ((j+.5-cx)/f, (i+.5-cy)/f)
+    jj, ii = np.meshgrid(np.arange(W), np.arange(H))
+    x = z * (jj + 0.5 - W / 2) / f
+    y = z * (ii + 0.5 - H / 2) / f
+    sig = 0.5 * (x * x + y * y) / (sc * sc)
+    hit = np.minimum(0.99, op * np.exp(-sig)) >= 1 / 255
+    assert out[0, 4] == hit.sum()
+    al64, img64 = al.astype(np.float64), img.astype(np.float64)
+    np.testing.assert_allclose(out[0, 3], al64[hit].sum(), rtol=1e-12)
+    for k in range(3):
+        np.testing.assert_allclose(out[0, k], (img64[..., k] * al64)[hit].sum(), rtol=1e-12)
+
+
+def test_texture_edit_channel_identities_and_depth_windows(oracle):
+    """White canvas: channels 0-2 equal channel 3; opaque canvas: channel 3 equals
+    channel 4; an empty depth window gives zero; two windows split at a depth
+    add up to the full window."""
+    from gstex_cuda_amd import synthetic
+
+    s = _edit_scene(synthetic.make_scene(1, num_points=300, img_w=64, img_h=48))
+    sc = oracle_scene(oracle, s, settings=0)
+    H, W = s.img_h, s.img_w
+    rng = np.random.default_rng(2)
+    img = rng.random((H, W, 3)).astype(np.float32)
+    al = rng.random((H, W)).astype(np.float32)
+    lo = np.zeros((H, W), np.float32)
+    hi = np.full((H, W), 1e9, np.float32)
+    full = oracle.texture_edit(sc, img, al, lo, hi, "f64")
+    assert full[:, 4].sum() > 0
+    white = oracle.texture_edit(sc, np.ones_like(img), al, lo, hi, "f64")
+    for k in range(3):
+        np.testing.assert_allclose(white[:, k], white[:, 3], rtol=1e-12)
+    opaque = oracle.texture_edit(sc, img, np.ones_like(al), lo, hi, "f64")
+    np.testing.assert_allclose(opaque[:, 3], opaque[:, 4], rtol=1e-12)
+    none = oracle.texture_edit(sc, img, al, hi, hi, "f64")
+    assert np.all(none == 0)
+    d = np.full((H, W), 4.0, np.float32)
+    near = oracle.texture_edit(sc, img, al, lo, d, "f64")
+    far = oracle.texture_edit(sc, img, al, np.nextafter(d, np.float32(np.inf)), hi, "f64")
+    np.testing.assert_allclose(near + far, full, rtol=1e-12, atol=1e-12)
