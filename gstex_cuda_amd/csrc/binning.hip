@@ -9,8 +9,15 @@
 namespace gstex {
 
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;  // per thread per block
-constexpr int kSortTile = kSortThreads * kSortItems;
+// Items per thread per block tile: 16 (4096-item tiles) for long inputs, 8
+// (2048-item tiles) below kSortSmallN, where 4096-item tiles would leave most
+// of the 256 CUs idle.
+constexpr int kSortSmallN = 1 << 22;
+static inline int sort_items(int n) { return n < kSortSmallN ? 8 : 16; }
+static inline int sort_blocks(int n) {
+    const int tile = kSortThreads * sort_items(n);
+    return (n + tile - 1) / tile;
+}
 
 // ------------------------------------------------------------------ fills
 template <typename T>
@@ -120,9 +127,10 @@ __device__ __forceinline__ uint32_t digit_of(K key, int shift, uint32_t mask) {
     return (uint32_t)(key >> shift) & mask;
 }
 
-// Per-block digit histogram of a 4096-item chunk, stored digit-major
-// (hist[d * nblocks + b]) so one exclusive scan yields scatter bases.
-template <typename K>
+// Per-block digit histogram of one block tile (ITEMS * 256 items), stored
+// digit-major (hist[d * nblocks + b]) so one exclusive scan yields scatter
+// bases.
+template <typename K, int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(int n, const K* __restrict__ keys,
                                                              K flip, int shift, int bits,
                                                              uint32_t* __restrict__ hist,
@@ -132,9 +140,9 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(int n, const K* __r
     cnt[tid] = 0;
     __syncthreads();
     const uint32_t mask = (1u << bits) - 1u;
-    const size_t base = (size_t)blockIdx.x * kSortTile;
+    const size_t base = (size_t)blockIdx.x * (kSortThreads * ITEMS);
 #pragma unroll 4
-    for (int r = 0; r < kSortItems; ++r) {
+    for (int r = 0; r < ITEMS; ++r) {
         const size_t idx = base + (size_t)r * kSortThreads + tid;
         if (idx < (size_t)n) atomicAdd(&cnt[digit_of(keys[idx] ^ flip, shift, mask)], 1u);
     }
@@ -142,31 +150,84 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(int n, const K* __r
     if (tid < (1 << bits)) hist[(size_t)tid * nblocks + blockIdx.x] = cnt[tid];
 }
 
-// Exclusive scan of a uint32 array in place by one 1024-lane workgroup
-// (sequential segments per lane, then a workgroup scan of segment sums).
+// Exclusive scan of up to 64K uint32 values in place by one 1024-lane
+// workgroup.  Wave w owns the PER * 64 contiguous values from w * PER * 64 on,
+// as PER / 4 chunks of 256: in chunk q lane l holds values q * 256 + 4 l ..
+// + 3, so each 16-byte load or store instruction covers 1 KB of contiguous
+// memory.  All loads are issued before any value is used (the scan costs about
+// two memory latencies).  Wave totals are exchanged through LDS; each chunk
+// is then scanned within the wave (four values per lane, a shuffle scan of the
+// lane sums) on top of a running carry, and written back from registers.  vec
+// is false when a is not 16-byte aligned (scalar accesses only).
+template <int PER>
 __global__ __launch_bounds__(1024) void k_scan_single(uint32_t* __restrict__ a, int n,
-                                                      uint32_t* __restrict__ total) {
-    __shared__ uint32_t sums[1024];
-    const int tid = threadIdx.x;
-    const int per = (n + 1023) / 1024;
-    const int lo = min(tid * per, n), hi = min(lo + per, n);
-    uint32_t s = 0;
-    for (int i = lo; i < hi; ++i) s += a[i];
-    sums[tid] = s;
+                                                      uint32_t* __restrict__ total, bool vec) {
+    __shared__ uint32_t wave_sums[1024 / kWave];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const int seg = wave * PER * kWave + 4 * lane;
+    uint32_t v[PER];
+#pragma unroll
+    for (int q = 0; q < PER / 4; ++q) {
+        const int i = seg + q * 256;
+        if (vec && i + 3 < n) {
+            const uint4 x = *reinterpret_cast<const uint4*>(a + i);
+            v[4 * q] = x.x;
+            v[4 * q + 1] = x.y;
+            v[4 * q + 2] = x.z;
+            v[4 * q + 3] = x.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[4 * q + j] = i + j < n ? a[i + j] : 0u;
+        }
+    }
+    uint32_t lsum = 0;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) lsum += v[j];
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) lsum += __shfl_xor(lsum, off);
+    if (lane == 0) wave_sums[wave] = lsum;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t v = tid >= off ? sums[tid - off] : 0u;
-        __syncthreads();
-        sums[tid] += v;
-        __syncthreads();
+    uint32_t carry = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < 1024 / kWave; ++w) {
+        const uint32_t t = wave_sums[w];
+        if (w < wave) carry += t;
+        tot += t;
     }
-    uint32_t run = tid == 0 ? 0u : sums[tid - 1];
-    for (int i = lo; i < hi; ++i) {
-        uint32_t v = a[i];
-        a[i] = run;
-        run += v;
+#pragma unroll
+    for (int q = 0; q < PER / 4; ++q) {
+        const uint32_t q0 = v[4 * q], q1 = q0 + v[4 * q + 1], q2 = q1 + v[4 * q + 2],
+                       q3 = q2 + v[4 * q + 3];
+        uint32_t inc = q3;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const uint32_t t = __shfl_up(inc, off);
+            if (lane >= off) inc += t;
+        }
+        const uint32_t b0 = carry + inc - q3;
+        carry += __shfl(inc, kWave - 1);
+        const uint32_t o[4] = {b0, b0 + q0, b0 + q1, b0 + q2};
+        const int i = seg + q * 256;
+        if (vec && i + 3 < n) {
+            *reinterpret_cast<uint4*>(a + i) = make_uint4(o[0], o[1], o[2], o[3]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (i + j < n) a[i + j] = o[j];
+        }
     }
-    if (tid == 1023 && total) *total = sums[1023];
+    if (tid == 0 && total) *total = tot;
+}
+
+// One-workgroup scan of n <= kScanSingleMax values, with the per-lane segment
+// length the smallest that covers n.
+static void launch_scan_single(uint32_t* a, int n, uint32_t* total, hipStream_t s) {
+    const bool vec = ((uintptr_t)a & 15u) == 0;
+    if (n <= 4 * 1024) k_scan_single<4><<<1, 1024, 0, s>>>(a, n, total, vec);
+    else if (n <= 8 * 1024) k_scan_single<8><<<1, 1024, 0, s>>>(a, n, total, vec);
+    else if (n <= 16 * 1024) k_scan_single<16><<<1, 1024, 0, s>>>(a, n, total, vec);
+    else if (n <= 32 * 1024) k_scan_single<32><<<1, 1024, 0, s>>>(a, n, total, vec);
+    else k_scan_single<64><<<1, 1024, 0, s>>>(a, n, total, vec);
 }
 
 // Device-wide exclusive scan in three passes: per-tile sums, a scan of the
@@ -242,78 +303,96 @@ static hipError_t scan_exclusive(uint32_t* a, int n, uint32_t* total, uint32_t* 
                                  hipStream_t s) {
     if (n <= 0) return hipSuccess;
     if (n <= kScanSingleMax) {
-        k_scan_single<<<1, 1024, 0, s>>>(a, n, total);
+        launch_scan_single(a, n, total, s);
         return hipGetLastError();
     }
     const int tiles = (n + kScanTile - 1) / kScanTile;
+    if (tiles > kScanSingleMax) return hipErrorInvalidValue;
     k_scan_tile_sums<<<tiles, kScanThreads, 0, s>>>(a, n, partials);
-    k_scan_single<<<1, 1024, 0, s>>>(partials, tiles, total);
+    launch_scan_single(partials, tiles, total, s);
     k_scan_tile_apply<<<tiles, kScanThreads, 0, s>>>(a, n, partials);
     return hipGetLastError();
 }
 
-// Stable scatter of one block tile (kSortTile items) in two phases.  Ranking:
-// each round ranks 256 items (one per lane) with wave ballots and combines the
-// four waves' per-digit counts in LDS, placing every item at its stable
-// position in an LDS copy of the tile sorted by digit.  Write-out: lanes take
+// Stable scatter of one block tile (ITEMS * 256 items) in two phases.  The
+// tile's keys and values are read once, into registers.  Ranking: each round
+// ranks 256 items (one per lane) with wave ballots and combines the four
+// waves' per-digit counts in LDS, placing every item at its stable position
+// in an LDS copy of the tile sorted by digit.  The per-wave counts and the
+// running per-digit offsets are double-buffered by round parity, so a round
+// needs two barriers: one after the counts are posted, one after they are
+// consumed (buffer r & 1 is read in round r and cleared or rewritten only in
+// round r + 1, after that round's first barrier).  Write-out: lanes take
 // consecutive LDS positions, so each digit's run goes to consecutive global
 // addresses (hist[digit][block] onward) in coalesced stores.
-template <typename K>
+template <typename K, int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     int n, const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     K* __restrict__ keys_out, uint32_t* __restrict__ vals_out, K flip, int shift, int bits,
     const uint32_t* __restrict__ hist, int nblocks) {
+    constexpr int kTile = kSortThreads * ITEMS;
+    constexpr int kWaves = kSortThreads / kWave;
     __shared__ uint32_t gbase[256];   // global start of each digit's run for this block
     __shared__ uint32_t lstart[256];  // start of each digit's run in the LDS tile
-    __shared__ uint32_t run[256];     // next free LDS slot per digit
-    __shared__ uint32_t wave_cnt[kSortThreads / kWave][256];
-    __shared__ K skeys[kSortTile];
-    __shared__ uint32_t svals[kSortTile];
+    __shared__ uint32_t run[2][256];  // next free LDS slot per digit, by round parity
+    __shared__ uint32_t wave_cnt[2][kWaves][256];
+    __shared__ K skeys[kTile];
+    __shared__ uint32_t svals[kTile];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const uint32_t mask = (1u << bits) - 1u;
     const int ndig = 1 << bits;
-    const size_t base = (size_t)blockIdx.x * kSortTile;
-    const int count = (int)min((size_t)kSortTile, (size_t)n - base);
+    const size_t base = (size_t)blockIdx.x * kTile;
+    const int count = (int)min((size_t)kTile, (size_t)n - base);
+
+    K key[ITEMS];
+    uint32_t val[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        const int li = r * kSortThreads + tid;
+        key[r] = li < count ? keys_in[base + li] : (K)0;
+        val[r] = li < count ? vals_in[base + li] : 0u;
+    }
 
     // local digit histogram -> exclusive prefix (one digit per lane)
-    run[tid] = 0;
-    __syncthreads();
-    for (int r = 0; r < kSortItems; ++r) {
-        const int li = r * kSortThreads + tid;
-        if (li < count) atomicAdd(&run[digit_of(keys_in[base + li] ^ flip, shift, mask)], 1u);
+    run[0][tid] = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        wave_cnt[0][w][tid] = 0;
+        wave_cnt[1][w][tid] = 0;
     }
     __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r)
+        if (r * kSortThreads + tid < count)
+            atomicAdd(&run[0][digit_of(key[r] ^ flip, shift, mask)], 1u);
+    __syncthreads();
     {
-        uint32_t v = run[tid];
+        const uint32_t c = run[0][tid];
+        uint32_t v = c;
         // inclusive scan over the 256 digits: within each wave, then across waves
 #pragma unroll
         for (int off = 1; off < kWave; off <<= 1) {
             const uint32_t t = __shfl_up(v, off);
             if (lane >= off) v += t;
         }
-        wave_cnt[0][tid] = v;
+        run[1][tid] = v;
         __syncthreads();
         uint32_t add = 0;
-        for (int w = 0; w < wave; ++w) add += wave_cnt[0][w * kWave + kWave - 1];
-        const uint32_t excl = v + add - run[tid];
+        for (int w = 0; w < wave; ++w) add += run[1][w * kWave + kWave - 1];
+        const uint32_t excl = v + add - c;
         __syncthreads();
         lstart[tid] = excl;
-        run[tid] = excl;
+        run[0][tid] = excl;
         gbase[tid] = tid < ndig ? hist[(size_t)tid * nblocks + blockIdx.x] : 0u;
     }
     __syncthreads();
 
     const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-    for (int r = 0; r < kSortItems; ++r) {
-        const int li = r * kSortThreads + tid;
-        const bool valid = li < count;
-        K key = 0;
-        uint32_t val = 0, d = 0;
-        if (valid) {
-            key = keys_in[base + li];
-            val = vals_in[base + li];
-            d = digit_of(key ^ flip, shift, mask);
-        }
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        const int cur = r & 1, nxt = cur ^ 1;
+        const bool valid = r * kSortThreads + tid < count;
+        const uint32_t d = valid ? digit_of(key[r] ^ flip, shift, mask) : 0u;
         uint64_t peers = __ballot(valid);
         for (int b = 0; b < bits; ++b) {
             const bool bit = (d >> b) & 1u;
@@ -322,34 +401,36 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
         }
         const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
         const uint32_t tot = (uint32_t)__popcll(peers);
-#pragma unroll
-        for (int w = 0; w < kSortThreads / kWave; ++w) wave_cnt[w][tid] = 0;
-        __syncthreads();
-        if (valid && rank == 0) wave_cnt[wave][d] = tot;
+        if (valid && rank == 0) wave_cnt[cur][wave][d] = tot;
         __syncthreads();
         if (valid) {
-            uint32_t pos = run[d] + rank;
-            for (int w = 0; w < wave; ++w) pos += wave_cnt[w][d];
-            skeys[pos] = key;
-            svals[pos] = val;
+            uint32_t pos = run[cur][d] + rank;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w)
+                if (w < wave) pos += wave_cnt[cur][w][d];
+            skeys[pos] = key[r];
+            svals[pos] = val[r];
         }
-        __syncthreads();
         {
             uint32_t add = 0;
 #pragma unroll
-            for (int w = 0; w < kSortThreads / kWave; ++w) add += wave_cnt[w][tid];
-            run[tid] += add;
+            for (int w = 0; w < kWaves; ++w) {
+                add += wave_cnt[cur][w][tid];
+                wave_cnt[nxt][w][tid] = 0;
+            }
+            run[nxt][tid] = run[cur][tid] + add;
         }
         __syncthreads();
     }
 
-    for (int r = 0; r < kSortItems; ++r) {
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
         const int p = r * kSortThreads + tid;
         if (p < count) {
-            const K key = skeys[p];
-            const uint32_t d = digit_of(key ^ flip, shift, mask);
+            const K k = skeys[p];
+            const uint32_t d = digit_of(k ^ flip, shift, mask);
             const uint32_t gp = gbase[d] + (uint32_t)p - lstart[d];
-            keys_out[gp] = key;
+            keys_out[gp] = k;
             vals_out[gp] = svals[p];
         }
     }
@@ -357,7 +438,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
 
 template <typename K>
 size_t radix_workspace_bytes(int n) {
-    const size_t nb = (size_t)(n + kSortTile - 1) / kSortTile;
+    const size_t nb = (size_t)sort_blocks(n);
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     return up(sizeof(K) * (size_t)n) + up(sizeof(uint32_t) * (size_t)n) +
            up(sizeof(uint32_t) * 256 * nb) + up(sizeof(uint32_t) * scan_partials_count((int)(256 * nb))) + 256;
@@ -372,7 +453,8 @@ hipError_t radix_sort_pairs(int n, const K* keys_in, const uint32_t* vals_in, K*
                             hipStream_t s) {
     if (n <= 0) return hipSuccess;
     if (ws_bytes < radix_workspace_bytes<K>(n)) return hipErrorInvalidValue;
-    const int nblocks = (n + kSortTile - 1) / kSortTile;
+    const int nblocks = sort_blocks(n);
+    const bool small = sort_items(n) == 8;
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     char* p = (char*)ws;
     K* tk = (K*)p;
@@ -400,11 +482,20 @@ hipError_t radix_sort_pairs(int n, const K* keys_in, const uint32_t* vals_in, K*
         const bool to_out = ((passes - 1 - pass) % 2) == 0;
         K* dk = to_out ? keys_out : tk;
         uint32_t* dv = to_out ? vals_out : tv;
-        k_radix_hist<K><<<nblocks, kSortThreads, 0, s>>>(n, sk, flip, shift, bits, hist, nblocks);
+        if (small)
+            k_radix_hist<K, 8><<<nblocks, kSortThreads, 0, s>>>(n, sk, flip, shift, bits, hist,
+                                                                 nblocks);
+        else
+            k_radix_hist<K, 16><<<nblocks, kSortThreads, 0, s>>>(n, sk, flip, shift, bits, hist,
+                                                                  nblocks);
         hipError_t es = scan_exclusive(hist, (1 << bits) * nblocks, nullptr, partials, s);
         if (es != hipSuccess) return es;
-        k_radix_scatter<K><<<nblocks, kSortThreads, 0, s>>>(n, sk, sv, dk, dv, flip, shift, bits,
-                                                             hist, nblocks);
+        if (small)
+            k_radix_scatter<K, 8><<<nblocks, kSortThreads, 0, s>>>(n, sk, sv, dk, dv, flip, shift,
+                                                                    bits, hist, nblocks);
+        else
+            k_radix_scatter<K, 16><<<nblocks, kSortThreads, 0, s>>>(n, sk, sv, dk, dv, flip,
+                                                                     shift, bits, hist, nblocks);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         sk = dk;
@@ -468,7 +559,10 @@ __global__ __launch_bounds__(256) void k_bin_prepare(
         w = min((x1 - x0) * (y1 - y0), max(counts[g], 0));
     }
     written[g] = w;
-    depth_keys[g] = w > 0 ? (uint32_t)__float_as_uint(depths[g]) : 0xFFFFFFFFu;
+    // 0xFFFFFFFF is kept for Gaussians that write nothing, so they all sort
+    // after every writing one (a writing Gaussian's depth is a positive float,
+    // whose bits are below 0x80000000; the clamp only touches a NaN pattern).
+    depth_keys[g] = w > 0 ? min((uint32_t)__float_as_uint(depths[g]), 0xFFFFFFFEu) : 0xFFFFFFFFu;
     order[g] = (uint32_t)g;
 }
 
@@ -480,58 +574,85 @@ __global__ __launch_bounds__(256) void k_gather_written(int n, const uint32_t* _
     out[p] = (uint32_t)written[order[p]];
 }
 
-// Emit (tile, id) for every depth-ordered Gaussian after the zero slots, which
-// occupy [0, m - total_written) and read as (tile 0, id 0).  A workgroup takes
-// 256 consecutive Gaussians and spreads their outputs evenly over its lanes:
-// output k of the group belongs to the Gaussian found by a binary search over
-// the group's output offsets (in LDS), and consecutive lanes write consecutive
-// slots.  A Gaussian's tiles go row by row over its rectangle, capped at its
-// count.
+// Emission of (tile, id) for every depth-ordered Gaussian after the zero
+// slots, which occupy [0, m - total_written) and read as (tile 0, id 0).
+// Work is split by output, not by Gaussian: workgroup b writes outputs
+// [b * kEmitOut, (b + 1) * kEmitOut) of the depth-ordered stream, so a few
+// screen-sized splats near the camera (thousands of tiles each) spread over
+// many workgroups instead of stalling one.
+constexpr int kEmitOut = 2048;
+
+// starts[b] = the depth-ordered position whose output range holds output
+// b * kEmitOut, and starts[ceil(total / kEmitOut)] = the last writing
+// position.  Output ranges [offs[p], offs[p] + w) tile [0, total) (writing
+// Gaussians all come before the ones that write nothing), so every entry is
+// written by exactly one lane.  Entries past max_b are not needed.
+__global__ __launch_bounds__(256) void k_bin_mark_starts(int n, const uint32_t* __restrict__ offs,
+                                                         const uint32_t* __restrict__ total_written,
+                                                         int max_b, uint32_t* __restrict__ starts) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t tw = *total_written, o = offs[p];
+    const uint32_t end = p + 1 < n ? offs[p + 1] : tw;
+    if (end <= o) return;
+    const uint32_t b0 = (o + kEmitOut - 1) / kEmitOut, b1 = (end - 1) / kEmitOut;
+    for (uint32_t b = b0; b <= b1 && b <= (uint32_t)max_b; ++b) starts[b] = (uint32_t)p;
+    const uint32_t last = (tw + kEmitOut - 1) / kEmitOut;
+    if (end == tw && last <= (uint32_t)max_b) starts[last] = (uint32_t)p;
+}
+
+// Workgroup b: its Gaussians are positions starts[b] .. starts[b + 1] (at
+// most kEmitOut + 1 of them, each with at least one output in the block or
+// at its end), staged in LDS with their output offsets and tile rectangles.
+// Output k belongs to the last staged Gaussian whose first output is <= k
+// (binary search in LDS); consecutive lanes write consecutive slots.  A
+// Gaussian's tiles go row by row over its rectangle, capped at its count.
 __global__ __launch_bounds__(256) void k_bin_emit(
     int n, int cap, const uint32_t* __restrict__ m_total, const uint32_t* __restrict__ order,
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ total_written,
-    const float* __restrict__ centers,
-    const float* __restrict__ extents, int tiles_x, int tiles_y, int bw, const int* __restrict__ written,
+    const uint32_t* __restrict__ starts, const float* __restrict__ centers,
+    const float* __restrict__ extents, int tiles_x, int tiles_y, int bw,
     uint32_t* __restrict__ tiles_out, uint32_t* __restrict__ ids_out) {
-    __shared__ uint32_t s_off[257];
-    __shared__ uint32_t s_g[256];
-    __shared__ int s_x0[256], s_y0[256], s_wd[256];
+    constexpr int kStage = kEmitOut + 1;
+    __shared__ uint32_t s_off[kStage], s_g[kStage];
+    __shared__ int s_x0[kStage], s_y0[kStage], s_wd[kStage];
     const int tid = threadIdx.x;
-    const int p0 = blockIdx.x * 256, p = p0 + tid;
-    const int cnt = min(256, n - p0);
     const uint32_t tw = *total_written, m = *m_total;
+    const uint32_t k0 = (uint32_t)blockIdx.x * kEmitOut;
+    if (k0 >= tw) return;
+    const uint32_t k1 = min(k0 + (uint32_t)kEmitOut, tw);
     const uint32_t zbase = tw <= m ? m - tw : 0u;
     const uint32_t limit = min(m, (uint32_t)cap);
-    if (tid < cnt) {
+    const uint32_t plo = starts[blockIdx.x];
+    const uint32_t phi = min(starts[blockIdx.x + 1], (uint32_t)n - 1u);
+    const int cnt = phi >= plo ? (int)min(phi - plo + 1u, (uint32_t)kStage) : 1;
+    for (int i = tid; i < cnt; i += 256) {
+        const uint32_t p = plo + (uint32_t)i;
         const uint32_t g = order[p];
-        const int w = max(written[g], 0);
         const float2 c = reinterpret_cast<const float2*>(centers)[g];
         const float2 e = reinterpret_cast<const float2*>(extents)[g];
         int x0, y0, x1, y1;
         tile_rect(c.x, c.y, e.x, e.y, bw, tiles_x, tiles_y, x0, y0, x1, y1);
-        s_off[tid] = offsets[p];
-        s_g[tid] = g;
-        s_x0[tid] = x0;
-        s_y0[tid] = y0;
-        s_wd[tid] = max(x1 - x0, 1);
-        if (tid == cnt - 1) s_off[cnt] = offsets[p] + (uint32_t)w;
+        s_off[i] = offsets[p];
+        s_g[i] = g;
+        s_x0[i] = x0;
+        s_y0[i] = y0;
+        s_wd[i] = max(x1 - x0, 1);
     }
     __syncthreads();
-    const uint32_t o0 = s_off[0], total = s_off[cnt] - o0;
-    for (uint32_t k = tid; k < total; k += 256) {
-        // last Gaussian whose first output is <= o0 + k
+    for (uint32_t k = k0 + tid; k < k1; k += 256) {
         int lo = 0, hi = cnt - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (s_off[mid] - o0 <= k) lo = mid;
+            if (s_off[mid] <= k) lo = mid;
             else hi = mid - 1;
         }
-        const uint32_t li = o0 + k - s_off[lo];
+        const uint32_t li = k - s_off[lo];
         const int wd = s_wd[lo];
         const int ty = s_y0[lo] + (int)(li / (uint32_t)wd);
         const int tx = s_x0[lo] + (int)(li % (uint32_t)wd);
         // entries past the capacity are dropped (the caller sees m > cap)
-        const uint32_t dst = zbase + o0 + k;
+        const uint32_t dst = zbase + k;
         if (dst < limit) {
             tiles_out[dst] = (uint32_t)(ty * tiles_x + tx);
             ids_out[dst] = s_g[lo];
@@ -540,35 +661,49 @@ __global__ __launch_bounds__(256) void k_bin_emit(
 }
 
 // Total intersect count sum(max(counts, 0)) into *m_total (zeroed beforehand by
-// a fill kernel): per-lane partial sums, a wave reduction, one atomic per wave.
+// a fill kernel): per-lane partial sums, a workgroup reduction, one atomic per
+// workgroup (launched with at most kCountBlocks workgroups, so the atomics on
+// the one address stay few).
+constexpr int kCountBlocks = 256;
 __global__ __launch_bounds__(256) void k_count_total(int n, const int* __restrict__ counts,
                                                      uint32_t* __restrict__ m_total) {
+    __shared__ uint32_t wsum[256 / kWave];
     uint32_t acc = 0;
     for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x)
         acc += (uint32_t)max(counts[g], 0);
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
-    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(m_total, acc);
+    if ((threadIdx.x & (kWave - 1)) == 0) wsum[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < 256 / kWave; ++w) t += wsum[w];
+        atomicAdd(m_total, t);
+    }
 }
 
 // Slots [0, cap) of the (tile, id) list before the tile sort: the zero slots
 // (tile 0, id 0) first, then the emitted entries (k_bin_emit), and past the
 // intersect total m (if m < cap) sentinel entries with tile = num_tiles that
-// sort last and belong to no bin.
+// sort last and belong to no bin.  Only the zero and sentinel ranges are
+// written: a grid-stride walk over their concatenation.
 __global__ __launch_bounds__(256) void k_zero_slots(int cap, const uint32_t* __restrict__ m_total,
                                                     const uint32_t* __restrict__ total_written,
                                                     int num_tiles, uint32_t* __restrict__ tiles_out,
                                                     uint32_t* __restrict__ ids_out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cap) return;
-    const uint32_t m = *m_total, tw = *total_written;
-    const uint32_t z = tw <= m ? m - tw : 0u;
-    if ((uint32_t)i < z) {
-        tiles_out[i] = 0u;
-        ids_out[i] = 0u;
-    } else if ((uint32_t)i >= m) {
-        tiles_out[i] = (uint32_t)num_tiles;
-        ids_out[i] = 0u;
+    const uint32_t m = *m_total, tw = *total_written, c = (uint32_t)cap;
+    const uint32_t z = min(tw <= m ? m - tw : 0u, c);
+    const uint32_t tail = m < c ? c - m : 0u;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < z + tail;
+         k += gridDim.x * blockDim.x) {
+        if (k < z) {
+            tiles_out[k] = 0u;
+            ids_out[k] = 0u;
+        } else {
+            tiles_out[m + (k - z)] = (uint32_t)num_tiles;
+            ids_out[m + (k - z)] = 0u;
+        }
     }
 }
 
@@ -579,14 +714,16 @@ static inline int bits_for(int v) {  // bits needed to represent 0..v-1
 }
 
 struct BinLayout {
-    size_t off[8];
+    size_t off[9];
     size_t total;
 };
+
+static int emit_blocks(int cap) { return (cap + kEmitOut - 1) / kEmitOut; }
 
 static BinLayout bin_layout(int n, int m) {
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     BinLayout L;
-    size_t sizes[8] = {
+    size_t sizes[9] = {
         up(4 * (size_t)n) * 2,  // depth keys in/out
         up(4 * (size_t)n) * 2,  // order in/out
         up(4 * (size_t)n),      // written
@@ -594,10 +731,11 @@ static BinLayout bin_layout(int n, int m) {
         up(4 * (size_t)m) * 2,  // tile keys in/out
         up(4 * (size_t)m),      // ids in (pre-sort)
         up(std::max(radix_workspace_bytes<uint32_t>(n), radix_workspace_bytes<uint32_t>(m))),
+        up(4 * ((size_t)emit_blocks(m) + 2)),  // emission block starts
         256,
     };
     size_t acc = 0;
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < 9; ++i) {
         L.off[i] = acc;
         acc += sizes[i];
     }
@@ -621,7 +759,7 @@ hipError_t bin_gaussians(int n, int cap, const float* centers, const float* exte
             if (ws_bytes < L.total) return hipErrorInvalidValue;
             uint32_t* mt = (uint32_t*)((char*)ws + L.off[3]) + n + 1;
             if ((e = fill<uint32_t>(mt, 1, 0u, s)) != hipSuccess) return e;
-            k_count_total<<<std::min((n + 255) / 256, 1024), 256, 0, s>>>(n, counts, mt);
+            k_count_total<<<std::min((n + 255) / 256, kCountBlocks), 256, 0, s>>>(n, counts, mt);
             if ((e = hipGetLastError()) != hipSuccess) return e;
             return hipMemcpyAsync(m_out, mt, sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
         }
@@ -648,7 +786,7 @@ hipError_t bin_gaussians(int n, int cap, const float* centers, const float* exte
     const int gn = (n + 255) / 256, gm = (cap + 255) / 256;
 
     if ((e = fill<uint32_t>(m_total, 1, 0u, s)) != hipSuccess) return e;
-    k_count_total<<<std::min(gn, 1024), 256, 0, s>>>(n, counts, m_total);
+    k_count_total<<<std::min(gn, kCountBlocks), 256, 0, s>>>(n, counts, m_total);
     k_bin_prepare<<<gn, 256, 0, s>>>(n, centers, extents, depths, counts, tiles_x, tiles_y, bw,
                                      dkey, ord, written);
     if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -658,9 +796,12 @@ hipError_t bin_gaussians(int n, int cap, const float* centers, const float* exte
     if ((e = scan_exclusive(offs, n, total, (uint32_t*)(b + L.off[3] + up(4 * (size_t)n + 8)),
                             s)) != hipSuccess)
         return e;
-    k_zero_slots<<<gm, 256, 0, s>>>(cap, m_total, total, num_tiles, tkey, tid_in);
-    k_bin_emit<<<gn, 256, 0, s>>>(n, cap, m_total, ord2, offs, total, centers, extents, tiles_x,
-                                  tiles_y, bw, written, tkey, tid_in);
+    k_zero_slots<<<std::min(gm, 1024), 256, 0, s>>>(cap, m_total, total, num_tiles, tkey, tid_in);
+    const int eb = emit_blocks(cap);
+    uint32_t* starts = (uint32_t*)(b + L.off[7]);
+    k_bin_mark_starts<<<gn, 256, 0, s>>>(n, offs, total, eb, starts);
+    k_bin_emit<<<eb, 256, 0, s>>>(n, cap, m_total, ord2, offs, total, starts, centers, extents,
+                                  tiles_x, tiles_y, bw, tkey, tid_in);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // the sentinel tile (num_tiles) needs one more key value
     e = radix_sort_pairs<uint32_t>(cap, tkey, tid_in, tkey2, (uint32_t*)ids_sorted,

@@ -65,7 +65,9 @@ __global__ __launch_bounds__(256) void k_texture_edit(RasterArgs a, EditIn ed) {
             const Splat s = splat_record(a, g);
             bool hit = false;
             float wt[4] = {0.f, 0.f, 0.f, 0.f};
-            int row[4] = {0, 0, 0, 0};
+            int row[4] = {0, 0, 0, 0};  // byte offsets inside the chart (float values)
+            // out has the texture's (X, C) float layout; charts outside it get nothing
+            const Chart ch = chart_of<false>(ed.out, s, C, a.num_texels);
             if (!done) {
                 const Pair p = pair_eval(s, pr, BLUR, a.precise_edge);
                 const float next_T = T * (1.f - p.alpha);
@@ -76,12 +78,13 @@ __global__ __launch_bounds__(256) void k_texture_edit(RasterArgs a, EditIn ed) {
                         hit = true;
                         float u, v, fu, fv;
                         splat_uv(s, p.delta, u, v);
-                        chart_taps(s, u, v, true, a.num_texels, wt, row, fu, fv);
+                        chart_taps(s, ch, u, v, true, wt, row, fu, fv);
                     }
                     T = next_T;
                 }
             }
-            if (!__any(hit)) continue;
+            if (!__any(hit) || !ch.ok) continue;
+            float* const out_chart = ed.out + (size_t)s.start * C;
             const QuadMerge qm = quad_merge_setup(hit ? row[0] : -1, lane);
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
@@ -90,7 +93,7 @@ __global__ __launch_bounds__(256) void k_texture_edit(RasterArgs a, EditIn ed) {
                 for (int t = 0; t < 4; ++t) q[t] = quad_merge(wt[t] * val[k], qm);
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
-                    if (q[t] != 0.f) unsafeAtomicAdd(ed.out + (size_t)row[t] * C + k, q[t]);
+                    if (q[t] != 0.f) unsafeAtomicAdd(out_chart + (row[t] >> 2) + k, q[t]);
             }
         }
     }

@@ -19,9 +19,9 @@ constexpr int kWave = 64;
 // and read (wave-uniformly) by the raster kernels.  All 32 slots are 32-bit.
 enum RecordSlot : int {
     R_AX1 = 0,     // 0..2  first in-plane axis
-    R_INV_SX = 3,  // 1 / (scale.x * glob_scale)
+    R_IQ1 = 3,     // is1 * is1, is1 = 1 / (scale.x * glob_scale)
     R_AX2 = 4,     // 4..6  second in-plane axis
-    R_INV_SY = 7,  // 1 / (scale.y * glob_scale)
+    R_IQ2 = 7,     // is2 * is2, is2 = 1 / (scale.y * glob_scale)
     R_AX3 = 8,     // 8..10 splat normal
     R_OPAC = 11,
     R_MEAN = 12,   // 12..14
@@ -34,7 +34,7 @@ enum RecordSlot : int {
     R_TEX_H = 28,  // int bits
     R_TEX_W = 29,  // int bits
     R_TEX_START = 30,
-    R_SPARE = 31,
+    R_NUM = 31,    // ax3 . (mean - camera origin): numerator of the ray/plane t
     kRecordFloats = 32
 };
 

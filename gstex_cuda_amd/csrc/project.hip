@@ -220,9 +220,11 @@ __global__ __launch_bounds__(256) void k_build_records(
     project_pix(fx, fy, cx, cy, pv, pu, pvv);
     float r[kRecordFloats];
     r[R_AX1 + 0] = a1.x; r[R_AX1 + 1] = a1.y; r[R_AX1 + 2] = a1.z;
-    r[R_INV_SX] = 1.f / (scales[3 * i] * glob_scale);
+    const float is1 = 1.f / (scales[3 * i] * glob_scale);
+    const float is2 = 1.f / (scales[3 * i + 1] * glob_scale);
+    r[R_IQ1] = is1 * is1;
     r[R_AX2 + 0] = a2.x; r[R_AX2 + 1] = a2.y; r[R_AX2 + 2] = a2.z;
-    r[R_INV_SY] = 1.f / (scales[3 * i + 1] * glob_scale);
+    r[R_IQ2] = is2 * is2;
     r[R_AX3 + 0] = a3.x; r[R_AX3 + 1] = a3.y; r[R_AX3 + 2] = a3.z;
     r[R_OPAC] = opacities[i];
     r[R_MEAN + 0] = m.x; r[R_MEAN + 1] = m.y; r[R_MEAN + 2] = m.z;
@@ -238,7 +240,8 @@ __global__ __launch_bounds__(256) void k_build_records(
     r[R_TEX_H] = __int_as_float(texture_dims[3 * i]);
     r[R_TEX_W] = __int_as_float(texture_dims[3 * i + 1]);
     r[R_TEX_START] = __int_as_float(texture_dims[3 * i + 2]);
-    r[R_SPARE] = 0.f;
+    const V3 o = v3(c2w[3], c2w[7], c2w[11]);
+    r[R_NUM] = dot3(a3, sub3(m, o));
     float4* dst = reinterpret_cast<float4*>(records + (size_t)i * kRecordFloats);
 #pragma unroll
     for (int k = 0; k < kRecordFloats / 4; ++k)

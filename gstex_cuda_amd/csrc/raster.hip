@@ -16,7 +16,8 @@
 namespace gstex {
 
 // GENERAL = false compiles out the blur (settings bit 9) and NDC distortion
-// (bit 10) code for the common settings; TB selects bfloat16 texels.
+// (bit 10) code for the common settings; TB selects bfloat16 texels.  CM <= 4
+// is the exact channel count; above 4, CM bounds a runtime count.
 template <int CM, bool GENERAL, bool TB>
 __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) {
     const int tile = blockIdx.x;
@@ -30,7 +31,8 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
     const bool use_blur = GENERAL && (a.settings & kSetBlur);
     const bool use_ndc = GENERAL && (a.settings & kSetNdc);
     const bool flip = a.settings & kSetNormalFlip, bilinear = !(a.settings & kSetNearest);
-    const int C = a.channels;
+    const int C = CM <= 4 ? CM : a.channels;
+    constexpr int E = TB ? 2 : 4;
 
     int r0, r1;
     tile_range(a, tile, r0, r1);
@@ -83,17 +85,19 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
                     rgb0 += s.rgb.x * vis;
                     rgb1 += s.rgb.y * vis;
                     rgb2 += s.rgb.z * vis;
-                    V3 hn = s.a3;
-                    if (flip && dot3(pr.r, s.a3) > 0.f) hn = v3(-hn.x, -hn.y, -hn.z);
-                    n0 += vis * hn.x;
-                    n1 += vis * hn.y;
-                    n2 += vis * hn.z;
+                    // normal facing the camera: the flip goes into the weight's
+                    // sign (exact), so the axis stays a scalar operand
+                    const float nvis = (flip && dot3(pr.r, s.a3) > 0.f) ? -vis : vis;
+                    n0 += nvis * s.a3.x;
+                    n1 += nvis * s.a3.y;
+                    n2 += nvis * s.a3.z;
                     const float t_view = p.t * pr.view_depth;
                     float u, v;
                     splat_uv(s, p.delta, u, v);
+                    const Chart ch = chart_of<TB>(a.texture, s, C, a.num_texels);
                     float wt[4], fu, fv;
-                    int row[4];
-                    chart_taps(s, u, v, bilinear, a.num_texels, wt, row, fu, fv);
+                    int off[4];
+                    chart_taps(s, ch, u, v, bilinear, wt, off, fu, fv);
                     // texels in chunks of up to 4 channels, all taps loaded before use
                     constexpr int CK = CM < 4 ? CM : 4;
 #pragma unroll
@@ -102,10 +106,11 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
                         float tv[4][CK];
 #pragma unroll
                         for (int k = 0; k < 4; ++k) {
-                            const int tk = row[k] * C + c0;
 #pragma unroll
                             for (int cc = 0; cc < CK; ++cc)
-                                tv[k][cc] = c0 + cc < C ? texel<TB>(a.texture, tk + cc) : 0.f;
+                                tv[k][cc] = c0 + cc < C
+                                                ? chart_texel<TB>(ch, off[k] + (c0 + cc) * E)
+                                                : 0.f;
                         }
 #pragma unroll
                         for (int cc = 0; cc < CK; ++cc) {
@@ -205,9 +210,10 @@ hipError_t raster_forward_stage(const RasterCall& c, const float* records, float
     const FwdOut o{out_img, out_depth, out_reg, out_texture, out_normal,
                    final_Ts, final_idx, depth_idx, out_reg_s};
     if (a.tiles_x * a.tiles_y <= 0) return hipSuccess;
-    if (c.channels <= 1) return launch_fwd_cm<1>(a, o, s);
-    if (c.channels <= 3) return launch_fwd_cm<3>(a, o, s);
-    if (c.channels <= 4) return launch_fwd_cm<4>(a, o, s);
+    if (c.channels == 1) return launch_fwd_cm<1>(a, o, s);
+    if (c.channels == 2) return launch_fwd_cm<2>(a, o, s);
+    if (c.channels == 3) return launch_fwd_cm<3>(a, o, s);
+    if (c.channels == 4) return launch_fwd_cm<4>(a, o, s);
     if (c.channels <= 8) return launch_fwd_cm<8>(a, o, s);
     if (c.channels <= 16) return launch_fwd_cm<16>(a, o, s);
     return launch_fwd_cm<64>(a, o, s);

@@ -46,7 +46,8 @@ __device__ __forceinline__ int reduce_slot32(int lane) {
 
 // ------------------------------------------------------------------ backward
 // GENERAL = false compiles out the blur (settings bit 9) and NDC distortion
-// (bit 10) code for the common settings; TB selects bfloat16 texels.
+// (bit 10) code for the common settings; TB selects bfloat16 texels.  CM <= 4
+// is the exact channel count; above 4, CM bounds a runtime count.
 template <int CM, bool GENERAL, bool TB>
 __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) {
     const int tile = blockIdx.x;
@@ -61,7 +62,9 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
     const bool use_blur = GENERAL && (a.settings & kSetBlur);
     const bool use_ndc = GENERAL && (a.settings & kSetNdc);
     const bool bilinear = !(a.settings & kSetNearest), prop_uv = a.settings & kSetUvGrad;
-    const int C = a.channels;
+    const int C = CM <= 4 ? CM : a.channels;
+    constexpr int E = TB ? 2 : 4;  // bytes per texel value
+    constexpr int kES = TB ? 1 : 2;  // chart byte offset >> kES = value index in the chart
     const int pid = inside ? i * a.img_w + j : 0;
 
     float T = 1.f, S0 = 0.f, S1 = 0.f, S2 = 0.f, vdep = 0.f, vreg = 0.f, vT = 0.f;
@@ -115,15 +118,16 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
         const uint64_t vmask = __ballot(valid);
         if (vmask == 0) return;
 
-        // Transmittance before this Gaussian and the texel taps.
+        // Transmittance before this Gaussian (1 - alpha >= 0.01) and the texel taps.
+        const Chart ch = chart_of<TB>(a.texture, s, C, a.num_texels);
         float wt[4] = {0.f, 0.f, 0.f, 0.f}, fu = 0.f, fv = 0.f, vis = 0.f;
-        int row[4] = {0, 0, 0, 0};
+        int row[4] = {0, 0, 0, 0};  // byte offsets inside the chart
         if (valid) {
-            T *= 1.f / (1.f - p.alpha);
+            T = div_fast(T, 1.f - p.alpha);
             vis = p.alpha * T;
             float u, v;
             splat_uv(s, p.delta, u, v);
-            chart_taps(s, u, v, bilinear, a.num_texels, wt, row, fu, fv);
+            chart_taps(s, ch, u, v, bilinear, wt, row, fu, fv);
         }
         // How the texel gradients reach HBM (wave-uniform choice):
         //   single  1x1 chart, C <= 4: C values in the reduction
@@ -132,16 +136,16 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
         //   lds     chart of <= kTexAccFloats values: LDS accumulation + flush
         //   direct  one atomic per tap
         const int first = __ffsll((unsigned long long)vmask) - 1;
-        const int hw = s.h * s.w;
-        const bool tex_single = hw == 1 && C >= 1 && C <= 4 && s.start >= 0 && s.start < a.num_texels;
+        const int hw = ch.ok ? s.h * s.w : 0;
+        const bool tex_single = hw == 1 && C >= 1 && C <= 4;
         const int row0 = __builtin_amdgcn_readlane(row[0], first);
         const int row1 = __builtin_amdgcn_readlane(row[1], first);
         const int row2 = __builtin_amdgcn_readlane(row[2], first);
         const int row3 = __builtin_amdgcn_readlane(row[3], first);
-        const bool quad32 = !tex_single && C >= 1 && C <= 3 && !use_blur &&
+        const bool quad32 = ch.ok && !tex_single && C >= 1 && C <= 3 && !use_blur &&
                             !__any(valid && row[0] != row0);
-        const bool tex_lds = !tex_single && !quad32 && C > 0 && hw * C <= kTexAccFloats &&
-                             s.start >= 0 && s.start + hw <= a.num_texels;
+        const bool tex_lds = ch.ok && !tex_single && !quad32 && C > 0 && hw * C <= kTexAccFloats;
+        float* const vtex_chart = b.v_texture + (size_t)(ch.ok ? s.start : 0) * C;
 
         float gr[kGradFloats];
 #pragma unroll
@@ -161,10 +165,10 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
                 float tv[4][CK];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    const int tk = row[k] * C + cb;
 #pragma unroll
                     for (int cc = 0; cc < CK; ++cc)
-                        tv[k][cc] = cb + cc < C ? texel<TB>(a.texture, tk + cc) : 0.f;
+                        tv[k][cc] = cb + cc < C ? chart_texel<TB>(ch, row[k] + (cb + cc) * E)
+                                                : 0.f;
                 }
 #pragma unroll
                 for (int cc = 0; cc < CK; ++cc) {
@@ -185,15 +189,14 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
 #pragma unroll
                             for (int k = 0; k < 4; ++k) {
                                 const float add = wt[k] * vval;
-                                if (add != 0.f) atomicAdd(acc + (row[k] - s.start) * C + c, add);
+                                if (add != 0.f) atomicAdd(acc + (row[k] >> kES) + c, add);
                             }
                         }
-                    } else {
+                    } else if (ch.ok) {
 #pragma unroll
                         for (int k = 0; k < 4; ++k) {
                             const float add = wt[k] * vval;
-                            if (add != 0.f)
-                                unsafeAtomicAdd(b.v_texture + (size_t)row[k] * C + c, add);
+                            if (add != 0.f) unsafeAtomicAdd(vtex_chart + (row[k] >> kES) + c, add);
                         }
                     }
                     if (bilinear && prop_uv) {
@@ -224,8 +227,8 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
 
             gr[G_OPAC] = (p.blur_sel ? p.Gb : p.G) * valpha;
             const float vsigma = p.blur_sel ? 0.f : -s.opac * p.G * valpha;
-            const float vl1 = vsigma * s.is1 * s.is1 * p.l1;
-            const float vl2 = vsigma * s.is2 * s.is2 * p.l2;
+            const float vl1 = vsigma * s.iq1 * p.l1;
+            const float vl2 = vsigma * s.iq2 * p.l2;
             const V3 vdelta = v3(vl1 * s.a1.x + vl2 * s.a2.x + s.umap.x * vu + s.vmap.x * vv,
                                  vl1 * s.a1.y + vl2 * s.a2.y + s.umap.y * vu + s.vmap.y * vv,
                                  vl1 * s.a1.z + vl2 * s.a2.z + s.umap.z * vu + s.vmap.z * vv);
@@ -260,7 +263,7 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
                 const QuadMerge qm = quad_merge_setup(valid ? row[0] : -1, lane);
                 int lofs[4];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) lofs[k] = (row[k] - s.start) * C;
+                for (int k = 0; k < 4; ++k) lofs[k] = row[k] >> kES;
 #pragma unroll
                 for (int c = 0; c < CM; ++c) {
                     if (c >= C) break;
@@ -280,12 +283,11 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
             if (slot < G_EXTRA || (use_blur && slot >= G_PIX)) {
                 unsafeAtomicAdd(b.grad_rec + (size_t)g * kGradFloats + slot, tot);
             } else if (tex_single) {
-                if (slot - G_EXTRA < C)
-                    unsafeAtomicAdd(b.v_texture + (size_t)s.start * C + (slot - G_EXTRA), tot);
+                if (slot - G_EXTRA < C) unsafeAtomicAdd(vtex_chart + (slot - G_EXTRA), tot);
             } else if (quad32) {
                 const int t = slot - G_EXTRA, k = t / 3, c = t - 3 * k;
                 const int rr = k == 0 ? row0 : k == 1 ? row1 : k == 2 ? row2 : row3;
-                if (c < C) unsafeAtomicAdd(b.v_texture + (size_t)rr * C + c, tot);
+                if (c < C) unsafeAtomicAdd(vtex_chart + (rr >> kES) + c, tot);
             }
         }
         if (tex_lds) {
@@ -299,11 +301,11 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
             rhi = wave_max(rhi);
             // every lane's LDS adds precede these reads in the wave's LDS queue
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            const int n_acc = (rhi + 1 - s.start) * C;
-            for (int k = (rlo - s.start) * C + lane; k < n_acc; k += kWave) {
+            const int n_acc = (rhi >> kES) + C;  // past the last value of the last touched texel
+            for (int k = (rlo >> kES) + lane; k < n_acc; k += kWave) {
                 const float v = acc[k];
                 if (v != 0.f) {
-                    unsafeAtomicAdd(b.v_texture + (size_t)s.start * C + k, v);
+                    unsafeAtomicAdd(vtex_chart + k, v);
                     acc[k] = 0.f;
                 }
             }
@@ -432,9 +434,10 @@ hipError_t raster_backward_stage(const RasterCall& c, const float* records,
     const BwdIn b{final_Ts, final_idx, depth_idx, final_s, v_img, v_depth, v_reg, v_alpha,
                   v_texture_out, v_normal, grad_rec, v_texture};
     if (a.tiles_x * a.tiles_y <= 0) return hipSuccess;
-    if (c.channels <= 1) return launch_bwd_cm<1>(a, b, s);
-    if (c.channels <= 3) return launch_bwd_cm<3>(a, b, s);
-    if (c.channels <= 4) return launch_bwd_cm<4>(a, b, s);
+    if (c.channels == 1) return launch_bwd_cm<1>(a, b, s);
+    if (c.channels == 2) return launch_bwd_cm<2>(a, b, s);
+    if (c.channels == 3) return launch_bwd_cm<3>(a, b, s);
+    if (c.channels == 4) return launch_bwd_cm<4>(a, b, s);
     if (c.channels <= 8) return launch_bwd_cm<8>(a, b, s);
     if (c.channels <= 16) return launch_bwd_cm<16>(a, b, s);
     return launch_bwd_cm<64>(a, b, s);

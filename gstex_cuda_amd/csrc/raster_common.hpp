@@ -109,7 +109,7 @@ __device__ __forceinline__ PixelRay pixel_ray(const float* c2w, const float* vm,
 
 struct Splat {
     V3 a1, a2, a3, m, rgb, umap, vmap;
-    float is1, is2, opac, pix_x, pix_y, u0, v0;
+    float iq1, iq2, num, opac, pix_x, pix_y, u0, v0;  // iq = squared inverse scale
     int h, w, start;
 };
 
@@ -130,9 +130,9 @@ template <typename P>
 __device__ __forceinline__ Splat load_splat(P q) {
     Splat s;
     s.a1 = v3(q[R_AX1], q[R_AX1 + 1], q[R_AX1 + 2]);
-    s.is1 = q[R_INV_SX];
+    s.iq1 = q[R_IQ1];
     s.a2 = v3(q[R_AX2], q[R_AX2 + 1], q[R_AX2 + 2]);
-    s.is2 = q[R_INV_SY];
+    s.iq2 = q[R_IQ2];
     s.a3 = v3(q[R_AX3], q[R_AX3 + 1], q[R_AX3 + 2]);
     s.opac = q[R_OPAC];
     s.m = v3(q[R_MEAN], q[R_MEAN + 1], q[R_MEAN + 2]);
@@ -146,7 +146,19 @@ __device__ __forceinline__ Splat load_splat(P q) {
     s.h = __float_as_int(q[R_TEX_H]);
     s.w = __float_as_int(q[R_TEX_W]);
     s.start = __float_as_int(q[R_TEX_START]);
+    s.num = q[R_NUM];
     return s;
+}
+
+// a / b for normal operands of moderate range (no overflow or denormal
+// quotients): hardware reciprocal (1 ulp), quotient, and one residual
+// correction q + (a - q b) r, which gives the IEEE quotient in nearly every
+// case at four instructions instead of the ten of the full divide (whose
+// scaling steps only matter near the exponent limits).
+__device__ __forceinline__ float div_fast(float a, float b) {
+    const float r = __builtin_amdgcn_rcpf(b);
+    const float q = a * r;
+    return fmaf(fmaf(-q, b, a), r, q);
 }
 
 struct Pair {
@@ -182,13 +194,15 @@ __device__ __forceinline__ Pair pair_eval(const Splat& s, const PixelRay& pr, bo
         p.l1 = (float)(dx * s.a1.x + dy * s.a1.y + dz * s.a1.z);
         p.l2 = (float)(dx * s.a2.x + dy * s.a2.y + dz * s.a2.z);
     } else {
-        p.t = dot3(s.a3, sub3(s.m, pr.o)) / den;
+        // The numerator n . (m - o) is per splat and view (built with the
+        // record); |den| >= 1e-6 keeps the quotient far from overflow.
+        p.t = div_fast(s.num, den);
         const V3 pos = v3(pr.o.x + p.t * pr.r.x, pr.o.y + p.t * pr.r.y, pr.o.z + p.t * pr.r.z);
         p.delta = sub3(pos, s.m);
         p.l1 = dot3(p.delta, s.a1);
         p.l2 = dot3(p.delta, s.a2);
     }
-    const float sigma = 0.5f * (s.is1 * s.is1 * p.l1 * p.l1 + s.is2 * s.is2 * p.l2 * p.l2);
+    const float sigma = 0.5f * (s.iq1 * p.l1 * p.l1 + s.iq2 * p.l2 * p.l2);
     p.G = __expf(-sigma);
     p.Gb = 0.f;
     p.blur_sel = false;
@@ -203,18 +217,65 @@ __device__ __forceinline__ Pair pair_eval(const Splat& s, const PixelRay& pr, bo
     return p;
 }
 
-// Bilinear (or nearest) taps of one chart at clamped (u, v): weights and the
-// texel rows (start + i*w + j), clamped into [0, X) for memory safety.
-__device__ __forceinline__ void chart_taps(const Splat& s, float u, float v, bool bilinear,
-                                           int num_texels, float wt[4], int row[4], float& fu,
+// Texels of one chart (wave-uniform), read through a raw buffer resource over
+// the chart's h * w * C values: taps are byte offsets inside the chart, and a
+// read past the resource returns 0 (the hardware range check), so no per-tap
+// clamping is needed for memory safety.  A chart that does not lie inside the
+// texture (start < 0, h or w outside [1, 2^24), start + h * w > num_texels) or
+// that spans 2^31 bytes or more gets an empty resource: its texels read as 0
+// and it receives no texel gradient (the reference indexes out of bounds
+// there).  Descriptor word 3 is the gfx9-family raw-buffer format (32-bit data
+// format, so the range check applies).
+constexpr int kBufferWord3 = 0x00020000;
+
+struct Chart {
+    __amdgpu_buffer_rsrc_t rsrc;
+    int ce;   // bytes per texel (C values)
+    int wce;  // bytes per chart row (w texels)
+    bool ok;
+};
+
+template <bool TB>
+__device__ __forceinline__ Chart chart_of(const void* texture, const Splat& s, int C,
+                                          int num_texels) {
+    constexpr int E = TB ? 2 : 4;
+    Chart ch;
+    const long long n = (long long)s.h * (long long)s.w;
+    ch.ce = C * E;
+    ch.ok = s.start >= 0 && s.h > 0 && s.w > 0 && s.h < (1 << 24) && s.w < (1 << 24) &&
+            (long long)s.start + n <= (long long)num_texels && n * ch.ce < 0x80000000ll;
+    const char* base = static_cast<const char*>(texture) + (ch.ok ? (size_t)s.start * ch.ce : 0);
+    ch.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(base), (short)0,
+                                                ch.ok ? (int)(n * ch.ce) : 0, kBufferWord3);
+    ch.wce = s.w * ch.ce;
+    return ch;
+}
+
+// Texel value at byte offset off of a chart (bfloat16 widened).
+template <bool TB>
+__device__ __forceinline__ float chart_texel(const Chart& ch, int off) {
+    if constexpr (TB) {
+        const unsigned short h = __builtin_amdgcn_raw_buffer_load_b16(ch.rsrc, off, 0, 0);
+        return __uint_as_float((unsigned)h << 16);
+    } else {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ch.rsrc, off, 0, 0));
+    }
+}
+
+// Bilinear (or nearest) taps of one chart at clamped (u, v) in [0, 1]: weights
+// and the byte offsets of texels (i, j) = (iu, iv), (iu, jv), (ju, iv),
+// (ju, jv) inside the chart, with the replicate-padded neighbour indices
+// (texture_helpers.cuh:162-215).  For a chart inside the texture tu = h u
+// lies in [0, h], so only the upper clamps can act.
+__device__ __forceinline__ void chart_taps(const Splat& s, const Chart& ch, float u, float v,
+                                           bool bilinear, float wt[4], int off[4], float& fu,
                                            float& fv) {
     const float tu = (float)s.h * u, tv = (float)s.w * v;
-    int iu = f2i_trunc(tu), iv = f2i_trunc(tv);
-    fu = tu - (float)iu;
-    fv = tv - (float)iv;
-    const int ju = max(min(iu + 1, s.h - 1), 0), jv = max(min(iv + 1, s.w - 1), 0);
-    iu = max(min(iu, s.h - 1), 0);
-    iv = max(min(iv, s.w - 1), 0);
+    const int iu0 = f2i_trunc(tu), iv0 = f2i_trunc(tv);
+    fu = tu - (float)iu0;
+    fv = tv - (float)iv0;
+    const int iu = min(iu0, s.h - 1), iv = min(iv0, s.w - 1);
+    const int ju = min(iu0 + 1, s.h - 1), jv = min(iv0 + 1, s.w - 1);
     const float wii = (1.f - fu) * (1.f - fv), wij = (1.f - fu) * fv, wji = fu * (1.f - fv),
                 wjj = fu * fv;
     if (bilinear) {
@@ -226,11 +287,12 @@ __device__ __forceinline__ void chart_taps(const Splat& s, float u, float v, boo
         else if (wji >= wii && wji >= wij && wji >= wjj) wt[2] = 1.f;
         else wt[3] = 1.f;
     }
-    const int lim = num_texels - 1;
-    row[0] = min(max(s.start + iu * s.w + iv, 0), lim);
-    row[1] = min(max(s.start + iu * s.w + jv, 0), lim);
-    row[2] = min(max(s.start + ju * s.w + iv, 0), lim);
-    row[3] = min(max(s.start + ju * s.w + jv, 0), lim);
+    // iu * wce < h * w * C * E < 2^31; iv < 2^24 and ce <= 256 fit the 24-bit multiply
+    off[0] = iu * ch.wce + (int)__umul24((unsigned)iv, (unsigned)ch.ce);
+    const int dv = jv != iv ? ch.ce : 0;
+    off[1] = off[0] + dv;
+    off[2] = off[0] + (ju != iu ? ch.wce : 0);
+    off[3] = off[2] + dv;
 }
 
 __device__ __forceinline__ void splat_uv(const Splat& s, V3 delta, float& u, float& v) {
@@ -342,14 +404,14 @@ __device__ __forceinline__ void pin_s(float a, float b, float c, float d) {
 // Record of Gaussian g (wave-uniform) as scalar loads, all in flight at once.
 __device__ __forceinline__ Splat splat_record(const RasterArgs& a, int g) {
     const Splat s = load_splat(as_const_space(a.records) + (size_t)g * kRecordFloats);
-    pin_s(s.a1.x, s.a1.y, s.a1.z, s.is1);
-    pin_s(s.a2.x, s.a2.y, s.a2.z, s.is2);
+    pin_s(s.a1.x, s.a1.y, s.a1.z, s.iq1);
+    pin_s(s.a2.x, s.a2.y, s.a2.z, s.iq2);
     pin_s(s.a3.x, s.a3.y, s.a3.z, s.opac);
     pin_s(s.m.x, s.m.y, s.m.z, s.rgb.x);
     pin_s(s.rgb.y, s.rgb.z, s.u0, s.v0);
     pin_s(s.umap.x, s.umap.y, s.umap.z, s.vmap.x);
     pin_s(s.vmap.y, s.vmap.z, __int_as_float(s.h), __int_as_float(s.w));
-    pin_s(__int_as_float(s.start), s.pix_x, s.pix_y, 0.f);
+    pin_s(__int_as_float(s.start), s.pix_x, s.pix_y, s.num);
     return s;
 }
 

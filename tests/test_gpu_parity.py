@@ -112,6 +112,16 @@ def test_binning_c2_scale(oracle):
     assert _binning_case(oracle, s, 16) > 1_000_000
 
 
+def test_binning_screen_sized_splats(oracle):
+    # the twelve nearest splats scaled up to 5K-8K tiles each (nine cover all
+    # 8160 tiles of the 1080p screen): one Gaussian's entries span several
+    # emission blocks and the first blocks hold only a few Gaussians
+    s = scene(2, num_points=20_000, seed=77)
+    depth = (s.means @ s.viewmat[:3, :3].T + s.viewmat[:3, 3])[:, 2]
+    s.scales[np.argsort(depth)[:12]] *= 60.0
+    assert _binning_case(oracle, s, 16) > 500_000
+
+
 def test_sort_full_64bit_signed():
     from gstex_cuda_amd import cuda as _C
 

@@ -102,8 +102,9 @@ def test_pipeline_matches_oracle(oracle, case):
         assert psnr(fwd[k], f[k]) >= 50.0 or np.mean(err < 1e-4) >= 0.999, k
     g = O.texture_backward(sc, f, cot["img"], cot["depth"], cot["reg"], cot["alpha"], cot["tex"],
                            cot["normal"], "f64")
-    for k in GRAD_NAMES:
-        ok, l2, fr = grad_close(grads[k], g[k].reshape(grads[k].shape))
+    res = {k: grad_close(grads[k], g[k].reshape(grads[k].shape)) for k in GRAD_NAMES}
+    log("  rel L2 vs f64 oracle: " + " ".join(f"{k}={r[1]:.2e}" for k, r in res.items()))
+    for k, (ok, l2, fr) in res.items():
         assert ok, (k, l2, fr)
 
 
