@@ -109,7 +109,11 @@ hipError_t texture_edit(const RasterCall& c, const float* upd_img, const float* 
     if (e != hipSuccess) return e;
     float* records = (float*)ws;
     if ((e = build_raster_records(c, records, s)) != hipSuccess) return e;
-    const RasterArgs a = make_raster_args(c, records);
+    RasterArgs a = make_raster_args(c, records);
+    if (c.settings & 1) {  // the edit kernel's blur flag: the box covers the blur disc
+        a.ellipses = nullptr;
+        a.ellipse_c = nullptr;
+    }
     const int tiles = a.tiles_x * a.tiles_y;
     if (tiles <= 0) return hipSuccess;
     const EditIn ed{upd_img, upd_alpha, depth_lower, depth_upper, out};

@@ -24,7 +24,7 @@ enum RecordSlot : int {
     R_IQ2 = 7,     // is2 * is2, is2 = 1 / (scale.y * glob_scale)
     R_AX3 = 8,     // 8..10 splat normal
     R_OPAC = 11,
-    R_MEAN = 12,   // 12..14
+    R_MO = 12,     // 12..14 mean - camera origin (per view)
     R_PIX_X = 15,  // projected mean (blur term)
     R_RGB = 16,    // 16..18
     R_PIX_Y = 19,

@@ -140,16 +140,36 @@ hipError_t grad_finalize(int n, const float* grad_rec, const float* records, con
                          float fy, float* v_colors, float* v_opacity, float* v_means,
                          float* v_scales, float* v_quats, float* v_uv0, float* v_umap,
                          float* v_vmap, hipStream_t s);
+// Per-view splat data, one buffer: n records (128 B), n screen ellipses
+// (float4 centre x, y, A, B: 16 B), n screen boxes (8 B), n ellipse C values
+// (4 B).  Each array starts at a multiple of its element size.
 constexpr size_t kRecordBytes = 128;
+constexpr size_t kCullConicBytes = 16;
 constexpr size_t kCullBoxBytes = 8;
+constexpr size_t kCullConicCBytes = 4;
 inline size_t splat_records_bytes(int n) {
-    return n > 0 ? (size_t)n * (kRecordBytes + kCullBoxBytes) : 0;
+    return n > 0 ? (size_t)n * (kRecordBytes + kCullConicBytes + kCullBoxBytes + kCullConicCBytes)
+                 : 0;
+}
+__host__ __device__ inline float4* record_conics(float* records, int n) {
+    return reinterpret_cast<float4*>(reinterpret_cast<char*>(records) + (size_t)n * kRecordBytes);
+}
+__host__ __device__ inline const float4* record_conics(const float* records, int n) {
+    return record_conics(const_cast<float*>(records), n);
 }
 __host__ __device__ inline uint2* record_boxes(float* records, int n) {
-    return reinterpret_cast<uint2*>(records + (size_t)n * (kRecordBytes / sizeof(float)));
+    return reinterpret_cast<uint2*>(reinterpret_cast<char*>(records) +
+                                    (size_t)n * (kRecordBytes + kCullConicBytes));
 }
 __host__ __device__ inline const uint2* record_boxes(const float* records, int n) {
-    return reinterpret_cast<const uint2*>(records + (size_t)n * (kRecordBytes / sizeof(float)));
+    return record_boxes(const_cast<float*>(records), n);
+}
+__host__ __device__ inline float* record_conic_c(float* records, int n) {
+    return reinterpret_cast<float*>(reinterpret_cast<char*>(records) +
+                                    (size_t)n * (kRecordBytes + kCullConicBytes + kCullBoxBytes));
+}
+__host__ __device__ inline const float* record_conic_c(const float* records, int n) {
+    return record_conic_c(const_cast<float*>(records), n);
 }
 constexpr size_t kGradRecordBytes = 128;
 

@@ -128,6 +128,29 @@ __global__ __launch_bounds__(256) void k_project_points(int n, const float* __re
 // Splat records: axes from the (unnormalised) quaternion exactly as the
 // raster loop of texture.cu:166-169 derives them, plus every per-Gaussian
 // field the raster needs, in one 128-byte line.
+// Inverse of a 3x3 matrix by the adjugate; false when (nearly) singular.
+__device__ bool inv3d(const double M[3][3], double Mi[3][3]) {
+    const double c00 = M[1][1] * M[2][2] - M[1][2] * M[2][1];
+    const double c01 = M[0][2] * M[2][1] - M[0][1] * M[2][2];
+    const double c02 = M[0][1] * M[1][2] - M[0][2] * M[1][1];
+    const double c10 = M[1][2] * M[2][0] - M[1][0] * M[2][2];
+    const double c11 = M[0][0] * M[2][2] - M[0][2] * M[2][0];
+    const double c12 = M[0][2] * M[1][0] - M[0][0] * M[1][2];
+    const double c20 = M[1][0] * M[2][1] - M[1][1] * M[2][0];
+    const double c21 = M[0][1] * M[2][0] - M[0][0] * M[2][1];
+    const double c22 = M[0][0] * M[1][1] - M[0][1] * M[1][0];
+    const double det = M[0][0] * c00 + M[0][1] * c10 + M[0][2] * c20;
+    double scale = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) scale = fmax(scale, fabs(M[i][j]));
+    if (!(fabs(det) > 1e-12 * scale * scale * scale)) return false;
+    const double id = 1.0 / det;
+    Mi[0][0] = c00 * id; Mi[0][1] = c01 * id; Mi[0][2] = c02 * id;
+    Mi[1][0] = c10 * id; Mi[1][1] = c11 * id; Mi[1][2] = c12 * id;
+    Mi[2][0] = c20 * id; Mi[2][1] = c21 * id; Mi[2][2] = c22 * id;
+    return true;
+}
+
 // Screen box (pixel index ranges, int16 pairs) outside of which the Gaussian
 // cannot pass the raster's alpha test, so a wave whose pixels all lie outside
 // may skip it.  alpha = min(0.99, opac * exp(-sigma)) >= 1/255 needs
@@ -144,7 +167,8 @@ __global__ __launch_bounds__(256) void k_project_points(int n, const float* __re
 // non-finite input; an empty box when opac < 1/255 (never visible).
 __device__ uint2 cull_box(V3 m, V3 a1, V3 a2, V3 a3, float s1, float s2, float opac,
                           const float* c2w, float fx, float fy, float cx, float cy, float pu,
-                          float pv) {
+                          float pv, bool& bounded) {
+    bounded = false;
     const uint2 all = make_uint2(pack_i16(-32768, 32767), pack_i16(-32768, 32767));
     const uint2 none = make_uint2(pack_i16(32767, -32768), pack_i16(32767, -32768));
     // non-finite or zero scales / opacity give NaN alphas (0.99 after the clamp)
@@ -197,8 +221,68 @@ __device__ uint2 cull_box(V3 m, V3 a1, V3 a2, V3 a3, float s1, float s2, float o
         umax = fminf(umax, 30000.f); vmax = fminf(vmax, 30000.f);
         if (!(umin <= umax && vmin <= vmax)) return all;
     }
+    bounded = true;
     return make_uint2(pack_i16((int)floorf(umin) - 2, (int)ceilf(umax) + 2),
                       pack_i16((int)floorf(vmin) - 2, (int)ceilf(vmax) + 2));
+}
+
+// Screen ellipse holding every pixel whose ray can pass the alpha test (blur
+// off), for a per-wave test tighter than the box.  On the splat plane that
+// region is the ellipse (l1 / (r s1))^2 + (l2 / (r s2))^2 <= 1 with r as in
+// cull_box.  Plane coordinates (l1, l2, 1) map to homogeneous pixel
+// coordinates by G = K Rinv [a1 a2 m-o] (the raster's ray model: pixel
+// (fx x / z + cx, fy y / z + cy) for camera-frame point (x, y, z)), so pixel p
+// lies on a ray line meeting the ellipse iff
+//   [p 1] G^-T diag(q1, q2, -1) G^-1 [p 1]^T <= 0,   q = 1 / (r s)^2.
+// When that conic is a bounded ellipse it is returned as centre (x, y) and
+// normalised shape (A, B, C): (p - c)^T [[A, B], [B, C]] (p - c) <= 1, in
+// continuous pixel coordinates (pixel index j has centre j + 0.5).  Float64
+// throughout.  A = -1 means "use the box only": cull_box found no bound, the
+// conic is not an ellipse (the region reaches the camera's vanishing line),
+// its eigenvalue ratio exceeds 4e4 (a float32 evaluation of the quadratic
+// would lose the 2% radius margin to cancellation), or a value is not finite.
+// The raster's own float32 error, mapped to the screen, is about 1e-7 * focal
+// pixels, which the 0.1-pixel pad of the wave rectangle covers.
+__device__ void cull_conic(V3 m, V3 a1, V3 a2, float s1, float s2, float opac, const float* c2w,
+                           float fx, float fy, float cx, float cy, float4& ell, float& ell_c) {
+    ell = make_float4(0.f, 0.f, -1.f, 0.f);
+    ell_c = 0.f;
+    double R[3][3] = {{c2w[0], c2w[1], c2w[2]}, {c2w[4], c2w[5], c2w[6]},
+                      {c2w[8], c2w[9], c2w[10]}};
+    double Ri[3][3];
+    if (!inv3d(R, Ri)) return;
+    const double mo[3] = {(double)m.x - c2w[3], (double)m.y - c2w[7], (double)m.z - c2w[11]};
+    const double cols[3][3] = {{a1.x, a1.y, a1.z}, {a2.x, a2.y, a2.z}, {mo[0], mo[1], mo[2]}};
+    double G[3][3];
+    for (int j = 0; j < 3; ++j) {
+        double x[3];
+        for (int i = 0; i < 3; ++i)
+            x[i] = Ri[i][0] * cols[j][0] + Ri[i][1] * cols[j][1] + Ri[i][2] * cols[j][2];
+        G[0][j] = fx * x[0] + cx * x[2];
+        G[1][j] = fy * x[1] + cy * x[2];
+        G[2][j] = x[2];
+    }
+    double Gi[3][3];
+    if (!inv3d(G, Gi)) return;
+    const double r = 1.02 * sqrt(2.0 * log(255.0 * (double)opac)) + 1e-3;
+    const double d[3] = {1.0 / (r * s1 * r * s1), 1.0 / (r * s2 * r * s2), -1.0};
+    double Q[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            Q[i][j] = d[0] * Gi[0][i] * Gi[0][j] + d[1] * Gi[1][i] * Gi[1][j] +
+                      d[2] * Gi[2][i] * Gi[2][j];
+    const double det = Q[0][0] * Q[1][1] - Q[0][1] * Q[0][1];
+    if (!(Q[0][0] > 0.0 && Q[1][1] > 0.0 && det > 1e-4 * Q[0][0] * Q[1][1])) return;
+    const double c0 = -(Q[1][1] * Q[0][2] - Q[0][1] * Q[1][2]) / det;
+    const double c1 = -(Q[0][0] * Q[1][2] - Q[0][1] * Q[0][2]) / det;
+    const double f = Q[2][2] + Q[0][2] * c0 + Q[1][2] * c1;
+    if (!(f < 0.0)) return;
+    const double A = Q[0][0] / -f, B = Q[0][1] / -f, C = Q[1][1] / -f;
+    if (!(fabs(c0) < 1e7 && fabs(c1) < 1e7 && A < 1e30 && C < 1e30 && fabs(B) < 1e30 &&
+          A > 1e-30 && C > 1e-30))
+        return;
+    ell = make_float4((float)c0, (float)c1, (float)A, (float)B);
+    ell_c = (float)C;
 }
 
 __global__ __launch_bounds__(256) void k_build_records(
@@ -208,7 +292,8 @@ __global__ __launch_bounds__(256) void k_build_records(
     const float* __restrict__ umap, const float* __restrict__ vmap,
     const int* __restrict__ texture_dims, const float* __restrict__ viewmat,
     const float* __restrict__ c2w, float fx, float fy, float cx, float cy,
-    float* __restrict__ records, uint2* __restrict__ boxes) {
+    float* __restrict__ records, uint2* __restrict__ boxes, float4* __restrict__ ellipses,
+    float* __restrict__ ellipse_c) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 q = reinterpret_cast<const float4*>(quats)[i];
@@ -227,7 +312,9 @@ __global__ __launch_bounds__(256) void k_build_records(
     r[R_IQ2] = is2 * is2;
     r[R_AX3 + 0] = a3.x; r[R_AX3 + 1] = a3.y; r[R_AX3 + 2] = a3.z;
     r[R_OPAC] = opacities[i];
-    r[R_MEAN + 0] = m.x; r[R_MEAN + 1] = m.y; r[R_MEAN + 2] = m.z;
+    const V3 o = v3(c2w[3], c2w[7], c2w[11]);
+    const V3 mo = sub3(m, o);
+    r[R_MO + 0] = mo.x; r[R_MO + 1] = mo.y; r[R_MO + 2] = mo.z;
     r[R_PIX_X] = pu;
     // colours may be absent (texture_edit does not read them)
     r[R_RGB + 0] = colors ? colors[3 * i] : 0.f;
@@ -240,14 +327,19 @@ __global__ __launch_bounds__(256) void k_build_records(
     r[R_TEX_H] = __int_as_float(texture_dims[3 * i]);
     r[R_TEX_W] = __int_as_float(texture_dims[3 * i + 1]);
     r[R_TEX_START] = __int_as_float(texture_dims[3 * i + 2]);
-    const V3 o = v3(c2w[3], c2w[7], c2w[11]);
-    r[R_NUM] = dot3(a3, sub3(m, o));
+    r[R_NUM] = dot3(a3, mo);
     float4* dst = reinterpret_cast<float4*>(records + (size_t)i * kRecordFloats);
 #pragma unroll
     for (int k = 0; k < kRecordFloats / 4; ++k)
         dst[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
-    boxes[i] = cull_box(m, a1, a2, a3, scales[3 * i] * glob_scale, scales[3 * i + 1] * glob_scale,
-                        r[R_OPAC], c2w, fx, fy, cx, cy, pu, pvv);
+    const float s1 = scales[3 * i] * glob_scale, s2 = scales[3 * i + 1] * glob_scale;
+    bool bounded;
+    boxes[i] = cull_box(m, a1, a2, a3, s1, s2, r[R_OPAC], c2w, fx, fy, cx, cy, pu, pvv, bounded);
+    float4 ell = make_float4(0.f, 0.f, -1.f, 0.f);
+    float ell_c = 0.f;
+    if (bounded) cull_conic(m, a1, a2, s1, s2, r[R_OPAC], c2w, fx, fy, cx, cy, ell, ell_c);
+    ellipses[i] = ell;
+    ellipse_c[i] = ell_c;
 }
 
 hipError_t launch_get_aabb_2d(int n, const float* means, const float* scales, float glob_scale,
@@ -299,7 +391,8 @@ hipError_t launch_build_records(int n, const float* means, const float* scales, 
     if (n <= 0) return hipSuccess;
     k_build_records<<<(n + 255) / 256, 256, 0, stream>>>(
         n, means, scales, glob_scale, quats, colors, opacities, uv0, umap, vmap, texture_dims,
-        viewmat, c2w, fx, fy, cx, cy, records, record_boxes(records, n));
+        viewmat, c2w, fx, fy, cx, cy, records, record_boxes(records, n),
+        record_conics(records, n), record_conic_c(records, n));
     return hipGetLastError();
 }
 

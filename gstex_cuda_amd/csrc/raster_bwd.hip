@@ -2,9 +2,9 @@
 // and the per-Gaussian gradient finalize.
 //
 // Per (wave, Gaussian) the lanes' contributions are summed with one butterfly
-// reduce-scatter over 32 values, whose first rounds are DPP lane permutes (no
-// LDS traffic); lane L then holds the wave total of one value and issues its
-// atomic.  The 32 values are 20 per-Gaussian scalars from which the finalize
+// reduce-scatter over 32 values built from permlane swaps and DPP lane
+// permutes (all VALU, no LDS traffic); every other lane then holds the wave
+// total of one value and issues its atomic.  The 32 values are 20 per-Gaussian scalars from which the finalize
 // kernel rebuilds every parameter gradient, plus up to 12 texel-gradient
 // values.  Using delta = l1*ax1 + l2*ax2 (the ray hits the splat plane), each
 // outer product sum_pixels v * delta becomes ax1 * sum(v*l1) + ax2 * sum(v*l2).
@@ -24,24 +24,38 @@ __device__ __forceinline__ void halve(float (&v)[NV], bool upper) {
     }
 }
 
-// Reduce-scatter 32 per-lane values over the wave.  The partner masks 15, 7,
-// 2, 1 (DPP), 16 (swizzle) and 32 (bpermute) span all six lane bits, so every
-// result is a sum over all 64 lanes.  Lane L ends with value
-//   16*b3 + 8*b2 + 4*b1 + 2*b0 + b4      (b_i = bit i of L)
-__device__ __forceinline__ float wave_reduce_scatter(float (&v)[32], int lane) {
-    halve<kDppRowMirror, 32, 16>(v, lane & 8);
-    halve<kDppRowHalfMirror, 32, 8>(v, lane & 4);
-    halve<kDppQuadXor2, 32, 4>(v, lane & 2);
-    halve<kDppQuadXor1, 32, 2>(v, lane & 1);
-    const float lo = v[0] + swizzle_xor16(v[0]);
-    const float hi = v[1] + swizzle_xor16(v[1]);
-    const float r = (lane & 16) ? hi : lo;
-    return r + __shfl_xor(r, 32);
+// One halving round over lane bit 5 (bit 4) by a permlane swap: lanes with
+// the bit clear end with a summed with the partner lane's a, lanes with it set
+// with b summed with the partner's b.  The swap moves the halves each lane
+// gives up in one instruction, so a pair costs a swap and an add.
+__device__ __forceinline__ float swap32_sum(float a, float b) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b),
+                                                    false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float swap16_sum(float a, float b) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b),
+                                                    false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
 }
 
-__device__ __forceinline__ int reduce_slot32(int lane) {
-    return ((lane >> 3) & 1) * 16 + ((lane >> 2) & 1) * 8 + ((lane >> 1) & 1) * 4 +
-           (lane & 1) * 2 + ((lane >> 4) & 1);
+// Reduce-scatter 32 per-lane values over the wave: permlane swaps over lane
+// bits 5 and 4 (32 -> 8 values), DPP halvings with partner masks 15, 7 and 2
+// selecting on bits 3, 2 and 1 (8 -> 1), then a DPP add over bit 0.  Each
+// partner mask leaves the bits already selected on unchanged and together they
+// span all six lane bits, so lane L ends with the wave total of value
+// 16*b5 + 8*b4 + 4*b3 + 2*b2 + b1 = L >> 1 (lanes 2j and 2j+1 both hold j).
+__device__ __forceinline__ float wave_reduce_scatter(float (&v)[32], int lane) {
+    float w[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) w[k] = swap32_sum(v[k], v[k + 16]);
+    float x[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = swap16_sum(w[k], w[k + 8]);
+    halve<kDppRowMirror, 8, 4>(x, lane & 8);
+    halve<kDppRowHalfMirror, 8, 2>(x, lane & 4);
+    halve<kDppQuadXor2, 8, 1>(x, lane & 2);
+    return x[0] + dpp<kDppQuadXor1>(x[0]);
 }
 
 // ------------------------------------------------------------------ backward
@@ -109,26 +123,24 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
 
     // One list entry (wave-uniform index e, Gaussian g with record s).
     auto entry = [&](const int e, const int g, const Splat& s) {
-        bool valid = inside && e <= bfin;
-        Pair p;
-        if (valid) {
-            p = pair_eval(s, pr, use_blur, GENERAL && a.precise_edge);
-            valid = !p.skip;
-        }
+        // The pair math and the taps run for every lane: they are finite for
+        // any pixel ray, and tap offsets stay inside the chart.  A lane that
+        // does not contribute keeps T and vT and gets zero seeds (vis, valpha,
+        // vt, vu, vv), so every gradient value below is zero for it without a
+        // branch or per-value selects.
+        const Pair p = pair_eval(s, pr, use_blur, GENERAL && a.precise_edge);
+        const bool valid = inside && e <= bfin && !p.skip;
         const uint64_t vmask = __ballot(valid);
         if (vmask == 0) return;
 
         // Transmittance before this Gaussian (1 - alpha >= 0.01) and the texel taps.
         const Chart ch = chart_of<TB>(a.texture, s, C, a.num_texels);
-        float wt[4] = {0.f, 0.f, 0.f, 0.f}, fu = 0.f, fv = 0.f, vis = 0.f;
-        int row[4] = {0, 0, 0, 0};  // byte offsets inside the chart
-        if (valid) {
-            T = div_fast(T, 1.f - p.alpha);
-            vis = p.alpha * T;
-            float u, v;
-            splat_uv(s, p.delta, u, v);
-            chart_taps(s, ch, u, v, bilinear, wt, row, fu, fv);
-        }
+        if (valid) T = div_fast(T, 1.f - p.alpha);
+        const float vis = valid ? p.alpha * T : 0.f;
+        float u, v, wt[4], fu, fv;
+        int row[4];  // byte offsets inside the chart
+        splat_uv(s, p.delta, u, v);
+        chart_taps(s, ch, u, v, bilinear, wt, row, fu, fv);
         // How the texel gradients reach HBM (wave-uniform choice):
         //   single  1x1 chart, C <= 4: C values in the reduction
         //   quad32  every valid lane samples the same texel quad, C <= 3, no
@@ -149,8 +161,8 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
 
         float gr[kGradFloats];
 #pragma unroll
-        for (int k = 0; k < kGradFloats; ++k) gr[k] = 0.f;
-        if (valid) {
+        for (int k = G_EXTRA; k < kGradFloats; ++k) gr[k] = 0.f;  // route and blur slots
+        {
             gr[G_RGB + 0] = vis * vo.x;
             gr[G_RGB + 1] = vis * vo.y;
             gr[G_RGB + 2] = vis * vo.z;
@@ -210,6 +222,7 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
                 }
             }
 
+            if (!valid) vu = vv = 0.f;  // NaN-safe zero seeds
             float valpha = T * vvis - T * vT;
             float vTcur = p.alpha * vvis + (1.f - p.alpha) * vT;
             const float t_view = p.t * pr.view_depth;
@@ -218,12 +231,14 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
             const float vw = (dval * dval * S0 - 2.f * dval * S1 + S2) * vreg;
             valpha += vw * T;
             vTcur += vw * p.alpha;
-            vT = vTcur;
+            if (valid) vT = vTcur;
+            valpha = valid ? valpha : 0.f;
 
             float vt = use_ndc ? 0.f : vd_val;
             float vtview = (e == dfin && dfin != -1) ? vdep : 0.f;
             if (use_ndc) vtview += (tf * tn) / ((tf - tn) * t_view * t_view) * vd_val;
             vt += pr.view_depth * vtview;
+            vt = valid ? vt : 0.f;
 
             gr[G_OPAC] = (p.blur_sel ? p.Gb : p.G) * valpha;
             const float vsigma = p.blur_sel ? 0.f : -s.opac * p.G * valpha;
@@ -278,8 +293,8 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
             }
         }
         const float tot = wave_reduce_scatter(gr, lane);
-        const int slot = reduce_slot32(lane);
-        if (lane < 32 && tot != 0.f) {
+        const int slot = lane >> 1;
+        if ((lane & 1) == 0 && tot != 0.f) {
             if (slot < G_EXTRA || (use_blur && slot >= G_PIX)) {
                 unsafeAtomicAdd(b.grad_rec + (size_t)g * kGradFloats + slot, tot);
             } else if (tex_single) {
@@ -357,7 +372,7 @@ __global__ __launch_bounds__(256) void k_grad_finalize(
     }
     const Splat s = load_splat(records + (size_t)g * kRecordFloats);
     const V3 o = v3(c2w[3], c2w[7], c2w[11]);
-    const V3 mo = sub3(s.m, o);
+    const V3 mo = s.mo;
     v_colors[3 * g] = q[G_RGB];
     v_colors[3 * g + 1] = q[G_RGB + 1];
     v_colors[3 * g + 2] = q[G_RGB + 2];
@@ -376,7 +391,7 @@ __global__ __launch_bounds__(256) void k_grad_finalize(
                    s.a3.z * q[G_VNUM]);
     if (q[G_PIX] != 0.f || q[G_PIX + 1] != 0.f) {
         // blur term: d projected mean / d mean (helpers.cuh:114-121,155-164)
-        const V3 pv = xform43(viewmat, s.m);
+        const V3 pv = xform43(viewmat, v3(s.mo.x + o.x, s.mo.y + o.y, s.mo.z + o.z));
         const float rw = 1.f / (pv.z + 1e-6f);
         const float vpx = fx * q[G_PIX], vpy = fy * q[G_PIX + 1];
         const V3 vview = v3(vpx * rw, vpy * rw, -(vpx * pv.x + vpy * pv.y) * rw * rw);

@@ -22,6 +22,8 @@ struct RasterArgs {
     uint64_t* wave_masks;  // kMaxTileWaves planes of mask_plane words, or nullptr
     int mask_plane;
     const uint2* boxes;    // per-Gaussian screen boxes (culling), or nullptr
+    const float4* ellipses;  // screen ellipses (x, y, A, B) for culling with blur off, or nullptr
+    const float* ellipse_c;  // their C values
     bool texture_bf16;
     bool precise_edge;     // float64 plane hits for near-edge-on splats
 };
@@ -108,7 +110,7 @@ __device__ __forceinline__ PixelRay pixel_ray(const float* c2w, const float* vm,
 }
 
 struct Splat {
-    V3 a1, a2, a3, m, rgb, umap, vmap;
+    V3 a1, a2, a3, mo, rgb, umap, vmap;  // mo = mean - camera origin
     float iq1, iq2, num, opac, pix_x, pix_y, u0, v0;  // iq = squared inverse scale
     int h, w, start;
 };
@@ -135,7 +137,7 @@ __device__ __forceinline__ Splat load_splat(P q) {
     s.iq2 = q[R_IQ2];
     s.a3 = v3(q[R_AX3], q[R_AX3 + 1], q[R_AX3 + 2]);
     s.opac = q[R_OPAC];
-    s.m = v3(q[R_MEAN], q[R_MEAN + 1], q[R_MEAN + 2]);
+    s.mo = v3(q[R_MO], q[R_MO + 1], q[R_MO + 2]);
     s.pix_x = q[R_PIX_X];
     s.rgb = v3(q[R_RGB], q[R_RGB + 1], q[R_RGB + 2]);
     s.pix_y = q[R_PIX_Y];
@@ -185,8 +187,7 @@ __device__ __forceinline__ Pair pair_eval(const Splat& s, const PixelRay& pr, bo
         double dd = ax * rx + ay * ry + az * rz;
         if (0.0 <= dd && dd < 1e-6) dd = 1e-6;
         else if (-1e-6 < dd && dd <= 0.0) dd = -1e-6;
-        const double mx = (double)s.m.x - pr.o.x, my = (double)s.m.y - pr.o.y,
-                     mz = (double)s.m.z - pr.o.z;
+        const double mx = s.mo.x, my = s.mo.y, mz = s.mo.z;
         const double t = (ax * mx + ay * my + az * mz) / dd;
         const double dx = t * rx - mx, dy = t * ry - my, dz = t * rz - mz;
         p.t = (float)t;
@@ -194,11 +195,12 @@ __device__ __forceinline__ Pair pair_eval(const Splat& s, const PixelRay& pr, bo
         p.l1 = (float)(dx * s.a1.x + dy * s.a1.y + dz * s.a1.z);
         p.l2 = (float)(dx * s.a2.x + dy * s.a2.y + dz * s.a2.z);
     } else {
-        // The numerator n . (m - o) is per splat and view (built with the
-        // record); |den| >= 1e-6 keeps the quotient far from overflow.
+        // The numerator n . (m - o) and m - o are per splat and view (built
+        // with the record); |den| >= 1e-6 keeps the quotient far from
+        // overflow.  The hit offset o + t r - m is formed as t r - (m - o).
         p.t = div_fast(s.num, den);
-        const V3 pos = v3(pr.o.x + p.t * pr.r.x, pr.o.y + p.t * pr.r.y, pr.o.z + p.t * pr.r.z);
-        p.delta = sub3(pos, s.m);
+        p.delta = v3(fmaf(p.t, pr.r.x, -s.mo.x), fmaf(p.t, pr.r.y, -s.mo.y),
+                     fmaf(p.t, pr.r.z, -s.mo.z));
         p.l1 = dot3(p.delta, s.a1);
         p.l2 = dot3(p.delta, s.a2);
     }
@@ -407,7 +409,7 @@ __device__ __forceinline__ Splat splat_record(const RasterArgs& a, int g) {
     pin_s(s.a1.x, s.a1.y, s.a1.z, s.iq1);
     pin_s(s.a2.x, s.a2.y, s.a2.z, s.iq2);
     pin_s(s.a3.x, s.a3.y, s.a3.z, s.opac);
-    pin_s(s.m.x, s.m.y, s.m.z, s.rgb.x);
+    pin_s(s.mo.x, s.mo.y, s.mo.z, s.rgb.x);
     pin_s(s.rgb.y, s.rgb.z, s.u0, s.v0);
     pin_s(s.umap.x, s.umap.y, s.umap.z, s.vmap.x);
     pin_s(s.vmap.y, s.vmap.z, __int_as_float(s.h), __int_as_float(s.w));
@@ -442,6 +444,32 @@ __device__ __forceinline__ bool box_meets(uint2 bx, const WaveRect& r) {
 // [lo, hi]), the wave mask of the entries in [lo, hi], and of those the ones
 // whose screen box meets the wave rectangle (all of them without boxes or
 // with cull = false).
+// Whether the screen ellipse (p - c)^T [[A, B], [B, C]] (p - c) <= 1 meets
+// the pixel centres of the wave rectangle (A <= 0: no ellipse, so yes).  The
+// quadratic's minimum over the rectangle, padded by 0.1 pixel, is 0 when the
+// centre is inside and otherwise lies on an edge, where it is a clamped 1-D
+// minimum.  The argmins use the hardware reciprocal; evaluating the quadratic
+// at a slightly moved point on the edge changes the value only to second
+// order.
+__device__ __forceinline__ bool ellipse_meets(float4 el, float C, const WaveRect& r) {
+    if (!(el.z > 0.f)) return true;
+    const float A = el.z, B = el.w;
+    const float x0 = (float)r.x0 + 0.4f - el.x, x1 = (float)r.x1 + 0.6f - el.x;
+    const float y0 = (float)r.y0 + 0.4f - el.y, y1 = (float)r.y1 + 0.6f - el.y;
+    if (x0 <= 0.f && x1 >= 0.f && y0 <= 0.f && y1 >= 0.f) return true;
+    const float ia = __builtin_amdgcn_rcpf(A), ic = __builtin_amdgcn_rcpf(C);
+    auto on_x = [&](float x) {  // edge x = const
+        const float y = fminf(fmaxf(-B * x * ic, y0), y1);
+        return A * x * x + y * (2.f * B * x + C * y);
+    };
+    auto on_y = [&](float y) {  // edge y = const
+        const float x = fminf(fmaxf(-B * y * ia, x0), x1);
+        return C * y * y + x * (2.f * B * y + A * x);
+    };
+    const float q = fminf(fminf(on_x(x0), on_x(x1)), fminf(on_y(y0), on_y(y1)));
+    return q <= 1.f;
+}
+
 struct Window {
     uint64_t in_range, cand;
     int gid;
@@ -459,6 +487,11 @@ __device__ __forceinline__ Window window_candidates(const RasterArgs& a, int bas
         if (cull) {
             const uint2 bx = a.boxes[w.gid];  // one 8-byte load
             cand = box_meets(bx, rect);
+            if (a.ellipses != nullptr) {
+                const float4 el = a.ellipses[w.gid];
+                const float ec = a.ellipse_c[w.gid];
+                cand = cand && ellipse_meets(el, ec, rect);
+            }
         }
     }
     w.in_range = __ballot(in_range);
@@ -498,6 +531,11 @@ inline RasterArgs make_raster_args(const RasterCall& c, const float* records) {
                       c.img_h < 32000
                   ? record_boxes(records, c.num_points)
                   : nullptr;
+    // the ellipses bound the Gaussian alpha only; with blur the box also
+    // covers the blur disc
+    const bool ell = a.boxes != nullptr && !(c.settings & kSetBlur);
+    a.ellipses = ell ? record_conics(records, c.num_points) : nullptr;
+    a.ellipse_c = ell ? record_conic_c(records, c.num_points) : nullptr;
     return a;
 }
 

@@ -1,12 +1,14 @@
-"""Per-wave culling by screen boxes leaves every raster output unchanged.
+"""Per-wave culling by screen boxes and ellipses leaves every raster output unchanged.
 
-The raster kernels skip a list entry for a wave when the Gaussian's screen box
-misses all of the wave's pixels.  These tests render scenes with culling on
-and off and require identical forward outputs (bit for bit) and backward
-gradients (up to the order of float atomics), on random scenes and on scenes
-built to stress the bound: edge-on splats, splats reaching behind the camera,
-extreme and sub-threshold opacities, zero scales, unnormalised quaternions,
-and saturated pixels that end on a near-invisible Gaussian.
+The raster kernels skip a list entry for a wave when the Gaussian's screen box,
+or (blur off) its screen ellipse, misses all of the wave's pixels.  These tests
+render scenes with culling on and off and require identical forward outputs
+(bit for bit) and backward gradients (up to the order of float atomics), on
+random scenes and on scenes built to stress the bounds: edge-on and grazing
+splats, splats reaching behind the camera, thin splats at every in-plane
+angle, sub-pixel splats, extreme and sub-threshold opacities, zero scales,
+unnormalised quaternions, and saturated pixels that end on a near-invisible
+Gaussian.
 """
 from __future__ import annotations
 
@@ -71,6 +73,31 @@ def _scene(kind, settings=1 << 8):
     elif kind == "degenerate":
         s.scales[pick & (rng.random(n) < 0.5), 0] = 0.0
         s.quats[pick] *= 1.3
+    elif kind == "thin_rotated":
+        s.scales[pick, 1] = s.scales[pick, 0] * rng.uniform(0.005, 0.05, pick.sum())
+        s.scales[pick, 0] *= 3.0
+        _rebuild_uv_maps(s)
+    elif kind == "grazing":
+        # |a3 . view direction| between 0.02 and 0.3
+        m = s.means.astype(np.float64)
+        d = m / np.linalg.norm(m, axis=1, keepdims=True)
+        side = np.cross(d, rng.standard_normal(d.shape))
+        side /= np.linalg.norm(side, axis=1, keepdims=True)
+        c = rng.uniform(0.02, 0.3, n)[:, None] * np.sign(rng.standard_normal((n, 1)))
+        a3 = c * d + np.sqrt(1 - c * c) * side
+        z = np.array([0.0, 0.0, 1.0])
+        axis = np.cross(z, a3)
+        sn = np.linalg.norm(axis, axis=1, keepdims=True)
+        ang = np.arctan2(sn[:, 0], a3 @ z)
+        axis = axis / np.maximum(sn, 1e-12)
+        q = np.concatenate([np.cos(ang / 2)[:, None], np.sin(ang / 2)[:, None] * axis], 1)
+        s.quats[pick] = q[pick].astype(np.float32)
+        s.scales[pick, :2] *= 2.0
+        _rebuild_uv_maps(s)
+    elif kind == "tiny":
+        s.scales[pick, :2] *= rng.uniform(0.02, 0.2, (pick.sum(), 1)).astype(np.float32)
+        s.opacities[pick, 0] = rng.uniform(0.3, 1.0, pick.sum()).astype(np.float32)
+        _rebuild_uv_maps(s)
     elif kind == "saturated":
         s.opacities[:, 0] = 0.99
         s.scales[:, :2] *= 3.0
@@ -111,7 +138,8 @@ def _grads_match(a, b, rel):
     return ok, (l2, fr)
 
 
-KINDS = ["random", "edge_on", "near_camera", "opacity_extremes", "degenerate", "saturated"]
+KINDS = ["random", "edge_on", "near_camera", "opacity_extremes", "degenerate", "saturated",
+         "thin_rotated", "grazing", "tiny"]
 
 
 @pytest.mark.parametrize("kind", KINDS)
