@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--batched-views", type=int, default=8,
                     help="also time a batch of this many views with bfloat16 texels/SH, "
                          "captured as one graph (0 disables)")
+    ap.add_argument("--train-steps", type=int, default=10,
+                    help="timed steps of the training-step variant (0 = skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the timed region, run one device-to-device copy of this many MiB "
                          "(a kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE)")
@@ -107,6 +109,40 @@ def batched_variant(scene, dev, num_views, steps, warmup):
            "ms_per_batch": round(ms, 4), "dtype": "bf16 storage, f32 compute",
            "graph": True, "capacity_ok": bool(ok)}
     del mv
+    torch.cuda.empty_cache()
+    return out
+
+
+def train_variant(scene, dev, steps, warmup):
+    """Iterations/s of the whole training step of example.py's trainer on this
+    scene: preprocessing of the raw parameters, the render forward + backward
+    (example loss: texture MSE + distortion + normal terms), the preprocessing
+    adjoint and an Adam step on every raw tensor (means, log scales, quaternions,
+    opacity logits, SH coefficients, texel logits)."""
+    import torch
+
+    from gstex_cuda_amd.pipeline import Camera
+    from gstex_cuda_amd.train import TrainStep, raw_from_scene
+
+    raw, dims = raw_from_scene(scene, dev)
+    cam = Camera(viewmat=torch.from_numpy(scene.viewmat).to(dev),
+                 c2w=torch.from_numpy(scene.c2w).to(dev), fx=scene.fx, fy=scene.fy,
+                 cx=scene.cx, cy=scene.cy, img_h=scene.img_h, img_w=scene.img_w)
+    target = torch.from_numpy(scene.target).to(dev)
+    ts = TrainStep(raw, dims, cam, target, lr=1e-3, sh_degree=scene.sh_degree,
+                   block_width=scene.block_width, settings=scene.settings)
+    for _ in range(warmup):
+        ts.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts.step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    out = {"iter_per_s": round(steps / elapsed, 3), "ms_per_step": round(1e3 * elapsed / steps, 4),
+           "steps": steps, "dtype": "f32", "optimizer": "adam (fused, one kernel per tensor)",
+           "loss": round(ts.loss(), 6)}
+    del ts
     torch.cuda.empty_cache()
     return out
 
@@ -208,6 +244,10 @@ def main():
                             max(args.warmup // 2, 1))
         b["vs_plain"] = round(b["views_per_s"] / value, 3)
         variants["bf16_batched_graph"] = b
+    if args.train_steps > 0:
+        t = train_variant(scene, dev, args.train_steps, max(args.warmup // 2, 1))
+        t["vs_plain"] = round(t["iter_per_s"] / value, 3)
+        variants["train_step_adam"] = t
     cpu = None
     if args.cpu_baseline_seconds > 0:
         cpu = cpu_baseline(scene, step, args.cpu_baseline_seconds)

@@ -19,10 +19,14 @@ P = C.c_void_p
 I = C.c_int
 F = C.c_float
 Z = C.c_size_t
+D = C.c_double
 
 # name -> (restype, argtypes)
 _SIGNATURES = {
     "gstex_last_error": (C.c_char_p, []),
+    "gstex_preprocess_gaussians": (I, [I, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "gstex_preprocess_gaussians_backward": (I, [I] + [P] * 17 + [P]),
+    "gstex_adam_step": (I, [Z, P, P, P, P, D, D, D, D, I, I, P, I, P]),
     "gstex_abi_version": (I, []),
     "gstex_get_aabb_2d": (I, [I, P, P, F, P, P, F, F, F, F, P, P, P]),
     "gstex_num_tiles_hit": (I, [I, P, P, I, I, I, P, P]),

@@ -590,4 +590,66 @@ int gstex_texture_sample_backward(int num_charts, int num_probs, int channels,
                      "texture_sample_backward");
 }
 
+int gstex_preprocess_gaussians(int num_points, const float* log_scales, const float* quats_raw,
+                               const float* mapping, const float* rgb_logits,
+                               const float* opacity_logits, float* scales, float* quats,
+                               float* uv0, float* umap, float* vmap, float* colors,
+                               float* opacities, void* stream) {
+    REQUIRE(num_points >= 0, "num_points must be >= 0");
+    if (num_points == 0) return GSTEX_OK;
+    REQUIRE(log_scales && quats_raw && mapping && opacity_logits && scales && quats && uv0 &&
+                umap && vmap && opacities && (!rgb_logits || colors),
+            "null pointer argument");
+    REQUIRE(((uintptr_t)quats_raw % 16) == 0 && ((uintptr_t)mapping % 16) == 0 &&
+                ((uintptr_t)quats % 16) == 0,
+            "quats and mapping must be 16-byte aligned");
+    return check_hip(gstex::preprocess_gaussians(num_points, log_scales, quats_raw, mapping,
+                                                 rgb_logits, opacity_logits, scales, quats, uv0,
+                                                 umap, vmap, colors, opacities,
+                                                 as_stream(stream)),
+                     "preprocess_gaussians");
+}
+
+int gstex_preprocess_gaussians_backward(
+    int num_points, const float* log_scales, const float* quats_raw, const float* mapping,
+    const float* rgb_logits, const float* opacity_logits, const float* v_scales,
+    const float* v_quats, const float* v_uv0, const float* v_umap, const float* v_vmap,
+    const float* v_colors, const float* v_opacities, float* v_log_scales, float* v_quats_raw,
+    float* v_mapping, float* v_rgb_logits, float* v_opacity_logits, void* stream) {
+    REQUIRE(num_points >= 0, "num_points must be >= 0");
+    if (num_points == 0) return GSTEX_OK;
+    REQUIRE(log_scales && quats_raw && mapping && opacity_logits && v_scales && v_quats &&
+                v_uv0 && v_umap && v_vmap && v_opacities && v_log_scales && v_quats_raw &&
+                v_opacity_logits && (!rgb_logits || (v_colors && v_rgb_logits)),
+            "null pointer argument");
+    REQUIRE(((uintptr_t)quats_raw % 16) == 0 && ((uintptr_t)mapping % 16) == 0 &&
+                ((uintptr_t)v_quats % 16) == 0 && ((uintptr_t)v_quats_raw % 16) == 0 &&
+                ((uintptr_t)v_mapping % 16) == 0,
+            "quats, mapping and their gradients must be 16-byte aligned");
+    return check_hip(gstex::preprocess_gaussians_backward(
+                         num_points, log_scales, quats_raw, mapping, rgb_logits, opacity_logits,
+                         v_scales, v_quats, v_uv0, v_umap, v_vmap, v_colors, v_opacities,
+                         v_log_scales, v_quats_raw, v_mapping, v_rgb_logits, v_opacity_logits,
+                         as_stream(stream)),
+                     "preprocess_gaussians_backward");
+}
+
+int gstex_adam_step(size_t n, float* param, const float* grad, float* exp_avg, float* exp_avg_sq,
+                    double lr, double beta1, double beta2, double eps, int step, int grad_mode,
+                    void* values, int values_format, void* stream) {
+    if (n == 0) return GSTEX_OK;
+    REQUIRE(param && grad && exp_avg && exp_avg_sq, "null pointer argument");
+    REQUIRE(step >= 1, "step must be >= 1 (the first step is 1), got %d", step);
+    REQUIRE(beta1 >= 0.0 && beta1 < 1.0 && beta2 >= 0.0 && beta2 < 1.0,
+            "betas must be in [0, 1)");
+    REQUIRE(grad_mode == GSTEX_GRAD_PLAIN || grad_mode == GSTEX_GRAD_SIGMOID,
+            "grad_mode must be GSTEX_GRAD_PLAIN or GSTEX_GRAD_SIGMOID");
+    REQUIRE(values_format == GSTEX_TEXTURE_F32 || values_format == GSTEX_TEXTURE_BF16,
+            "values_format must be GSTEX_TEXTURE_F32 or GSTEX_TEXTURE_BF16");
+    return check_hip(gstex::adam_step(n, param, grad, exp_avg, exp_avg_sq, lr, beta1, beta2, eps,
+                                      step, grad_mode, values,
+                                      values_format == GSTEX_TEXTURE_BF16, as_stream(stream)),
+                     "adam_step");
+}
+
 }  // extern "C"

@@ -173,6 +173,22 @@ __host__ __device__ inline const float* record_conic_c(const float* records, int
 }
 constexpr size_t kGradRecordBytes = 128;
 
+// training-step glue (example.py:126-143 preprocessing, torch.optim.Adam)
+hipError_t preprocess_gaussians(int n, const float* log_scales, const float* quats_raw,
+                                const float* mapping, const float* rgb_logits,
+                                const float* opacity_logits, float* scales, float* quats,
+                                float* uv0, float* umap, float* vmap, float* colors,
+                                float* opacities, hipStream_t s);
+hipError_t preprocess_gaussians_backward(
+    int n, const float* log_scales, const float* quats_raw, const float* mapping,
+    const float* rgb_logits, const float* opacity_logits, const float* v_scales,
+    const float* v_quats, const float* v_uv0, const float* v_umap, const float* v_vmap,
+    const float* v_colors, const float* v_opacities, float* v_log_scales, float* v_quats_raw,
+    float* v_mapping, float* v_rgb_logits, float* v_opacity_logits, hipStream_t s);
+hipError_t adam_step(size_t n, float* param, const float* grad, float* exp_avg, float* exp_avg_sq,
+                     double lr, double beta1, double beta2, double eps, int step, int grad_mode,
+                     void* values, int values_bf16, hipStream_t s);
+
 // texture edit (texture_edit.cu semantics; out is X x C with C >= 5, zeroed here)
 size_t texture_edit_workspace_bytes(int n);
 hipError_t texture_edit(const RasterCall& c, const float* upd_img, const float* upd_alpha,

@@ -278,6 +278,47 @@ int gstex_texture_sample_backward(int num_charts, int num_probs, int channels,
                                   const float *texture, int num_texels, const float *v_output,
                                   float *v_texture, void *stream);
 
+/* ---- Training-step glue ------------------------------------------------
+ * The reference does these steps with torch ops in its training loop
+ * (example.py:126-143 preprocessing, example.py:231-233,277 torch.optim.Adam);
+ * they have no native counterpart there. */
+
+/* Raw parameters -> render inputs (example.py:126-143): scales (N,3) =
+ * exp(log_scales.xy) with z = 1e-5 * mean(xy); quats (N,4) = q / |q|; uv0 (N,2),
+ * umap, vmap (N,3) from mapping (N,4) = (uv0, log uv scale, theta) and the
+ * rotation axes; colors (N,3) = sigmoid(rgb_logits) when rgb_logits is given
+ * (NULL: colours come from elsewhere, e.g. SH, and colors is not written);
+ * opacities (N,1) = sigmoid(opacity_logits).  quats_raw and mapping must be
+ * 16-byte aligned. */
+int gstex_preprocess_gaussians(int num_points, const float *log_scales, const float *quats_raw,
+                               const float *mapping, const float *rgb_logits,
+                               const float *opacity_logits, float *scales, float *quats,
+                               float *uv0, float *umap, float *vmap, float *colors,
+                               float *opacities, void *stream);
+
+/* Adjoint of gstex_preprocess_gaussians: gradients of the render inputs ->
+ * gradients of the raw parameters (log_scales.z gets 0: scales.z is detached).
+ * v_mapping may be NULL (mapping frozen, as in example.py:117); v_rgb_logits is
+ * written only when rgb_logits is given.  v_quats_raw and v_mapping 16-byte
+ * aligned. */
+int gstex_preprocess_gaussians_backward(
+    int num_points, const float *log_scales, const float *quats_raw, const float *mapping,
+    const float *rgb_logits, const float *opacity_logits, const float *v_scales,
+    const float *v_quats, const float *v_uv0, const float *v_umap, const float *v_vmap,
+    const float *v_colors, const float *v_opacities, float *v_log_scales, float *v_quats_raw,
+    float *v_mapping, float *v_rgb_logits, float *v_opacity_logits, void *stream);
+
+/* One torch.optim.Adam step (no weight decay, no amsgrad) over n float32 values,
+ * step >= 1 counting from 1.  grad_mode GSTEX_GRAD_SIGMOID treats grad as the
+ * gradient of sigmoid(param).  values (may be NULL) receives sigmoid(updated
+ * param) in values_format (GSTEX_TEXTURE_F32 / GSTEX_TEXTURE_BF16), e.g. the
+ * texels the next render reads. */
+#define GSTEX_GRAD_PLAIN 0
+#define GSTEX_GRAD_SIGMOID 1
+int gstex_adam_step(size_t n, float *param, const float *grad, float *exp_avg, float *exp_avg_sq,
+                    double lr, double beta1, double beta2, double eps, int step, int grad_mode,
+                    void *values, int values_format, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
