@@ -3,6 +3,8 @@
 //
 // Reference semantics: forward.cu:13-98, helpers.cuh:37-51,75-92,
 // utils.py:40-59,106-162 (torch.sort is replaced by the stable radix sort).
+#include <climits>
+
 #include "gstex_common.hpp"
 #include "gstex_kernels.hpp"
 
@@ -52,6 +54,26 @@ __device__ __forceinline__ void tile_rect(float cx, float cy, float ex, float ey
     y1 = min(max(0, f2i_trunc(tcy + tey + 1.f)), tiles_y);
 }
 
+// get_tile_bbox_wrapped (helpers.cuh:53-73,94-111): the same rectangle
+// unclamped, with the lower corner moved one tile further when it is <= 0;
+// tile indices are then taken modulo the tile grid.
+__device__ __forceinline__ void tile_rect_wrapped(float cx, float cy, float ex, float ey, int bw,
+                                                  int& x0, int& y0, int& x1, int& y1) {
+    const float fbw = (float)bw;
+    const float tcx = cx / fbw, tcy = cy / fbw, tex = ex / fbw, tey = ey / fbw;
+    x0 = f2i_trunc(tcx - tex);
+    if (x0 <= 0 && x0 > INT_MIN) x0 -= 1;
+    x1 = f2i_trunc(tcx + tex + 1.f);
+    y0 = f2i_trunc(tcy - tey);
+    if (y0 <= 0 && y0 > INT_MIN) y0 -= 1;
+    y1 = f2i_trunc(tcy + tey + 1.f);
+}
+
+__device__ __forceinline__ int wrap_tile(int t, int n) {
+    t %= n;
+    return t < 0 ? t + n : t;
+}
+
 __device__ __forceinline__ bool writes_intersects(float ex, float ey) {
     // forward.cu:32 compares float extents with the double literal 1e-4.
     return !((double)ex <= 1e-4 && (double)ey <= 1e-4);
@@ -62,29 +84,59 @@ __device__ __forceinline__ bool writes_intersects(float ex, float ey) {
 // its tile rectangle from cum_tiles_hit[g-1]; outputs are pre-zeroed so slots
 // a Gaussian counts but does not write stay (0, 0).  Writes past the
 // Gaussian's own count window are dropped rather than spilling into the next
-// Gaussian's slots.
+// Gaussian's slots (the walk stops there: a wrapped rectangle is unclamped).
 __global__ __launch_bounds__(256) void k_map_intersects(
     int n, int num_isects, const float* __restrict__ centers, const float* __restrict__ extents,
     const float* __restrict__ depths, const int* __restrict__ cum, int tiles_x, int tiles_y,
-    int bw, int64_t* __restrict__ keys, int* __restrict__ ids) {
+    int bw, bool wrapped, int64_t* __restrict__ keys, int* __restrict__ ids) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const float2 c = reinterpret_cast<const float2*>(centers)[g];
     const float2 e = reinterpret_cast<const float2*>(extents)[g];
     if (!writes_intersects(e.x, e.y)) return;
     int x0, y0, x1, y1;
-    tile_rect(c.x, c.y, e.x, e.y, bw, tiles_x, tiles_y, x0, y0, x1, y1);
+    if (wrapped) tile_rect_wrapped(c.x, c.y, e.x, e.y, bw, x0, y0, x1, y1);
+    else tile_rect(c.x, c.y, e.x, e.y, bw, tiles_x, tiles_y, x0, y0, x1, y1);
     int cur = g == 0 ? 0 : cum[g - 1];
     const int end = min(cum[g], num_isects);
     const int64_t did = (int64_t)__float_as_int(depths[g]);
-    for (int ty = y0; ty < y1; ++ty)
-        for (int tx = x0; tx < x1; ++tx) {
-            if (cur >= 0 && cur < end) {
-                keys[cur] = ((int64_t)(ty * tiles_x + tx) << 32) | did;
+    for (int ty = y0; ty < y1 && cur < end; ++ty)
+        for (int tx = x0; tx < x1 && cur < end; ++tx) {
+            const int ry = wrapped ? wrap_tile(ty, tiles_y) : ty;
+            const int rx = wrapped ? wrap_tile(tx, tiles_x) : tx;
+            if (cur >= 0) {
+                keys[cur] = ((int64_t)(ry * tiles_x + rx) << 32) | did;
                 ids[cur] = g;
             }
             ++cur;
         }
+}
+
+// Tile count of the wrapped rectangle (0 for a Gaussian that writes nothing):
+// the count that makes wrapped binning write every tile of its rectangle.
+__global__ __launch_bounds__(256) void k_num_tiles_hit_wrapped(int n,
+                                                               const float* __restrict__ centers,
+                                                               const float* __restrict__ extents,
+                                                               int bw, int* __restrict__ out) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const float2 c = reinterpret_cast<const float2*>(centers)[g];
+    const float2 e = reinterpret_cast<const float2*>(extents)[g];
+    int cnt = 0;
+    if (writes_intersects(e.x, e.y)) {
+        int x0, y0, x1, y1;
+        tile_rect_wrapped(c.x, c.y, e.x, e.y, bw, x0, y0, x1, y1);
+        const long long a = (long long)max(x1 - x0, 0) * (long long)max(y1 - y0, 0);
+        cnt = (int)min(a, 0x7fffffffll);
+    }
+    out[g] = cnt;
+}
+
+hipError_t num_tiles_hit_wrapped(int n, const float* centers, const float* extents, int bw,
+                                 int* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_num_tiles_hit_wrapped<<<(n + 255) / 256, 256, 0, s>>>(n, centers, extents, bw, out);
+    return hipGetLastError();
 }
 
 // forward.cu:76-98 on sorted 64-bit keys (tile in the high word).
@@ -515,14 +567,14 @@ hipError_t sort_isects(int n, const int64_t* keys_in, const int* ids_in, int64_t
 
 hipError_t map_intersects(int n, int num_isects, const float* centers, const float* extents,
                           const float* depths, const int* cum, int tiles_x, int tiles_y, int bw,
-                          int64_t* keys, int* ids, hipStream_t s) {
+                          bool wrapped, int64_t* keys, int* ids, hipStream_t s) {
     hipError_t e = fill<int64_t>(keys, (size_t)num_isects, (int64_t)0, s);
     if (e != hipSuccess) return e;
     e = fill<int>(ids, (size_t)num_isects, 0, s);
     if (e != hipSuccess) return e;
     if (n <= 0) return hipSuccess;
     k_map_intersects<<<(n + 255) / 256, 256, 0, s>>>(n, num_isects, centers, extents, depths, cum,
-                                                    tiles_x, tiles_y, bw, keys, ids);
+                                                    tiles_x, tiles_y, bw, wrapped, keys, ids);
     return hipGetLastError();
 }
 

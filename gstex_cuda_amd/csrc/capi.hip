@@ -52,8 +52,7 @@ int check_raster_common(int tiles_x, int tiles_y, int block_width, int img_width
             channels);
     REQUIRE(num_points >= 0 && num_ids >= 0, "negative sizes");
     REQUIRE(channels == 0 || num_texels > 0, "texture has no rows");
-    REQUIRE(!(settings & gstex::kSetAlphaViz),
-            "settings bit 16 (alpha visualisation mode) is not supported");
+    REQUIRE(settings >= 0 && settings < (1 << 30), "settings bits 30 and 31 are not defined");
     return GSTEX_OK;
 }
 
@@ -87,6 +86,16 @@ int gstex_num_tiles_hit(int num_points, const float* centers, const float* exten
                                                  tiles_of(img_height, block_width), num_tiles_hit,
                                                  as_stream(stream)),
                      "num_tiles_hit");
+}
+
+int gstex_num_tiles_hit_wrapped(int num_points, const float* centers, const float* extents,
+                                int block_width, int32_t* num_tiles_hit, void* stream) {
+    REQUIRE(num_points >= 0 && block_width > 0, "invalid sizes");
+    if (num_points == 0) return GSTEX_OK;
+    REQUIRE(centers && extents && num_tiles_hit, "null pointer argument");
+    return check_hip(gstex::num_tiles_hit_wrapped(num_points, centers, extents, block_width,
+                                                  num_tiles_hit, as_stream(stream)),
+                     "num_tiles_hit_wrapped");
 }
 
 int gstex_project_gaussians(int num_points, const float* means, const float* scales,
@@ -165,15 +174,16 @@ int gstex_map_gaussian_to_intersects(int num_points, int num_intersects, const f
                                      int block_width, int wrapped, int64_t* isect_ids,
                                      int32_t* gaussian_ids, void* stream) {
     REQUIRE(num_points >= 0 && num_intersects >= 0, "negative sizes");
-    REQUIRE(!wrapped, "wrapped tile binning is not supported");
     REQUIRE(block_width > 0 && tiles_x >= 0 && tiles_y >= 0, "invalid tile bounds");
+    REQUIRE(!wrapped || (tiles_x > 0 && tiles_y > 0), "wrapped binning needs a non-empty tile grid");
     if (num_intersects == 0) return GSTEX_OK;
     REQUIRE(isect_ids && gaussian_ids, "null pointer argument");
     REQUIRE(num_points == 0 || (centers && extents && depths && cum_tiles_hit),
             "null pointer argument");
     return check_hip(gstex::map_intersects(num_points, num_intersects, centers, extents, depths,
-                                           cum_tiles_hit, tiles_x, tiles_y, block_width, isect_ids,
-                                           gaussian_ids, as_stream(stream)),
+                                           cum_tiles_hit, tiles_x, tiles_y, block_width,
+                                           wrapped != 0, isect_ids, gaussian_ids,
+                                           as_stream(stream)),
                      "map_gaussian_to_intersects");
 }
 

@@ -75,7 +75,16 @@ constexpr int kSetUvGrad = 1 << 8;
 constexpr int kSetBlur = 1 << 9;
 constexpr int kSetNdc = 1 << 10;
 constexpr int kSetNormalFlip = 1 << 15;
-constexpr int kSetAlphaViz = 1 << 16;
+constexpr int kSetAlphaViz = 1 << 16;      // alpha visualisation (texture.cu:56-65)
+constexpr int kSetWhiteOutline = 1 << 24;  //   outline drawn white instead of hidden
+constexpr int kSetOpacityThresh = 1 << 25; //   opacity < 0.5 hidden
+// alpha bound = bits 17..21 / 8 (sigma threshold 0.5 bound^2); outline bound = bits 26..29 / 4
+__host__ __device__ inline float viz_alpha_bound(int settings) {
+    return (float)((settings >> 17) & 31) / 8.0f;
+}
+__host__ __device__ inline float viz_outline_bound(int settings) {
+    return (float)((settings >> 26) & 15) / 4.0f;
+}
 
 // ---------------------------------------------------------------- small vector helpers
 struct V3 {

@@ -52,7 +52,9 @@ hipError_t fill_i32(int* p, size_t n, int v, hipStream_t s);
 // binning
 hipError_t map_intersects(int n, int num_isects, const float* centers, const float* extents,
                           const float* depths, const int* cum, int tiles_x, int tiles_y, int bw,
-                          int64_t* keys, int* ids, hipStream_t s);
+                          bool wrapped, int64_t* keys, int* ids, hipStream_t s);
+hipError_t num_tiles_hit_wrapped(int n, const float* centers, const float* extents, int bw,
+                                 int* out, hipStream_t s);
 size_t sort_isects_workspace_bytes(int n);
 hipError_t sort_isects(int n, const int64_t* keys_in, const int* ids_in, int64_t* keys_out,
                        int* ids_out, int end_bit, void* ws, size_t ws_bytes, hipStream_t s);

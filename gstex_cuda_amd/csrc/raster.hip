@@ -31,6 +31,13 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
     const bool use_blur = GENERAL && (a.settings & kSetBlur);
     const bool use_ndc = GENERAL && (a.settings & kSetNdc);
     const bool flip = a.settings & kSetNormalFlip, bilinear = !(a.settings & kSetNearest);
+    // alpha visualisation (texture.cu:56-65,192-212,226-241): alpha 0.99 inside
+    // sigma <= 0.5 bound^2, else 0; near the edge the splat is drawn white or not
+    const bool viz = GENERAL && (a.settings & kSetAlphaViz);
+    const float viz_bound = viz_alpha_bound(a.settings);
+    const float viz_thresh = 0.5f * viz_bound * viz_bound;
+    const float viz_outline = viz_outline_bound(a.settings);
+    const bool viz_white = a.settings & kSetWhiteOutline, viz_opac = a.settings & kSetOpacityThresh;
     const int C = CM <= 4 ? CM : a.channels;
     constexpr int E = TB ? 2 : 4;
 
@@ -76,15 +83,29 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
             const Splat s = splat_record(a, g);
             bool comp = false;
             if (!done) {
-                const Pair p = pair_eval(s, pr, use_blur, GENERAL && a.precise_edge);
+                Pair p = pair_eval(s, pr, use_blur, GENERAL && a.precise_edge);
+                if (viz) {
+                    const float sigma = 0.5f * (s.iq1 * p.l1 * p.l1 + s.iq2 * p.l2 * p.l2);
+                    p.alpha = (sigma > viz_thresh || (viz_opac && s.opac < 0.5f)) ? 0.f : 0.99f;
+                    p.skip = p.t < 0.01f || p.t > 1000.0f || p.alpha < 1.f / 255.f;
+                }
                 const float next_T = T * (1.f - p.alpha);
                 if (next_T <= 1e-4f) done = true;
                 comp = !(p.skip || done);
                 if (comp) {
                     const float vis = p.alpha * T;
-                    rgb0 += s.rgb.x * vis;
-                    rgb1 += s.rgb.y * vis;
-                    rgb2 += s.rgb.z * vis;
+                    // colour and texture weights: as given, or (visualisation,
+                    // within the outline bound of the edge) white or nothing
+                    float cr = s.rgb.x, cg = s.rgb.y, cb = s.rgb.z;
+                    bool tex_on = true, tex_white = false;
+                    if (viz && !(outline_distance(a, s, j, i, viz_thresh) > viz_outline)) {
+                        tex_white = viz_white;
+                        tex_on = viz_white;
+                        cr = cg = cb = viz_white ? 1.f : 0.f;
+                    }
+                    rgb0 += cr * vis;
+                    rgb1 += cg * vis;
+                    rgb2 += cb * vis;
                     // normal facing the camera: the flip goes into the weight's
                     // sign (exact), so the axis stays a scalar operand
                     const float nvis = (flip && dot3(pr.r, s.a3) > 0.f) ? -vis : vis;
@@ -116,7 +137,8 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
                         for (int cc = 0; cc < CK; ++cc) {
                             const float val = wt[0] * tv[0][cc] + wt[1] * tv[1][cc] +
                                               wt[2] * tv[2][cc] + wt[3] * tv[3][cc];
-                            tex[c0 + cc] += vis * val;
+                            if (!viz) tex[c0 + cc] += vis * val;
+                            else if (tex_on) tex[c0 + cc] += vis * (tex_white ? 1.f : val);
                         }
                     }
                     if (T > 0.5f) {
@@ -170,7 +192,7 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
 template <int CM>
 static hipError_t launch_fwd_cm(const RasterArgs& a, const FwdOut& o, hipStream_t s) {
     const int tiles = a.tiles_x * a.tiles_y;
-    const bool general = (a.settings & (kSetBlur | kSetNdc)) || a.precise_edge;
+    const bool general = (a.settings & (kSetBlur | kSetNdc | kSetAlphaViz)) || a.precise_edge;
     const dim3 grid(tiles), block(raster_block_threads(a.bw));
     if (a.texture_bf16) {
         if (general) k_raster_forward<CM, true, true><<<grid, block, 0, s>>>(a, o);

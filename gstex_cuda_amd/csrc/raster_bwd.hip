@@ -445,7 +445,11 @@ hipError_t raster_backward_stage(const RasterCall& c, const float* records,
                                  const float* v_depth, const float* v_reg, const float* v_alpha,
                                  const float* v_texture_out, const float* v_normal,
                                  float* grad_rec, float* v_texture, hipStream_t s) {
-    const RasterArgs a = make_raster_args(c, records);
+    RasterArgs a = make_raster_args(c, records);
+    // The backward differentiates the Gaussian alpha whatever the alpha
+    // visualisation bits say (texture.cu:381-387 reads only bits 2, 8, 9, 10);
+    // masks written by a visualisation forward would mark other entries.
+    if (c.settings & kSetAlphaViz) a.wave_masks = nullptr;
     const BwdIn b{final_Ts, final_idx, depth_idx, final_s, v_img, v_depth, v_reg, v_alpha,
                   v_texture_out, v_normal, grad_rec, v_texture};
     if (a.tiles_x * a.tiles_y <= 0) return hipSuccess;

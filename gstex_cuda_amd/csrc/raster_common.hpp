@@ -470,6 +470,29 @@ __device__ __forceinline__ bool ellipse_meets(float4 el, float C, const WaveRect
     return q <= 1.f;
 }
 
+// local_outline (texture_helpers.cuh:390-417): distance in pixels from pixel
+// (j, i) to the nearest pixel of its 7 x 7 neighbourhood whose ray meets the
+// splat plane outside sigma <= thresh (1000 when there is none), with the ray
+// model and plane hit of the raster.
+__device__ inline float outline_distance(const RasterArgs& a, const Splat& s, int j, int i,
+                                         float thresh) {
+    float best = 1000.f;
+    for (int dx = -3; dx <= 3; ++dx)
+        for (int dy = -3; dy <= 3; ++dy) {
+            const PixelRay q = pixel_ray(a.c2w, a.viewmat, a.fx, a.fy, a.cx, a.cy, j + dx, i + dy);
+            float den = dot3(s.a3, q.r);
+            if (0.f <= den && den < 1e-6f) den = 1e-6f;
+            else if (-1e-6f < den && den <= 0.f) den = -1e-6f;
+            const float t = div_fast(s.num, den);
+            const V3 d = v3(fmaf(t, q.r.x, -s.mo.x), fmaf(t, q.r.y, -s.mo.y),
+                            fmaf(t, q.r.z, -s.mo.z));
+            const float l1 = dot3(d, s.a1), l2 = dot3(d, s.a2);
+            if (0.5f * (s.iq1 * l1 * l1 + s.iq2 * l2 * l2) > thresh)
+                best = fminf(best, sqrtf((float)(dx * dx + dy * dy)));
+        }
+    return best;
+}
+
 struct Window {
     uint64_t in_range, cand;
     int gid;
@@ -527,8 +550,10 @@ inline RasterArgs make_raster_args(const RasterCall& c, const float* records) {
     a.wave_masks = c.wave_masks;
     a.mask_plane = (int)raster_wave_mask_words(c.num_ids, a.tiles_x * a.tiles_y);
     // boxes hold int16 pixel indices, so culling is off for larger images
+    // the alpha visualisation mode draws splats out to its own sigma bound,
+    // beyond the alpha >= 1/255 region the culling bounds hold
     a.boxes = raster_culling_enabled() && c.num_points > 0 && c.img_w < 32000 &&
-                      c.img_h < 32000
+                      c.img_h < 32000 && !(c.settings & kSetAlphaViz)
                   ? record_boxes(records, c.num_points)
                   : nullptr;
     // the ellipses bound the Gaussian alpha only; with blur the box also

@@ -111,15 +111,21 @@ def get_aabb_2d(means, scales, glob_scale, quats, viewmat, fx, fy, cx, cy):
     return centers, extents
 
 
-def num_tiles_hit(centers, extents, img_height, img_width, block_width):
-    """get_aabb_2d.py:70-92 on the GPU -> (N,) int32."""
+def num_tiles_hit(centers, extents, img_height, img_width, block_width, wrapped=False):
+    """get_aabb_2d.py:70-92 on the GPU -> (N,) int32.  ``wrapped``: the tile
+    count of each Gaussian's wrapped (unclamped) rectangle instead, which
+    map_gaussian_to_intersects(..., wrapped=True) then fills completely."""
     _check(centers, "centers")
     _check(extents, "extents")
     n = centers.size(0)
     out = torch.empty((n,), dtype=torch.int32, device=centers.device)
     with torch.cuda.device(centers.device):
-        call("gstex_num_tiles_hit", n, _ptr(centers), _ptr(extents), int(img_height),
-             int(img_width), int(block_width), _ptr(out), _stream(centers.device))
+        if wrapped:
+            call("gstex_num_tiles_hit_wrapped", n, _ptr(centers), _ptr(extents),
+                 int(block_width), _ptr(out), _stream(centers.device))
+        else:
+            call("gstex_num_tiles_hit", n, _ptr(centers), _ptr(extents), int(img_height),
+                 int(img_width), int(block_width), _ptr(out), _stream(centers.device))
     return out
 
 

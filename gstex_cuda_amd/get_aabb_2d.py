@@ -32,10 +32,12 @@ def get_aabb_2d(means: Tensor, scales: Tensor, glob_scale, quats: Tensor, viewma
 
 
 def get_num_tiles_hit_2d(centers: Tensor, extents: Tensor, img_height: int, img_width: int,
-                         block_width: int) -> Tensor:
-    """Tiles touched by each AABB (get_aabb_2d.py:70-92), int32."""
+                         block_width: int, wrapped: bool = False) -> Tensor:
+    """Tiles touched by each AABB (get_aabb_2d.py:70-92), int32.  ``wrapped``
+    (not in the reference) counts the wrapped rectangle that
+    ``map_gaussian_to_intersects(..., wrapped=True)`` walks."""
     return _C.num_tiles_hit(centers.contiguous(), extents.contiguous(), img_height, img_width,
-                            block_width)
+                            block_width, wrapped=wrapped)
 
 
 def project_gaussians(means: Tensor, scales: Tensor, glob_scale, quats: Tensor, viewmat: Tensor,

@@ -52,6 +52,12 @@ int gstex_get_aabb_2d(int num_points, const float *means, const float *scales, f
 int gstex_num_tiles_hit(int num_points, const float *centers, const float *extents, int img_height,
                         int img_width, int block_width, int32_t *num_tiles_hit, void *stream);
 
+/* Tile count of each Gaussian's wrapped rectangle (0 when max(extent) <= 1e-4),
+ * the counts wrapped binning needs; no counterpart in the reference, whose
+ * get_num_tiles_hit_2d (get_aabb_2d.py:70-92) clamps to the image. */
+int gstex_num_tiles_hit_wrapped(int num_points, const float *centers, const float *extents,
+                                int block_width, int32_t *num_tiles_hit, void *stream);
+
 /* Fused caller-side projection (example.py:146-152): get_aabb_2d plus the
  * view-space depth of project_points (get_aabb_2d.py:22-32) plus
  * get_num_tiles_hit_2d (get_aabb_2d.py:70-92), one pass over the Gaussians. */
@@ -89,7 +95,12 @@ int gstex_compute_sh_backward(int num_points, int degree, int degrees_to_use,
 
 /* Replaces map_gaussian_to_intersects_tensor (cuda/csrc/bindings.cu:77-121).
  * isect_ids (I,) int64 and gaussian_ids (I,) int32 are fully written
- * (zero where a Gaussian counts a slot but writes none).  wrapped must be 0. */
+ * (zero where a Gaussian counts a slot but writes none).  wrapped != 0 takes the
+ * unclamped rectangle of get_tile_bbox_wrapped (helpers.cuh:53-73,94-111) with
+ * tile indices modulo the grid (forward.cu:34-62).  Each Gaussian writes at
+ * most its own count window [cum[g-1], cum[g]); the reference writes its whole
+ * rectangle there, past the window when it holds more tiles than the count.
+ * Use gstex_num_tiles_hit_wrapped for counts that hold every wrapped tile. */
 int gstex_map_gaussian_to_intersects(int num_points, int num_intersects, const float *centers,
                                      const float *extents, const float *depths,
                                      const int32_t *cum_tiles_hit, int tiles_x, int tiles_y,

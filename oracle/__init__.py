@@ -162,11 +162,16 @@ def get_aabb_2d(means, scales, glob_scale, quats, viewmat, fx, fy, cx, cy, preci
     return centers, extents
 
 
-def num_tiles_hit(centers, extents, img_h, img_w, block_width):
-    """get_aabb_2d.py:70-92 in float32 -> (N,) int32."""
+def num_tiles_hit(centers, extents, img_h, img_w, block_width, wrapped=False):
+    """get_aabb_2d.py:70-92 in float32 -> (N,) int32; ``wrapped``: tile count of
+    the wrapped rectangle (helpers.cuh:53-73,94-111) instead."""
     centers, extents = _f32(centers), _f32(extents)
     n = centers.shape[0]
     out = np.zeros(n, np.int32)
+    if wrapped:
+        lib().orc_num_tiles_hit_wrapped(C.c_int(n), _p(centers), _p(extents),
+                                        C.c_int(block_width), _p(out))
+        return out
     lib().orc_num_tiles_hit(C.c_int(n), _p(centers), _p(extents), C.c_int(img_h), C.c_int(img_w),
                             C.c_int(block_width), _p(out))
     return out
@@ -174,17 +179,19 @@ def num_tiles_hit(centers, extents, img_h, img_w, block_width):
 
 # --------------------------------------------------------------------------- binning
 def map_gaussian_to_intersects(centers, extents, depths, cum_tiles_hit, tile_bounds, block_width,
-                               num_intersects=None):
-    """forward.cu:13-71 -> (isect_ids int64, gaussian_ids int32), zero-initialised."""
+                               num_intersects=None, wrapped=False):
+    """forward.cu:13-71 -> (isect_ids int64, gaussian_ids int32), zero-initialised;
+    ``wrapped``: forward.cu:34-62 with writes bounded by each Gaussian's window."""
     centers, extents, depths = _f32(centers), _f32(extents), _f32(depths)
     cum = _i32(cum_tiles_hit)
     n = centers.shape[0]
     m = int(cum[-1]) if num_intersects is None else int(num_intersects)
     keys = np.zeros(m, np.int64)
     ids = np.zeros(m, np.int32)
-    lib().orc_map_gaussian_to_intersects(
-        C.c_int(n), C.c_int(m), _p(centers), _p(extents), _p(depths), _p(cum),
-        C.c_int(tile_bounds[0]), C.c_int(tile_bounds[1]), C.c_int(block_width), _p(keys), _p(ids))
+    fn = (lib().orc_map_gaussian_to_intersects_wrapped if wrapped
+          else lib().orc_map_gaussian_to_intersects)
+    fn(C.c_int(n), C.c_int(m), _p(centers), _p(extents), _p(depths), _p(cum),
+       C.c_int(tile_bounds[0]), C.c_int(tile_bounds[1]), C.c_int(block_width), _p(keys), _p(ids))
     return keys, ids
 
 

@@ -90,6 +90,68 @@ static void orc_tile_rect(float cxp, float cyp, float ex, float ey, int tiles_x,
     }
 }
 
+/* helpers.cuh:53-73,94-111 (get_tile_bbox_wrapped): the rectangle unclamped,
+ * with the lower corner one tile further when it is <= 0 (not below INT_MIN). */
+static void orc_tile_rect_wrapped(float cxp, float cyp, float ex, float ey, int bw, int mn[2],
+                                  int mx[2]) {
+    float fbw = (float)bw;
+    float tc[2] = {cxp / fbw, cyp / fbw}, te[2] = {ex / fbw, ey / fbw};
+    for (int k = 0; k < 2; ++k) {
+        int a = orc_sat_i32(tc[k] - te[k]);
+        if (a <= 0 && a > INT32_MIN) a -= 1;
+        mn[k] = a;
+        mx[k] = orc_sat_i32(tc[k] + te[k] + 1.0f);
+    }
+}
+
+/* Tile count of the wrapped rectangle (0 when max(extent) <= 1e-4). */
+void orc_num_tiles_hit_wrapped(int n, const float *centers, const float *extents, int bw, int *out) {
+    for (int g = 0; g < n; ++g) {
+        float ex = extents[2 * g], ey = extents[2 * g + 1];
+        out[g] = 0;
+        if ((double)ex <= 1e-4 && (double)ey <= 1e-4) continue;
+        int mn[2], mx[2];
+        orc_tile_rect_wrapped(centers[2 * g], centers[2 * g + 1], ex, ey, bw, mn, mx);
+        long long w = (long long)mx[0] - mn[0], h = (long long)mx[1] - mn[1];
+        long long a = (w > 0 ? w : 0) * (h > 0 ? h : 0);
+        out[g] = a > 0x7fffffffLL ? 0x7fffffff : (int)a;
+    }
+}
+
+/* forward.cu:34-62 (wrapped=true): the wrapped rectangle row by row, tile
+ * indices modulo the grid.  Each Gaussian writes at most its own count window
+ * [cum[g-1], cum[g]); the reference writes the whole rectangle from cum[g-1],
+ * past the window (into the next Gaussian's slots) when it holds more tiles. */
+void orc_map_gaussian_to_intersects_wrapped(int n, int num_isects, const float *centers,
+                                            const float *extents, const float *depths,
+                                            const int *cum, int tiles_x, int tiles_y, int bw,
+                                            int64_t *keys, int32_t *ids) {
+    memset(keys, 0, sizeof(int64_t) * (size_t)num_isects);
+    memset(ids, 0, sizeof(int32_t) * (size_t)num_isects);
+    for (int g = 0; g < n; ++g) {
+        float ex = extents[2 * g], ey = extents[2 * g + 1];
+        if ((double)ex <= 1e-4 && (double)ey <= 1e-4) continue;
+        int mn[2], mx[2];
+        orc_tile_rect_wrapped(centers[2 * g], centers[2 * g + 1], ex, ey, bw, mn, mx);
+        int cur = g == 0 ? 0 : cum[g - 1];
+        int end = cum[g] < num_isects ? cum[g] : num_isects;
+        int32_t dbits;
+        memcpy(&dbits, depths + g, 4);
+        int64_t did = (int64_t)dbits;
+        for (int i = mn[1]; i < mx[1] && cur < end; ++i)
+            for (int j = mn[0]; j < mx[0] && cur < end; ++j) {
+                int ti = i % tiles_y, tj = j % tiles_x;
+                if (ti < 0) ti += tiles_y;
+                if (tj < 0) tj += tiles_x;
+                if (cur >= 0) {
+                    keys[cur] = ((int64_t)(ti * tiles_x + tj) << 32) | did;
+                    ids[cur] = g;
+                }
+                ++cur;
+            }
+    }
+}
+
 /* forward.cu:13-71 (wrapped=false): keys (tile << 32 | depth bits) and ids,
  * written from cum_tiles_hit[g-1]; slots a Gaussian counts but never writes stay 0. */
 void orc_map_gaussian_to_intersects(int n, int num_isects, const float *centers, const float *extents,

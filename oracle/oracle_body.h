@@ -384,6 +384,28 @@ static int SFX(pair_skip)(const SFX(pair_t) *P) {
     return P->t < (REAL)0.01f || P->t > (REAL)1000.0f || P->alpha < (REAL)1.f / (REAL)255.f;
 }
 
+/* local_outline (texture_helpers.cuh:390-417) with get_pixel_to_texel
+ * (:355-366): distance in pixels to the nearest pixel of the 7 x 7
+ * neighbourhood whose plane hit lies outside sigma <= thresh (1000 if none). */
+static REAL SFX(local_outline)(const orc_scene *S, const SFX(pair_t) *P, REAL px, REAL py,
+                               REAL thresh) {
+    REAL gs = (REAL)S->glob_scale, s1 = P->sx * gs, s2 = P->sy * gs, best = (REAL)1000.0f;
+    for (int dx = -3; dx <= 3; ++dx)
+        for (int dy = -3; dy <= 3; ++dy) {
+            REAL o[3], r[3], d[3];
+            SFX(get_ray)(SF(c2w), (REAL)S->fx, (REAL)S->fy, (REAL)S->cx, (REAL)S->cy,
+                         px + (REAL)dx, py + (REAL)dy, o, r);
+            REAL t = SFX(compute_tt)(o, r, P->mean, P->ax3, NULL);
+            for (int k = 0; k < 3; ++k) d[k] = (o[k] + t * r[k]) - P->mean[k];
+            REAL as1 = SFX(dot3)(d, P->ax1) / s1, as2 = SFX(dot3)(d, P->ax2) / s2;
+            if ((REAL)0.5f * (as1 * as1 + as2 * as2) > thresh) {
+                REAL dis = SQRT((REAL)(dx * dx + dy * dy));
+                if (best > dis) best = dis;
+            }
+        }
+    return best;
+}
+
 /* texture.cu:11-329: front-to-back compositing of one image. */
 void SFX(orc_texture_forward)(const orc_scene *S, REAL *out_img, REAL *out_depth, REAL *out_reg,
                               REAL *out_texture, REAL *out_normal, REAL *final_Ts, int *final_idx,
@@ -392,6 +414,12 @@ void SFX(orc_texture_forward)(const orc_scene *S, REAL *out_img, REAL *out_depth
     const int tiles_x = (W + bw - 1) / bw, tiles_y = (H + bw - 1) / bw;
     const int use_blur = (S->settings >> 9) & 1, use_ndc = (S->settings >> 10) & 1;
     const int viz = (S->settings >> 15) & 1, bilinear = !((S->settings >> 2) & 1);
+    /* alpha visualisation mode, texture.cu:56-65 */
+    const int aviz = (S->settings >> 16) & 1, white = (S->settings >> 24) & 1;
+    const int opac_thresh = (S->settings >> 25) & 1;
+    const REAL abound = (REAL)(float)((S->settings >> 17) & 31) / (REAL)8.0f;
+    const REAL obound = (REAL)(float)((S->settings >> 26) & 15) / (REAL)4.0f;
+    const REAL sthresh = (REAL)0.5f * abound * abound;
     REAL *tex = (REAL *)malloc(sizeof(REAL) * (C > 0 ? C : 1));
     for (int ty = 0; ty < tiles_y; ++ty)
         for (int tx = 0; tx < tiles_x; ++tx) {
@@ -415,12 +443,28 @@ void SFX(orc_texture_forward)(const orc_scene *S, REAL *out_img, REAL *out_depth
                         int g = S->ids_sorted[e];
                         SFX(pair_t) P;
                         SFX(pair_geom)(S, g, o, r, px, py, use_blur, 0, &P);
+                        REAL pixel_dis = (REAL)1000.0f;
+                        if (aviz) { /* texture.cu:201-212 */
+                            REAL gs = (REAL)S->glob_scale;
+                            REAL is1 = (REAL)1 / (P.sx * gs), is2 = (REAL)1 / (P.sy * gs);
+                            REAL sigma = (REAL)0.5f * (is1 * is1 * P.l1 * P.l1 + is2 * is2 * P.l2 * P.l2);
+                            P.alpha = (REAL)0.99f;
+                            if (sigma > sthresh) P.alpha = 0;
+                            if (opac_thresh && P.opac < (REAL)0.5f) P.alpha = 0;
+                            pixel_dis = SFX(local_outline)(S, &P, px, py, sthresh);
+                        }
                         int skip = SFX(pair_skip)(&P);
                         REAL next_T = T * ((REAL)1 - P.alpha);
                         if (next_T <= (REAL)1e-4f) done = 1;
                         if (skip || done) continue;
                         REAL vis = P.alpha * T;
-                        for (int k = 0; k < 3; ++k) rgb[k] = rgb[k] + (REAL)SF(colors)[3 * g + k] * vis;
+                        /* texture.cu:226-235,268-273: colour and texture as given, or
+                         * within the outline bound white (bit 24) or nothing */
+                        int plain = !aviz || pixel_dis > obound, outline_white = !plain && white;
+                        for (int k = 0; k < 3; ++k) {
+                            if (plain) rgb[k] = rgb[k] + (REAL)SF(colors)[3 * g + k] * vis;
+                            else if (outline_white) rgb[k] = rgb[k] + vis;
+                        }
                         REAL hn[3] = {P.ax3[0], P.ax3[1], P.ax3[2]};
                         if (viz && SFX(dot3)(r, P.ax3) > 0) {
                             hn[0] = -hn[0]; hn[1] = -hn[1]; hn[2] = -hn[2];
@@ -441,7 +485,8 @@ void SFX(orc_texture_forward)(const orc_scene *S, REAL *out_img, REAL *out_depth
                             SFX(chart_taps)(d[0], d[1], d[2], C, c, u, v, bilinear, wt, idx);
                             REAL val = wt[0] * (REAL)SF(texture)[idx[0]] + wt[1] * (REAL)SF(texture)[idx[1]] +
                                        wt[2] * (REAL)SF(texture)[idx[2]] + wt[3] * (REAL)SF(texture)[idx[3]];
-                            tex[c] += vis * val;
+                            if (plain) tex[c] += vis * val;
+                            else if (outline_white) tex[c] += vis;
                         }
                         if (T > (REAL)0.5f) {
                             pdepth = t_view;

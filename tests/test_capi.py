@@ -72,7 +72,7 @@ def _call_forward(**over):
     (dict(probs=2), "num_probs"),
     (dict(C=65), "texture channels"),
     (dict(ws=8), "workspace too small"),
-    (dict(settings=1 << 16), "alpha visualisation"),
+    (dict(settings=1 << 30), "settings bits 30 and 31"),
 ])
 def test_forward_rejects_bad_arguments(over, msg):
     rc, err = _call_forward(**over)

@@ -122,6 +122,40 @@ def test_binning_screen_sized_splats(oracle):
     assert _binning_case(oracle, s, 16) > 500_000
 
 
+@pytest.mark.parametrize("counts", ["wrapped", "clamped"])
+def test_binning_wrapped(oracle, counts):
+    # forward.cu:34-62: Gaussians spread past every image edge; wrapped counts
+    # fill each wrapped rectangle, clamped counts cut it to the window
+    from gstex_cuda_amd import cuda as _C
+    from gstex_cuda_amd.utils import bin_and_sort_gaussians, compute_cumulative_intersects
+
+    s = scene(1, num_points=3000, seed=61, img_w=200, img_h=136)
+    s.means[:, :2] *= 1.6
+    bw = 16
+    pb = project_and_bin(s, bw)
+    c, e, d = pb["centers"], pb["extents"], pb["depths"]
+    tb = tile_bounds(s.img_w, s.img_h, bw)
+    nt = _C.num_tiles_hit(c, e, s.img_h, s.img_w, bw, wrapped=counts == "wrapped")
+    o_nt = oracle.num_tiles_hit(np_(c), np_(e), s.img_h, s.img_w, bw, wrapped=counts == "wrapped")
+    assert np.array_equal(np_(nt), o_nt)
+    m, cum = compute_cumulative_intersects(nt)
+    log(f"wrapped binning ({counts} counts): I = {m}")
+    keys, ids, ks, isd, bins = bin_and_sort_gaussians(s.num_points, m, c, e, d, cum, tb, bw,
+                                                      wrapped=True)
+    o_keys, o_ids = oracle.map_gaussian_to_intersects(np_(c), np_(e), np_(d), np_(cum), tb, bw,
+                                                      wrapped=True)
+    assert np.array_equal(np_(keys), o_keys)
+    assert np.array_equal(np_(ids), o_ids)
+    o_ks, o_is = oracle.sort_pairs(o_keys, o_ids)
+    assert np.array_equal(np_(ks), o_ks)
+    assert np.array_equal(np_(isd), o_is)
+    assert np.array_equal(np_(bins), oracle.tile_bin_edges(o_ks, tb[0] * tb[1]))
+    # Gaussians crossing an edge really do wrap: some tiles get entries from
+    # the far side of the image
+    assert (np_(nt) > oracle.num_tiles_hit(np_(c), np_(e), s.img_h, s.img_w, bw)).any() == (
+        counts == "wrapped")
+
+
 def test_sort_full_64bit_signed():
     from gstex_cuda_amd import cuda as _C
 

@@ -159,6 +159,30 @@ def test_zero_slot_quirk_sorts_first_in_tile0(oracle):
     assert bins[0, 0] == 0 and bins[0, 1] >= 1
 
 
+def test_wrapped_binning_hand_case(oracle):
+    # forward.cu:34-62 with get_tile_bbox_wrapped (helpers.cuh:53-73,94-111) on a
+    # 4 x 3 tile grid, bw 16.  Gaussian 0: centre (5, 5), extent (10, 3) spans
+    # tiles x, y in [-1, 1) (a lower corner <= 0 moves one tile further), which
+    # wrap to x 3, 0 and y 2, 0: row by row tiles 11, 8, 3, 0.  Gaussian 1:
+    # centre (40, 20), extent (2, 2) is the single tile (2, 1) = 6.
+    c = np.array([[5.0, 5.0], [40.0, 20.0]], np.float32)
+    e = np.array([[10.0, 3.0], [2.0, 2.0]], np.float32)
+    d = np.array([2.0, 3.0], np.float32)
+    dbits = d.view(np.int32).astype(np.int64)
+    cnt = oracle.num_tiles_hit(c, e, 48, 64, 16, wrapped=True)
+    assert cnt.tolist() == [4, 1]
+    assert oracle.num_tiles_hit(c, e, 48, 64, 16).tolist() == [1, 1]
+    keys, ids = oracle.map_gaussian_to_intersects(c, e, d, np.cumsum(cnt).astype(np.int32),
+                                                  (4, 3, 1), 16, wrapped=True)
+    tiles = np.array([11, 8, 3, 0, 6], np.int64)
+    assert np.array_equal(keys, (tiles << 32) | dbits[[0, 0, 0, 0, 1]])
+    assert ids.tolist() == [0, 0, 0, 0, 1]
+    # with the clamped counts each Gaussian fills only its own window
+    keys, ids = oracle.map_gaussian_to_intersects(c, e, d, np.array([1, 2], np.int32),
+                                                  (4, 3, 1), 16, wrapped=True)
+    assert np.array_equal(keys, (np.array([11, 6], np.int64) << 32) | dbits)
+
+
 def test_sort_matches_numpy_stable(oracle):
     rng = np.random.default_rng(11)
     keys = (rng.integers(0, 40, 5000).astype(np.int64) << 32) | rng.integers(0, 50, 5000)
@@ -228,6 +252,45 @@ def test_single_splat_closed_form(oracle):
         np.testing.assert_allclose(f["out_depth"][hit], k["mean"][2], rtol=1e-5)
         assert np.all(f["out_depth"][~hit] == 0) and np.all(f["depth_idx"][~hit] == -1)
         assert np.all(f["out_reg"] == 0)
+
+
+@pytest.mark.parametrize("outline_q,white", [(0, False), (6, False), (6, True), (11, True)])
+def test_alpha_visualisation_single_splat_geometry(oracle, outline_q, white):
+    # texture.cu:56-65,192-241 with local_outline (texture_helpers.cuh:390-417):
+    # a camera-facing circular splat at depth d with scale s is drawn with
+    # alpha 0.99 inside radius R = B s f / d pixels (alpha bound B); a pixel
+    # whose 7 x 7 neighbourhood reaches outside the circle within the outline
+    # bound is drawn white (bit 24) or not at all.  Expected values come from
+    # circle geometry on the pixel grid, not from the oracle's ray solves.
+    from helpers import camera_facing_scene
+
+    W = H = 64
+    f, d, sc, B = 64.0, 4.0, 0.3, 2.5
+    R = B * sc * f / d  # 12 px; centres at half-integer offsets never lie on it
+    col, tex, bg = [0.2, 0.4, 0.6], [0.3, 0.5, 0.7], [0.1, 0.1, 0.1]
+    s = camera_facing_scene([[0.0, 0.0, d]], [[sc, sc, 1e-3]], [1.0], [col], W, H, f, W / 2,
+                            H / 2, texel=tex, background=bg)
+    settings = (1 << 8) | (1 << 16) | (int(B * 8) << 17) | (outline_q << 26) | (white << 24)
+    jj, ii = np.meshgrid(np.arange(W) + 0.5 - W / 2, np.arange(H) + 0.5 - H / 2)
+    inside = jj ** 2 + ii ** 2 <= R * R
+    dis = np.full((H, W), 1000.0)
+    for dx in range(-3, 4):
+        for dy in range(-3, 4):
+            out = (jj + dx) ** 2 + (ii + dy) ** 2 > R * R
+            dis = np.where(out, np.minimum(dis, np.hypot(dx, dy)), dis)
+    plain = dis > outline_q / 4.0
+    a = float(np.float32(0.99))  # the reference's 0.99f
+    rgb = np.where(plain[..., None], a * np.array(col), a if white else 0.0)
+    img = np.where(inside[..., None], rgb + (1 - a) * np.array(bg), np.array(bg))
+    texo = np.where(inside[..., None],
+                    np.where(plain[..., None], a * np.array(tex), a if white else 0.0), 0.0)
+    outlined = int((~plain & inside).sum())
+    assert inside.sum() > 400 and (outlined == 0 if outline_q == 0 else outlined > 50)
+    for prec, tol in (("f64", 1e-9), ("f32", 2e-5)):
+        f_ = oracle.texture_forward(oracle_scene(oracle, s, settings=settings), prec)
+        np.testing.assert_allclose(f_["out_img"], img, atol=tol)
+        np.testing.assert_allclose(f_["out_texture"], texo, atol=tol)
+        np.testing.assert_allclose(f_["final_Ts"], np.where(inside, 1 - a, 1.0), atol=tol)
 
 
 def test_two_splats_compositing_and_distortion(oracle):
