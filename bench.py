@@ -55,6 +55,9 @@ def parse():
                          "captured as one graph (0 disables)")
     ap.add_argument("--train-steps", type=int, default=10,
                     help="timed steps of the training-step variant (0 = skip)")
+    ap.add_argument("--config-steps", type=int, default=5,
+                    help="timed steps of each of the 1600x1200 regularised, 16-view orbit and "
+                         "8-view 3M bf16 workloads (0 = skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the timed region, run one device-to-device copy of this many MiB "
                          "(a kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE)")
@@ -143,6 +146,78 @@ def train_variant(scene, dev, steps, warmup):
            "steps": steps, "dtype": "f32", "optimizer": "adam (fused, one kernel per tensor)",
            "loss": round(ts.loss(), 6)}
     del ts
+    torch.cuda.empty_cache()
+    return out
+
+
+def _timed(fn, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def regularised_variant(dev, steps, warmup):
+    """Iterations/s of fwd+bwd on one 1600x1200 view of 1M untextured Gaussians
+    (one texel each) with the L2 loss plus the distortion (weight 100) and
+    normal (weight 0.05) regularisers."""
+    import torch
+
+    from gstex_cuda_amd import synthetic
+    from gstex_cuda_amd.pipeline import scene_to_step
+
+    scene = synthetic.make_scene(3)
+    step = scene_to_step(scene, dev)
+    target = torch.from_numpy(scene.target).to(dev)
+    elapsed = _timed(lambda: step.regularised_step(target, 100.0, 0.05), steps, warmup)
+    out = {"workload": f"1 view {scene.img_w}x{scene.img_h}, {scene.num_points} untextured "
+                       "Gaussians, distortion 100 + normal 0.05",
+           "iter_per_s": round(steps / elapsed, 3), "ms_per_step": round(1e3 * elapsed / steps, 4),
+           "intersects": int(step.num_intersects), "dtype": "f32", "steps": steps}
+    del step
+    torch.cuda.empty_cache()
+    return out
+
+
+def multiview_variant(dev, workload, steps, warmup):
+    """Batches/s of a multi-view workload captured as one graph: 16 orbit views
+    of mixed resolution over 1M Gaussians in float32 (workload 4), or 8 views at
+    1920x1080 over 3M Gaussians with bfloat16 texels and SH coefficients and the
+    float64 ray/plane solve for near-edge-on pairs (workload 5)."""
+    import torch
+
+    from gstex_cuda_amd import cuda as _C
+    from gstex_cuda_amd import synthetic
+    from gstex_cuda_amd.pipeline import scene_to_multiview
+
+    bf16 = workload == 5
+    scene = synthetic.make_scene(workload)
+    views = synthetic.workload_views(workload, scene)
+    prev = _C.set_precise_edge_on(bf16)
+    try:
+        mv = scene_to_multiview(scene, views, dev, bf16=bf16)
+        mv.capture()
+        elapsed = _timed(mv.replay, steps, warmup)
+        ok = mv.capacity_ok()
+        isects = [int(v.num_intersects) for v in mv.views]
+    finally:
+        _C.set_precise_edge_on(prev)
+    res = sorted({f"{v.img_w}x{v.img_h}" for v in views})
+    out = {"workload": f"{len(views)} views ({', '.join(res)}), {scene.num_points} Gaussians, "
+                       f"{scene.texture.shape[0]} texels, SH degree {scene.sh_degree}"
+                       + (", bf16 storage, f64 edge-on solve" if bf16 else ", f32"),
+           "batches_per_s": round(steps / elapsed, 3),
+           "views_per_s": round(len(views) * steps / elapsed, 3),
+           "ms_per_batch": round(1e3 * elapsed / steps, 4),
+           "intersects_per_batch": int(sum(isects)), "intersects_min_max": [min(isects), max(isects)],
+           "graph": True, "capacity_ok": bool(ok), "steps": steps}
+    del mv
     torch.cuda.empty_cache()
     return out
 
@@ -248,6 +323,11 @@ def main():
         t = train_variant(scene, dev, args.train_steps, max(args.warmup // 2, 1))
         t["vs_plain"] = round(t["iter_per_s"] / value, 3)
         variants["train_step_adam"] = t
+    if args.config_steps > 0:
+        w = max(args.warmup // 2, 1)
+        variants["c3_regularised_1600x1200_1m"] = regularised_variant(dev, args.config_steps, w)
+        variants["c4_orbit_16_views_1m_graph"] = multiview_variant(dev, 4, args.config_steps, w)
+        variants["c5_bf16_8x1080p_3m_graph"] = multiview_variant(dev, 5, args.config_steps, w)
     cpu = None
     if args.cpu_baseline_seconds > 0:
         cpu = cpu_baseline(scene, step, args.cpu_baseline_seconds)

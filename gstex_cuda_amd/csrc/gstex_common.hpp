@@ -125,6 +125,33 @@ __device__ __forceinline__ void quat_axes(float w, float x, float y, float z, V3
     a3 = V3{2.f * (x * z + w * y), 2.f * (y * z - w * x), 1.f - 2.f * (x * x + y * y)};
 }
 
+// The same three functions with every product and sum rounded on its own (no
+// fused multiply-add).  They feed the integer results of the projection, the
+// tile rectangles and the depth sort keys, which then follow the written
+// operation order bit for bit whatever the compiler would contract.
+__device__ __forceinline__ V3 xform43_strict(const float* m, V3 p) {
+#pragma clang fp contract(off)
+    return V3{m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3],
+              m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+              m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]};
+}
+
+__device__ __forceinline__ void project_pix_strict(float fx, float fy, float cx, float cy, V3 pv,
+                                                   float& u, float& v) {
+#pragma clang fp contract(off)
+    float rw = 1.f / (pv.z + 1e-6f);
+    u = pv.x * rw * fx + cx;
+    v = pv.y * rw * fy + cy;
+}
+
+__device__ __forceinline__ void quat_axes_strict(float w, float x, float y, float z, V3& a1,
+                                                 V3& a2, V3& a3) {
+#pragma clang fp contract(off)
+    a1 = V3{1.f - 2.f * (y * y + z * z), 2.f * (x * y + w * z), 2.f * (x * z - w * y)};
+    a2 = V3{2.f * (x * y - w * z), 1.f - 2.f * (x * x + z * z), 2.f * (y * z + w * x)};
+    a3 = V3{2.f * (x * z + w * y), 2.f * (y * z - w * x), 1.f - 2.f * (x * x + y * y)};
+}
+
 // Adjoint of quat_axes: g1, g2, g3 are the gradients of the three axes.
 __device__ __forceinline__ void quat_axes_vjp(float w, float x, float y, float z, V3 g1, V3 g2,
                                               V3 g3, float out[4]) {

@@ -8,20 +8,23 @@ namespace gstex {
 
 // Screen AABB of the four corners mean +- 3*g*s.x*ax1 +- 3*g*s.y*ax2, each
 // projected with view z clamped to >= 0.01; a mean at view z <= 0.01 gives
-// (projected mean, zero extent).  Semantics of get_aabb_2d.cu:11-89.
+// (projected mean, zero extent).  Semantics of get_aabb_2d.cu:11-89.  No
+// multiply-add is fused here: the AABB and depth become tile counts and sort
+// keys, so they keep the written float32 operation order exactly.
 __device__ __forceinline__ void aabb_one(const float* __restrict__ means,
                                          const float* __restrict__ scales, float glob_scale,
                                          const float* __restrict__ quats, const float* vm,
                                          float fx, float fy, float cx, float cy, int i,
                                          float& ccx, float& ccy, float& ex, float& ey, float& depth) {
+#pragma clang fp contract(off)
     const float t_near = 0.01f;
     V3 p = v3(means[3 * i], means[3 * i + 1], means[3 * i + 2]);
-    V3 pv = xform43(vm, p);
+    V3 pv = xform43_strict(vm, p);
     depth = pv.z;
     const bool clipped = pv.z <= t_near;
     const float4 q = reinterpret_cast<const float4*>(quats)[i];
     V3 a1, a2, a3;
-    quat_axes(q.x, q.y, q.z, q.w, a1, a2, a3);
+    quat_axes_strict(q.x, q.y, q.z, q.w, a1, a2, a3);
     const float ell = 3.0f * glob_scale;
     const float e1 = ell * scales[3 * i], e2 = ell * scales[3 * i + 1];
     const V3 d1 = v3(e1 * a1.x, e1 * a1.y, e1 * a1.z);
@@ -32,17 +35,17 @@ __device__ __forceinline__ void aabb_one(const float* __restrict__ means,
         const float s1 = (c & 2) ? -1.f : 1.f, s2 = (c & 1) ? -1.f : 1.f;
         V3 corner = v3(p.x + s1 * d1.x + s2 * d2.x, p.y + s1 * d1.y + s2 * d2.y,
                        p.z + s1 * d1.z + s2 * d2.z);
-        V3 cv = xform43(vm, corner);
+        V3 cv = xform43_strict(vm, corner);
         cv.z = fmaxf(cv.z, t_near);
         float u, v;
-        project_pix(fx, fy, cx, cy, cv, u, v);
+        project_pix_strict(fx, fy, cx, cy, cv, u, v);
         mxx = fmaxf(mxx, u);
         mxy = fmaxf(mxy, v);
         mnx = fminf(mnx, u);
         mny = fminf(mny, v);
     }
     if (clipped) {
-        project_pix(fx, fy, cx, cy, pv, ccx, ccy);
+        project_pix_strict(fx, fy, cx, cy, pv, ccx, ccy);
         ex = 0.f;
         ey = 0.f;
     } else {
@@ -117,10 +120,10 @@ __global__ __launch_bounds__(256) void k_project_points(int n, const float* __re
                                                         float* __restrict__ depths) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    V3 pv = xform43(viewmat, v3(points[3 * i], points[3 * i + 1], points[3 * i + 2]));
+    V3 pv = xform43_strict(viewmat, v3(points[3 * i], points[3 * i + 1], points[3 * i + 2]));
     if (clip) pv.z = fminf(fmaxf(pv.z, 0.01f), 1000.0f);
     float u, v;
-    project_pix(fx, fy, cx, cy, pv, u, v);
+    project_pix_strict(fx, fy, cx, cy, pv, u, v);
     reinterpret_cast<float2*>(pix)[i] = make_float2(u, v);
     depths[i] = pv.z;
 }

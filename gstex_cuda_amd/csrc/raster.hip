@@ -68,10 +68,21 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
     // from the first live pixel at that level on, every entry is evaluated.
     constexpr float kNearDoneT = 1.0041e-4f;
     bool cull = a.boxes != nullptr;
+    // Copies of one entry in a row: the reference leaves the slots of the
+    // Gaussians it counts but never writes (zero screen extent) zeroed, so they
+    // all hold Gaussian 0 and sort to the front of tile 0.  Once an entry has
+    // left every pixel of the wave as it was (each live pixel skipped it, or
+    // it was culled), a copy of it does the same, so the windows that follow
+    // and hold nothing but copies of it are passed over, eight per read (their
+    // mask words written as zero).
+    int dup_g = -1;
+    bool dup_noop = false;
     for (int base = mentry0; base < r1; base += 64) {
         if (!__any(!done)) break;
         if (cull && __any(!done && T <= kNearDoneT)) cull = false;
         const Window w = window_candidates(a, base, r0, r1 - 1, rect, lane, cull);
+        int ev_e = -1, ev_g = -1;
+        bool ev_noop = false;
         uint64_t cand = w.cand;
         uint64_t mbits = 0;
         while (cand != 0) {
@@ -158,13 +169,54 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
                     cur_idx = e;
                 }
             }
-            if (mplane && __any(comp)) mbits |= 1ull << bit;
+            const bool any_comp = __any(comp);
+            if (any_comp) mbits |= 1ull << bit;
+            ev_e = e;
+            ev_g = g;
+            ev_noop = !any_comp;
             if (cull && __any(!done && T <= kNearDoneT)) {
                 cull = false;
                 cand = bit == 63 ? 0ull : w.in_range & ~((2ull << bit) - 1);
             }
         }
         if (mplane && lane == 0) mplane[mword0 + ((base - mentry0) >> 6)] = mbits;
+        // what a copy of the window's last entry would do: as the evaluation
+        // of that entry found, or nothing if it was culled (culling still on)
+        bool dup_run = false;
+        if (w.in_range != 0) {
+            const int last = 63 - __builtin_clzll(w.in_range);
+            if (ev_e == base + last) {
+                dup_g = ev_g;
+                dup_noop = ev_noop;
+            } else {
+                dup_g = __builtin_amdgcn_readlane(w.gid, last);
+                dup_noop = cull;
+            }
+            // a run of copies shows as two equal ids at the window's end
+            // (distinct Gaussians otherwise), so the common case reads nothing
+            dup_run = last > 0 && ((w.in_range >> (last - 1)) & 1ull) &&
+                      __builtin_amdgcn_readlane(w.gid, last - 1) == dup_g;
+        }
+        // pass over the windows made only of copies of that entry, eight
+        // windows per read
+        if (dup_noop && dup_run && __any(!done)) {
+            int nb = base + 64;
+            while (nb < r1) {
+                const int cnt = min(r1 - nb, 512);
+                const int* ids = a.ids_sorted + nb + lane;
+                unsigned same = 0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const bool s_k = 64 * k + lane >= cnt || ids[64 * k] == dup_g;
+                    if (__all(s_k)) same |= 1u << k;
+                }
+                const int lead = min(__builtin_ctz(~same), (cnt + 63) >> 6);
+                if (mplane && lane < lead) mplane[mword0 + ((nb - mentry0) >> 6) + lane] = 0ull;
+                nb += 64 * lead;
+                if (lead < 8) break;
+            }
+            base = nb - 64;
+        }
     }
     if (!inside) return;
     const int pid = i * a.img_w + j;

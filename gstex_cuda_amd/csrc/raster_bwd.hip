@@ -333,16 +333,44 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
     // only the entries the forward composited for this wave are visited;
     // without them, the entries whose screen box meets the wave's pixels (a
     // culled entry has alpha < 1/255 at all of them, so it adds nothing).
-    const bool use_masks = a.wave_masks != nullptr;
-    const GSTEX_CONST uint64_t* mplane =
-        use_masks ? as_const_space(a.wave_masks) + (size_t)wave_index() * a.mask_plane
-                  : nullptr;
     const int mword0 = (r0 >> 6) + tile, mentry0 = r0 & ~63;
     const WaveRect rect = wave_rect(inside, j, i);
+    if (a.wave_masks != nullptr) {
+        // The mask words of 64 windows are read at once, lane k holding window
+        // top - k, so windows this wave never composited (long runs of them in
+        // lists the forward culled or passed over) cost no read of their ids.
+        const size_t plane = (size_t)wave_index() * a.mask_plane + mword0;
+        const uint64_t* mplane = a.wave_masks + plane;
+        const GSTEX_CONST uint64_t* mplane_s = as_const_space(a.wave_masks) + plane;
+        const int top = e_hi >= r0 ? ((e_hi & ~63) - mentry0) >> 6 : -1;
+        for (int wt = top; wt >= 0; wt -= 64) {
+            const int wi = wt - lane;
+            const int wb = mentry0 + (wi << 6);
+            const int lo = max(r0 - wb, 0), hi = min(e_hi - wb, 63);
+            bool any = false;
+            if (wi >= 0 && hi >= lo) any = (mplane[wi] & ((~0ull >> (63 - hi)) & (~0ull << lo))) != 0;
+            uint64_t nz = __ballot(any);
+            while (nz != 0) {
+                const int k = __ffsll((unsigned long long)nz) - 1;
+                nz &= nz - 1;
+                // the word again, through the scalar path: no vector register
+                // holds it while the entries are processed
+                const int base = mentry0 + ((wt - k) << 6);
+                const Window w = window_candidates(a, base, r0, e_hi, rect, lane, false);
+                uint64_t m = mplane_s[wt - k] & w.in_range;
+                while (m != 0) {
+                    const int bit = 63 - __builtin_clzll(m);
+                    m &= ~(1ull << bit);
+                    const int g = __builtin_amdgcn_readlane(w.gid, bit);
+                    entry(base + bit, g, splat_record(a, g));
+                }
+            }
+        }
+        return;
+    }
     for (int base = e_hi >= r0 ? (e_hi & ~63) : mentry0 - 64; base >= mentry0; base -= 64) {
-        const Window w = window_candidates(a, base, r0, e_hi, rect, lane,
-                                           !use_masks && a.boxes != nullptr);
-        uint64_t m = use_masks ? (mplane[mword0 + ((base - mentry0) >> 6)] & w.in_range) : w.cand;
+        const Window w = window_candidates(a, base, r0, e_hi, rect, lane, a.boxes != nullptr);
+        uint64_t m = w.cand;
         while (m != 0) {
             const int bit = 63 - __builtin_clzll(m);
             m &= ~(1ull << bit);

@@ -123,6 +123,8 @@ class RenderStep:
         self.v_tex = torch.empty((H, W, self.C), **f32)
         self.v_zero1 = torch.zeros((H, W), **f32)
         self.v_zero3 = torch.zeros((H, W, 3), **f32)
+        self.v_reg = None  # regulariser cotangents, allocated on first use
+        self.v_normal = None
         # gradients (reference layouts)
         self.grads = dict(
             v_colors=torch.empty((n, 3), **f32), v_opacity=torch.empty((n, 1), **f32),
@@ -219,6 +221,32 @@ class RenderStep:
             self.v_tex.zero_()
             self.v_tex[..., :3].copy_(self.v_img)
 
+    def regularised_loss_grad(self, target: torch.Tensor, reg_weight: float = 100.0,
+                              normal_weight: float = 0.05):
+        """Cotangents of mean((out_img + out_texture - target)^2)
+        + reg_weight * mean(out_reg)
+        + normal_weight * mean(n_x^2 + n_y^2 + (1 - n_z)^2),
+        the distortion and normal terms of example.py:199-207 with weights;
+        returns (v_reg, v_normal) for ``backward``."""
+        self.l2_loss_grad(target)
+        H, W = self.cam.img_h, self.cam.img_w
+        if self.v_reg is None:
+            self.v_reg = torch.empty_like(self.v_zero1)
+            self.v_normal = torch.empty_like(self.v_zero3)
+        self.v_reg.fill_(reg_weight / (H * W))
+        w = normal_weight / (H * W)
+        torch.mul(self.out_normal, 2.0 * w, out=self.v_normal)
+        self.v_normal[..., 2].sub_(2.0 * w)
+        return self.v_reg, self.v_normal
+
+    def regularised_step(self, target: torch.Tensor, reg_weight: float = 100.0,
+                         normal_weight: float = 0.05, timer: Optional[StageTimer] = None):
+        """Forward, gradient of the L2 loss plus the distortion and normal
+        regularisers (``regularised_loss_grad``), backward."""
+        self.forward(timer)
+        v_reg, v_normal = self.regularised_loss_grad(target, reg_weight, normal_weight)
+        self.backward(timer, v_reg=v_reg, v_normal=v_normal)
+
     def backward(self, timer: Optional[StageTimer] = None, v_depth=None, v_reg=None,
                  v_alpha=None, v_normal=None):
         p, cam, s, g = self.p, self.cam, self._stream(), self.grads
@@ -311,12 +339,18 @@ class MultiViewStep:
     whole batch as one graph, so ``replay()`` costs one launch from the host.
     The capacities come from one eager pass per view (``capacity_factor`` of
     the measured intersect count); ``replay_checked()`` verifies them.
+
+    The views are dealt round-robin onto ``streams`` HIP streams (default
+    priority) forked from and joined back into the caller's stream, so the
+    tiles of one view fill the compute units another view's longest tiles
+    leave idle; in a captured graph they become parallel branches.  Every view
+    owns its buffers, so the order across streams does not change any result.
     """
 
     def __init__(self, params: Dict[str, torch.Tensor], cams, targets, block_width: int = 16,
                  settings: int = 1 << 8, background: Optional[torch.Tensor] = None,
                  sh_degree: int = -1, capacity_factor: float = 1.1,
-                 use_wave_masks: bool = True):
+                 use_wave_masks: bool = True, streams: int = 4):
         self.targets = list(targets)
         kw = dict(block_width=block_width, settings=settings, background=background,
                   sh_degree=sh_degree, use_wave_masks=use_wave_masks)
@@ -329,10 +363,23 @@ class MultiViewStep:
             self.views.append(RenderStep(params, cam, capacity=cap, **kw))
         self.grads = {k: torch.zeros_like(v) for k, v in self.views[0].grads.items()}
         self.graph = None
+        dev = self.views[0].dev
+        k = min(max(int(streams), 1), len(self.views))
+        self.streams = [torch.cuda.Stream(dev) for _ in range(k)] if k > 1 else []
 
     def step(self):
-        for v, tgt in zip(self.views, self.targets):
-            v.step(tgt)
+        if self.streams:
+            main = torch.cuda.current_stream(self.views[0].dev)
+            for st in self.streams:
+                st.wait_stream(main)
+            for i, (v, tgt) in enumerate(zip(self.views, self.targets)):
+                with torch.cuda.stream(self.streams[i % len(self.streams)]):
+                    v.step(tgt)
+            for st in self.streams:
+                main.wait_stream(st)
+        else:
+            for v, tgt in zip(self.views, self.targets):
+                v.step(tgt)
         for k, acc in self.grads.items():
             acc.copy_(self.views[0].grads[k])
             for v in self.views[1:]:

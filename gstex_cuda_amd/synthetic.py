@@ -10,6 +10,11 @@ Workloads (sizes are the defaults; tests shrink them):
      ~N(0,0.1)), a 2M-texel budget split by projected area into 1x1..16x16
      grids, fx=fy=1080
   3  one 1600x1200 view, 1M untextured Gaussians (one texel each), fx=fy=1200
+  4  1M Gaussians, SH degree 3, a 4M-texel budget; rendered from 16 cameras on
+     a sphere of radius 8 looking at the origin (``make_orbit_views``), each
+     at 512x512, 960x540, 1280x720 or 1920x1080 with fx=fy=H
+  5  3M Gaussians, SH degree 3, an 8M-texel budget; rendered from 8 cameras at
+     1920x1080 jittered around the identity pose (``make_views``)
 Shared distributions: means xy~U[-1,1]; log-scales of the two in-plane axes
 ~U[-4,-2]; uniformly random unit quaternions; opacity = sigmoid(N(0,1));
 uv0 = (0.5, 0.5); umap = ax1 / (6 s_x), vmap = ax2 / (6 s_y) so the +-3 sigma
@@ -49,7 +54,15 @@ WORKLOADS: Dict[int, SceneSpec] = {
                  sh_degree=3, texel_budget=2_000_000),
     3: SceneSpec(num_points=1_000_000, img_w=1600, img_h=1200, focal=1200.0,
                  z_range=(2.0, 20.0), texel_side=1),
+    4: SceneSpec(num_points=1_000_000, img_w=1920, img_h=1080, focal=1080.0,
+                 z_range=(2.0, 20.0), sh_degree=3, texel_budget=4_000_000),
+    5: SceneSpec(num_points=3_000_000, img_w=1920, img_h=1080, focal=1080.0,
+                 z_range=(2.0, 20.0), sh_degree=3, texel_budget=8_000_000),
 }
+
+# (W, H) the orbit cameras of the multi-view workload draw from
+ORBIT_RESOLUTIONS = [(512, 512), (960, 540), (1280, 720), (1920, 1080)]
+
 
 
 def quat_axes_np(q: np.ndarray):
@@ -231,3 +244,65 @@ def make_views(scene: Scene, num_views: int, seed: int = 7, max_angle_deg: float
                           img_w=int(W),
                           target=rng.uniform(0.0, 1.0, (H, W, 3)).astype(np.float32)))
     return views
+
+
+def _look_at(eye: np.ndarray, target: np.ndarray) -> np.ndarray:
+    """Camera-to-world 4x4 (float64) at ``eye`` whose +z axis points at
+    ``target``; x and y complete a right-handed frame (det R = +1)."""
+    z = target - eye
+    z = z / np.linalg.norm(z)
+    ref = np.array([0.0, 1.0, 0.0]) if abs(z[1]) < 0.95 else np.array([1.0, 0.0, 0.0])
+    x = np.cross(z, ref)
+    x /= np.linalg.norm(x)
+    y = np.cross(z, x)
+    c2w = np.eye(4)
+    c2w[:3, 0], c2w[:3, 1], c2w[:3, 2], c2w[:3, 3] = x, y, z, eye
+    return c2w
+
+
+def make_orbit_views(num_views: int = 16, radius: float = 8.0, seed: int = 4,
+                     resolutions: Optional[list] = None, jitter: float = 0.05) -> list:
+    """Seeded cameras on a sphere of ``radius`` looking at the origin.
+
+    Directions follow a Fibonacci lattice, each perturbed by N(0, ``jitter``)
+    per coordinate and renormalised; the radius is scaled by U[0.95, 1.05] and
+    the look-at point drawn from U[-0.1, 0.1]^3.  Each view draws its (W, H)
+    from ``resolutions`` (default 512x512, 960x540, 1280x720, 1920x1080) and
+    uses fx = fy = H, cx = W/2, cy = H/2.  Targets are U[0,1]."""
+    rng = np.random.default_rng(seed)
+    res = ORBIT_RESOLUTIONS if resolutions is None else resolutions
+    golden = np.pi * (3.0 - np.sqrt(5.0))
+    views = []
+    for i in range(num_views):
+        yy = 1.0 - 2.0 * (i + 0.5) / num_views
+        r = np.sqrt(max(1.0 - yy * yy, 0.0))
+        d = np.array([r * np.cos(golden * i), yy, r * np.sin(golden * i)])
+        d = d + rng.normal(0.0, jitter, 3)
+        d /= np.linalg.norm(d)
+        eye = d * radius * rng.uniform(0.95, 1.05)
+        c2w = _look_at(eye, rng.uniform(-0.1, 0.1, 3))
+        W, H = res[int(rng.integers(0, len(res)))]
+        views.append(View(viewmat=np.linalg.inv(c2w).astype(np.float32),
+                          c2w=c2w.astype(np.float32), fx=float(H), fy=float(H), cx=W / 2.0,
+                          cy=H / 2.0, img_h=int(H), img_w=int(W),
+                          target=rng.uniform(0.0, 1.0, (H, W, 3)).astype(np.float32)))
+    return views
+
+
+def workload_views(workload: int, scene: Scene) -> list:
+    """The cameras of a multi-view workload: 16 orbit views for 4, 8 views
+    around the identity pose at the scene's 1920x1080 for 5."""
+    if workload == 4:
+        return make_orbit_views(16, radius=8.0, seed=4)
+    if workload == 5:
+        return make_views(scene, 8, seed=5)
+    raise ValueError(f"workload {workload} has a single view")
+
+
+def view_scene(scene: Scene, view: View) -> Scene:
+    """The scene seen from ``view`` (shares every parameter array)."""
+    import dataclasses
+
+    return dataclasses.replace(scene, viewmat=view.viewmat, c2w=view.c2w, fx=view.fx,
+                               fy=view.fy, cx=view.cx, cy=view.cy, img_h=view.img_h,
+                               img_w=view.img_w, target=view.target)

@@ -19,7 +19,7 @@ import torch
 
 from gpu_helpers import (DEV, gpu_backward, gpu_forward, grad_close, log, np_, oracle_with,
                          project_and_bin, random_grads, raster_inputs, to_dev)
-from helpers import camera_facing_scene, frac_close, load_kat, psnr, tile_bounds
+from helpers import camera_facing_scene, frac_close, load_kat, psnr, tile_bounds, view_depths
 
 pytestmark = pytest.mark.gpu
 
@@ -110,6 +110,27 @@ def test_binning_c2_scale(oracle):
     # full-resolution 1080p with 200k Gaussians: exact against the oracle
     s = scene(2, num_points=200_000)
     assert _binning_case(oracle, s, 16) > 1_000_000
+
+
+@pytest.mark.parametrize("view", [0, 3, 6, 13])
+def test_projection_bit_exact_rotated_cameras(oracle, view):
+    # orbit cameras with general rotations (products and sums that a fused
+    # multiply-add would round differently): AABBs, depths, tile counts and the
+    # binning equal the oracle's bit for bit, Gaussians behind the camera included
+    from gstex_cuda_amd import synthetic
+
+    s = scene(4, num_points=200_000)
+    s = synthetic.view_scene(s, synthetic.make_orbit_views(16, radius=8.0, seed=4)[view])
+    pb = project_and_bin(s)
+    c, e = oracle.get_aabb_2d(s.means, s.scales, 1.0, s.quats, s.viewmat, s.fx, s.fy, s.cx, s.cy)
+    assert np.array_equal(np_(pb["centers"]), c)
+    assert np.array_equal(np_(pb["extents"]), e)
+    assert np.array_equal(np_(pb["depths"]), view_depths(s.means, s.viewmat))
+    nt = oracle.num_tiles_hit(c, e, s.img_h, s.img_w, 16)
+    assert np.array_equal(np_(pb["num_tiles_hit"]), nt)
+    log(f"view {view} {s.img_w}x{s.img_h} I={int(nt.sum())} behind="
+        f"{int((np_(pb['depths']) < 0).sum())}")
+    _binning_case(oracle, s, 16)
 
 
 def test_binning_screen_sized_splats(oracle):
