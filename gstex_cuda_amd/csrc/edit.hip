@@ -50,11 +50,13 @@ __global__ __launch_bounds__(256) void k_texture_edit(RasterArgs a, EditIn ed) {
     bool done = !inside;
     float T = 1.f;
     const WaveRect rect = wave_rect(inside, j, i);
-    constexpr float kNearDoneT = 1.0041e-4f;  // see k_raster_forward
-    bool cull = a.boxes != nullptr;
+    // a pixel with T (1 - 1/255) <= 1e-4 can take no further entry (each one
+    // with alpha >= 1/255 ends it), so it stops at once and culled entries
+    // never matter (as in k_raster_forward)
+    constexpr float kKeep = 1.f - 1.f / 255.f;
+    const bool cull = a.boxes != nullptr;
     for (int base = r0 & ~63; base < r1; base += 64) {
         if (!__any(!done)) break;
-        if (cull && __any(!done && T <= kNearDoneT)) cull = false;
         const Window w = window_candidates(a, base, r0, r1 - 1, rect, lane, cull);
         uint64_t cand = w.cand;
         while (cand != 0) {
@@ -81,6 +83,7 @@ __global__ __launch_bounds__(256) void k_texture_edit(RasterArgs a, EditIn ed) {
                         chart_taps(s, ch, u, v, true, wt, row, fu, fv);
                     }
                     T = next_T;
+                    if (T * kKeep <= 1e-4f) done = true;
                 }
             }
             if (!__any(hit) || !ch.ok) continue;

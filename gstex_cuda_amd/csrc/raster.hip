@@ -63,11 +63,14 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
     const int mword0 = (r0 >> 6) + tile, mentry0 = r0 & ~63;
     const WaveRect rect = wave_rect(inside, j, i);
 
-    // A culled entry has alpha < 1/255 at every pixel of the wave, so it can
-    // still end a pixel (T (1 - alpha) <= 1e-4) only once T <= 1e-4 / (1 - 1/255);
-    // from the first live pixel at that level on, every entry is evaluated.
-    constexpr float kNearDoneT = 1.0041e-4f;
-    bool cull = a.boxes != nullptr;
+    // A pixel whose T (1 - 1/255) <= 1e-4 can composite nothing more: every
+    // entry with alpha >= 1/255 ends it (rounding is monotonic), and ending
+    // changes none of its outputs.  Such a pixel stops at once.  Every live
+    // pixel then has T (1 - 1/255) > 1e-4, so an entry with alpha < 1/255 at
+    // all of the wave's pixels (a culled one) can neither composite nor end
+    // any of them, and culling stays on for the whole walk.
+    constexpr float kKeep = 1.f - 1.f / 255.f;
+    const bool cull = a.boxes != nullptr;
     // Copies of one entry in a row: the reference leaves the slots of the
     // Gaussians it counts but never writes (zero screen extent) zeroed, so they
     // all hold Gaussian 0 and sort to the front of tile 0.  Once an entry has
@@ -79,7 +82,6 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
     bool dup_noop = false;
     for (int base = mentry0; base < r1; base += 64) {
         if (!__any(!done)) break;
-        if (cull && __any(!done && T <= kNearDoneT)) cull = false;
         const Window w = window_candidates(a, base, r0, r1 - 1, rect, lane, cull);
         int ev_e = -1, ev_g = -1;
         bool ev_noop = false;
@@ -167,6 +169,7 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
                     s2 += vis * dval * dval;
                     T = next_T;
                     cur_idx = e;
+                    if (T * kKeep <= 1e-4f) done = true;
                 }
             }
             const bool any_comp = __any(comp);
@@ -174,14 +177,10 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
             ev_e = e;
             ev_g = g;
             ev_noop = !any_comp;
-            if (cull && __any(!done && T <= kNearDoneT)) {
-                cull = false;
-                cand = bit == 63 ? 0ull : w.in_range & ~((2ull << bit) - 1);
-            }
         }
         if (mplane && lane == 0) mplane[mword0 + ((base - mentry0) >> 6)] = mbits;
         // what a copy of the window's last entry would do: as the evaluation
-        // of that entry found, or nothing if it was culled (culling still on)
+        // of that entry found, or nothing if it was culled
         bool dup_run = false;
         if (w.in_range != 0) {
             const int last = 63 - __builtin_clzll(w.in_range);

@@ -476,3 +476,33 @@ def test_texture_edit_channel_identities_and_depth_windows(oracle):
     near = oracle.texture_edit(sc, img, al, lo, d, "f64")
     far = oracle.texture_edit(sc, img, al, np.nextafter(d, np.float32(np.inf)), hi, "f64")
     np.testing.assert_allclose(near + far, full, rtol=1e-12, atol=1e-12)
+
+
+def test_saturated_pixel_stops_exactly():
+    """A pixel with fl(T (1 - 1/255)) <= 1e-4 is ended by every entry with
+    alpha >= 1/255 (texture.cu:213-221: the end test precedes compositing), so
+    the raster stops it at once; a live pixel is never ended by an entry with
+    alpha < 1/255, so culled entries never matter.  Checked in float32 on
+    random pairs and on the float neighbourhood of the boundary."""
+    f = np.float32
+    keep = f(1) - f(1) / f(255)
+    lim = f(1e-4)
+    amin = f(1) / f(255)
+    rng = np.random.default_rng(0)
+    # every float T in a band around the boundary, with the extreme alphas
+    T = np.arange(0, 120000, dtype=np.int64) + np.float32(9.99e-5).view(np.int32)
+    T = T.astype(np.int32).view(np.float32)
+    stopped = (T * keep) <= lim
+    assert stopped.any() and (~stopped).any()
+    for alpha in (amin, np.nextafter(amin, f(1)), f(0.5), f(0.99)):
+        assert np.all((T[stopped] * (f(1) - alpha)) <= lim)
+    below = np.nextafter(amin, f(0))
+    for alpha in (below, f(1e-3), f(0)):
+        assert np.all((T[~stopped] * (f(1) - alpha)) > lim)
+    # random pairs over the whole ranges
+    Tr = rng.uniform(9.9e-5, 1.02e-4, 2_000_000).astype(f)
+    ar = rng.uniform(float(amin), 0.99, 2_000_000).astype(f)
+    st = (Tr * keep) <= lim
+    assert np.all((Tr[st] * (f(1) - ar[st])) <= lim)
+    ab = rng.uniform(0.0, float(below), 2_000_000).astype(f)
+    assert np.all((Tr[~st] * (f(1) - ab[~st])) > lim)
