@@ -366,17 +366,18 @@ static hipError_t scan_exclusive(uint32_t* a, int n, uint32_t* total, uint32_t* 
     return hipGetLastError();
 }
 
-// Stable scatter of one block tile (ITEMS * 256 items) in two phases.  The
-// tile's keys and values are read once, into registers.  Ranking: each round
-// ranks 256 items (one per lane) with wave ballots and combines the four
-// waves' per-digit counts in LDS, placing every item at its stable position
-// in an LDS copy of the tile sorted by digit.  The per-wave counts and the
-// running per-digit offsets are double-buffered by round parity, so a round
-// needs two barriers: one after the counts are posted, one after they are
-// consumed (buffer r & 1 is read in round r and cleared or rewritten only in
-// round r + 1, after that round's first barrier).  Write-out: lanes take
-// consecutive LDS positions, so each digit's run goes to consecutive global
-// addresses (hist[digit][block] onward) in coalesced stores.
+// Stable scatter of one block tile (ITEMS * 256 items).  Wave w reads the
+// ITEMS * 64 consecutive items from w * ITEMS * 64 on (item r of lane l is
+// number w * ITEMS * 64 + r * 64 + l, one coalesced read per round), so the
+// order (wave, round, lane) is the input order.  Ranking needs no barrier per
+// round: each wave ranks its items round by round with ballots over the digit
+// bits against a wave-private running count per digit in LDS (all lanes read
+// the count, then the leader of each digit group adds the group size; one
+// wave's LDS operations complete in program order).  One block-wide pass then
+// turns the per-wave counts into each wave's start inside each digit's run of
+// the tile, the items are placed in an LDS copy of the tile sorted by digit,
+// and lanes write consecutive LDS positions out, so each digit's run goes to
+// consecutive global addresses (hist[digit][block] onwards).
 template <typename K, int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     int n, const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
@@ -386,8 +387,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     constexpr int kWaves = kSortThreads / kWave;
     __shared__ uint32_t gbase[256];   // global start of each digit's run for this block
     __shared__ uint32_t lstart[256];  // start of each digit's run in the LDS tile
-    __shared__ uint32_t run[2][256];  // next free LDS slot per digit, by round parity
-    __shared__ uint32_t wave_cnt[2][kWaves][256];
+    __shared__ uint32_t wrun[kWaves][256];  // per wave: running count, then start offset
     __shared__ K skeys[kTile];
     __shared__ uint32_t svals[kTile];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
@@ -395,55 +395,26 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const int ndig = 1 << bits;
     const size_t base = (size_t)blockIdx.x * kTile;
     const int count = (int)min((size_t)kTile, (size_t)n - base);
+    const int wbase = wave * ITEMS * kWave;
 
     K key[ITEMS];
     uint32_t val[ITEMS];
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
-        const int li = r * kSortThreads + tid;
+        const int li = wbase + r * kWave + lane;
         key[r] = li < count ? keys_in[base + li] : (K)0;
         val[r] = li < count ? vals_in[base + li] : 0u;
     }
-
-    // local digit histogram -> exclusive prefix (one digit per lane)
-    run[0][tid] = 0;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
-        wave_cnt[0][w][tid] = 0;
-        wave_cnt[1][w][tid] = 0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < ITEMS; ++r)
-        if (r * kSortThreads + tid < count)
-            atomicAdd(&run[0][digit_of(key[r] ^ flip, shift, mask)], 1u);
-    __syncthreads();
-    {
-        const uint32_t c = run[0][tid];
-        uint32_t v = c;
-        // inclusive scan over the 256 digits: within each wave, then across waves
-#pragma unroll
-        for (int off = 1; off < kWave; off <<= 1) {
-            const uint32_t t = __shfl_up(v, off);
-            if (lane >= off) v += t;
-        }
-        run[1][tid] = v;
-        __syncthreads();
-        uint32_t add = 0;
-        for (int w = 0; w < wave; ++w) add += run[1][w * kWave + kWave - 1];
-        const uint32_t excl = v + add - c;
-        __syncthreads();
-        lstart[tid] = excl;
-        run[0][tid] = excl;
-        gbase[tid] = tid < ndig ? hist[(size_t)tid * nblocks + blockIdx.x] : 0u;
-    }
+    for (int w = 0; w < kWaves; ++w) wrun[w][tid] = 0;
     __syncthreads();
 
+    // wave-private ranking: rank[r] = items of digit d before item r in this wave
     const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    uint32_t rank[ITEMS];
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
-        const int cur = r & 1, nxt = cur ^ 1;
-        const bool valid = r * kSortThreads + tid < count;
+        const bool valid = wbase + r * kWave + lane < count;
         const uint32_t d = valid ? digit_of(key[r] ^ flip, shift, mask) : 0u;
         uint64_t peers = __ballot(valid);
         for (int b = 0; b < bits; ++b) {
@@ -451,29 +422,47 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
             const uint64_t bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
         }
-        const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
-        const uint32_t tot = (uint32_t)__popcll(peers);
-        if (valid && rank == 0) wave_cnt[cur][wave][d] = tot;
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = run[cur][d] + rank;
+        const uint32_t before = (uint32_t)__popcll(peers & lt_mask);
+        const uint32_t seen = valid ? wrun[wave][d] : 0u;
+        if (valid && before == 0) wrun[wave][d] = seen + (uint32_t)__popcll(peers);
+        rank[r] = seen + before;
+    }
+    __syncthreads();
+    // per digit (one per thread): each wave's start inside the digit's run,
+    // and the exclusive scan of the digit totals over the tile
+    {
+        uint32_t c = 0;
 #pragma unroll
-            for (int w = 0; w < kWaves; ++w)
-                if (w < wave) pos += wave_cnt[cur][w][d];
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t t = wrun[w][tid];
+            wrun[w][tid] = c;
+            c += t;
+        }
+        uint32_t v = c;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const uint32_t t = __shfl_up(v, off);
+            if (lane >= off) v += t;
+        }
+        lstart[tid] = v;  // inclusive within the wave for now
+        gbase[tid] = tid < ndig ? hist[(size_t)tid * nblocks + blockIdx.x] : 0u;
+        __syncthreads();
+        uint32_t add = 0;
+        for (int w = 0; w < wave; ++w) add += lstart[w * kWave + kWave - 1];
+        __syncthreads();
+        lstart[tid] = v + add - c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        if (wbase + r * kWave + lane < count) {
+            const uint32_t d = digit_of(key[r] ^ flip, shift, mask);
+            const uint32_t pos = lstart[d] + wrun[wave][d] + rank[r];
             skeys[pos] = key[r];
             svals[pos] = val[r];
         }
-        {
-            uint32_t add = 0;
-#pragma unroll
-            for (int w = 0; w < kWaves; ++w) {
-                add += wave_cnt[cur][w][tid];
-                wave_cnt[nxt][w][tid] = 0;
-            }
-            run[nxt][tid] = run[cur][tid] + add;
-        }
-        __syncthreads();
     }
+    __syncthreads();
 
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
