@@ -156,20 +156,35 @@ __global__ __launch_bounds__(256) void k_bin_edges64(int m, const int64_t* __res
     }
 }
 
-// Same edges on sorted 32-bit tile ids.
+// Same edges on sorted 32-bit tile ids, four per thread (one 16-byte load
+// when the ids are 16-byte aligned, plus the id before them).
 __global__ __launch_bounds__(256) void k_bin_edges32(int m, const uint32_t* __restrict__ tiles,
-                                                     int num_tiles, int* __restrict__ bins) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    const int cur = (int)tiles[i];
-    const bool cur_ok = cur >= 0 && cur < num_tiles;
-    if (i == 0 && cur_ok) bins[2 * cur] = 0;
-    if (i == m - 1 && cur_ok) bins[2 * cur + 1] = m;
-    if (i == 0) return;
-    const int prev = (int)tiles[i - 1];
-    if (prev != cur) {
-        if (prev >= 0 && prev < num_tiles) bins[2 * prev + 1] = i;
-        if (cur_ok) bins[2 * cur] = i;
+                                                     int num_tiles, int* __restrict__ bins,
+                                                     bool vec) {
+    const int i0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+    if (i0 >= m) return;
+    int cur[4];
+    if (vec && i0 + 3 < m) {
+        const uint4 v = *reinterpret_cast<const uint4*>(tiles + i0);
+        cur[0] = (int)v.x; cur[1] = (int)v.y; cur[2] = (int)v.z; cur[3] = (int)v.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) cur[k] = i0 + k < m ? (int)tiles[i0 + k] : -1;
+    }
+    int prev = i0 > 0 ? (int)tiles[i0 - 1] : -1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int i = i0 + k;
+        if (i >= m) break;
+        const int c = cur[k];
+        const bool c_ok = c >= 0 && c < num_tiles;
+        if (i == 0 && c_ok) bins[2 * c] = 0;
+        if (i == m - 1 && c_ok) bins[2 * c + 1] = m;
+        if (i > 0 && prev != c) {
+            if (prev >= 0 && prev < num_tiles) bins[2 * prev + 1] = i;
+            if (c_ok) bins[2 * c] = i;
+        }
+        prev = c;
     }
 }
 
@@ -848,7 +863,8 @@ hipError_t bin_gaussians(int n, int cap, const float* centers, const float* exte
     e = radix_sort_pairs<uint32_t>(cap, tkey, tid_in, tkey2, (uint32_t*)ids_sorted,
                                    bits_for(num_tiles + 1), rws, rws_bytes, s);
     if (e != hipSuccess) return e;
-    k_bin_edges32<<<gm, 256, 0, s>>>(cap, tkey2, num_tiles, tile_bins);
+    k_bin_edges32<<<(cap + 1023) / 1024, 256, 0, s>>>(
+        cap, tkey2, num_tiles, tile_bins, (reinterpret_cast<uintptr_t>(tkey2) & 15) == 0);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (m_out) return hipMemcpyAsync(m_out, m_total, sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
     return hipSuccess;
