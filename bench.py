@@ -84,6 +84,21 @@ def measured_traffic(workload: int, kernel: str):
     return None, None
 
 
+def measured_valu(workload: int, kernel: str):
+    """(lane utilisation, VALU issue share) of ``kernel`` from the committed
+    rocprofv3 SQ counter summary for this workload, or (None, None)."""
+    path = os.path.join(ROOT, "profiles", f"sq_counters_c{workload}.md")
+    if not os.path.exists(path):
+        return None, None
+    key = f"- `k_{kernel}`: lane utilisation "
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith(key):
+                util, share = line[len(key):].split(", VALU issue share ")
+                return float(util), float(share)
+    return None, None
+
+
 def batched_variant(scene, dev, num_views, steps, warmup):
     """Views/s of a multi-view batch of the same scene (seeded cameras around the
     original one, same resolution), bfloat16 texels and SH coefficients, the
@@ -309,10 +324,13 @@ def main():
     achieved = nbytes / (kern[dom] * 1e-3) / 1e9
     traffic, traffic_src = (measured_traffic(args.workload, dom) if args.num_points is None
                             else (None, None))
+    util, share = (measured_valu(args.workload, dom) if args.num_points is None
+                   else (None, None))
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": traffic, "traffic_source": traffic_src, "algorithmic_bytes": int(nbytes),
-            "kernel_ms": {k: round(v, 4) for k, v in kern.items()}}
+            "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
+            "valu_issue_share": share, "valu_lane_utilisation": util}
     variants = {}
     if args.batched_views > 0:
         b = batched_variant(scene, dev, args.batched_views, max(args.steps // 4, 3),
