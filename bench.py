@@ -10,7 +10,7 @@ timed region.  Prints one JSON line.
 Roofline: the slower of the two raster kernels, timed live with HIP events on
 the stream it runs on, against its algorithmic bytes per launch (``traffic`` is
 the measured HBM bytes per launch of that kernel, read from the committed
-rocprofv3 FETCH_SIZE / WRITE_SIZE summary profiles/hbm_traffic_c<workload>.md):
+rocprofv3 FETCH_SIZE / WRITE_SIZE summary profiles/hbm_traffic_<workload tag>.md):
   forward   I*(4+128) + T*8 + P*(56 + 4C) + X*C*4
   backward  I*(4+128) + T*8 + P*(60 + 4C) + 2*X*C*4 + N*128
 (I intersects, T tiles, P pixels, X texels, C channels, N Gaussians).
@@ -70,10 +70,14 @@ def raster_bytes(kind, I, T, P, X, C, N):
     return I * (4 + 128) + T * 8 + P * (60 + 4 * C) + 2 * X * C * 4 + N * 128
 
 
+# file-name tag of each workload's committed rocprofv3 summaries
+PROFILE_TAGS = {1: "256px_1k", 2: "1080p_500k", 3: "1600x1200_1m"}
+
+
 def measured_traffic(workload: int, kernel: str):
     """HBM bytes per launch of ``kernel`` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary for this workload (copy-calibrated), or None."""
-    path = os.path.join(ROOT, "profiles", f"hbm_traffic_c{workload}.md")
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{PROFILE_TAGS.get(workload, '')}.md")
     if not os.path.exists(path):
         return None, None
     key = f"k_{kernel}_traffic_bytes_per_launch:"
@@ -87,7 +91,7 @@ def measured_traffic(workload: int, kernel: str):
 def measured_valu(workload: int, kernel: str):
     """(lane utilisation, VALU issue share) of ``kernel`` from the committed
     rocprofv3 SQ counter summary for this workload, or (None, None)."""
-    path = os.path.join(ROOT, "profiles", f"sq_counters_c{workload}.md")
+    path = os.path.join(ROOT, "profiles", f"sq_counters_{PROFILE_TAGS.get(workload, '')}.md")
     if not os.path.exists(path):
         return None, None
     key = f"- `k_{kernel}`: lane utilisation "

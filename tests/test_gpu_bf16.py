@@ -51,7 +51,7 @@ def _run(step, tgt):
             {k: np_(v) for k, v in step.grads.items()})
 
 
-@pytest.mark.parametrize("full", [False, True], ids=["small", "c2_full"])
+@pytest.mark.parametrize("full", [False, True], ids=["small", "1080p_500k"])
 def test_bf16_storage_equals_rounded_f32(full):
     from gstex_cuda_amd import synthetic
 

@@ -160,8 +160,8 @@ def test_culling_changes_nothing(kind, settings):
             assert ok, (k, info)
 
 
-def test_culling_full_size_c2():
-    """C2 (1920x1080, 500k Gaussians): identical forward with culling on and off."""
+def test_culling_full_size_1080p_500k():
+    """1920x1080 with 500k Gaussians: identical forward with culling on and off."""
     from gstex_cuda_amd import synthetic
 
     s = synthetic.make_scene(2)

@@ -45,7 +45,7 @@ def _assert_grads(a, b, rel=1e-4):
         assert ok, (k, l2, fr)
 
 
-@pytest.mark.parametrize("full", [False, True], ids=["small", "c2_full"])
+@pytest.mark.parametrize("full", [False, True], ids=["small", "1080p_500k"])
 def test_capacity_mode_equals_eager(full):
     from gstex_cuda_amd.pipeline import scene_to_step
 

@@ -344,13 +344,13 @@ def test_forward_closed_form_single_splat():
     np.testing.assert_allclose(1 - np_(g["final_Ts"]), np.array(k["alpha"]), atol=2e-6)
 
 
-def test_forward_invariants_c2_full_size(oracle):
+def test_forward_invariants_1080p_500k(oracle):
     # 1080p, 500k Gaussians: sum of weights + T = 1 everywhere, and a sample of
     # tiles rendered exactly by the oracle
     s = scene(2)
     s.colors[:] = 1.0
     pb = project_and_bin(s)
-    log("c2 intersects", pb["num_intersects"])
+    log("1080p intersects", pb["num_intersects"])
     g, _ = gpu_forward(s, pb["ids"], pb["bins"])
     T = np_(g["final_Ts"])
     np.testing.assert_allclose(np_(g["out_img"]), np.repeat((1 - T)[..., None], 3, -1), atol=2e-5)
@@ -371,7 +371,7 @@ def test_forward_invariants_c2_full_size(oracle):
         ty, tx = divmod(int(t), tiles[0])
         sel[ty * 16:(ty + 1) * 16, tx * 16:(tx + 1) * 16] = True
     fc = frac_close(np_(g["out_texture"])[sel], o["out_texture"][sel], 1e-4)
-    log("c2 tile sample frac", fc, "busiest tile entries", int(counts.max()))
+    log("1080p tile sample frac", fc, "busiest tile entries", int(counts.max()))
     assert fc >= 0.999
 
 

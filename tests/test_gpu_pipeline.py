@@ -108,8 +108,8 @@ def test_pipeline_matches_oracle(oracle, case):
         assert ok, (k, l2, fr)
 
 
-def test_wave_masks_full_size_c2():
-    """C2 (1920x1080, 500k Gaussians): masked and full walks agree."""
+def test_wave_masks_full_size_1080p_500k():
+    """1920x1080 with 500k Gaussians: masked and full walks agree."""
     from gstex_cuda_amd import synthetic
     from gstex_cuda_amd.pipeline import scene_to_step
 
