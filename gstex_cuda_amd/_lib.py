@@ -36,6 +36,7 @@ _SIGNATURES = {
     "gstex_compute_sh_forward": (I, [I, I, I, P, P, P, P]),
     "gstex_compute_sh_forward_bf16": (I, [I, I, I, P, P, P, P]),
     "gstex_compute_sh_backward": (I, [I, I, I, P, P, P, P]),
+    "gstex_compute_sh_backward_views": (I, [I, I, I, I, P, P, P, P]),
     "gstex_map_gaussian_to_intersects": (I, [I, I, P, P, P, P, I, I, I, I, P, P, P]),
     "gstex_sort_intersects_workspace_size": (Z, [I]),
     "gstex_sort_intersects": (I, [I, I, P, P, P, P, P, Z, P]),

@@ -168,6 +168,23 @@ int gstex_compute_sh_backward(int num_points, int degree, int degrees_to_use,
                      "compute_sh_backward");
 }
 
+int gstex_compute_sh_backward_views(int num_points, int num_views, int degree,
+                                    int degrees_to_use, const float* viewdirs,
+                                    const float* v_colors, float* v_coeffs, void* stream) {
+    REQUIRE(num_points >= 0, "num_points must be >= 0");
+    REQUIRE(num_views >= 1, "num_views must be >= 1, got %d", num_views);
+    REQUIRE(degree >= 0 && degree <= 4, "SH degree must be in [0, 4], got %d", degree);
+    REQUIRE(degrees_to_use >= 0 && degrees_to_use <= degree,
+            "degrees_to_use must be in [0, degree], got %d", degrees_to_use);
+    REQUIRE((int64_t)num_points * num_views * 3 < (int64_t)1 << 40, "input too large");
+    if (num_points == 0) return GSTEX_OK;
+    REQUIRE(viewdirs && v_colors && v_coeffs, "null pointer argument");
+    return check_hip(gstex::launch_sh_backward_views(num_points, num_views, degree,
+                                                     degrees_to_use, viewdirs, v_colors,
+                                                     v_coeffs, as_stream(stream)),
+                     "compute_sh_backward_views");
+}
+
 int gstex_map_gaussian_to_intersects(int num_points, int num_intersects, const float* centers,
                                      const float* extents, const float* depths,
                                      const int32_t* cum_tiles_hit, int tiles_x, int tiles_y,

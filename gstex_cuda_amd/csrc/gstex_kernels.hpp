@@ -43,6 +43,9 @@ hipError_t launch_sh_forward_bf16(int n, int degree, int degrees_to_use, const f
                                   hipStream_t stream);
 hipError_t launch_sh_backward(int n, int degree, int degrees_to_use, const float* viewdirs,
                               const float* v_colors, float* v_coeffs, hipStream_t stream);
+hipError_t launch_sh_backward_views(int n, int views, int degree, int degrees_to_use,
+                                    const float* viewdirs, const float* v_colors,
+                                    float* v_coeffs, hipStream_t stream);
 int sh_num_bases(int degree);
 
 // fills

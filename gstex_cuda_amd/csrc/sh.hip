@@ -142,6 +142,44 @@ __global__ __launch_bounds__(256) void k_sh_backward(int n, int nb, const float*
     }
 }
 
+// The sum over V views of k_sh_backward: viewdirs and v_colors are (V, n, 3)
+// stacks, v_coeffs receives sum_v basis(dir_v) x v_colors_v.  The products are
+// rounded on their own and added in view order (no fused multiply-add), so
+// the result equals adding the per-view outputs one after another; each
+// coefficient is written once instead of once per view and once per sum.
+template <int D>
+__global__ __launch_bounds__(256) void k_sh_backward_views(int n, int nb, int views,
+                                                           const float* __restrict__ viewdirs,
+                                                           const float* __restrict__ v_colors,
+                                                           float* __restrict__ v_coeffs) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    constexpr int used = D <= 0 ? 1 : D == 1 ? 4 : D == 2 ? 9 : D == 3 ? 16 : 25;
+    float acc[3 * used];
+    for (int v = 0; v < views; ++v) {
+        const size_t o = ((size_t)v * n + i) * 3;
+        float b[25];
+        b[0] = kShC0;
+        if constexpr (D >= 1) {
+            float x, y, z;
+            sh_basis<D>(viewdirs[o], viewdirs[o + 1], viewdirs[o + 2], b, x, y, z);
+        }
+        const float g0 = v_colors[o], g1 = v_colors[o + 1], g2 = v_colors[o + 2];
+#pragma unroll
+        for (int k = 0; k < used; ++k) {
+            const float p0 = b[k] * g0, p1 = b[k] * g1, p2 = b[k] * g2;
+            acc[3 * k] = v == 0 ? p0 : acc[3 * k] + p0;
+            acc[3 * k + 1] = v == 0 ? p1 : acc[3 * k + 1] + p1;
+            acc[3 * k + 2] = v == 0 ? p2 : acc[3 * k + 2] + p2;
+        }
+    }
+    float* out = v_coeffs + (size_t)i * nb * 3;
+#pragma unroll
+    for (int k = 0; k < 3 * used; ++k) out[k] = acc[k];
+    for (int k = 3 * used; k < 3 * nb; ++k) out[k] = 0.f;
+}
+
 template <typename CT>
 static hipError_t sh_forward_t(int n, int degree, int degrees_to_use, const float* viewdirs,
                                const CT* coeffs, float* colors, hipStream_t stream) {
@@ -179,6 +217,21 @@ hipError_t launch_sh_backward(int n, int degree, int degrees_to_use, const float
         case 2: k_sh_backward<2><<<g, 256, 0, stream>>>(n, nb, viewdirs, v_colors, v_coeffs); break;
         case 3: k_sh_backward<3><<<g, 256, 0, stream>>>(n, nb, viewdirs, v_colors, v_coeffs); break;
         default: k_sh_backward<4><<<g, 256, 0, stream>>>(n, nb, viewdirs, v_colors, v_coeffs); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sh_backward_views(int n, int views, int degree, int degrees_to_use,
+                                    const float* viewdirs, const float* v_colors,
+                                    float* v_coeffs, hipStream_t stream) {
+    if (n <= 0 || views <= 0) return hipSuccess;
+    const int nb = num_sh_bases(degree), g = (n + 255) / 256;
+    switch (degrees_to_use) {
+        case 0: k_sh_backward_views<0><<<g, 256, 0, stream>>>(n, nb, views, viewdirs, v_colors, v_coeffs); break;
+        case 1: k_sh_backward_views<1><<<g, 256, 0, stream>>>(n, nb, views, viewdirs, v_colors, v_coeffs); break;
+        case 2: k_sh_backward_views<2><<<g, 256, 0, stream>>>(n, nb, views, viewdirs, v_colors, v_coeffs); break;
+        case 3: k_sh_backward_views<3><<<g, 256, 0, stream>>>(n, nb, views, viewdirs, v_colors, v_coeffs); break;
+        default: k_sh_backward_views<4><<<g, 256, 0, stream>>>(n, nb, views, viewdirs, v_colors, v_coeffs); break;
     }
     return hipGetLastError();
 }

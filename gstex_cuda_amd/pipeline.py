@@ -123,6 +123,10 @@ class RenderStep:
         self.v_tex = torch.empty((H, W, self.C), **f32)
         self.v_zero1 = torch.zeros((H, W), **f32)
         self.v_zero3 = torch.zeros((H, W, 3), **f32)
+        # a batch that shares the texel gradient or runs the SH adjoint for all
+        # of its views at once turns these off
+        self.zero_texture_grad = True
+        self.sh_backward = True
         self.v_reg = None  # regulariser cotangents, allocated on first use
         self.v_normal = None
         # gradients (reference layouts)
@@ -252,7 +256,8 @@ class RenderStep:
         p, cam, s, g = self.p, self.cam, self._stream(), self.grads
         n, m = self.n, self._list_len
         call("gstex_fill_f32", _p(self.grad_records), self.grad_records.numel(), 0.0, s)
-        call("gstex_fill_f32", _p(g["v_texture"]), g["v_texture"].numel(), 0.0, s)
+        if self.zero_texture_grad:
+            call("gstex_fill_f32", _p(g["v_texture"]), g["v_texture"].numel(), 0.0, s)
         v_depth = self.v_zero1 if v_depth is None else v_depth
         v_reg = self.v_zero1 if v_reg is None else v_reg
         v_alpha = self.v_zero1 if v_alpha is None else v_alpha
@@ -272,7 +277,7 @@ class RenderStep:
              _p(p["quats"]), _p(p["scales"]), _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy,
              _p(g["v_colors"]), _p(g["v_opacity"]), _p(g["v_means"]), _p(g["v_scales"]),
              _p(g["v_quats"]), _p(g["v_uv0"]), _p(g["v_umap"]), _p(g["v_vmap"]), s)
-        if self.sh_degree >= 0:
+        if self.sh_degree >= 0 and self.sh_backward:
             call("gstex_compute_sh_backward", n, self.sh_degree, self.sh_degree,
                  _p(self._viewdirs), _p(g["v_colors"]), _p(g["v_sh_coeffs"]), s)
 
@@ -343,8 +348,14 @@ class MultiViewStep:
     The views are dealt round-robin onto ``streams`` HIP streams (default
     priority) forked from and joined back into the caller's stream, so the
     tiles of one view fill the compute units another view's longest tiles
-    leave idle; in a captured graph they become parallel branches.  Every view
-    owns its buffers, so the order across streams does not change any result.
+    leave idle; in a captured graph they become parallel branches.
+
+    Gradient traffic is shared across the batch: every view's backward adds
+    its texel gradients into the one ``grads["v_texture"]`` (zeroed once per
+    batch), and with SH the views' colour gradients and view directions sit in
+    (V, N, 3) stacks that one ``gstex_compute_sh_backward_views`` turns into
+    ``grads["v_sh_coeffs"]``.  The other gradients are per-view buffers summed
+    in view order.
     """
 
     def __init__(self, params: Dict[str, torch.Tensor], cams, targets, block_width: int = 16,
@@ -364,10 +375,26 @@ class MultiViewStep:
         self.grads = {k: torch.zeros_like(v) for k, v in self.views[0].grads.items()}
         self.graph = None
         dev = self.views[0].dev
+        n, nv = self.views[0].n, len(self.views)
+        self.sh_degree = sh_degree
+        self._shared = {"v_texture"}
+        if sh_degree >= 0:
+            self._shared.add("v_sh_coeffs")
+            self._viewdirs = torch.empty((nv, n, 3), dtype=torch.float32, device=dev)
+            self._v_colors = torch.empty((nv, n, 3), dtype=torch.float32, device=dev)
+        for i, v in enumerate(self.views):
+            v.grads["v_texture"] = self.grads["v_texture"]
+            v.zero_texture_grad = False
+            if sh_degree >= 0:
+                v._viewdirs = self._viewdirs[i]
+                v.grads["v_colors"] = self._v_colors[i]
+                del v.grads["v_sh_coeffs"]
+                v.sh_backward = False
         k = min(max(int(streams), 1), len(self.views))
         self.streams = [torch.cuda.Stream(dev) for _ in range(k)] if k > 1 else []
 
     def step(self):
+        self.grads["v_texture"].zero_()
         if self.streams:
             main = torch.cuda.current_stream(self.views[0].dev)
             for st in self.streams:
@@ -381,9 +408,16 @@ class MultiViewStep:
             for v, tgt in zip(self.views, self.targets):
                 v.step(tgt)
         for k, acc in self.grads.items():
+            if k in self._shared:
+                continue
             acc.copy_(self.views[0].grads[k])
             for v in self.views[1:]:
                 acc.add_(v.grads[k])
+        if self.sh_degree >= 0:
+            v0 = self.views[0]
+            call("gstex_compute_sh_backward_views", v0.n, len(self.views), self.sh_degree,
+                 self.sh_degree, _p(self._viewdirs), _p(self._v_colors),
+                 _p(self.grads["v_sh_coeffs"]), v0._stream())
 
     def capacity_ok(self) -> bool:
         return all(v.capacity_ok() for v in self.views)

@@ -93,6 +93,17 @@ int gstex_compute_sh_backward(int num_points, int degree, int degrees_to_use,
                               const float *viewdirs, const float *v_colors, float *v_coeffs,
                               void *stream);
 
+/* The sum over num_views views of gstex_compute_sh_backward, for batched
+ * multi-view steps.  No counterpart in the reference, whose trainer renders
+ * one view per step (example.py:227-255).  viewdirs and v_colors are
+ * (num_views, N, 3) stacks; v_coeffs (N, bases(degree), 3) receives
+ * sum_v d colours_v / d coeffs, each coefficient written once.  The per-view
+ * products are added in view order without fused multiply-add, so the result
+ * equals summing the per-view gstex_compute_sh_backward outputs in order. */
+int gstex_compute_sh_backward_views(int num_points, int num_views, int degree,
+                                    int degrees_to_use, const float *viewdirs,
+                                    const float *v_colors, float *v_coeffs, void *stream);
+
 /* Replaces map_gaussian_to_intersects_tensor (cuda/csrc/bindings.cu:77-121).
  * isect_ids (I,) int64 and gaussian_ids (I,) int32 are fully written
  * (zero where a Gaussian counts a slot but writes none).  wrapped != 0 takes the

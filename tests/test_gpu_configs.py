@@ -9,7 +9,7 @@ own sorted lists (float32 for the forward; float64 for gradients and image
 PSNR), and asserts size-independent properties of the whole
 frame: with unit colours the image equals 1 - T, T stays in (1e-4, 1], the
 distortion is non-negative, the graph replay equals the eager batch, and the
-gradients of a batch equal the sum of its views.
+per-view gradients of a batch sum to the batch's.
 """
 from __future__ import annotations
 
@@ -187,7 +187,9 @@ def test_orbit_16_views_1m_full_size(oracle):
     mv.step()
     torch.cuda.synchronize()
     eager = {k: v.clone() for k, v in mv.grads.items()}
-    per_view = {k: sum(v.grads[k].double() for v in mv.views) for k in eager}
+    # texel and SH gradients are shared by the views; the rest are per view
+    per_view = {k: sum(v.grads[k].double() for v in mv.views) for k in eager
+                if k not in mv._shared}
     counts = [int(v.intersect_total()) for v in mv.views]
     log(f"  per-view intersects min {min(counts)} max {max(counts)} total {sum(counts)}")
     assert min(counts) * 4 < max(counts)  # uneven work per view
@@ -198,8 +200,10 @@ def test_orbit_16_views_1m_full_size(oracle):
         ok, l2, fr = grad_close(np_(mv.grads[k]), np_(v), rel_l2=1e-4, elem_tol=1e-4, frac=0.999)
         log(f"  replay {k}: relL2 {l2:.2e}")
         assert ok, (k, l2, fr)
-        ok, l2, fr = grad_close(np_(v), np_(per_view[k]), rel_l2=1e-5, elem_tol=1e-5, frac=1.0)
-        assert ok, (k, l2, fr)
+        if k in per_view:
+            ok, l2, fr = grad_close(np_(v), np_(per_view[k]), rel_l2=1e-5, elem_tol=1e-5,
+                                    frac=1.0)
+            assert ok, (k, l2, fr)
 
 
 # ------------------------------------------------------------------ 8 x 1080p, 3M, bf16

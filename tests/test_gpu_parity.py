@@ -106,7 +106,7 @@ def test_binning_zero_slots_and_ties(oracle):
     _binning_case(oracle, s, 16)
 
 
-def test_binning_c2_scale(oracle):
+def test_binning_1080p_200k(oracle):
     # full-resolution 1080p with 200k Gaussians: exact against the oracle
     s = scene(2, num_points=200_000)
     assert _binning_case(oracle, s, 16) > 1_000_000
@@ -225,6 +225,31 @@ def test_sh_matches_oracle(oracle, degree, used):
     np.testing.assert_allclose(np_(col), oracle.sh_forward(degree, used, dirs, co, "f64"), atol=1e-5)
     vco = _C.compute_sh_backward(n, degree, used, to_dev(dirs), to_dev(vc))
     np.testing.assert_allclose(np_(vco), oracle.sh_backward(degree, used, dirs, vc, "f64"), atol=1e-5)
+
+
+@pytest.mark.parametrize("degree,used,views", [(3, 3, 5), (4, 2, 3), (0, 0, 2), (2, 2, 1)])
+def test_sh_backward_views_sums_single_views(oracle, degree, used, views):
+    # gstex_compute_sh_backward_views equals the per-view adjoints added in
+    # view order (bit for bit) and the float64 oracle's sum (within 1e-5)
+    from gstex_cuda_amd import cuda as _C
+    from gstex_cuda_amd._lib import call
+
+    rng = np.random.default_rng(degree * 11 + views)
+    n, nb = 4099, (degree + 1) ** 2
+    dirs = rng.standard_normal((views, n, 3)).astype(np.float32)
+    vc = rng.standard_normal((views, n, 3)).astype(np.float32)
+    out = torch.full((n, nb, 3), float("nan"), device=DEV)
+    d_dirs, d_vc = to_dev(dirs), to_dev(vc)  # held for the duration of the call
+    call("gstex_compute_sh_backward_views", n, views, degree, used, d_dirs.data_ptr(),
+         d_vc.data_ptr(), out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    acc = None
+    for v in range(views):
+        one = _C.compute_sh_backward(n, degree, used, to_dev(dirs[v]), to_dev(vc[v]))
+        acc = one.clone() if acc is None else acc.add_(one)
+    assert torch.equal(out, acc)
+    want = sum(oracle.sh_backward(degree, used, dirs[v], vc[v], "f64") for v in range(views))
+    np.testing.assert_allclose(np_(out), want, atol=1e-5 * views)
 
 
 def test_sh_adam_converges_like_reference_test():
