@@ -660,8 +660,9 @@ __global__ __launch_bounds__(256) void k_bin_mark_starts(int n, const uint32_t* 
 // Workgroup b: its Gaussians are positions starts[b] .. starts[b + 1] (at
 // most kEmitOut + 1 of them, each with at least one output in the block or
 // at its end), staged in LDS with their output offsets and tile rectangles.
-// Output k belongs to the last staged Gaussian whose first output is <= k
-// (binary search in LDS); consecutive lanes write consecutive slots.  A
+// Output k belongs to the last staged Gaussian whose first output is <= k.
+// Each thread takes kEmitOut / 256 consecutive outputs: a binary search in
+// LDS for the first, then a forward walk over the staged offsets.  A
 // Gaussian's tiles go row by row over its rectangle, capped at its count.
 __global__ __launch_bounds__(256) void k_bin_emit(
     int n, int cap, const uint32_t* __restrict__ m_total, const uint32_t* __restrict__ order,
@@ -670,8 +671,9 @@ __global__ __launch_bounds__(256) void k_bin_emit(
     const float* __restrict__ extents, int tiles_x, int tiles_y, int bw,
     uint32_t* __restrict__ tiles_out, uint32_t* __restrict__ ids_out) {
     constexpr int kStage = kEmitOut + 1;
-    __shared__ uint32_t s_off[kStage], s_g[kStage];
-    __shared__ int s_x0[kStage], s_y0[kStage], s_wd[kStage];
+    constexpr int kPer = kEmitOut / 256;
+    __shared__ uint32_t s_off[kStage], s_g[kStage], s_xy[kStage], s_wd[kStage];
+    __shared__ uint32_t s_tile[kEmitOut], s_id[kEmitOut];
     const int tid = threadIdx.x;
     const uint32_t tw = *total_written, m = *m_total;
     const uint32_t k0 = (uint32_t)blockIdx.x * kEmitOut;
@@ -691,27 +693,41 @@ __global__ __launch_bounds__(256) void k_bin_emit(
         tile_rect(c.x, c.y, e.x, e.y, bw, tiles_x, tiles_y, x0, y0, x1, y1);
         s_off[i] = offsets[p];
         s_g[i] = g;
-        s_x0[i] = x0;
-        s_y0[i] = y0;
-        s_wd[i] = max(x1 - x0, 1);
+        s_xy[i] = (uint32_t)x0 | ((uint32_t)y0 << 16);  // tile indices < 65536
+        s_wd[i] = (uint32_t)max(x1 - x0, 1);
     }
     __syncthreads();
-    for (uint32_t k = k0 + tid; k < k1; k += 256) {
+    // Each thread resolves kPer consecutive outputs: a binary search in LDS
+    // for the first, then a forward walk over the staged offsets (a
+    // Gaussian's outputs are consecutive, so the walk mostly stays put).
+    // The results go to LDS and leave in consecutive-lane order.
+    const uint32_t kt = k0 + (uint32_t)(kPer * tid);
+    if (kt < k1) {
         int lo = 0, hi = cnt - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (s_off[mid] <= k) lo = mid;
+            if (s_off[mid] <= kt) lo = mid;
             else hi = mid - 1;
         }
-        const uint32_t li = k - s_off[lo];
-        const int wd = s_wd[lo];
-        const int ty = s_y0[lo] + (int)(li / (uint32_t)wd);
-        const int tx = s_x0[lo] + (int)(li % (uint32_t)wd);
+        const uint32_t kend = min(kt + (uint32_t)kPer, k1);
+        for (uint32_t k = kt; k < kend; ++k) {
+            while (lo + 1 < cnt && s_off[lo + 1] <= k) ++lo;
+            const uint32_t li = k - s_off[lo];
+            const uint32_t wd = s_wd[lo], xy = s_xy[lo];
+            const uint32_t ty = (xy >> 16) + li / wd;
+            const uint32_t tx = (xy & 0xffffu) + li % wd;
+            s_tile[k - k0] = ty * (uint32_t)tiles_x + tx;
+            s_id[k - k0] = s_g[lo];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const uint32_t idx = (uint32_t)(r * 256 + tid), k = k0 + idx;
         // entries past the capacity are dropped (the caller sees m > cap)
-        const uint32_t dst = zbase + k;
-        if (dst < limit) {
-            tiles_out[dst] = (uint32_t)(ty * tiles_x + tx);
-            ids_out[dst] = s_g[lo];
+        if (k < k1 && zbase + k < limit) {
+            tiles_out[zbase + k] = s_tile[idx];
+            ids_out[zbase + k] = s_id[idx];
         }
     }
 }
