@@ -73,6 +73,8 @@ _SIGNATURES = {
                                   P, P, P, P, P, P,  # six output gradients
                                   P, P, P, Z, P]),  # grad_records, v_texture, masks, bytes, stream
     "gstex_finalize_gradients": (I, [I, P, P, P, P, P, P, F, F, P, P, P, P, P, P, P, P, P]),
+    "gstex_finalize_gradients_views": (I, [I, I, P, P, Z, P, P, P, P, P, P, P, P, P, P, P, P, P,
+                                           P, P]),
     "gstex_fill_f32": (I, [P, Z, F, P]),
     "gstex_texture_edit_workspace_size": (Z, [I]),
     "gstex_texture_edit": (I, [I, I, I, I, I, I, I, I, I,  # tiles, bw, img, texture_info, X

@@ -580,6 +580,31 @@ int gstex_finalize_gradients(int num_points, const float* grad_records, const vo
                      "finalize_gradients");
 }
 
+int gstex_finalize_gradients_views(int num_points, int num_views, const float* grad_records,
+                                   const void* records, size_t records_stride,
+                                   const float* quats, const float* scales,
+                                   const float* viewmats, const float* c2ws,
+                                   const float* focals, float* v_colors, float* v_opacity,
+                                   float* v_means, float* v_scales, float* v_quats,
+                                   float* v_uv0, float* v_umap, float* v_vmap,
+                                   float* colors_views, void* stream) {
+    REQUIRE(num_points >= 0, "num_points must be >= 0");
+    REQUIRE(num_views >= 1, "num_views must be >= 1, got %d", num_views);
+    if (num_points == 0) return GSTEX_OK;
+    REQUIRE(records_stride >= gstex::splat_records_bytes(num_points) && records_stride % 16 == 0,
+            "records_stride must be a multiple of 16 of at least gstex_splat_records_size(N)");
+    REQUIRE(grad_records && records && scales && quats && viewmats && c2ws && focals &&
+                v_colors && v_opacity && v_means && v_scales && v_quats && v_uv0 && v_umap &&
+                v_vmap,
+            "null pointer argument");
+    return check_hip(gstex::grad_finalize_views(num_points, num_views, grad_records, records,
+                                                records_stride, quats, scales, viewmats, c2ws,
+                                                focals, v_colors, v_opacity, v_means, v_scales,
+                                                v_quats, v_uv0, v_umap, v_vmap, colors_views,
+                                                as_stream(stream)),
+                     "finalize_gradients_views");
+}
+
 int gstex_fill_f32(float* dst, size_t n, float v, void* stream) {
     if (n == 0) return GSTEX_OK;
     REQUIRE(dst, "null pointer argument");

@@ -145,6 +145,12 @@ hipError_t grad_finalize(int n, const float* grad_rec, const float* records, con
                          float fy, float* v_colors, float* v_opacity, float* v_means,
                          float* v_scales, float* v_quats, float* v_uv0, float* v_umap,
                          float* v_vmap, hipStream_t s);
+hipError_t grad_finalize_views(int n, int views, const float* grad_rec, const void* records,
+                               size_t rec_stride, const float* quats, const float* scales,
+                               const float* viewmats, const float* c2ws, const float* focals,
+                               float* v_colors, float* v_opacity, float* v_means, float* v_scales,
+                               float* v_quats, float* v_uv0, float* v_umap, float* v_vmap,
+                               float* colors_views, hipStream_t s);
 // Per-view splat data, one buffer: n records (128 B), n screen ellipses
 // (float4 centre x, y, A, B: 16 B), n screen boxes (8 B), n ellipse C values
 // (4 B).  Each array starts at a multiple of its element size.

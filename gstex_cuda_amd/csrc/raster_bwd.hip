@@ -382,15 +382,17 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
 
 // Rebuild the reference gradient tensors from the reduced scalars and the
 // splat record (axes, mean, uv maps, inverse scales).
-__global__ __launch_bounds__(256) void k_grad_finalize(
-    int n, const float* __restrict__ grad_rec, const float* __restrict__ records,
-    const float* __restrict__ quats, const float* __restrict__ scales,
-    const float* __restrict__ viewmat, const float* __restrict__ c2w, float fx, float fy,
-    float* __restrict__ v_colors, float* __restrict__ v_opacity, float* __restrict__ v_means,
-    float* __restrict__ v_scales, float* __restrict__ v_quats, float* __restrict__ v_uv0,
-    float* __restrict__ v_umap, float* __restrict__ v_vmap) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
+// The 22 reference gradient values of Gaussian g for one view, in the order
+// colour (3), opacity, mean (3), scale (3), quaternion (4), uv0 (2), umap (3),
+// vmap (3).
+constexpr int kFinalFloats = 22;
+__device__ __forceinline__ void finalize_one(int g, const float* __restrict__ grad_rec,
+                                             const float* __restrict__ records,
+                                             const float* __restrict__ quats,
+                                             const float* __restrict__ scales,
+                                             const float* __restrict__ viewmat,
+                                             const float* __restrict__ c2w, float fx, float fy,
+                                             float (&out)[kFinalFloats]) {
     float q[kGradFloats];
     const float4* src = reinterpret_cast<const float4*>(grad_rec + (size_t)g * kGradFloats);
 #pragma unroll
@@ -399,12 +401,11 @@ __global__ __launch_bounds__(256) void k_grad_finalize(
         q[4 * k] = t.x; q[4 * k + 1] = t.y; q[4 * k + 2] = t.z; q[4 * k + 3] = t.w;
     }
     const Splat s = load_splat(records + (size_t)g * kRecordFloats);
-    const V3 o = v3(c2w[3], c2w[7], c2w[11]);
     const V3 mo = s.mo;
-    v_colors[3 * g] = q[G_RGB];
-    v_colors[3 * g + 1] = q[G_RGB + 1];
-    v_colors[3 * g + 2] = q[G_RGB + 2];
-    v_opacity[g] = q[G_OPAC];
+    out[0] = q[G_RGB];
+    out[1] = q[G_RGB + 1];
+    out[2] = q[G_RGB + 2];
+    out[3] = q[G_OPAC];
     const V3 ga1 = v3(s.a1.x * q[G_P11] + s.a2.x * q[G_P12], s.a1.y * q[G_P11] + s.a2.y * q[G_P12],
                       s.a1.z * q[G_P11] + s.a2.z * q[G_P12]);
     const V3 ga2 = v3(s.a1.x * q[G_P21] + s.a2.x * q[G_P22], s.a1.y * q[G_P21] + s.a2.y * q[G_P22],
@@ -419,6 +420,7 @@ __global__ __launch_bounds__(256) void k_grad_finalize(
                    s.a3.z * q[G_VNUM]);
     if (q[G_PIX] != 0.f || q[G_PIX + 1] != 0.f) {
         // blur term: d projected mean / d mean (helpers.cuh:114-121,155-164)
+        const V3 o = v3(c2w[3], c2w[7], c2w[11]);
         const V3 pv = xform43(viewmat, v3(s.mo.x + o.x, s.mo.y + o.y, s.mo.z + o.z));
         const float rw = 1.f / (pv.z + 1e-6f);
         const float vpx = fx * q[G_PIX], vpy = fy * q[G_PIX + 1];
@@ -426,25 +428,105 @@ __global__ __launch_bounds__(256) void k_grad_finalize(
         const V3 vmx = xform43_rot_t(viewmat, vview);
         vm = v3(vm.x + vmx.x, vm.y + vmx.y, vm.z + vmx.z);
     }
-    v_means[3 * g] = vm.x;
-    v_means[3 * g + 1] = vm.y;
-    v_means[3 * g + 2] = vm.z;
+    out[4] = vm.x;
+    out[5] = vm.y;
+    out[6] = vm.z;
+    // sigma = 0.5 (l1^2 is1^2 + ...) with is1 = 1 / (s_x g): d/ds_x = -(sum vl1 l1) / s_x
+    out[7] = -q[G_P11] / scales[3 * g];
+    out[8] = -q[G_P22] / scales[3 * g + 1];
+    out[9] = 0.f;
     const float4 qt = reinterpret_cast<const float4*>(quats)[g];
     float vq[4];
     quat_axes_vjp(qt.x, qt.y, qt.z, qt.w, ga1, ga2, ga3, vq);
-    reinterpret_cast<float4*>(v_quats)[g] = make_float4(vq[0], vq[1], vq[2], vq[3]);
-    // sigma = 0.5 (l1^2 is1^2 + ...) with is1 = 1 / (s_x g): d/ds_x = -(sum vl1 l1) / s_x
-    v_scales[3 * g] = -q[G_P11] / scales[3 * g];
-    v_scales[3 * g + 1] = -q[G_P22] / scales[3 * g + 1];
-    v_scales[3 * g + 2] = 0.f;
-    v_uv0[2 * g] = q[G_VU];
-    v_uv0[2 * g + 1] = q[G_VV];
-    v_umap[3 * g] = s.a1.x * q[G_U1] + s.a2.x * q[G_U2];
-    v_umap[3 * g + 1] = s.a1.y * q[G_U1] + s.a2.y * q[G_U2];
-    v_umap[3 * g + 2] = s.a1.z * q[G_U1] + s.a2.z * q[G_U2];
-    v_vmap[3 * g] = s.a1.x * q[G_V1] + s.a2.x * q[G_V2];
-    v_vmap[3 * g + 1] = s.a1.y * q[G_V1] + s.a2.y * q[G_V2];
-    v_vmap[3 * g + 2] = s.a1.z * q[G_V1] + s.a2.z * q[G_V2];
+    out[10] = vq[0];
+    out[11] = vq[1];
+    out[12] = vq[2];
+    out[13] = vq[3];
+    out[14] = q[G_VU];
+    out[15] = q[G_VV];
+    out[16] = s.a1.x * q[G_U1] + s.a2.x * q[G_U2];
+    out[17] = s.a1.y * q[G_U1] + s.a2.y * q[G_U2];
+    out[18] = s.a1.z * q[G_U1] + s.a2.z * q[G_U2];
+    out[19] = s.a1.x * q[G_V1] + s.a2.x * q[G_V2];
+    out[20] = s.a1.y * q[G_V1] + s.a2.y * q[G_V2];
+    out[21] = s.a1.z * q[G_V1] + s.a2.z * q[G_V2];
+}
+
+struct FinalOut {
+    float *v_colors, *v_opacity, *v_means, *v_scales, *v_quats, *v_uv0, *v_umap, *v_vmap;
+};
+
+__device__ __forceinline__ void store_final(const FinalOut& o, int g,
+                                            const float (&f)[kFinalFloats]) {
+    o.v_colors[3 * g] = f[0];
+    o.v_colors[3 * g + 1] = f[1];
+    o.v_colors[3 * g + 2] = f[2];
+    o.v_opacity[g] = f[3];
+    o.v_means[3 * g] = f[4];
+    o.v_means[3 * g + 1] = f[5];
+    o.v_means[3 * g + 2] = f[6];
+    o.v_scales[3 * g] = f[7];
+    o.v_scales[3 * g + 1] = f[8];
+    o.v_scales[3 * g + 2] = f[9];
+    reinterpret_cast<float4*>(o.v_quats)[g] = make_float4(f[10], f[11], f[12], f[13]);
+    o.v_uv0[2 * g] = f[14];
+    o.v_uv0[2 * g + 1] = f[15];
+    o.v_umap[3 * g] = f[16];
+    o.v_umap[3 * g + 1] = f[17];
+    o.v_umap[3 * g + 2] = f[18];
+    o.v_vmap[3 * g] = f[19];
+    o.v_vmap[3 * g + 1] = f[20];
+    o.v_vmap[3 * g + 2] = f[21];
+}
+
+// Rebuild the reference gradient tensors from the reduced scalars and the
+// splat record (axes, mean, uv maps, inverse scales).
+__global__ __launch_bounds__(256) void k_grad_finalize(
+    int n, const float* __restrict__ grad_rec, const float* __restrict__ records,
+    const float* __restrict__ quats, const float* __restrict__ scales,
+    const float* __restrict__ viewmat, const float* __restrict__ c2w, float fx, float fy,
+    FinalOut o) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    float f[kFinalFloats];
+    finalize_one(g, grad_rec, records, quats, scales, viewmat, c2w, fx, fy, f);
+    store_final(o, g, f);
+}
+
+// The sum over V views of k_grad_finalize: view v's gradient records start at
+// grad_rec + v n 32 and its splat records rec_stride bytes after view v - 1's;
+// cameras are (V, 16) row-major matrices and (V, 2) focal lengths.  Each
+// view's values are formed exactly as k_grad_finalize forms them and added in
+// view order without fused multiply-add, so the result equals adding the
+// per-view outputs one after another.  colors_views (V, n, 3), when given,
+// receives each view's colour gradient (the input of the SH adjoint).
+__global__ __launch_bounds__(256) void k_grad_finalize_views(
+    int n, int views, const float* __restrict__ grad_rec, const char* __restrict__ records,
+    size_t rec_stride, const float* __restrict__ quats, const float* __restrict__ scales,
+    const float* __restrict__ viewmats, const float* __restrict__ c2ws,
+    const float* __restrict__ focals, FinalOut o, float* __restrict__ colors_views) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    float acc[kFinalFloats];
+    for (int v = 0; v < views; ++v) {
+        float f[kFinalFloats];
+        finalize_one(g, grad_rec + (size_t)v * n * kGradFloats,
+                     reinterpret_cast<const float*>(records + (size_t)v * rec_stride), quats,
+                     scales, viewmats + 16 * v, c2ws + 16 * v, focals[2 * v], focals[2 * v + 1],
+                     f);
+        if (colors_views) {
+            float* cv = colors_views + ((size_t)v * n + g) * 3;
+            cv[0] = f[0];
+            cv[1] = f[1];
+            cv[2] = f[2];
+        }
+        {
+#pragma clang fp contract(off)
+#pragma unroll
+            for (int k = 0; k < kFinalFloats; ++k) acc[k] = v == 0 ? f[k] : acc[k] + f[k];
+        }
+    }
+    store_final(o, g, acc);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -496,9 +578,23 @@ hipError_t grad_finalize(int n, const float* grad_rec, const float* records, con
                          float* v_scales, float* v_quats, float* v_uv0, float* v_umap,
                          float* v_vmap, hipStream_t s) {
     if (n <= 0) return hipSuccess;
+    const FinalOut o{v_colors, v_opacity, v_means, v_scales, v_quats, v_uv0, v_umap, v_vmap};
     k_grad_finalize<<<(n + 255) / 256, 256, 0, s>>>(n, grad_rec, records, quats, scales, viewmat,
-                                                   c2w, fx, fy, v_colors, v_opacity, v_means,
-                                                   v_scales, v_quats, v_uv0, v_umap, v_vmap);
+                                                   c2w, fx, fy, o);
+    return hipGetLastError();
+}
+
+hipError_t grad_finalize_views(int n, int views, const float* grad_rec, const void* records,
+                               size_t rec_stride, const float* quats, const float* scales,
+                               const float* viewmats, const float* c2ws, const float* focals,
+                               float* v_colors, float* v_opacity, float* v_means, float* v_scales,
+                               float* v_quats, float* v_uv0, float* v_umap, float* v_vmap,
+                               float* colors_views, hipStream_t s) {
+    if (n <= 0 || views <= 0) return hipSuccess;
+    const FinalOut o{v_colors, v_opacity, v_means, v_scales, v_quats, v_uv0, v_umap, v_vmap};
+    k_grad_finalize_views<<<(n + 255) / 256, 256, 0, s>>>(
+        n, views, grad_rec, (const char*)records, rec_stride, quats, scales, viewmats, c2ws,
+        focals, o, colors_views);
     return hipGetLastError();
 }
 

@@ -127,6 +127,7 @@ class RenderStep:
         # of its views at once turns these off
         self.zero_texture_grad = True
         self.sh_backward = True
+        self.finalize = True
         self.v_reg = None  # regulariser cotangents, allocated on first use
         self.v_normal = None
         # gradients (reference layouts)
@@ -273,6 +274,8 @@ class RenderStep:
              _p(self.grad_records), _p(g["v_texture"]), *self._masks(), s)
         if timer:
             timer.stop("raster_backward")
+        if not self.finalize:
+            return
         call("gstex_finalize_gradients", n, _p(self.grad_records), _p(self.records),
              _p(p["quats"]), _p(p["scales"]), _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy,
              _p(g["v_colors"]), _p(g["v_opacity"]), _p(g["v_means"]), _p(g["v_scales"]),
@@ -350,12 +353,12 @@ class MultiViewStep:
     tiles of one view fill the compute units another view's longest tiles
     leave idle; in a captured graph they become parallel branches.
 
-    Gradient traffic is shared across the batch: every view's backward adds
-    its texel gradients into the one ``grads["v_texture"]`` (zeroed once per
-    batch), and with SH the views' colour gradients and view directions sit in
-    (V, N, 3) stacks that one ``gstex_compute_sh_backward_views`` turns into
-    ``grads["v_sh_coeffs"]``.  The other gradients are per-view buffers summed
-    in view order.
+    Gradient passes run once per batch, not once per view: every view's
+    backward adds its texel gradients into the one ``grads["v_texture"]``
+    (zeroed once per batch); the views' gradient records and splat records sit
+    in stacks that one ``gstex_finalize_gradients_views`` turns into the summed
+    per-Gaussian gradients; with SH, the views' directions and colour
+    gradients sit in (V, N, 3) stacks for one ``gstex_compute_sh_backward_views``.
     """
 
     def __init__(self, params: Dict[str, torch.Tensor], cams, targets, block_width: int = 16,
@@ -376,20 +379,30 @@ class MultiViewStep:
         self.graph = None
         dev = self.views[0].dev
         n, nv = self.views[0].n, len(self.views)
+        f32 = dict(dtype=torch.float32, device=dev)
         self.sh_degree = sh_degree
-        self._shared = {"v_texture"}
+        self.params = params
+        # per-view splat records and gradient records, stacked
+        rec_size = self.views[0].records.numel()
+        self._rec_stride = (rec_size + 255) // 256 * 256
+        self._records = torch.empty((nv * self._rec_stride,), dtype=torch.uint8, device=dev)
+        self._grad_records = torch.empty((nv, n, 32), **f32)
+        self._viewmats = torch.stack([v.cam.viewmat.to(**f32) for v in self.views]).contiguous()
+        self._c2ws = torch.stack([v.cam.c2w.to(**f32) for v in self.views]).contiguous()
+        self._focals = torch.tensor([[v.cam.fx, v.cam.fy] for v in self.views], **f32)
+        self._v_colors = None
         if sh_degree >= 0:
-            self._shared.add("v_sh_coeffs")
-            self._viewdirs = torch.empty((nv, n, 3), dtype=torch.float32, device=dev)
-            self._v_colors = torch.empty((nv, n, 3), dtype=torch.float32, device=dev)
+            self._viewdirs = torch.empty((nv, n, 3), **f32)
+            self._v_colors = torch.empty((nv, n, 3), **f32)
         for i, v in enumerate(self.views):
-            v.grads["v_texture"] = self.grads["v_texture"]
+            v.records = self._records[i * self._rec_stride:i * self._rec_stride + rec_size]
+            v.grad_records = self._grad_records[i]
+            v.grads = {"v_texture": self.grads["v_texture"]}
             v.zero_texture_grad = False
+            v.finalize = False
+            v.sh_backward = False
             if sh_degree >= 0:
                 v._viewdirs = self._viewdirs[i]
-                v.grads["v_colors"] = self._v_colors[i]
-                del v.grads["v_sh_coeffs"]
-                v.sh_backward = False
         k = min(max(int(streams), 1), len(self.views))
         self.streams = [torch.cuda.Stream(dev) for _ in range(k)] if k > 1 else []
 
@@ -407,17 +420,17 @@ class MultiViewStep:
         else:
             for v, tgt in zip(self.views, self.targets):
                 v.step(tgt)
-        for k, acc in self.grads.items():
-            if k in self._shared:
-                continue
-            acc.copy_(self.views[0].grads[k])
-            for v in self.views[1:]:
-                acc.add_(v.grads[k])
+        v0, g, p = self.views[0], self.grads, self.params
+        call("gstex_finalize_gradients_views", v0.n, len(self.views), _p(self._grad_records),
+             _p(self._records), self._rec_stride, _p(p["quats"]), _p(p["scales"]),
+             _p(self._viewmats), _p(self._c2ws), _p(self._focals), _p(g["v_colors"]),
+             _p(g["v_opacity"]), _p(g["v_means"]), _p(g["v_scales"]), _p(g["v_quats"]),
+             _p(g["v_uv0"]), _p(g["v_umap"]), _p(g["v_vmap"]), _p(self._v_colors),
+             v0._stream())
         if self.sh_degree >= 0:
-            v0 = self.views[0]
             call("gstex_compute_sh_backward_views", v0.n, len(self.views), self.sh_degree,
                  self.sh_degree, _p(self._viewdirs), _p(self._v_colors),
-                 _p(self.grads["v_sh_coeffs"]), v0._stream())
+                 _p(g["v_sh_coeffs"]), v0._stream())
 
     def capacity_ok(self) -> bool:
         return all(v.capacity_ok() for v in self.views)

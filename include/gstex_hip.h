@@ -261,6 +261,26 @@ int gstex_finalize_gradients(int num_points, const float *grad_records, const vo
                              const float *c2w, float fx, float fy, float *v_colors,
                              float *v_opacity, float *v_means, float *v_scales, float *v_quats,
                              float *v_uv0, float *v_umap, float *v_vmap, void *stream);
+/* The sum over num_views views of gstex_finalize_gradients, for batched
+ * multi-view steps (no counterpart in the reference, whose trainer renders one
+ * view per step, example.py:227-255).  grad_records holds num_views stacked
+ * N x 32 blocks; view v's splat records start records_stride bytes after view
+ * v - 1's (records_stride >= gstex_splat_records_size(N), a multiple of 16);
+ * viewmats and c2ws are (num_views, 4, 4) row-major, focals (num_views, 2)
+ * holds fx, fy; all are device pointers.  The outputs receive the sums over
+ * the views, formed as gstex_finalize_gradients forms each view's values and
+ * added in view order without fused multiply-add (equal to summing the
+ * per-view outputs in order).  colors_views (num_views, N, 3), if not NULL,
+ * receives each view's colour gradient, the input of
+ * gstex_compute_sh_backward_views. */
+int gstex_finalize_gradients_views(int num_points, int num_views, const float *grad_records,
+                                   const void *records, size_t records_stride,
+                                   const float *quats, const float *scales,
+                                   const float *viewmats, const float *c2ws,
+                                   const float *focals, float *v_colors, float *v_opacity,
+                                   float *v_means, float *v_scales, float *v_quats,
+                                   float *v_uv0, float *v_umap, float *v_vmap,
+                                   float *colors_views, void *stream);
 /* Fills n float32 values with v (a plain kernel, safe inside captured graphs). */
 int gstex_fill_f32(float *dst, size_t n, float v, void *stream);
 

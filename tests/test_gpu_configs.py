@@ -8,8 +8,7 @@ a sample of tiles (random ones plus the busiest) with the oracle on the GPU's
 own sorted lists (float32 for the forward; float64 for gradients and image
 PSNR), and asserts size-independent properties of the whole
 frame: with unit colours the image equals 1 - T, T stays in (1e-4, 1], the
-distortion is non-negative, the graph replay equals the eager batch, and the
-per-view gradients of a batch sum to the batch's.
+distortion is non-negative, and the graph replay equals the eager batch.
 """
 from __future__ import annotations
 
@@ -181,15 +180,13 @@ def test_orbit_16_views_1m_full_size(oracle):
         _sampled_forward_backward(oracle, vs, pb, colors, _regularised_cotangents(vs, 0.0, 0.0),
                                   f"orbit view {i}")
 
-    # the whole batch as one graph: capacities hold, replay equals the eager
-    # batch, and the batch gradient is the sum of the views'
+    # the whole batch as one graph: capacities hold and replay equals the
+    # eager batch (the batch gradient as the sum of single-view steps is
+    # checked at a smaller size in test_gpu_graph.py)
     mv = scene_to_multiview(s, views, DEV)
     mv.step()
     torch.cuda.synchronize()
     eager = {k: v.clone() for k, v in mv.grads.items()}
-    # texel and SH gradients are shared by the views; the rest are per view
-    per_view = {k: sum(v.grads[k].double() for v in mv.views) for k in eager
-                if k not in mv._shared}
     counts = [int(v.intersect_total()) for v in mv.views]
     log(f"  per-view intersects min {min(counts)} max {max(counts)} total {sum(counts)}")
     assert min(counts) * 4 < max(counts)  # uneven work per view
@@ -200,10 +197,6 @@ def test_orbit_16_views_1m_full_size(oracle):
         ok, l2, fr = grad_close(np_(mv.grads[k]), np_(v), rel_l2=1e-4, elem_tol=1e-4, frac=0.999)
         log(f"  replay {k}: relL2 {l2:.2e}")
         assert ok, (k, l2, fr)
-        if k in per_view:
-            ok, l2, fr = grad_close(np_(v), np_(per_view[k]), rel_l2=1e-5, elem_tol=1e-5,
-                                    frac=1.0)
-            assert ok, (k, l2, fr)
 
 
 # ------------------------------------------------------------------ 8 x 1080p, 3M, bf16

@@ -252,6 +252,56 @@ def test_sh_backward_views_sums_single_views(oracle, degree, used, views):
     np.testing.assert_allclose(np_(out), want, atol=1e-5 * views)
 
 
+def test_finalize_views_sums_single_views():
+    # gstex_finalize_gradients_views equals the per-view gstex_finalize_gradients
+    # outputs added in view order, bit for bit, blur (camera-dependent) slots
+    # included, and hands back each view's colour gradient
+    from gstex_cuda_amd import synthetic
+    from gstex_cuda_amd._lib import call, lib
+
+    s = synthetic.make_scene(1, seed=5, num_points=3001, img_w=160, img_h=120)
+    views = synthetic.make_views(s, 3, seed=9)
+    n, V = s.num_points, len(views)
+    r = raster_inputs(s)
+    st = torch.cuda.current_stream().cuda_stream
+    size = lib().gstex_splat_records_size(n)
+    stride = (size + 255) // 256 * 256
+    recs = torch.zeros((V * stride,), dtype=torch.uint8, device=DEV)
+    rng = np.random.default_rng(2)
+    grad = to_dev(rng.standard_normal((V, n, 32)).astype(np.float32))
+    grad[:, : n // 2, 30:32] = 0.0  # half the Gaussians without the blur term
+    vms = torch.stack([to_dev(v.viewmat) for v in views]).contiguous()
+    c2ws = torch.stack([to_dev(v.c2w) for v in views]).contiguous()
+    foc = to_dev(np.array([[v.fx, v.fy] for v in views], np.float32))
+    shapes = dict(v_colors=(n, 3), v_opacity=(n, 1), v_means=(n, 3), v_scales=(n, 3),
+                  v_quats=(n, 4), v_uv0=(n, 1, 2), v_umap=(n, 1, 3), v_vmap=(n, 1, 3))
+    acc = None
+    for i, v in enumerate(views):
+        base = recs.data_ptr() + i * stride
+        call("gstex_build_splat_records", n, r["means"].data_ptr(), r["scales"].data_ptr(), 1.0,
+             r["quats"].data_ptr(), r["colors"].data_ptr(), r["opacities"].data_ptr(),
+             r["uv0"].data_ptr(), r["umap"].data_ptr(), r["vmap"].data_ptr(),
+             r["texture_dims"].data_ptr(), vms[i].data_ptr(), c2ws[i].data_ptr(), v.fx, v.fy,
+             v.cx, v.cy, base, st)
+        one = {k: torch.empty(sh, device=DEV) for k, sh in shapes.items()}
+        call("gstex_finalize_gradients", n, grad[i].data_ptr(), base, r["quats"].data_ptr(),
+             r["scales"].data_ptr(), vms[i].data_ptr(), c2ws[i].data_ptr(), v.fx, v.fy,
+             *[one[k].data_ptr() for k in shapes], st)
+        acc = one if acc is None else {k: acc[k].add_(one[k]) for k in shapes}
+        if i == 0:
+            first_colors = one["v_colors"].clone()
+    out = {k: torch.full(sh, float("nan"), device=DEV) for k, sh in shapes.items()}
+    cviews = torch.full((V, n, 3), float("nan"), device=DEV)
+    call("gstex_finalize_gradients_views", n, V, grad.data_ptr(), recs.data_ptr(), stride,
+         r["quats"].data_ptr(), r["scales"].data_ptr(), vms.data_ptr(), c2ws.data_ptr(),
+         foc.data_ptr(), *[out[k].data_ptr() for k in shapes], cviews.data_ptr(), st)
+    torch.cuda.synchronize()
+    for k in shapes:
+        assert torch.equal(out[k], acc[k]), k
+    assert torch.equal(cviews[0], first_colors)
+    assert torch.equal(cviews.sum(0), cviews[0] + cviews[1] + cviews[2])
+
+
 def test_sh_adam_converges_like_reference_test():
     # tests/test_sh.py of the reference: degree 4, one point, 1000 Adam steps to colour 0.5
     from gstex_cuda_amd import sh
