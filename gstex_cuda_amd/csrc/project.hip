@@ -297,8 +297,13 @@ __global__ __launch_bounds__(256) void k_build_records(
     const float* __restrict__ c2w, float fx, float fy, float cx, float cy,
     float* __restrict__ records, uint2* __restrict__ boxes, float4* __restrict__ ellipses,
     float* __restrict__ ellipse_c) {
+    // records go out through LDS: each thread writes its 32 floats there
+    // (row stride 33 floats, so the 64 lanes of a wave hit distinct banks) and
+    // the block's 256 records leave as contiguous 16-byte stores
+    __shared__ float srec[256][kRecordFloats + 1];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int i0 = blockIdx.x * blockDim.x, tid = threadIdx.x;
+    if (i < n) {
     const float4 q = reinterpret_cast<const float4*>(quats)[i];
     V3 a1, a2, a3;
     quat_axes(q.x, q.y, q.z, q.w, a1, a2, a3);
@@ -331,10 +336,8 @@ __global__ __launch_bounds__(256) void k_build_records(
     r[R_TEX_W] = __int_as_float(texture_dims[3 * i + 1]);
     r[R_TEX_START] = __int_as_float(texture_dims[3 * i + 2]);
     r[R_NUM] = dot3(a3, mo);
-    float4* dst = reinterpret_cast<float4*>(records + (size_t)i * kRecordFloats);
 #pragma unroll
-    for (int k = 0; k < kRecordFloats / 4; ++k)
-        dst[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
+    for (int k = 0; k < kRecordFloats; ++k) srec[tid][k] = r[k];
     const float s1 = scales[3 * i] * glob_scale, s2 = scales[3 * i + 1] * glob_scale;
     bool bounded;
     boxes[i] = cull_box(m, a1, a2, a3, s1, s2, r[R_OPAC], c2w, fx, fy, cx, cy, pu, pvv, bounded);
@@ -343,6 +346,17 @@ __global__ __launch_bounds__(256) void k_build_records(
     if (bounded) cull_conic(m, a1, a2, s1, s2, r[R_OPAC], c2w, fx, fy, cx, cy, ell, ell_c);
     ellipses[i] = ell;
     ellipse_c[i] = ell_c;
+    }
+    __syncthreads();
+    const int cnt = min(256, n - i0);
+    float4* dst = reinterpret_cast<float4*>(records + (size_t)i0 * kRecordFloats);
+#pragma unroll
+    for (int k = 0; k < kRecordFloats / 4; ++k) {
+        const int f = k * 256 + tid;  // 16-byte word of the block's records
+        const int rr = f / (kRecordFloats / 4), c = 4 * (f % (kRecordFloats / 4));
+        if (rr < cnt)
+            dst[f] = make_float4(srec[rr][c], srec[rr][c + 1], srec[rr][c + 2], srec[rr][c + 3]);
+    }
 }
 
 hipError_t launch_get_aabb_2d(int n, const float* means, const float* scales, float glob_scale,

@@ -386,7 +386,24 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
 // colour (3), opacity, mean (3), scale (3), quaternion (4), uv0 (2), umap (3),
 // vmap (3).
 constexpr int kFinalFloats = 22;
-__device__ __forceinline__ void finalize_one(int g, const float* __restrict__ grad_rec,
+// Gradient records of a block's 256 Gaussians, staged in LDS by contiguous
+// 16-byte loads (row stride 33 floats: the lanes of a wave read distinct banks).
+using GradRows = float[256][kGradFloats + 1];
+__device__ __forceinline__ void stage_grad_rows(const float* __restrict__ grad_rec, int g0, int n,
+                                                GradRows& rows) {
+    const int cnt = min(256, n - g0), tid = threadIdx.x;
+    const float4* src = reinterpret_cast<const float4*>(grad_rec + (size_t)g0 * kGradFloats);
+#pragma unroll
+    for (int k = 0; k < kGradFloats / 4; ++k) {
+        const int f = k * 256 + tid, rr = f / (kGradFloats / 4), c = 4 * (f % (kGradFloats / 4));
+        if (rr < cnt) {
+            const float4 t = src[f];
+            rows[rr][c] = t.x; rows[rr][c + 1] = t.y; rows[rr][c + 2] = t.z; rows[rr][c + 3] = t.w;
+        }
+    }
+}
+
+__device__ __forceinline__ void finalize_one(int g, const float* q_row,
                                              const float* __restrict__ records,
                                              const float* __restrict__ quats,
                                              const float* __restrict__ scales,
@@ -394,12 +411,8 @@ __device__ __forceinline__ void finalize_one(int g, const float* __restrict__ gr
                                              const float* __restrict__ c2w, float fx, float fy,
                                              float (&out)[kFinalFloats]) {
     float q[kGradFloats];
-    const float4* src = reinterpret_cast<const float4*>(grad_rec + (size_t)g * kGradFloats);
 #pragma unroll
-    for (int k = 0; k < kGradFloats / 4; ++k) {
-        const float4 t = src[k];
-        q[4 * k] = t.x; q[4 * k + 1] = t.y; q[4 * k + 2] = t.z; q[4 * k + 3] = t.w;
-    }
+    for (int k = 0; k < kGradFloats; ++k) q[k] = q_row[k];
     const Splat s = load_splat(records + (size_t)g * kRecordFloats);
     const V3 mo = s.mo;
     out[0] = q[G_RGB];
@@ -486,10 +499,13 @@ __global__ __launch_bounds__(256) void k_grad_finalize(
     const float* __restrict__ quats, const float* __restrict__ scales,
     const float* __restrict__ viewmat, const float* __restrict__ c2w, float fx, float fy,
     FinalOut o) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ GradRows rows;
+    const int g0 = blockIdx.x * blockDim.x, g = g0 + threadIdx.x;
+    stage_grad_rows(grad_rec, g0, n, rows);
+    __syncthreads();
     if (g >= n) return;
     float f[kFinalFloats];
-    finalize_one(g, grad_rec, records, quats, scales, viewmat, c2w, fx, fy, f);
+    finalize_one(g, rows[threadIdx.x], records, quats, scales, viewmat, c2w, fx, fy, f);
     store_final(o, g, f);
 }
 
@@ -505,12 +521,16 @@ __global__ __launch_bounds__(256) void k_grad_finalize_views(
     size_t rec_stride, const float* __restrict__ quats, const float* __restrict__ scales,
     const float* __restrict__ viewmats, const float* __restrict__ c2ws,
     const float* __restrict__ focals, FinalOut o, float* __restrict__ colors_views) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
+    __shared__ GradRows rows;
+    const int g0 = blockIdx.x * blockDim.x, g = g0 + threadIdx.x;
     float acc[kFinalFloats];
     for (int v = 0; v < views; ++v) {
+        if (v > 0) __syncthreads();  // the previous view's rows are consumed
+        stage_grad_rows(grad_rec + (size_t)v * n * kGradFloats, g0, n, rows);
+        __syncthreads();
+        if (g >= n) continue;
         float f[kFinalFloats];
-        finalize_one(g, grad_rec + (size_t)v * n * kGradFloats,
+        finalize_one(g, rows[threadIdx.x],
                      reinterpret_cast<const float*>(records + (size_t)v * rec_stride), quats,
                      scales, viewmats + 16 * v, c2ws + 16 * v, focals[2 * v], focals[2 * v + 1],
                      f);
@@ -526,7 +546,7 @@ __global__ __launch_bounds__(256) void k_grad_finalize_views(
             for (int k = 0; k < kFinalFloats; ++k) acc[k] = v == 0 ? f[k] : acc[k] + f[k];
         }
     }
-    store_final(o, g, acc);
+    if (g < n) store_final(o, g, acc);
 }
 
 // ------------------------------------------------------------------ launchers
