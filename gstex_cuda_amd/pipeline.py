@@ -348,10 +348,12 @@ class MultiViewStep:
     The capacities come from one eager pass per view (``capacity_factor`` of
     the measured intersect count); ``replay_checked()`` verifies them.
 
-    The views are dealt round-robin onto ``streams`` HIP streams (default
-    priority) forked from and joined back into the caller's stream, so the
-    tiles of one view fill the compute units another view's longest tiles
-    leave idle; in a captured graph they become parallel branches.
+    The views run on ``streams`` HIP streams (default priority) forked from
+    and joined back into the caller's stream, so the tiles of one view fill
+    the compute units another view's longest tiles leave idle; in a captured
+    graph they become parallel branches.  With ``balance`` the views are
+    timed once and placed longest first on the least loaded stream; otherwise
+    they are dealt round-robin.
 
     Gradient passes run once per batch, not once per view: every view's
     backward adds its texel gradients into the one ``grads["v_texture"]``
@@ -364,7 +366,7 @@ class MultiViewStep:
     def __init__(self, params: Dict[str, torch.Tensor], cams, targets, block_width: int = 16,
                  settings: int = 1 << 8, background: Optional[torch.Tensor] = None,
                  sh_degree: int = -1, capacity_factor: float = 1.1,
-                 use_wave_masks: bool = True, streams: int = 4):
+                 use_wave_masks: bool = True, streams: int = 4, balance: bool = True):
         self.targets = list(targets)
         kw = dict(block_width=block_width, settings=settings, background=background,
                   sh_degree=sh_degree, use_wave_masks=use_wave_masks)
@@ -405,6 +407,28 @@ class MultiViewStep:
                 v._viewdirs = self._viewdirs[i]
         k = min(max(int(streams), 1), len(self.views))
         self.streams = [torch.cuda.Stream(dev) for _ in range(k)] if k > 1 else []
+        # views -> streams: longest first onto the least loaded stream, by the
+        # time of one eager step per view (tiles of very uneven length make
+        # equal-sized views cost very different times)
+        self.assign = [(i, i % max(k, 1)) for i in range(nv)]
+        if k > 1 and balance:
+            cost = []
+            for v, tgt in zip(self.views, self.targets):
+                v.step(tgt)  # warm
+                t0 = torch.cuda.Event(enable_timing=True)
+                t1 = torch.cuda.Event(enable_timing=True)
+                t0.record()
+                v.step(tgt)
+                t1.record()
+                t1.synchronize()
+                cost.append(t0.elapsed_time(t1))
+            load = [0.0] * k
+            self.assign = []
+            for i in sorted(range(nv), key=lambda j: -cost[j]):
+                st = min(range(k), key=lambda j: load[j])
+                load[st] += cost[i]
+                self.assign.append((i, st))
+            self.view_ms = cost
 
     def step(self):
         self.grads["v_texture"].zero_()
@@ -412,9 +436,9 @@ class MultiViewStep:
             main = torch.cuda.current_stream(self.views[0].dev)
             for st in self.streams:
                 st.wait_stream(main)
-            for i, (v, tgt) in enumerate(zip(self.views, self.targets)):
-                with torch.cuda.stream(self.streams[i % len(self.streams)]):
-                    v.step(tgt)
+            for i, st in self.assign:
+                with torch.cuda.stream(self.streams[st]):
+                    self.views[i].step(self.targets[i])
             for st in self.streams:
                 main.wait_stream(st)
         else:
