@@ -75,6 +75,7 @@ _SIGNATURES = {
     "gstex_finalize_gradients": (I, [I, P, P, P, P, P, P, F, F, P, P, P, P, P, P, P, P, P]),
     "gstex_finalize_gradients_views": (I, [I, I, P, P, Z, P, P, P, P, P, P, P, P, P, P, P, P, P,
                                            P, P]),
+    "gstex_image_loss_grad": (I, [I, I, I, P, P, P, F, P, F, P, P, P, P]),
     "gstex_fill_f32": (I, [P, Z, F, P]),
     "gstex_texture_edit_workspace_size": (Z, [I]),
     "gstex_texture_edit": (I, [I, I, I, I, I, I, I, I, I,  # tiles, bw, img, texture_info, X

@@ -686,6 +686,23 @@ int gstex_preprocess_gaussians_backward(
                      "preprocess_gaussians_backward");
 }
 
+int gstex_image_loss_grad(int num_pixels, int channels, int with_img, const float* out_img,
+                          const float* out_texture, const float* target, float scale,
+                          const float* out_normal, float normal_scale, float* v_img,
+                          float* v_texture_out, float* v_normal, void* stream) {
+    REQUIRE(num_pixels >= 0, "num_pixels must be >= 0");
+    REQUIRE(channels >= 1 && channels <= 64, "channels must be in [1, 64], got %d", channels);
+    if (num_pixels == 0) return GSTEX_OK;
+    REQUIRE(out_texture && target && v_img && v_texture_out && (out_img || !with_img),
+            "null pointer argument");
+    REQUIRE(!out_normal || v_normal, "v_normal is required with out_normal");
+    return check_hip(gstex::image_loss_grad(num_pixels, channels, with_img != 0, out_img,
+                                            out_texture, target, scale, out_normal,
+                                            normal_scale, v_img, v_texture_out, v_normal,
+                                            as_stream(stream)),
+                     "image_loss_grad");
+}
+
 int gstex_adam_step(size_t n, float* param, const float* grad, float* exp_avg, float* exp_avg_sq,
                     double lr, double beta1, double beta2, double eps, int step, int grad_mode,
                     void* values, int values_format, void* stream) {

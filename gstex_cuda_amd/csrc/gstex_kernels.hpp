@@ -196,6 +196,10 @@ hipError_t preprocess_gaussians_backward(
     const float* v_quats, const float* v_uv0, const float* v_umap, const float* v_vmap,
     const float* v_colors, const float* v_opacities, float* v_log_scales, float* v_quats_raw,
     float* v_mapping, float* v_rgb_logits, float* v_opacity_logits, hipStream_t s);
+hipError_t image_loss_grad(int npix, int C, int with_img, const float* out_img,
+                           const float* out_tex, const float* target, float scale,
+                           const float* out_normal, float normal_scale, float* v_img,
+                           float* v_tex, float* v_normal, hipStream_t s);
 hipError_t adam_step(size_t n, float* param, const float* grad, float* exp_avg, float* exp_avg_sq,
                      double lr, double beta1, double beta2, double eps, int step, int grad_mode,
                      void* values, int values_bf16, hipStream_t s);

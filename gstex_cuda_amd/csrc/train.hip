@@ -206,6 +206,45 @@ hipError_t preprocess_gaussians_backward(
     return hipGetLastError();
 }
 
+// Per pixel: d = (with_img ? out_img + tex[:3] : tex[:3]) - target, rounded
+// as torch's add / sub / mul steps are; v_img = with_img ? scale d : 0,
+// v_texture_out[:3] = scale d and 0 above; v_normal = normal_scale n with
+// normal_scale subtracted from z (when out_normal is given).
+__global__ __launch_bounds__(256) void k_image_loss_grad(
+    int npix, int C, int with_img, const float* __restrict__ out_img,
+    const float* __restrict__ out_tex, const float* __restrict__ target, float scale,
+    const float* __restrict__ out_normal, float normal_scale, float* __restrict__ v_img,
+    float* __restrict__ v_tex, float* __restrict__ v_normal) {
+#pragma clang fp contract(off)
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float t = c < C ? out_tex[(size_t)p * C + c] : 0.f;
+        const float img = with_img ? out_img[3 * p + c] + t : t;
+        const float g = (img - target[3 * p + c]) * scale;
+        v_img[3 * p + c] = with_img ? g : 0.f;
+        if (c < C) v_tex[(size_t)p * C + c] = g;
+    }
+    for (int c = 3; c < C; ++c) v_tex[(size_t)p * C + c] = 0.f;
+    if (out_normal) {
+        v_normal[3 * p] = out_normal[3 * p] * normal_scale;
+        v_normal[3 * p + 1] = out_normal[3 * p + 1] * normal_scale;
+        v_normal[3 * p + 2] = out_normal[3 * p + 2] * normal_scale - normal_scale;
+    }
+}
+
+hipError_t image_loss_grad(int npix, int C, int with_img, const float* out_img,
+                           const float* out_tex, const float* target, float scale,
+                           const float* out_normal, float normal_scale, float* v_img,
+                           float* v_tex, float* v_normal, hipStream_t s) {
+    if (npix <= 0) return hipSuccess;
+    k_image_loss_grad<<<(npix + 255) / 256, 256, 0, s>>>(npix, C, with_img, out_img, out_tex,
+                                                         target, scale, out_normal, normal_scale,
+                                                         v_img, v_tex, v_normal);
+    return hipGetLastError();
+}
+
 hipError_t adam_step(size_t n, float* param, const float* grad, float* exp_avg, float* exp_avg_sq,
                      double lr, double beta1, double beta2, double eps, int step, int grad_mode,
                      void* values, int values_bf16, hipStream_t s) {

@@ -130,6 +130,7 @@ class RenderStep:
         self.finalize = True
         self.v_reg = None  # regulariser cotangents, allocated on first use
         self.v_normal = None
+        self._reg_weight = None
         # gradients (reference layouts)
         self.grads = dict(
             v_colors=torch.empty((n, 3), **f32), v_opacity=torch.empty((n, 1), **f32),
@@ -215,16 +216,18 @@ class RenderStep:
         if timer:
             timer.stop("raster_forward")
 
-    def l2_loss_grad(self, target: torch.Tensor):
-        """d/d(img, texture) of mean((out_img + out_texture - target)^2)."""
-        scale = 2.0 / target.numel()
-        torch.add(self.out_img, self.out_texture, out=self.v_img)
-        self.v_img.sub_(target).mul_(scale)
-        if self.C == 3:
-            self.v_tex.copy_(self.v_img)
-        else:
-            self.v_tex.zero_()
-            self.v_tex[..., :3].copy_(self.v_img)
+    def l2_loss_grad(self, target: torch.Tensor, v_normal: Optional[torch.Tensor] = None,
+                     normal_scale: float = 0.0):
+        """d/d(img, texture) of mean((out_img + out_texture[..., :3] - target)^2),
+        in one kernel; with ``v_normal``, also the normal term's cotangent
+        ``normal_scale * n`` minus ``normal_scale`` in z."""
+        H, W = self.cam.img_h, self.cam.img_w
+        if target.dtype != torch.float32 or not target.is_contiguous():
+            target = target.to(torch.float32).contiguous()
+        call("gstex_image_loss_grad", H * W, self.C, 1, _p(self.out_img), _p(self.out_texture),
+             _p(target), 2.0 / target.numel(), _p(self.out_normal if v_normal is not None
+                                                   else None),
+             normal_scale, _p(self.v_img), _p(self.v_tex), _p(v_normal), self._stream())
 
     def regularised_loss_grad(self, target: torch.Tensor, reg_weight: float = 100.0,
                               normal_weight: float = 0.05):
@@ -233,15 +236,14 @@ class RenderStep:
         + normal_weight * mean(n_x^2 + n_y^2 + (1 - n_z)^2),
         the distortion and normal terms of example.py:199-207 with weights;
         returns (v_reg, v_normal) for ``backward``."""
-        self.l2_loss_grad(target)
         H, W = self.cam.img_h, self.cam.img_w
-        if self.v_reg is None:
-            self.v_reg = torch.empty_like(self.v_zero1)
-            self.v_normal = torch.empty_like(self.v_zero3)
-        self.v_reg.fill_(reg_weight / (H * W))
-        w = normal_weight / (H * W)
-        torch.mul(self.out_normal, 2.0 * w, out=self.v_normal)
-        self.v_normal[..., 2].sub_(2.0 * w)
+        if self.v_reg is None or self._reg_weight != reg_weight:
+            if self.v_reg is None:
+                self.v_reg = torch.empty_like(self.v_zero1)
+                self.v_normal = torch.empty_like(self.v_zero3)
+            self.v_reg.fill_(reg_weight / (H * W))  # constant: refilled on change only
+            self._reg_weight = reg_weight
+        self.l2_loss_grad(target, self.v_normal, 2.0 * normal_weight / (H * W))
         return self.v_reg, self.v_normal
 
     def regularised_step(self, target: torch.Tensor, reg_weight: float = 100.0,

@@ -254,16 +254,10 @@ class TrainStep:
     def loss_grad(self):
         """Cotangents of the example loss (``example.py:188-210``)."""
         r, H, W = self.render, self.cam.img_h, self.cam.img_w
-        C = r.C
-        r.v_img.zero_()
-        scale = 2.0 / (H * W * 3)
-        torch.sub(r.out_texture[..., :3], self.target, out=r.v_tex[..., :3])
-        r.v_tex[..., :3].mul_(scale)
-        if C > 3:
-            r.v_tex[..., 3:].zero_()
-        w = self.normal_weight / (H * W)
-        torch.mul(r.out_normal, 2.0 * w, out=self.v_normal)
-        self.v_normal[..., 2].sub_(2.0 * w)
+        # texture MSE (the example renders out_texture only) and the normal term
+        call("gstex_image_loss_grad", H * W, r.C, 0, None, _p(r.out_texture), _p(self.target),
+             2.0 / (H * W * 3), _p(r.out_normal), 2.0 * self.normal_weight / (H * W),
+             _p(r.v_img), _p(r.v_tex), _p(self.v_normal), _stream(self.target))
 
     def loss(self) -> float:
         """The example loss of the last forward (a host synchronisation)."""

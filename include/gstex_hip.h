@@ -350,6 +350,19 @@ int gstex_preprocess_gaussians_backward(
     const float *v_colors, const float *v_opacities, float *v_log_scales, float *v_quats_raw,
     float *v_mapping, float *v_rgb_logits, float *v_opacity_logits, void *stream);
 
+/* Cotangents of the image losses of a training step in one pass (no
+ * counterpart in the reference, which forms them with torch autograd,
+ * example.py:188-210).  Per pixel, with t = out_texture[:3] (missing channels
+ * 0): d = (with_img ? out_img + t : t) - target; v_img = with_img ? scale d : 0;
+ * v_texture_out[:3] = scale d, its other channels 0.  When out_normal is
+ * given, v_normal = normal_scale * out_normal with normal_scale subtracted
+ * from z (the gradient of the normal term, normal_scale = 2 weight / pixels).
+ * Each step is rounded as the separate float32 add, subtract and multiply. */
+int gstex_image_loss_grad(int num_pixels, int channels, int with_img, const float *out_img,
+                          const float *out_texture, const float *target, float scale,
+                          const float *out_normal, float normal_scale, float *v_img,
+                          float *v_texture_out, float *v_normal, void *stream);
+
 /* One torch.optim.Adam step (no weight decay, no amsgrad) over n float32 values,
  * step >= 1 counting from 1.  grad_mode GSTEX_GRAD_SIGMOID treats grad as the
  * gradient of sigmoid(param).  values (may be NULL) receives sigmoid(updated

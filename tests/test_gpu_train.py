@@ -220,3 +220,35 @@ def test_train_step_follows_example_loop():
         # zero may step the other way: require all but a few elements to agree
         assert close.mean() >= 0.999, k
     assert not torch.equal(mine["means"], raw["means"])
+
+
+@pytest.mark.parametrize("channels,with_img", [(3, 1), (5, 1), (1, 1), (3, 0), (7, 0)])
+def test_image_loss_grad_matches_torch_steps(channels, with_img):
+    # gstex_image_loss_grad equals the float32 torch sequence it replaces:
+    # add, subtract, multiply (and the normal term's multiply and subtract)
+    from gstex_cuda_amd._lib import call
+
+    g = torch.Generator().manual_seed(channels * 10 + with_img)
+    H, W = 37, 53
+    img = torch.rand((H, W, 3), generator=g).to(DEV)
+    tex = torch.rand((H, W, channels), generator=g).to(DEV)
+    tgt = torch.rand((H, W, 3), generator=g).to(DEV)
+    nrm = torch.randn((H, W, 3), generator=g).to(DEV)
+    scale, ns = 2.0 / (H * W * 3), 2.0 * 0.05 / (H * W)
+    v_img = torch.full((H, W, 3), float("nan"), device=DEV)
+    v_tex = torch.full((H, W, channels), float("nan"), device=DEV)
+    v_n = torch.full((H, W, 3), float("nan"), device=DEV)
+    call("gstex_image_loss_grad", H * W, channels, with_img, img.data_ptr(), tex.data_ptr(),
+         tgt.data_ptr(), scale, nrm.data_ptr(), ns, v_img.data_ptr(), v_tex.data_ptr(),
+         v_n.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    t3 = torch.zeros((H, W, 3), device=DEV)
+    t3[..., :min(channels, 3)] = tex[..., :3]
+    want = ((img + t3) if with_img else t3).sub(tgt).mul(scale)
+    assert torch.equal(v_img, want if with_img else torch.zeros_like(want))
+    assert torch.equal(v_tex[..., :min(channels, 3)], want[..., :min(channels, 3)])
+    if channels > 3:
+        assert torch.equal(v_tex[..., 3:], torch.zeros_like(v_tex[..., 3:]))
+    wn = torch.mul(nrm, ns)
+    wn[..., 2].sub_(ns)
+    assert torch.equal(v_n, wn)
