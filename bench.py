@@ -279,9 +279,27 @@ def cpu_baseline(scene, step, target_seconds):
             break
         share = min(1.0, share * max(2.0, 0.5 * target_seconds / max(elapsed, 1e-3)))
     frame_s = elapsed * total / max(chosen, 1)
+    # the timed run's own image, checked against the float64 oracle on 16 of
+    # the sampled tiles (outside the timed region)
+    chk = np.zeros(len(counts), np.uint8)
+    chk[order[:16]] = 1
+    sc.tile_mask = chk
+    f64 = O.texture_forward(sc, "f64")
+    bw, tx = scene.block_width, (W + scene.block_width - 1) // scene.block_width
+    sel = np.zeros((H, W), bool)
+    for t in order[:16]:
+        ty, tx0 = divmod(int(t), tx)
+        sel[ty * bw:(ty + 1) * bw, tx0 * bw:(tx0 + 1) * bw] = True
+    img = (step.out_img + step.out_texture[..., :3]).cpu().numpy().astype(np.float64)
+    ref = f64["out_img"] + f64["out_texture"][..., :3]
+    mse = float(np.mean((img[sel] - ref[sel]) ** 2))
+    peak = max(float(np.abs(ref[sel]).max()), 1e-12)
+    psnr = float("inf") if mse == 0 else 10.0 * np.log10(peak * peak / mse)
     return {"value": 1.0 / frame_s, "unit": "iter/s", "cores": 1, "kind": "port",
             "sample": f"{k} of {len(counts)} tiles ({chosen} of {total} intersects), "
-                      f"FP32 oracle fwd+bwd {elapsed:.1f} s, scaled by intersect share"}
+                      f"FP32 oracle fwd+bwd {elapsed:.1f} s, scaled by intersect share",
+            "image_psnr_db_vs_f64_oracle": round(psnr, 2),
+            "psnr_sample": f"{int(sel.sum())} pixels of 16 sampled tiles, out_img + out_texture"}
 
 
 def main():
