@@ -15,11 +15,14 @@
 
 namespace gstex {
 
-// GENERAL = false compiles out the blur (settings bit 9) and NDC distortion
-// (bit 10) code for the common settings; TB selects bfloat16 texels.  CM <= 4
-// is the exact channel count; above 4, CM bounds a runtime count.
-template <int CM, bool GENERAL, bool TB>
+// MODE 0 compiles out the blur (settings bit 9), NDC distortion (bit 10), alpha visualisation (bits 16-29)
+// and float64 edge-on code for the common settings; MODE 1 keeps all of it;
+// MODE 2 is MODE 0 plus the float64 edge-on solve.  TB selects bfloat16
+// texels.  CM <= 4 is the exact channel count; above 4, CM bounds a runtime
+// count.
+template <int CM, int MODE, bool TB>
 __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) {
+    constexpr bool GENERAL = MODE == 1, PRECISE = MODE != 0;
     const int tile = blockIdx.x;
     const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
     int lx, ly;
@@ -96,7 +99,7 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
             const Splat s = splat_record(a, g);
             bool comp = false;
             if (!done) {
-                Pair p = pair_eval(s, pr, use_blur, GENERAL && a.precise_edge);
+                Pair p = pair_eval(s, pr, use_blur, PRECISE && a.precise_edge);
                 if (viz) {
                     const float sigma = 0.5f * (s.iq1 * p.l1 * p.l1 + s.iq2 * p.l2 * p.l2);
                     p.alpha = (sigma > viz_thresh || (viz_opac && s.opac < 0.5f)) ? 0.f : 0.99f;
@@ -243,14 +246,18 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
 template <int CM>
 static hipError_t launch_fwd_cm(const RasterArgs& a, const FwdOut& o, hipStream_t s) {
     const int tiles = a.tiles_x * a.tiles_y;
-    const bool general = (a.settings & (kSetBlur | kSetNdc | kSetAlphaViz)) || a.precise_edge;
+    // 1: blur, NDC distortion or alpha visualisation (and the float64 edge-on
+    // solve if on); 2: the plain path with the float64 edge-on solve
+    const int mode = (a.settings & (kSetBlur | kSetNdc | kSetAlphaViz)) ? 1 : a.precise_edge ? 2 : 0;
     const dim3 grid(tiles), block(raster_block_threads(a.bw));
     if (a.texture_bf16) {
-        if (general) k_raster_forward<CM, true, true><<<grid, block, 0, s>>>(a, o);
-        else k_raster_forward<CM, false, true><<<grid, block, 0, s>>>(a, o);
+        if (mode == 1) k_raster_forward<CM, 1, true><<<grid, block, 0, s>>>(a, o);
+        else if (mode == 2) k_raster_forward<CM, 2, true><<<grid, block, 0, s>>>(a, o);
+        else k_raster_forward<CM, 0, true><<<grid, block, 0, s>>>(a, o);
     } else {
-        if (general) k_raster_forward<CM, true, false><<<grid, block, 0, s>>>(a, o);
-        else k_raster_forward<CM, false, false><<<grid, block, 0, s>>>(a, o);
+        if (mode == 1) k_raster_forward<CM, 1, false><<<grid, block, 0, s>>>(a, o);
+        else if (mode == 2) k_raster_forward<CM, 2, false><<<grid, block, 0, s>>>(a, o);
+        else k_raster_forward<CM, 0, false><<<grid, block, 0, s>>>(a, o);
     }
     return hipGetLastError();
 }

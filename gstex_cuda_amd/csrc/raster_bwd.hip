@@ -59,11 +59,17 @@ __device__ __forceinline__ float wave_reduce_scatter(float (&v)[32], int lane) {
 }
 
 // ------------------------------------------------------------------ backward
-// GENERAL = false compiles out the blur (settings bit 9) and NDC distortion
-// (bit 10) code for the common settings; TB selects bfloat16 texels.  CM <= 4
-// is the exact channel count; above 4, CM bounds a runtime count.
-template <int CM, bool GENERAL, bool TB>
-__global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) {
+// MODE 0 compiles out the blur (settings bit 9), NDC distortion (bit 10)
+// and float64 edge-on code for the common settings; MODE 1 keeps all of it;
+// MODE 2 is MODE 0 plus the float64 edge-on solve.  TB selects bfloat16
+// texels.  CM <= 4 is the exact channel count; above 4, CM bounds a runtime
+// count.  The plain kernels for CM <= 4 are held to 96 VGPRs (5 waves per
+// SIMD), which they need at most one register more than.
+template <int CM, int MODE, bool TB>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(MODE == 0 && CM <= 4 ? 5 : 1)))
+void k_raster_backward(RasterArgs a, BwdIn b) {
+    constexpr bool GENERAL = MODE == 1, PRECISE = MODE != 0;
     const int tile = blockIdx.x;
     const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
     const int lane = threadIdx.x & (kWave - 1);
@@ -128,7 +134,7 @@ __global__ __launch_bounds__(256) void k_raster_backward(RasterArgs a, BwdIn b) 
         // does not contribute keeps T and vT and gets zero seeds (vis, valpha,
         // vt, vu, vv), so every gradient value below is zero for it without a
         // branch or per-value selects.
-        const Pair p = pair_eval(s, pr, use_blur, GENERAL && a.precise_edge);
+        const Pair p = pair_eval(s, pr, use_blur, PRECISE && a.precise_edge);
         const bool valid = inside && e <= bfin && !p.skip;
         const uint64_t vmask = __ballot(valid);
         if (vmask == 0) return;
@@ -553,14 +559,18 @@ __global__ __launch_bounds__(256) void k_grad_finalize_views(
 template <int CM>
 static hipError_t launch_bwd_cm(const RasterArgs& a, const BwdIn& b, hipStream_t s) {
     const int tiles = a.tiles_x * a.tiles_y;
-    const bool general = (a.settings & (kSetBlur | kSetNdc)) || a.precise_edge;
+    // 1: blur, NDC distortion or alpha visualisation (and the float64 edge-on
+    // solve if on); 2: the plain path with the float64 edge-on solve
+    const int mode = (a.settings & (kSetBlur | kSetNdc)) ? 1 : a.precise_edge ? 2 : 0;
     const dim3 grid(tiles), block(raster_block_threads(a.bw));
     if (a.texture_bf16) {
-        if (general) k_raster_backward<CM, true, true><<<grid, block, 0, s>>>(a, b);
-        else k_raster_backward<CM, false, true><<<grid, block, 0, s>>>(a, b);
+        if (mode == 1) k_raster_backward<CM, 1, true><<<grid, block, 0, s>>>(a, b);
+        else if (mode == 2) k_raster_backward<CM, 2, true><<<grid, block, 0, s>>>(a, b);
+        else k_raster_backward<CM, 0, true><<<grid, block, 0, s>>>(a, b);
     } else {
-        if (general) k_raster_backward<CM, true, false><<<grid, block, 0, s>>>(a, b);
-        else k_raster_backward<CM, false, false><<<grid, block, 0, s>>>(a, b);
+        if (mode == 1) k_raster_backward<CM, 1, false><<<grid, block, 0, s>>>(a, b);
+        else if (mode == 2) k_raster_backward<CM, 2, false><<<grid, block, 0, s>>>(a, b);
+        else k_raster_backward<CM, 0, false><<<grid, block, 0, s>>>(a, b);
     }
     return hipGetLastError();
 }
