@@ -108,14 +108,16 @@ def test_graph_replays_match_eager():
             eager.p["opacities"].copy_(g_step.p["opacities"])
 
 
-@pytest.mark.parametrize("bf16", [False, True], ids=["f32", "bf16"])
-def test_multiview_batch_sums_single_views(bf16):
+@pytest.mark.parametrize("bf16,streams", [(False, 4), (True, 4), (False, 1)],
+                         ids=["f32", "bf16", "f32_one_stream"])
+def test_multiview_batch_sums_single_views(bf16, streams):
     from gstex_cuda_amd import synthetic
     from gstex_cuda_amd.pipeline import RenderStep, scene_to_multiview
 
     s = _scene()
     views = synthetic.make_views(s, 4, resolutions=[(240, 136), (160, 120), (200, 100)])
-    mv = scene_to_multiview(s, views, DEV, bf16=bf16)
+    mv = scene_to_multiview(s, views, DEV, bf16=bf16, streams=streams)
+    assert sorted(i for i, _ in mv.assign) == list(range(len(views)))
     mv.step()
     torch.cuda.synchronize()
     assert mv.capacity_ok()
