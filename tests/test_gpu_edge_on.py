@@ -74,3 +74,19 @@ def test_precise_edge_on_tracks_f64_oracle(oracle):
         f"mean {err_lo.mean():.3e} / {err_hi.mean():.3e}")
     assert err_hi.mean() <= err_lo.mean()
     assert np.mean(err_hi < 1e-4) >= 0.999
+
+
+def test_precise_mode_kernels_agree():
+    # the float64 solve lives in two kernel builds: the plain one (mode 2) and
+    # the general one (mode 1, chosen here by the NDC-distortion bit, which
+    # changes only the distortion outputs); everything else is bit-identical,
+    # and so are all gradients but the ones the distortion feeds
+    import dataclasses
+
+    s = _edge_on_scene()
+    ndc = dataclasses.replace(s, settings=s.settings | (1 << 10))
+    a, b = _render(s, True), _render(ndc, True)
+    for k in ("out_img", "out_texture", "out_normal", "out_depth", "final_Ts", "final_idx",
+              "depth_idx"):
+        assert torch.equal(getattr(a, k), getattr(b, k)), k
+    assert not torch.equal(a.out_reg, b.out_reg)
