@@ -79,8 +79,7 @@ def test_precise_edge_on_tracks_f64_oracle(oracle):
 def test_precise_mode_kernels_agree():
     # the float64 solve lives in two kernel builds: the plain one (mode 2) and
     # the general one (mode 1, chosen here by the NDC-distortion bit, which
-    # changes only the distortion outputs); everything else is bit-identical,
-    # and so are all gradients but the ones the distortion feeds
+    # changes only the distortion outputs); every other output is bit-identical
     import dataclasses
 
     s = _edge_on_scene()
