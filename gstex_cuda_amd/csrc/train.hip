@@ -1,6 +1,7 @@
 // Training-step glue around the raster path: the per-Gaussian parameter
 // preprocessing of a GStex training step (raw parameters -> the reference's
-// render inputs), its adjoint, and a fused Adam update.
+// render inputs), its adjoint, the image-loss cotangents and a fused Adam
+// update.
 //
 // Preprocessing follows example.py:126-143:
 //   scales.xy = exp(log_scales.xy), scales.z = 1e-5 * mean(scales.xy) (no gradient)
@@ -14,6 +15,8 @@
 // path (example.py:231-233,277 use it with default betas and eps):
 //   m = lerp(m, g, 1 - b1);  v = v * b2 + (1 - b2) g g
 //   p = p - (lr / bc1) * m / (sqrt(v) / sqrt(bc2) + eps),  bc_k = 1 - b_k^step
+#include <cstdint>
+
 #include "gstex_common.hpp"
 #include "gstex_kernels.hpp"
 
@@ -234,11 +237,68 @@ __global__ __launch_bounds__(256) void k_image_loss_grad(
     }
 }
 
+// The same for three texture channels, over the flat (3 npix) arrays: four
+// consecutive floats per thread, 16-byte loads and stores.
+__global__ __launch_bounds__(256) void k_image_loss_grad3(
+    size_t nflat, int with_img, const float* __restrict__ out_img,
+    const float* __restrict__ out_tex, const float* __restrict__ target, float scale,
+    const float* __restrict__ out_normal, float normal_scale, float* __restrict__ v_img,
+    float* __restrict__ v_tex, float* __restrict__ v_normal) {
+#pragma clang fp contract(off)
+    const size_t k0 = 4 * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (k0 >= nflat) return;
+    if (k0 + 4 <= nflat) {
+        const float4 t = *reinterpret_cast<const float4*>(out_tex + k0);
+        const float4 tg = *reinterpret_cast<const float4*>(target + k0);
+        float4 d = t;
+        if (with_img) {
+            const float4 im = *reinterpret_cast<const float4*>(out_img + k0);
+            d = make_float4(im.x + t.x, im.y + t.y, im.z + t.z, im.w + t.w);
+        }
+        const float4 g = make_float4((d.x - tg.x) * scale, (d.y - tg.y) * scale,
+                                     (d.z - tg.z) * scale, (d.w - tg.w) * scale);
+        *reinterpret_cast<float4*>(v_tex + k0) = g;
+        *reinterpret_cast<float4*>(v_img + k0) = with_img ? g : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (out_normal) {
+            const float4 nv = *reinterpret_cast<const float4*>(out_normal + k0);
+            float r[4] = {nv.x * normal_scale, nv.y * normal_scale, nv.z * normal_scale,
+                          nv.w * normal_scale};
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if ((k0 + j) % 3 == 2) r[j] = r[j] - normal_scale;  // z of its pixel
+            *reinterpret_cast<float4*>(v_normal + k0) = make_float4(r[0], r[1], r[2], r[3]);
+        }
+        return;
+    }
+    for (size_t k = k0; k < nflat; ++k) {
+        const float t = out_tex[k];
+        const float d = with_img ? out_img[k] + t : t;
+        const float g = (d - target[k]) * scale;
+        v_tex[k] = g;
+        v_img[k] = with_img ? g : 0.f;
+        if (out_normal) {
+            const float r = out_normal[k] * normal_scale;
+            v_normal[k] = k % 3 == 2 ? r - normal_scale : r;
+        }
+    }
+}
+
 hipError_t image_loss_grad(int npix, int C, int with_img, const float* out_img,
                            const float* out_tex, const float* target, float scale,
                            const float* out_normal, float normal_scale, float* v_img,
                            float* v_tex, float* v_normal, hipStream_t s) {
     if (npix <= 0) return hipSuccess;
+    auto aligned = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+    if (C == 3 && aligned(out_tex) && aligned(target) && aligned(v_tex) && aligned(v_img) &&
+        (!with_img || aligned(out_img)) && (!out_normal || (aligned(out_normal) &&
+                                                            aligned(v_normal)))) {
+        const size_t nflat = 3 * (size_t)npix;
+        const size_t threads = (nflat + 3) / 4;
+        k_image_loss_grad3<<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(
+            nflat, with_img, out_img, out_tex, target, scale, out_normal, normal_scale, v_img,
+            v_tex, v_normal);
+        return hipGetLastError();
+    }
     k_image_loss_grad<<<(npix + 255) / 256, 256, 0, s>>>(npix, C, with_img, out_img, out_tex,
                                                          target, scale, out_normal, normal_scale,
                                                          v_img, v_tex, v_normal);
