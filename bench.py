@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--config-steps", type=int, default=5,
                     help="timed steps of each of the 1600x1200 regularised, 16-view orbit and "
                          "8-view 3M bf16 workloads (0 = skip)")
+    ap.add_argument("--edit-sample", type=int, default=1,
+                    help="also time texture_edit on this view and texture_sample on 16M "
+                         "queries (0 = skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the timed region, run one device-to-device copy of this many MiB "
                          "(a kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE)")
@@ -241,6 +244,72 @@ def multiview_variant(dev, workload, steps, warmup):
     return out
 
 
+def edit_sample_variant(scene, step, dev, reps=5):
+    """Timing of texture_edit (texture_edit.cu:229-346) over the benched view's
+    sorted lists, and of texture_sample forward and texel adjoint
+    (texture_sample.cu:11-141) on 16M queries over the view's charts; HIP
+    events on the launch stream, mean of ``reps`` calls after one warm-up."""
+    import torch
+
+    from gstex_cuda_amd import cuda as _C
+
+    def timed_ms(fn):
+        fn()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        b.synchronize()
+        return a.elapsed_time(b) / reps
+
+    t = scene.to_torch(dev)
+    H, W, N = scene.img_h, scene.img_w, scene.num_points
+    X, C = scene.texture.shape
+    torch.manual_seed(0)
+    img = torch.rand((H, W, 3), device=dev)
+    alpha = torch.ones((H, W), device=dev)
+    lo = torch.zeros((H, W), device=dev)
+    hi = torch.full((H, W), 1e9, device=dev)
+    ids = step.ids[: step.num_intersects]
+    bw = scene.block_width
+
+    def edit():
+        return _C.texture_edit(step.tiles, (bw, bw, 1), (W, H, 1), (N, 1, 5), X,
+                               t["texture_dims"], img, alpha, lo, hi, ids, step.tile_bins,
+                               t["opacities"], t["means"], t["scales"], 1.0, t["quats"],
+                               t["uv0"], t["umap"], t["vmap"], t["viewmat"], t["c2w"],
+                               scene.fx, scene.fy, scene.cx, scene.cy, 0, t["background"])
+
+    edit_ms = timed_ms(edit)
+    Q = 1 << 24
+    dims = t["texture_dims"][torch.randint(0, N, (Q,), device=dev)].contiguous()
+    uvs = torch.rand((Q, 2), device=dev)
+    v_out = torch.rand((Q, C), device=dev)
+    fwd_ms = timed_ms(lambda: _C.texture_sample_forward((N, 1, C), dims, uvs, t["texture"]))
+    bwd_ms = timed_ms(lambda: _C.texture_sample_backward((N, 1, C), dims, uvs, t["texture"],
+                                                         v_out))
+    # per query: chart row 12 B, uv 8 B, 4 taps x C values read (forward) or
+    # added (adjoint), C values written (forward) or read (adjoint); the adjoint
+    # also clears the (X, C) texel gradient
+    q_bytes = 12 + 8 + 16 * C + 4 * C
+    fwd_gbs = Q * q_bytes / (fwd_ms * 1e-3) / 1e9
+    bwd_gbs = (Q * q_bytes + X * C * 4) / (bwd_ms * 1e-3) / 1e9
+    out = {"texture_edit_ms": round(edit_ms, 4),
+           "texture_edit_workload": f"{W}x{H}, {int(step.num_intersects)} intersects, 5 "
+                                    "channels, depth window unbounded (records built per call)",
+           "texture_sample_queries": Q,
+           "texture_sample_fwd_ms": round(fwd_ms, 4),
+           "texture_sample_fwd_GBps": round(fwd_gbs, 1),
+           "texture_sample_fwd_frac_hbm": round(fwd_gbs / HBM_PEAK_GBS, 4),
+           "texture_sample_bwd_ms": round(bwd_ms, 4),
+           "texture_sample_bwd_GBps": round(bwd_gbs, 1),
+           "texture_sample_bwd_frac_hbm": round(bwd_gbs / HBM_PEAK_GBS, 4)}
+    del dims, uvs, v_out
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(scene, step, target_seconds):
     """Time the FP32 oracle on a sample of tiles and scale to one full frame."""
     import oracle as O
@@ -363,6 +432,8 @@ def main():
         t = train_variant(scene, dev, args.train_steps, max(args.warmup // 2, 1))
         t["vs_plain"] = round(t["iter_per_s"] / value, 3)
         variants["train_step_adam"] = t
+    if args.edit_sample:
+        variants["texture_edit_and_sample"] = edit_sample_variant(scene, step, dev)
     if args.config_steps > 0:
         w = max(args.warmup // 2, 1)
         variants["c3_regularised_1600x1200_1m"] = regularised_variant(dev, args.config_steps, w)
