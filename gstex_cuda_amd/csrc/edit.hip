@@ -50,6 +50,12 @@ __global__ __launch_bounds__(256) void k_texture_edit(RasterArgs a, EditIn ed) {
     bool done = !inside;
     float T = 1.f;
     const WaveRect rect = wave_rect(inside, j, i);
+    // wave-private texel accumulator, kept zero between Gaussians by the flush
+    __shared__ float tex_acc[256 / kWave][kTexAccFloats];
+    float* acc = tex_acc[wave_index()];
+    for (int k = lane; k < kTexAccFloats; k += kWave) acc[k] = 0.f;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    const float inv_c4 = 1.f / (float)(4 * C);  // chart byte offset -> texel index
     // a pixel with T (1 - 1/255) <= 1e-4 can take no further entry (each one
     // with alpha >= 1/255 ends it), so it stops at once and culled entries
     // never matter (as in k_raster_forward)
@@ -89,14 +95,51 @@ __global__ __launch_bounds__(256) void k_texture_edit(RasterArgs a, EditIn ed) {
             if (!__any(hit) || !ch.ok) continue;
             float* const out_chart = ed.out + (size_t)s.start * C;
             const QuadMerge qm = quad_merge_setup(hit ? row[0] : -1, lane);
+            if (s.h * s.w * 5 <= kTexAccFloats) {
+                // charts of up to kTexAccFloats / 5 texels: merged lanes add
+                // into the wave's LDS accumulator (5 values per texel), then
+                // the texels some lane touched go out with one atomic each
+                int tix[4];
 #pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                float q[4];
+                for (int t = 0; t < 4; ++t) tix[t] = (int)((float)row[t] * inv_c4 + 0.5f);
 #pragma unroll
-                for (int t = 0; t < 4; ++t) q[t] = quad_merge(wt[t] * val[k], qm);
+                for (int k = 0; k < 5; ++k) {
+                    float q[4];
 #pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    if (q[t] != 0.f) unsafeAtomicAdd(out_chart + (row[t] >> 2) + k, q[t]);
+                    for (int t = 0; t < 4; ++t) q[t] = quad_merge(wt[t] * val[k], qm);
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        if (q[t] != 0.f) atomicAdd(acc + tix[t] * 5 + k, q[t]);
+                }
+                int tlo = 0x7fffffff, thi = -1;
+                if (hit) {
+                    tlo = min(min(tix[0], tix[1]), min(tix[2], tix[3]));
+                    thi = max(max(tix[0], tix[1]), max(tix[2], tix[3]));
+                }
+                tlo = wave_min(tlo);
+                thi = wave_max(thi);
+                // every lane's LDS adds precede these reads in the wave's LDS queue
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                const int nv = (thi - tlo + 1) * 5;
+                for (int idx = lane; idx < nv; idx += kWave) {
+                    const int tx = tlo + idx / 5, k = idx - (idx / 5) * 5;
+                    const float v = acc[tx * 5 + k];
+                    if (v != 0.f) {
+                        unsafeAtomicAdd(out_chart + (size_t)tx * C + k, v);
+                        acc[tx * 5 + k] = 0.f;
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            } else {
+#pragma unroll
+                for (int k = 0; k < 5; ++k) {
+                    float q[4];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) q[t] = quad_merge(wt[t] * val[k], qm);
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        if (q[t] != 0.f) unsafeAtomicAdd(out_chart + (row[t] >> 2) + k, q[t]);
+                }
             }
         }
     }
