@@ -56,6 +56,19 @@ __global__ __launch_bounds__(256) void k_sample_backward(int q, int C, int num_t
     int row[4];
     sample_taps(dims[3 * i], dims[3 * i + 1], dims[3 * i + 2], uvs[2 * i], uvs[2 * i + 1],
                 num_texels, wt, row);
+    // taps that clamp onto one texel (every tap of a 1x1 chart, the far
+    // edge of larger ones) are added once with their summed weight; a
+    // zero weight masks the lane off the atomic
+#pragma unroll
+    for (int k = 1; k < 4; ++k) {
+#pragma unroll
+        for (int j = 0; j < k; ++j) {
+            if (wt[k] != 0.f && wt[j] != 0.f && row[k] == row[j]) {
+                wt[j] += wt[k];
+                wt[k] = 0.f;
+            }
+        }
+    }
     for (int c = 0; c < C; ++c) {
         const float g = v_out[(size_t)i * C + c];
 #pragma unroll

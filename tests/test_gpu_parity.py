@@ -345,6 +345,30 @@ def test_texture_sample_reference_scenario(oracle):
                                atol=1e-5)
 
 
+def test_texture_sample_adjoint_one_texel_charts(oracle):
+    # charts of 1x1, 1x2, 2x1 and 2x2 texels: most taps clamp onto a shared
+    # texel and 4096 queries pile onto 8 charts
+    from gstex_cuda_amd import cuda as _C
+
+    rng = np.random.default_rng(7)
+    h = np.array([1, 1, 2, 2, 1, 1, 2, 1])
+    w = np.array([1, 2, 1, 2, 1, 1, 2, 1])
+    start = np.concatenate([[0], np.cumsum(h * w)[:-1]])
+    dims = np.stack([h, w, start], 1).astype(np.int32)
+    X = int((h * w).sum())
+    tex = rng.uniform(0, 1, (X, 3)).astype(np.float32)
+    qd = dims[rng.integers(0, len(h), 4096)]
+    uvs = rng.uniform(0, 1, (4096, 2)).astype(np.float32)
+    v = rng.standard_normal((4096, 3)).astype(np.float32)
+    got = _C.texture_sample_backward((len(h), 1, 3), to_dev(qd), to_dev(uvs), to_dev(tex),
+                                     to_dev(v))
+    want = oracle.texture_sample_vjp(qd, uvs, v, X, "f64")
+    # sums of about 500 terms of size 1 in float32, in atomic order
+    np.testing.assert_allclose(np_(got), want, atol=2e-4)
+    fwd = _C.texture_sample_forward((len(h), 1, 3), to_dev(qd), to_dev(uvs), to_dev(tex))
+    np.testing.assert_allclose(np_(fwd), oracle.texture_sample(qd, uvs, tex, "f64"), atol=1e-5)
+
+
 def test_texture_sample_affine_kat():
     from gstex_cuda_amd import cuda as _C
 
