@@ -18,6 +18,7 @@ CASES = {
     "w1_blur": (1, 800, 96, 64, 16, 1),
     "w1_bw8": (1, 1500, 80, 56, 8, 0),
     "w2_small": (2, 3000, 160, 90, 16, 0),
+    "w3_one_texel": (3, 2000, 160, 120, 16, 0),  # every chart 1x1
 }
 
 
