@@ -28,7 +28,6 @@ from typing import Dict, Iterable, Optional
 
 import torch
 
-from . import cuda as _C
 from .pipeline import Camera, RenderStep, StageTimer
 
 
@@ -50,22 +49,18 @@ def _key(params, cam: Camera, settings: int) -> str:
 
 def _time_config(params, cam, target, cfg: RasterConfig, settings, background, sh_degree,
                  trials: int) -> float:
-    prev = _C.set_raster_culling(cfg.culling)
-    try:
-        step = RenderStep(params, cam, block_width=cfg.block_width, settings=settings,
-                          background=background, sh_degree=sh_degree,
-                          use_wave_masks=cfg.use_wave_masks)
+    step = RenderStep(params, cam, block_width=cfg.block_width, settings=settings,
+                      background=background, sh_degree=sh_degree,
+                      use_wave_masks=cfg.use_wave_masks, culling=cfg.culling)
+    step.step(target)
+    torch.cuda.synchronize()
+    timer = StageTimer()
+    for _ in range(trials):
+        timer.start("all")
         step.step(target)
-        torch.cuda.synchronize()
-        timer = StageTimer()
-        for _ in range(trials):
-            timer.start("all")
-            step.step(target)
-            timer.stop("all")
-        torch.cuda.synchronize()
-        return timer.mean_ms("all")
-    finally:
-        _C.set_raster_culling(prev)
+        timer.stop("all")
+    torch.cuda.synchronize()
+    return timer.mean_ms("all")
 
 
 def tune(params, cam: Camera, target: torch.Tensor, settings: int = 1 << 8,
@@ -96,12 +91,22 @@ def tune(params, cam: Camera, target: torch.Tensor, settings: int = 1 << 8,
 
 
 def tuned_step(params, cam: Camera, target: torch.Tensor, **kw) -> RenderStep:
-    """A RenderStep built with the tuned configuration (culling switched process-wide)."""
+    """A RenderStep built with the tuned configuration.
+
+    ``block_widths`` (default: the caller's ``block_width``, else 16) are the tile
+    sides the tuner may choose among; the tuned choice replaces any
+    ``block_width`` / ``use_wave_masks`` / ``culling`` passed here.  The culling
+    choice belongs to the returned step (its raster launches set it and restore
+    the previous process-wide value), so other steps are unaffected."""
+    bw = kw.pop("block_width", None)
+    kw.pop("use_wave_masks", None)
+    kw.pop("culling", None)
+    widths = tuple(kw.pop("block_widths", (bw if bw is not None else 16,)))
     cfg = tune(params, cam, target, settings=kw.get("settings", 1 << 8),
-               background=kw.get("background"), sh_degree=kw.get("sh_degree", -1))
-    _C.set_raster_culling(cfg.culling)
+               background=kw.get("background"), sh_degree=kw.get("sh_degree", -1),
+               block_widths=widths)
     return RenderStep(params, cam, block_width=cfg.block_width,
-                      use_wave_masks=cfg.use_wave_masks, **kw)
+                      use_wave_masks=cfg.use_wave_masks, culling=cfg.culling, **kw)
 
 
 def last_timings() -> Dict[str, dict]:

@@ -36,6 +36,14 @@ hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 int tiles_of(int extent, int bw) { return (extent + bw - 1) / bw; }
 
+// Tile grids the kernels can index: tile coordinates are packed as 16-bit
+// halves in the binning emission, and tile indices (and the raster grid) are
+// 32-bit ints.
+#define REQUIRE_TILE_GRID(tx, ty)                                                           \
+    REQUIRE((tx) < 65536 && (ty) < 65536 && (int64_t)(tx) * (int64_t)(ty) <= 0x7fffffffll,   \
+            "tile grid %d x %d is too large (each side below 65536, at most 2^31 - 1 tiles)", \
+            (tx), (ty))
+
 int check_raster_common(int tiles_x, int tiles_y, int block_width, int img_width, int img_height,
                         int num_probs, int channels, int num_points, int num_ids, int num_texels,
                         int settings) {
@@ -43,6 +51,7 @@ int check_raster_common(int tiles_x, int tiles_y, int block_width, int img_width
             "block_width must be between 2 and 16, got %d", block_width);
     REQUIRE(img_width > 0 && img_height > 0, "image size must be positive, got %dx%d", img_width,
             img_height);
+    REQUIRE_TILE_GRID(tiles_x, tiles_y);
     REQUIRE(tiles_x == tiles_of(img_width, block_width) &&
                 tiles_y == tiles_of(img_height, block_width),
             "tile_bounds (%d, %d) do not match img_size (%d, %d) and block width %d", tiles_x,
@@ -192,6 +201,7 @@ int gstex_map_gaussian_to_intersects(int num_points, int num_intersects, const f
                                      int32_t* gaussian_ids, void* stream) {
     REQUIRE(num_points >= 0 && num_intersects >= 0, "negative sizes");
     REQUIRE(block_width > 0 && tiles_x >= 0 && tiles_y >= 0, "invalid tile bounds");
+    REQUIRE_TILE_GRID(tiles_x, tiles_y);
     REQUIRE(!wrapped || (tiles_x > 0 && tiles_y > 0), "wrapped binning needs a non-empty tile grid");
     if (num_intersects == 0) return GSTEX_OK;
     REQUIRE(isect_ids && gaussian_ids, "null pointer argument");
@@ -229,6 +239,7 @@ int gstex_sort_intersects(int num_intersects, int end_bit, const int64_t* isect_
 int gstex_get_tile_bin_edges(int num_intersects, const int64_t* isect_ids_sorted, int tiles_x,
                              int tiles_y, int32_t* tile_bins, void* stream) {
     REQUIRE(num_intersects >= 0 && tiles_x >= 0 && tiles_y >= 0, "negative sizes");
+    REQUIRE_TILE_GRID(tiles_x, tiles_y);
     if (tiles_x * tiles_y == 0) return GSTEX_OK;
     REQUIRE(tile_bins && (num_intersects == 0 || isect_ids_sorted), "null pointer argument");
     return check_hip(gstex::tile_bin_edges(num_intersects, isect_ids_sorted, tiles_x * tiles_y,
@@ -247,6 +258,7 @@ int gstex_bin_gaussians(int num_points, int num_intersects, const float* centers
                         int32_t* tile_bins, void* workspace, size_t workspace_bytes, void* stream) {
     REQUIRE(num_points >= 0 && num_intersects >= 0, "negative sizes");
     REQUIRE(block_width > 0 && tiles_x >= 0 && tiles_y >= 0, "invalid tile bounds");
+    REQUIRE_TILE_GRID(tiles_x, tiles_y);
     if (tiles_x * tiles_y == 0) return GSTEX_OK;
     REQUIRE(tile_bins, "null pointer argument");
     if (num_points > 0 && num_intersects > 0) {
@@ -270,6 +282,7 @@ int gstex_bin_gaussians_capped(int num_points, int capacity, const float* center
                                size_t workspace_bytes, void* stream) {
     REQUIRE(num_points >= 0 && capacity >= 0, "negative sizes");
     REQUIRE(block_width > 0 && tiles_x >= 0 && tiles_y >= 0, "invalid tile bounds");
+    REQUIRE_TILE_GRID(tiles_x, tiles_y);
     if (tiles_x * tiles_y == 0) return GSTEX_OK;
     REQUIRE(tile_bins && num_intersects_out, "null pointer argument");
     if (num_points > 0) {

@@ -242,6 +242,14 @@ _RASTER_INPUTS = ("texture_dims", "gaussian_ids_sorted", "tile_bins", "colors", 
                   "background")
 
 
+def _texture_shape(texture: torch.Tensor, channels: int) -> None:
+    """The kernels size every chart as h * w * channels values of ``texture``
+    rows, so its row width must be the channel count."""
+    if texture.dim() != 2 or texture.size(1) != int(channels):
+        raise RuntimeError(f"texture must have dimensions (X, {int(channels)}) to match "
+                           f"texture_info[2], got {tuple(texture.shape)}")
+
+
 def _raster_checks(named, bf16_texture_ok=False):
     for name in _RASTER_INPUTS:
         _check(named[name], name)
@@ -254,6 +262,7 @@ def _raster_checks(named, bf16_texture_ok=False):
         _texture_format(named["texture"])
     else:
         _dtype(named["texture"], "texture", torch.float32)
+    _texture_shape(named["texture"], named["texture_info"][2])
 
 
 def texture_forward(tile_bounds, block, img_size, texture_info, texture_dims,
@@ -391,6 +400,7 @@ def raster_forward(records, num_points, img_width, img_height, block_width, chan
                     ("texture", texture), ("viewmat", viewmat), ("c2w", c2w),
                     ("background", background)):
         _check(t, name)
+    _texture_shape(texture, channels)
     dev = records.device
     W, H, C = int(img_width), int(img_height), int(channels)
     tx, ty = (W + block_width - 1) // block_width, (H + block_width - 1) // block_width
@@ -425,6 +435,8 @@ def raster_backward(records, num_points, img_width, img_height, block_width, cha
     for name in ("final_Ts", "final_idx", "depth_idx", "final_s", "v_output", "v_output_depth",
                  "v_output_reg", "v_output_alpha", "v_output_texture", "v_output_normal"):
         _check(named[name], name)
+    _check(texture, "texture")
+    _texture_shape(texture, channels)
     dev = records.device
     n, C = int(num_points), int(channels)
     grad_records = torch.zeros((n, 32), dtype=torch.float32, device=dev)
@@ -461,13 +473,40 @@ def finalize_gradients(grad_records, records, quats, scales, viewmat, c2w, fx, f
 
 
 # --------------------------------------------------------------------------- sampling
+def _sample_checks(texture_info, texture_dims, uvs, texture, v_output=None) -> int:
+    """Dtype and shape checks of texture_sample_forward/backward (the reference
+    raises through data_ptr<int>/data_ptr<float>, texture_sample.cu:98-101,
+    133-137); returns the query count.  Each query reads its own texture_dims
+    row, and the kernels index rows of C = texture_info[2] float32 values."""
+    named = dict(texture_dims=texture_dims, uvs=uvs, texture=texture)
+    if v_output is not None:
+        named["v_output"] = v_output
+    for nm, t in named.items():
+        _check(t, nm)
+    _dtype(texture_dims, "texture_dims", torch.int32)
+    for nm in ("uvs", "texture", "v_output"):
+        if nm in named:
+            _dtype(named[nm], nm, torch.float32)
+    C = int(texture_info[2])
+    if uvs.dim() != 2 or uvs.size(1) != 2:
+        raise RuntimeError(f"uvs must have dimensions (Q, 2), got {tuple(uvs.shape)}")
+    q = uvs.size(0)
+    if texture_dims.dim() != 2 or texture_dims.size(1) != 3 or texture_dims.size(0) < q:
+        raise RuntimeError(f"texture_dims must have dimensions (Q, 3) with Q >= {q} queries, "
+                           f"got {tuple(texture_dims.shape)}")
+    _texture_shape(texture, C)
+    if v_output is not None and (v_output.dim() != 2 or v_output.size(0) != q
+                                 or v_output.size(1) != C):
+        raise RuntimeError(f"v_output must have dimensions ({q}, {C}), got "
+                           f"{tuple(v_output.shape)}")
+    return q
+
+
 def texture_sample_forward(texture_info, texture_dims, uvs, texture):
     """texture_sample.cu:72-105 -> (Q, C)."""
-    for t, nm in ((texture_dims, "texture_dims"), (uvs, "uvs"), (texture, "texture")):
-        _check(t, nm)
+    q = _sample_checks(texture_info, texture_dims, uvs, texture)
     dev = texture.device
     C = int(texture_info[2])
-    q = uvs.size(0)
     out = torch.empty((q, C), dtype=torch.float32, device=dev)
     with torch.cuda.device(dev):
         call("gstex_texture_sample_forward", int(texture_info[0]), int(texture_info[1]), C,
@@ -479,9 +518,7 @@ def texture_sample_forward(texture_info, texture_dims, uvs, texture):
 def texture_sample_backward(texture_info, texture_dims, uvs, texture, v_output):
     """Adjoint of texture_sample_forward w.r.t. the texels -> (X, C)
     (texture_sample.cu:107-141, reading v_output)."""
-    for t, nm in ((texture_dims, "texture_dims"), (uvs, "uvs"), (texture, "texture"),
-                  (v_output, "v_output")):
-        _check(t, nm)
+    _sample_checks(texture_info, texture_dims, uvs, texture, v_output)
     dev = texture.device
     C = int(texture_info[2])
     v_texture = torch.empty((texture.size(0), C), dtype=torch.float32, device=dev)

@@ -68,13 +68,16 @@ class RenderStep:
     quats (N,4), opacities (N,1), uv0 (N,1,2), umap (N,1,3), vmap (N,1,3),
     texture (X,C) float32 or bfloat16, texture_dims (N,3) int32, and either
     colors (N,3) or sh_coeffs (N,D,3) float32 or bfloat16 with ``sh_degree``.
+    ``culling`` (None: the library's process-wide setting) switches the raster
+    kernels' screen-box culling for this step's launches only.
     """
 
     def __init__(self, params: Dict[str, torch.Tensor], cam: Camera, block_width: int = 16,
                  settings: int = 1 << 8, background: Optional[torch.Tensor] = None,
                  sh_degree: int = -1, use_wave_masks: bool = True,
-                 capacity: Optional[int] = None):
+                 capacity: Optional[int] = None, culling: Optional[bool] = None):
         self.p = params
+        self.culling = culling
         self.cam = cam
         self.bw = block_width
         self.settings = settings
@@ -166,6 +169,16 @@ class RenderStep:
             self.wave_masks = torch.empty((int(need * 1.25) + 4096,), dtype=torch.uint8,
                                           device=self.dev)
 
+    def _raster(self, name: str, *args):
+        """One raster launch, under this step's culling choice when it has one."""
+        if self.culling is None:
+            return call(name, *args)
+        prev = _lib.lib().gstex_set_raster_culling(int(bool(self.culling)))
+        try:
+            return call(name, *args)
+        finally:
+            _lib.lib().gstex_set_raster_culling(prev)
+
     def _masks(self):
         if not self.use_wave_masks:
             return None, 0
@@ -206,7 +219,7 @@ class RenderStep:
              cam.cx, cam.cy, _p(self.records), s)
         if timer:
             timer.start("raster_forward")
-        call("gstex_raster_forward", cam.img_w, cam.img_h, self.bw, self.C, self.settings,
+        self._raster("gstex_raster_forward", cam.img_w, cam.img_h, self.bw, self.C, self.settings,
              _p(self.records), n, _p(self.ids), m, _p(self.tile_bins), _p(p["texture"]), self.X,
              self.texture_format, _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy, cam.cx, cam.cy,
              _p(self.background),
@@ -267,7 +280,7 @@ class RenderStep:
         v_normal = self.v_zero3 if v_normal is None else v_normal
         if timer:
             timer.start("raster_backward")
-        call("gstex_raster_backward", cam.img_w, cam.img_h, self.bw, self.C, self.settings,
+        self._raster("gstex_raster_backward", cam.img_w, cam.img_h, self.bw, self.C, self.settings,
              _p(self.records), n, _p(self.ids), m, _p(self.tile_bins), _p(p["texture"]), self.X,
              self.texture_format, _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy, cam.cx, cam.cy,
              _p(self.background),

@@ -118,3 +118,28 @@ def test_synthetic_scenes_are_deterministic():
     assert a.texture_dims[:, 0].min() >= 1 and a.texture_dims[:, 0].max() <= 16
     assert np.array_equal(a.texture_dims[1:, 2],
                           np.cumsum(a.texture_dims[:, 0] * a.texture_dims[:, 1])[:-1])
+
+
+def test_binning_rejects_tile_grids_beyond_the_packing():
+    """Tile coordinates travel as 16-bit halves in the binning emission and tile
+    indices as ints, so grids of 65536 tiles on a side (or above 2^31 - 1
+    tiles) are refused before any launch."""
+    from gstex_cuda_amd import _lib
+
+    L = _lib.lib()
+    fake = ctypes.c_void_p(0x1000)
+    for tx, ty in ((65536, 1), (1, 70000), (50000, 50000)):
+        rc = L.gstex_bin_gaussians(4, 8, fake, fake, fake, fake, tx, ty, 2, fake, fake, fake,
+                                   1 << 30, None)
+        assert rc == 1 and "tile grid" in L.gstex_last_error().decode(), (tx, ty)
+        rc = L.gstex_bin_gaussians_capped(4, 8, fake, fake, fake, fake, tx, ty, 2, fake, fake,
+                                          fake, fake, 1 << 30, None)
+        assert rc == 1 and "tile grid" in L.gstex_last_error().decode(), (tx, ty)
+        rc = L.gstex_map_gaussian_to_intersects(4, 8, fake, fake, fake, fake, tx, ty, 2, 0,
+                                                fake, fake, None)
+        assert rc == 1 and "tile grid" in L.gstex_last_error().decode(), (tx, ty)
+        rc = L.gstex_get_tile_bin_edges(8, fake, tx, ty, fake, None)
+        assert rc == 1 and "tile grid" in L.gstex_last_error().decode(), (tx, ty)
+    # a raster launch over such a grid (width 2 * 65536 at block width 2)
+    rc, err = _call_forward(tiles_x=65536, tiles_y=1, bw=2, W=131072, H=2)
+    assert rc == 1 and "tile grid" in err

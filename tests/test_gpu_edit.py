@@ -12,24 +12,29 @@ from helpers import oracle_scene, tile_bounds
 
 pytestmark = pytest.mark.gpu
 
-# name -> (workload, num_points, W, H, block_width, edit settings (bit 0 = blur))
+# name -> (workload, num_points, W, H, block_width, edit settings (bit 0 = blur), channels)
 CASES = {
-    "w1": (1, 800, 96, 64, 16, 0),
-    "w1_blur": (1, 800, 96, 64, 16, 1),
-    "w1_bw8": (1, 1500, 80, 56, 8, 0),
-    "w2_small": (2, 3000, 160, 90, 16, 0),
-    "w3_one_texel": (3, 2000, 160, 120, 16, 0),  # every chart 1x1
+    "w1": (1, 800, 96, 64, 16, 0, 5),
+    "w1_blur": (1, 800, 96, 64, 16, 1, 5),
+    "w1_bw8": (1, 1500, 80, 56, 8, 0, 5),
+    "w2_small": (2, 3000, 160, 90, 16, 0, 5),
+    "w3_one_texel": (3, 2000, 160, 120, 16, 0, 5),  # every chart 1x1
+    # C = 8: the LDS route's texel index (byte offset / 4C) and flush stride,
+    # and the direct route's row stride, at a width other than 5; the mixed
+    # 1x1..16x16 charts of workload 2 take both routes (h w 8 <= 1024 or not)
+    "w2_c8": (2, 3000, 160, 90, 16, 0, 8),
+    "w1_c8_bw8": (1, 1500, 80, 56, 8, 0, 8),
 }
 
 
 def _case(name):
     from gstex_cuda_amd import synthetic
 
-    wl, n, W, H, bw, st = CASES[name]
+    wl, n, W, H, bw, st, C = CASES[name]
     spec = dataclasses.replace(synthetic.WORKLOADS[wl], block_width=bw)
     s = synthetic.make_scene(wl, seed=20 + sum(map(ord, name)), num_points=n, img_w=W, img_h=H,
                              spec=spec)
-    s.texture = np.zeros((s.texture.shape[0], 5), np.float32)
+    s.texture = np.zeros((s.texture.shape[0], C), np.float32)
     rng = np.random.default_rng(sum(map(ord, name)))
     img = rng.random((H, W, 3)).astype(np.float32)
     al = (rng.random((H, W)) * (rng.random((H, W)) < 0.7)).astype(np.float32)
@@ -45,10 +50,14 @@ def test_texture_edit_matches_oracle(oracle, name):
     s, st, img, al, lo, hi = _case(name)
     log(f"case {name}")
     b = project_and_bin(s)
-    X = s.texture.shape[0]
+    X, C = s.texture.shape
     tb = tile_bounds(s.img_w, s.img_h, s.block_width)
+    if C > 5:
+        hw = s.texture_dims[:, 0].astype(np.int64) * s.texture_dims[:, 1]
+        log(f"  charts on the LDS route {int((hw * C <= 1024).sum())}, "
+            f"direct {int((hw * C > 1024).sum())}")
     out = _C.texture_edit(tb, (s.block_width, s.block_width, 1), (s.img_w, s.img_h, 1),
-                          (s.num_points, 1, 5), X, to_dev(s.texture_dims), to_dev(img),
+                          (s.num_points, 1, C), X, to_dev(s.texture_dims), to_dev(img),
                           to_dev(al), to_dev(lo), to_dev(hi), b["ids"], b["bins"],
                           to_dev(s.opacities), to_dev(s.means), to_dev(s.scales), 1.0,
                           to_dev(s.quats), to_dev(s.uv0), to_dev(s.umap), to_dev(s.vmap),
@@ -60,6 +69,9 @@ def test_texture_edit_matches_oracle(oracle, name):
     ref = oracle.texture_edit(sc, img, al, lo, hi, "f64")
     log(f"  weight total {ref[:, 4].sum():.1f}")
     assert ref[:, 4].sum() > 0
+    assert got.shape == (X, C)
+    # the edit writes channels 0..4 only; the rest of the output stays zero
+    assert not np.any(got[:, 5:]), "channels past 4 were written"
     # float atomics in any order and float32 pair math: relative to each channel's scale
     for k in range(5):
         scale = max(np.abs(ref[:, k]).max(), 1e-12)
