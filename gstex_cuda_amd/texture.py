@@ -86,7 +86,8 @@ class _TextureGaussians(Function):
                                tile_bins=tile_bins, colors=colors, opacities=opacity,
                                means=means, scales=scales, quats=quats, uv0=uv0, umap=umap,
                                vmap=vmap, texture=texture, viewmat=viewmat, c2w=c2w,
-                               background=background), bf16_texture_ok=True)
+                               background=background, texture_info=texture_info),
+                          bf16_texture_ok=True)
         records = _C.build_splat_records(means, scales, glob_scale, quats, colors, opacity, uv0,
                                          umap, vmap, texture_dims, viewmat, c2w, fx, fy, cx, cy)
         C = int(texture_info[2])
