@@ -19,6 +19,14 @@ here come from the mathematics the reference implements:
   bilinear  a texel grid that is affine in (i, j) is reproduced exactly by
             bilinear sampling inside the grid, and clamped (replicate-padded)
             outside it
+  splat_tex one camera-facing splat (identity, and turned in its plane by a
+            z-axis quaternion) over a 4 x 5 chart whose three channels are
+            affine in the texel indices: at a pixel the ray meets the plane
+            z = m_z at offset d = (m_z (px - cx)/f - m_x, m_z (py - cy)/f - m_y, 0)
+            from the mean, l1 = d.ax1, l2 = d.ax2, u = u0 + umap.d and
+            v = v0 + vmap.d clamped to [0, 1] (umap and vmap mix x and y), and
+            the bilinear lookup is A + B min(h u, h - 1) + C min(w v, w - 1);
+            out_texture = alpha * that, out_img = alpha * colour + (1 - alpha) bg
 Run: python tests/golden/make_golden.py
 """
 import json
@@ -148,9 +156,64 @@ def bilinear_vectors():
     return {"texture": rows, "queries": queries}
 
 
+def splat_tex_vectors():
+    """One camera-facing splat over an affine 4 x 5 x 3 chart (h = 4 indexes u,
+    w = 5 indexes v; texel (i, j) is row i * w + j)."""
+    W, H, f, cx, cy = 48, 40, 80.0, 24.0, 20.0
+    h, w = 4, 5
+    # dyadic coefficients: every texel value is exact in float32
+    A = [0.125, 0.875, 0.3125]
+    B = [0.1875, -0.09375, 0.0]     # per row step (u)
+    C = [0.046875, 0.03125, -0.0625]  # per column step (v)
+    rows = [[A[k] + B[k] * i + C[k] * j for k in range(3)] for i in range(h) for j in range(w)]
+    assert rows == r32(rows)
+    cases = []
+    for theta in (0.0, 0.6):
+        mean = r32([0.04, -0.03, 2.5])
+        sx, sy, opac = r32([0.15, 0.10, 0.8])
+        q = r32([math.cos(theta / 2), 0.0, 0.0, math.sin(theta / 2)])
+        qw, qz = q[0], q[3]
+        ax1 = [1 - 2 * qz * qz, 2 * qw * qz, 0.0]       # columns of R(q), unnormalised q
+        ax2 = [-2 * qw * qz, 1 - 2 * qz * qz, 0.0]
+        uv0 = r32([0.45, 0.55])
+        umap = r32([2.0, 0.8, 0.0])
+        vmap = r32([-0.6, 2.5, 0.0])
+        color, bg = r32([0.9, 0.2, 0.4]), r32([0.1, 0.3, 0.5])
+        tex_out = np.zeros((H, W, 3))
+        img = np.zeros((H, W, 3))
+        alpha_map = np.zeros((H, W))
+        check = np.ones((H, W), bool)
+        for i in range(H):
+            for j in range(W):
+                px, py = j + 0.5, i + 0.5
+                d = [mean[2] * (px - cx) / f - mean[0], mean[2] * (py - cy) / f - mean[1], 0.0]
+                l1 = sum(a * b for a, b in zip(d, ax1))
+                l2 = sum(a * b for a, b in zip(d, ax2))
+                a = min(0.99, opac * math.exp(-0.5 * (l1 * l1 / sx ** 2 + l2 * l2 / sy ** 2)))
+                # float rounding may flip the 1/255 test right at the threshold
+                check[i, j] = abs(a - 1.0 / 255.0) > 1e-5
+                if a < 1.0 / 255.0:
+                    a = 0.0
+                u = min(max(uv0[0] + sum(x * y for x, y in zip(umap, d)), 0.0), 1.0)
+                v = min(max(uv0[1] + sum(x * y for x, y in zip(vmap, d)), 0.0), 1.0)
+                ti, tj = min(h * u, h - 1.0), min(w * v, w - 1.0)
+                alpha_map[i, j] = a
+                for k in range(3):
+                    tex_out[i, j, k] = a * (A[k] + B[k] * ti + C[k] * tj)
+                    img[i, j, k] = a * color[k] + (1 - a) * bg[k]
+        cases.append({"W": W, "H": H, "f": f, "cx": cx, "cy": cy, "mean": mean,
+                      "scale": [sx, sy, 0.0], "quat": q, "opacity": opac, "color": color,
+                      "background": bg, "uv0": uv0, "umap": umap, "vmap": vmap,
+                      "dims": [h, w, 0], "texture": rows, "out_texture": tex_out.tolist(),
+                      "img": img.tolist(), "alpha": alpha_map.tolist(),
+                      "check": check.astype(int).tolist()})
+    return cases
+
+
 def main():
     data = {"sh": sh_vectors(), "aabb": aabb_vectors(), "splat1": splat1_vectors(),
-            "splat2": splat2_vectors(), "bilinear": bilinear_vectors()}
+            "splat2": splat2_vectors(), "bilinear": bilinear_vectors(),
+            "splat_tex": splat_tex_vectors()}
     with open(os.path.join(HERE, "kat.json"), "w") as fh:
         json.dump(data, fh, indent=1)
 

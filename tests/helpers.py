@@ -90,3 +90,21 @@ def camera_facing_scene(means, scales, opacities, colors, W, H, f, cx, cy, texel
                  texture_dims=dims, viewmat=eye.copy(), c2w=eye.copy(),
                  background=np.asarray(background, np.float32), target=np.zeros((H, W, 3), np.float32),
                  fx=f, fy=f, cx=cx, cy=cy, img_h=H, img_w=W, block_width=16, settings=1 << 8)
+
+
+def textured_splat_scene(k):
+    """The one-splat scene of a ``splat_tex`` known-answer case: a 4 x 5 chart
+    with explicit uv0 / umap / vmap and quaternion."""
+    from gstex_cuda_amd.synthetic import Scene
+
+    f32 = np.float32
+    H, W = k["H"], k["W"]
+    eye = np.eye(4, dtype=f32)
+    return Scene(means=np.array([k["mean"]], f32), scales=np.array([k["scale"]], f32),
+                 quats=np.array([k["quat"]], f32), opacities=np.array([[k["opacity"]]], f32),
+                 colors=np.array([k["color"]], f32), uv0=np.array([[k["uv0"]]], f32),
+                 umap=np.array([[k["umap"]]], f32), vmap=np.array([[k["vmap"]]], f32),
+                 texture=np.array(k["texture"], f32), texture_dims=np.array([k["dims"]], np.int32),
+                 viewmat=eye.copy(), c2w=eye.copy(), background=np.array(k["background"], f32),
+                 target=np.zeros((H, W, 3), f32), fx=k["f"], fy=k["f"], cx=k["cx"], cy=k["cy"],
+                 img_h=H, img_w=W, block_width=16, settings=1 << 8)

@@ -443,6 +443,22 @@ def test_forward_closed_form_single_splat():
     np.testing.assert_allclose(1 - np_(g["final_Ts"]), np.array(k["alpha"]), atol=2e-6)
 
 
+@pytest.mark.parametrize("case", [0, 1])
+def test_forward_closed_form_textured_splat(case):
+    # the splat_tex known answer (closed-form plane hits over an affine 4 x 5
+    # chart with u and v mixing x and y, clamped at both ends)
+    from helpers import textured_splat_scene
+
+    k = KAT["splat_tex"][case]
+    s = textured_splat_scene(k)
+    pb = project_and_bin(s)
+    g, _ = gpu_forward(s, pb["ids"], pb["bins"])
+    chk = np.array(k["check"], bool)
+    np.testing.assert_allclose(np_(g["out_texture"])[chk], np.array(k["out_texture"])[chk],
+                               atol=2e-5)
+    np.testing.assert_allclose(np_(g["out_img"])[chk], np.array(k["img"])[chk], atol=2e-5)
+
+
 def test_forward_invariants_1080p_500k(oracle):
     # 1080p, 500k Gaussians: sum of weights + T = 1 everywhere, and a sample of
     # tiles rendered exactly by the oracle

@@ -254,6 +254,26 @@ def test_single_splat_closed_form(oracle):
         assert np.all(f["out_reg"] == 0)
 
 
+@pytest.mark.parametrize("case", [0, 1])
+def test_textured_splat_closed_form(oracle, case):
+    # tests/golden/make_golden.py splat_tex: one camera-facing splat (case 1
+    # turned in its plane) over a 4 x 5 chart affine in its texel indices;
+    # the expected texture is alpha * (A + B min(h u, h-1) + C min(w v, w-1))
+    # with u, v from the closed-form plane hit, so a swapped u/v, umap/vmap,
+    # h/w or axis fails it
+    from helpers import textured_splat_scene
+
+    k = KAT["splat_tex"][case]
+    s = textured_splat_scene(k)
+    chk = np.array(k["check"], bool)
+    for prec, tol in (("f64", 1e-9), ("f32", 2e-5)):
+        f = oracle.texture_forward(oracle_scene(oracle, s), prec)
+        np.testing.assert_allclose(f["out_texture"][chk], np.array(k["out_texture"])[chk],
+                                   atol=tol)
+        np.testing.assert_allclose(f["out_img"][chk], np.array(k["img"])[chk], atol=tol)
+        np.testing.assert_allclose((1 - f["final_Ts"])[chk], np.array(k["alpha"])[chk], atol=tol)
+
+
 @pytest.mark.parametrize("outline_q,white", [(0, False), (6, False), (6, True), (11, True)])
 def test_alpha_visualisation_single_splat_geometry(oracle, outline_q, white):
     # texture.cu:56-65,192-241 with local_outline (texture_helpers.cuh:390-417):
