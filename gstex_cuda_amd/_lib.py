@@ -51,6 +51,12 @@ _SIGNATURES = {
                                   P, P, F, F, F, F, I, P,  # viewmat, c2w, intrinsics, settings, bg
                                   P, P, P, P, P, P, P, P, P,  # nine outputs
                                   P, Z, P]),
+    "gstex_texture_forward_noalloc": (I, [I, I, I, I, I, I, I, I,
+                                          P, P, I, P,
+                                          P, P, P, P, I, F, P, P, P, P, P, I,
+                                          P, P, F, F, F, F, I, P,
+                                          P, P, P, P, P, P, P, P, P,
+                                          P]),  # stream
     "gstex_texture_backward_workspace_size": (Z, [I]),
     "gstex_texture_backward": (I, [I, I, I, I, I, I,  # H, W, bw, texture_info
                                    P, P, I, P,

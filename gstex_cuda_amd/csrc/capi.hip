@@ -3,6 +3,8 @@
 #include <stdarg.h>
 #include <stdio.h>
 
+#include <mutex>
+
 #include "../../include/gstex_hip.h"
 #include "gstex_common.hpp"
 #include "gstex_kernels.hpp"
@@ -732,6 +734,78 @@ int gstex_adam_step(size_t n, float* param, const float* grad, float* exp_avg, f
                                       step, grad_mode, values,
                                       values_format == GSTEX_TEXTURE_BF16, as_stream(stream)),
                      "adam_step");
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free forward
+// gstex_texture_forward_noalloc keeps the forward's splat records (its only
+// scratch) in this file-scope device array instead of a caller workspace.
+// Each device the library runs on holds one copy until the process ends; its
+// device address comes from hipGetSymbolAddress, once per device, and only
+// that address is handed to kernels (in host code the array's name is a host
+// stand-in, not a device pointer).  The records are written in full before
+// they are read, so nothing in it needs clearing between calls.
+static __device__ __align__(256) unsigned char scratch[GSTEX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 256;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of this device's scratch array (looked up on the first call
+// made with the device current).
+int scratch_of_current_device(void** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return check_hip(e, "texture_forward_noalloc: hipGetDevice");
+    REQUIRE(dev >= 0 && dev < kMaxScratchDevices, "texture_forward_noalloc: device %d is out of "
+            "range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return check_hip(e, "texture_forward_noalloc: hipGetSymbolAddress");
+        REQUIRE(p != nullptr, "texture_forward_noalloc: no device address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return GSTEX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gstex_texture_forward_noalloc(
+    int tiles_x, int tiles_y, int block_width, int img_width, int img_height, int num_charts,
+    int num_probs, int channels, const int32_t* texture_dims, const int32_t* gaussian_ids_sorted,
+    int num_ids, const int32_t* tile_bins, const float* colors, const float* opacities,
+    const float* means, const float* scales, int num_points, float glob_scale,
+    const float* quats, const float* uv0, const float* umap, const float* vmap,
+    const float* texture, int num_texels, const float* viewmat, const float* c2w, float fx,
+    float fy, float cx, float cy, int settings, const float* background, float* out_img,
+    float* out_depth, float* out_reg, float* out_texture, float* out_normal, float* final_Ts,
+    int32_t* final_idx, int32_t* depth_idx, float* out_reg_s, void* stream) {
+    const size_t per_point = gstex::splat_records_bytes(1);
+    const size_t max_points = (GSTEX_NOALLOC_SCRATCH_BYTES - 256) / per_point;
+    REQUIRE(num_points >= 0 && (size_t)num_points <= max_points,
+            "texture_forward_noalloc: %d Gaussians need %zu bytes of scratch; its fixed scratch "
+            "of %zu bytes holds at most %zu Gaussians (use gstex_texture_forward with a "
+            "workspace)",
+            num_points, gstex::raster_forward_workspace_bytes(num_points > 0 ? num_points : 0),
+            (size_t)GSTEX_NOALLOC_SCRATCH_BYTES, max_points);
+    void* ws = nullptr;
+    const int rc = scratch_of_current_device(&ws);
+    if (rc) return rc;
+    return gstex_texture_forward(tiles_x, tiles_y, block_width, img_width, img_height, num_charts,
+                                 num_probs, channels, texture_dims, gaussian_ids_sorted, num_ids,
+                                 tile_bins, colors, opacities, means, scales, num_points,
+                                 glob_scale, quats, uv0, umap, vmap, texture, num_texels, viewmat,
+                                 c2w, fx, fy, cx, cy, settings, background, out_img, out_depth,
+                                 out_reg, out_texture, out_normal, final_Ts, final_idx, depth_idx,
+                                 out_reg_s, ws, GSTEX_NOALLOC_SCRATCH_BYTES, stream);
 }
 
 }  // extern "C"

@@ -174,6 +174,32 @@ int gstex_texture_forward(int tiles_x, int tiles_y, int block_width, int img_wid
                           int32_t *depth_idx, float *out_reg_s, void *workspace,
                           size_t workspace_bytes, void *stream);
 
+/* No counterpart in the reference.  gstex_texture_forward without a
+ * workspace argument, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the splat records live in a fixed device array of
+ * GSTEX_NOALLOC_SCRATCH_BYTES inside the library, one per device, held until
+ * the process ends.  It calls no allocating routine.  num_points above
+ * (GSTEX_NOALLOC_SCRATCH_BYTES - 256) / 156 = 645,275 is rejected with
+ * GSTEX_ERR_ARG.  Arguments, outputs and results are those of
+ * gstex_texture_forward; the device of `stream` must be current.  Every call
+ * on one device shares that device's array, so calls that overlap on one
+ * device (other streams or threads) race. */
+#define GSTEX_NOALLOC_SCRATCH_BYTES (96u << 20)
+int gstex_texture_forward_noalloc(int tiles_x, int tiles_y, int block_width, int img_width,
+                                  int img_height, int num_charts, int num_probs, int channels,
+                                  const int32_t *texture_dims, const int32_t *gaussian_ids_sorted,
+                                  int num_ids, const int32_t *tile_bins, const float *colors,
+                                  const float *opacities, const float *means, const float *scales,
+                                  int num_points, float glob_scale, const float *quats,
+                                  const float *uv0, const float *umap, const float *vmap,
+                                  const float *texture, int num_texels, const float *viewmat,
+                                  const float *c2w, float fx, float fy, float cx, float cy,
+                                  int settings, const float *background, float *out_img,
+                                  float *out_depth, float *out_reg, float *out_texture,
+                                  float *out_normal, float *final_Ts, int32_t *final_idx,
+                                  int32_t *depth_idx, float *out_reg_s, void *stream);
+
 /* Replaces texture_backward_tensor (cuda/csrc/texture.cu:903-1053).  The nine
  * gradient outputs are fully written (v_scales[:,2] = 0). */
 size_t gstex_texture_backward_workspace_size(int num_points);
