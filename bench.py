@@ -8,21 +8,33 @@ backward, gradient finalize, SH adjoint; inputs are resident in HBM before the
 timed region.  Prints one JSON line.
 
 Roofline: the slower of the two raster kernels, timed live with HIP events on
-the stream it runs on, against its algorithmic bytes per launch (``traffic`` is
-the measured HBM bytes per launch of that kernel, read from the committed
-rocprofv3 FETCH_SIZE / WRITE_SIZE summary profiles/hbm_traffic_<workload tag>.md):
-  forward   I*(4+128) + T*8 + P*(56 + 4C) + X*C*4
-  backward  I*(4+128) + T*8 + P*(60 + 4C) + 2*X*C*4 + N*128
-(I intersects, T tiles, P pixels, X texels, C channels, N Gaussians).
+the stream it runs on, against its algorithmic bytes per launch, counted per
+unit of work (I intersects, P pixels, T tiles, X texels, C channels, b bytes
+per texel value):
+  forward   I*108 + P*68 + T*8 + X*C*b      (id + 104-byte Gaussian record per
+  backward  I*108 + P*72 + T*8 + 2*X*C*b     intersect; per-pixel outputs / inputs)
+``record_model`` beside it counts this build's own 128-byte records and
+buffers.  ``step`` is the whole iteration against
+  N*(68 + 6*D*b + 24 + 88) + I*(12 + 24 + 8 + 216) + P*(68 + 72) + T*8 + 3*X*C*b.
+``traffic`` is the measured HBM bytes per launch of that kernel, read from the
+committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+profiles/hbm_traffic_<workload tag>.md.
 
 variants.bf16_batched_graph: views/s of an 8-view batch of the same scene
 (seeded cameras around the original, same resolution) with bfloat16 texels and
 SH coefficients, the whole batch captured as one graph and replayed; vs_plain
-is its views/s over the plain single-view iter/s.
+is its views/s over the plain single-view iter/s of the headline view, and
+vs_plain_same_cameras over the plain FP32 path rendering the same 8 cameras
+one view at a time.
 
-cpu_baseline: the FP32 C oracle (single thread, the reference's semantics) on a
-bounded sample of whole tiles of the same frame, forward + backward, scaled to
-the full frame by the sampled share of intersects.
+cpu_baseline: the PyTorch CPU path (oracle/torch_path.py: the same semantics,
+autograd backward) on the host's cores: the 256x256 / 1k-Gaussian workload
+whole (3 warm-up, median of 20), and the benched 1080p frame as full-frame
+preprocessing (SH, projection, binning) plus the render fwd+bwd of a random
+sample of tiles scaled by its share of intersects.  The same leg checks the
+GPU results against the float64 C oracle over sampled tiles, outside every
+timed region: image PSNR and per-gradient relative L2 (``vs_f64_oracle`` of
+the headline, bf16 batch and 3M bf16 lines).
 """
 from __future__ import annotations
 
@@ -67,10 +79,35 @@ def parse():
     return ap.parse_args()
 
 
-def raster_bytes(kind, I, T, P, X, C, N):
+def raster_bytes(kind, I, T, P, X, C, b=4):
+    """Algorithmic bytes of one raster launch, per unit of work: an id and a
+    104-byte Gaussian record per intersect, the per-pixel outputs (forward,
+    68 B) or inputs (backward, 72 B), a bin per tile, the texels read (and in
+    the backward their gradients written)."""
+    if kind == "raster_forward":
+        return I * 108 + P * 68 + T * 8 + X * C * b
+    return I * 108 + P * 72 + T * 8 + 2 * X * C * b
+
+
+def raster_bytes_record_model(kind, I, T, P, X, C, N):
+    """The same launch counted with this build's buffers: a 4-byte id and a
+    128-byte splat record per intersect, its own per-pixel buffers and the
+    128-byte gradient record per Gaussian."""
     if kind == "raster_forward":
         return I * (4 + 128) + T * 8 + P * (56 + 4 * C) + X * C * 4
     return I * (4 + 128) + T * 8 + P * (60 + 4 * C) + 2 * X * C * 4 + N * 128
+
+
+def step_bytes(N, I, P, T, X, C, D, b=4):
+    """Algorithmic bytes of one whole fwd+bwd iteration: projection in/out
+    (68), SH coefficients read and their gradient written (2 * 3 D b), colour
+    and its gradient (24), parameter gradients (88) per Gaussian; key + id
+    (12), one sort pass read + write (24), a bin edge (8) and the id + record
+    read in both passes (216) per intersect; forward outputs (68) and backward
+    inputs (72) per pixel; a bin per tile; texels read twice and their
+    gradient written once."""
+    return (N * (68 + 2 * 3 * D * b + 24 + 88) + I * (12 + 24 + 8 + 2 * 108)
+            + P * (68 + 72) + T * 8 + 3 * X * C * b)
 
 
 # file-name tag of each workload's committed rocprofv3 summaries
@@ -106,6 +143,67 @@ def measured_valu(workload: int, kernel: str):
     return None, None
 
 
+def _pick_tiles(bins, k=16, seed=0):
+    counts = bins[:, 1] - bins[:, 0]
+    nz = np.nonzero(counts > 0)[0]
+    rng = np.random.default_rng(seed)
+    return np.sort(rng.choice(nz, min(k, len(nz)), replace=False))
+
+
+def quality_job(step, target, seed=0, k_tiles=16):
+    """GPU side of the float64-oracle check of an exact-size RenderStep over
+    ``k_tiles`` random non-empty tiles: the step's image (out_img +
+    out_texture), its L2-loss cotangents, and the gradients of those tiles
+    alone (the other tiles' bins emptied before the backward).  Returns host
+    arrays; the oracle side runs in the cpu_baseline leg."""
+    import torch
+
+    step.forward()
+    step.l2_loss_grad(target)
+    bins = step.tile_bins.cpu().numpy()
+    pick = _pick_tiles(bins, k_tiles, seed)
+    keep = torch.zeros(bins.shape[0], dtype=torch.bool, device=step.dev)
+    keep[torch.from_numpy(pick).to(step.dev)] = True
+    step.tile_bins[~keep] = 0
+    step.backward()
+    torch.cuda.synchronize()
+    return dict(pick=pick, bins=bins, ids=step.ids[: step.num_intersects].cpu().numpy(),
+                img=(step.out_img + step.out_texture[..., :3]).cpu().numpy(),
+                v_img=step.v_img.cpu().numpy(), v_tex=step.v_tex.cpu().numpy(),
+                colors=step.colors.float().cpu().numpy(),
+                grads={k: v.float().cpu().numpy() for k, v in step.grads.items()
+                       if k != "v_sh_coeffs"})
+
+
+def _view_step(mv, i, **kw):
+    """An exact-size single-view RenderStep over a batch's parameter tensors
+    (same storage formats) and its view i's camera."""
+    from gstex_cuda_amd.pipeline import RenderStep
+
+    v = mv.views[i]
+    return RenderStep(mv.params, v.cam, block_width=v.bw, settings=v.settings,
+                      background=v.background, sh_degree=v.sh_degree, **kw)
+
+
+def plain_same_cameras(scene, views, dev, steps, warmup):
+    """Views/s of the plain FP32 single-view path (exact-size steps, one view
+    at a time) over the given cameras."""
+    import torch
+
+    from gstex_cuda_amd import synthetic
+    from gstex_cuda_amd.pipeline import scene_to_step
+
+    total = 0.0
+    for v in views:
+        vs = synthetic.view_scene(scene, v)
+        st = scene_to_step(vs, dev)
+        tgt = torch.from_numpy(vs.target).to(dev)
+        total += _timed(lambda: st.step(tgt), steps, warmup) / steps
+        del st
+    torch.cuda.empty_cache()
+    return len(views) / total
+
+
 def batched_variant(scene, dev, num_views, steps, warmup):
     """Views/s of a multi-view batch of the same scene (seeded cameras around the
     original one, same resolution), bfloat16 texels and SH coefficients, the
@@ -133,9 +231,16 @@ def batched_variant(scene, dev, num_views, steps, warmup):
     out = {"views": num_views, "views_per_s": round(num_views * steps / elapsed, 3),
            "ms_per_batch": round(ms, 4), "dtype": "bf16 storage, f32 compute",
            "graph": True, "capacity_ok": bool(ok)}
-    del mv
+    # precision check of view 0 (same storage formats, kernels and camera)
+    vstep = _view_step(mv, 0)
+    job = quality_job(vstep, mv.targets[0], seed=1)
+    job.update(view=views[0], bf16=True)
+    del vstep, mv
     torch.cuda.empty_cache()
-    return out
+    plain = plain_same_cameras(scene, views, dev, steps, warmup)
+    out["plain_same_cameras_views_per_s"] = round(plain, 3)
+    out["vs_plain_same_cameras"] = round(out["views_per_s"] / plain, 3)
+    return out, job
 
 
 def train_variant(scene, dev, steps, warmup):
@@ -222,12 +327,20 @@ def multiview_variant(dev, workload, steps, warmup):
     scene = synthetic.make_scene(workload)
     views = synthetic.workload_views(workload, scene)
     prev = _C.set_precise_edge_on(bf16)
+    job = None
     try:
         mv = scene_to_multiview(scene, views, dev, bf16=bf16)
         mv.capture()
         elapsed = _timed(mv.replay, steps, warmup)
         ok = mv.capacity_ok()
         isects = [int(v.num_intersects) for v in mv.views]
+        if bf16:
+            # precision check of view 0 (same storage formats, kernels, camera
+            # and float64 edge-on solve)
+            vstep = _view_step(mv, 0)
+            job = quality_job(vstep, mv.targets[0], seed=5)
+            job.update(view=views[0], bf16=True, scene=scene)
+            del vstep
     finally:
         _C.set_precise_edge_on(prev)
     res = sorted({f"{v.img_w}x{v.img_h}" for v in views})
@@ -241,7 +354,7 @@ def multiview_variant(dev, workload, steps, warmup):
            "graph": True, "capacity_ok": bool(ok), "steps": steps}
     del mv
     torch.cuda.empty_cache()
-    return out
+    return out, job
 
 
 def edit_sample_variant(scene, step, dev, reps=5):
@@ -310,65 +423,158 @@ def edit_sample_variant(scene, step, dev, reps=5):
     return out
 
 
-def cpu_baseline(scene, step, target_seconds):
-    """Time the FP32 oracle on a sample of tiles and scale to one full frame."""
-    import oracle as O
+def _oracle_quality(O, scene, job):
+    """The float64 C oracle against a quality job: image PSNR over the sampled
+    tiles and the relative L2 error of every gradient of those tiles, on the
+    stored values (bfloat16-rounded texels and SH coefficients for a bfloat16
+    job) and, for a bfloat16 job, on the original float32 values as well."""
+    import torch
 
-    O.build()
-    ids = step.ids[: step.num_intersects].cpu().numpy()
-    bins = step.tile_bins.cpu().numpy()
-    colors = step.colors.cpu().numpy()
-    counts = (bins[:, 1] - bins[:, 0]).astype(np.int64)
+    from gstex_cuda_amd import synthetic
+
+    vs = synthetic.view_scene(scene, job["view"]) if job.get("view") is not None else scene
+    H, W, bw = vs.img_h, vs.img_w, vs.block_width
+    tx = (W + bw - 1) // bw
+    pick = job["pick"]
+    mask = np.zeros(job["bins"].shape[0], np.uint8)
+    mask[pick] = 1
+    sel = np.zeros((H, W), bool)
+    for t in pick:
+        ty, t0 = divmod(int(t), tx)
+        sel[ty * bw:(ty + 1) * bw, t0 * bw:(t0 + 1) * bw] = True
+
+    def rounded(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(torch.bfloat16).float().numpy()
+
+    bf = bool(job.get("bf16"))
+    sh = vs.sh_coeffs
+    inputs = [("stored", rounded(vs.texture) if bf else vs.texture,
+               None if sh is None else (rounded(sh) if bf else sh))]
+    if bf:
+        inputs.append(("float32_inputs", vs.texture, sh))
+    z1 = np.zeros((H, W), np.float32)
+    z3 = np.zeros((H, W, 3), np.float32)
+    out = {"view": 0, "tiles": int(len(pick)), "pixels": int(sel.sum())}
+    for name, tex, coeffs in inputs:
+        if coeffs is not None:
+            cols = O.sh_forward(vs.sh_degree, vs.sh_degree, vs.means - vs.c2w[:3, 3], coeffs,
+                                "f64")
+        else:
+            cols = vs.colors
+        sc = O.Scene(means=vs.means, scales=vs.scales, quats=vs.quats, colors=cols,
+                     opacities=vs.opacities, uv0=vs.uv0, umap=vs.umap, vmap=vs.vmap, texture=tex,
+                     texture_dims=vs.texture_dims, viewmat=vs.viewmat, c2w=vs.c2w,
+                     background=vs.background, fx=vs.fx, fy=vs.fy, cx=vs.cx, cy=vs.cy, img_h=H,
+                     img_w=W, block_width=bw, settings=vs.settings, ids_sorted=job["ids"],
+                     tile_bins=job["bins"], tile_mask=mask)
+        f = O.texture_forward(sc, "f64")
+        ref = f["out_img"] + f["out_texture"][..., :3]
+        img = job["img"].astype(np.float64)
+        mse = float(np.mean((img[sel] - ref[sel]) ** 2))
+        peak = max(float(np.abs(ref[sel]).max()), 1e-12)
+        out[f"image_psnr_db_{name}"] = (round(10.0 * np.log10(peak * peak / mse), 2)
+                                        if mse > 0 else float("inf"))
+        g = O.texture_backward(sc, f, job["v_img"], z1, z1, z1, job["v_tex"], z3, "f64")
+        rel = {}
+        for k, ref_g in g.items():
+            got = job["grads"][k].astype(np.float64).ravel()
+            r = ref_g.astype(np.float64).ravel()
+            rel[k] = float(np.linalg.norm(got - r) / max(np.linalg.norm(r), 1e-30))
+        out[f"grad_rel_l2_{name}"] = {k: float(f"{v:.3e}") for k, v in rel.items()}
+        out[f"grad_rel_l2_max_{name}"] = float(f"{max(rel.values()):.3e}")
+    return out
+
+
+def cpu_threads():
+    """Threads for the CPU path: the host's CPUs available to this process,
+    capped by OMP_NUM_THREADS when it is set (the share of the host a job on a
+    shared GPU box is given)."""
+    import os as _os
+
+    n = _os.cpu_count() or 1
+    try:
+        n = min(n, len(_os.sched_getaffinity(0)))
+    except AttributeError:
+        pass
+    cap = _os.environ.get("OMP_NUM_THREADS", "")
+    if cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(n, 1)
+
+
+def cpu_baseline(scene, target_seconds, jobs):
+    """The PyTorch CPU path timed on the host cores (the 256x256 / 1k workload
+    whole, and the benched frame by a tile sample), then the float64 oracle
+    checks of the GPU quality jobs.  Returns (cpu_baseline, {job: check})."""
+    import torch
+
+    import oracle as O
+    from gstex_cuda_amd import synthetic
+    from oracle import torch_path as TP
+
+    threads = cpu_threads()
+    torch.set_num_threads(threads)
+    host = TP.host_cpu()
+    budget = 250_000  # pairs per chunk
+
+    # 256x256, 1k Gaussians, 4x4 RGB texels: 3 warm-up, median of 20
+    s1 = synthetic.make_scene(1)
+    p1, cam1, tgt1 = TP.leaf_params(s1), TP.camera_of(s1), torch.tensor(s1.target)
+    times = []
+    for _ in range(23):
+        t0 = time.perf_counter()
+        TP.train_iteration(p1, cam1, tgt1, s1.sh_degree, s1.block_width, s1.settings,
+                           pair_budget=budget)
+        times.append(time.perf_counter() - t0)
+    c1_ms = 1e3 * float(np.median(times[3:]))
+
+    # the benched frame: full-frame SH, projection and binning, then the render
+    # fwd+bwd of a growing random sample of tiles, scaled by intersect share
+    p2, cam2, tgt2 = TP.leaf_params(scene), TP.camera_of(scene), torch.tensor(scene.target)
+    t0 = time.perf_counter()
+    q, ids, bins = TP.preprocess(p2, cam2, scene.sh_degree, scene.block_width)
+    t_pre = time.perf_counter() - t0
+    col = q.get("colors")
+    if "sh_coeffs" in p2:
+        q = dict(q)
+        q["colors"] = col.detach().requires_grad_(True)
+    counts = (bins[:, 1] - bins[:, 0]).numpy().astype(np.int64)
     total = int(counts.sum())
-    rng = np.random.default_rng(0)
-    order = rng.permutation(len(counts))
-    H, W, C = scene.img_h, scene.img_w, scene.texture.shape[1]
-    sc = O.Scene(means=scene.means, scales=scene.scales, quats=scene.quats, colors=colors,
-                 opacities=scene.opacities, uv0=scene.uv0, umap=scene.umap, vmap=scene.vmap,
-                 texture=scene.texture, texture_dims=scene.texture_dims, viewmat=scene.viewmat,
-                 c2w=scene.c2w, background=scene.background, fx=scene.fx, fy=scene.fy,
-                 cx=scene.cx, cy=scene.cy, img_h=H, img_w=W, block_width=scene.block_width,
-                 settings=scene.settings, ids_sorted=ids, tile_bins=bins)
-    v = np.zeros((H, W, 3), np.float32)
-    vt = np.zeros((H, W, C), np.float32)
-    z = np.zeros((H, W), np.float32)
-    # grow the sample until the measured time reaches the target
-    share, chosen, elapsed = 0.002, 0, 0.0
+    order = np.random.default_rng(0).permutation(len(counts))
+    share, k, chosen, t_render = 0.002, 1, 0, 0.0
     while True:
         k = max(1, int(len(order) * share))
-        mask = np.zeros(len(counts), np.uint8)
-        mask[order[:k]] = 1
-        sc.tile_mask = mask
         t0 = time.perf_counter()
-        f = O.texture_forward(sc, "f32")
-        O.texture_backward(sc, f, v, z, z, z, vt, v, "f32")
-        elapsed = time.perf_counter() - t0
+        TP.render_backward(q, cam2, ids, bins, scene.block_width, scene.settings, target=tgt2,
+                           tiles=torch.from_numpy(order[:k]), pair_budget=budget)
+        t_render = time.perf_counter() - t0
         chosen = int(counts[order[:k]].sum())
-        if elapsed >= 0.3 * target_seconds or share >= 1.0 or chosen == total:
+        if t_render >= 0.3 * target_seconds or share >= 1.0 or chosen == total:
             break
-        share = min(1.0, share * max(2.0, 0.5 * target_seconds / max(elapsed, 1e-3)))
-    frame_s = elapsed * total / max(chosen, 1)
-    # the timed run's own image, checked against the float64 oracle on 16 of
-    # the sampled tiles (outside the timed region)
-    chk = np.zeros(len(counts), np.uint8)
-    chk[order[:16]] = 1
-    sc.tile_mask = chk
-    f64 = O.texture_forward(sc, "f64")
-    bw, tx = scene.block_width, (W + scene.block_width - 1) // scene.block_width
-    sel = np.zeros((H, W), bool)
-    for t in order[:16]:
-        ty, tx0 = divmod(int(t), tx)
-        sel[ty * bw:(ty + 1) * bw, tx0 * bw:(tx0 + 1) * bw] = True
-    img = (step.out_img + step.out_texture[..., :3]).cpu().numpy().astype(np.float64)
-    ref = f64["out_img"] + f64["out_texture"][..., :3]
-    mse = float(np.mean((img[sel] - ref[sel]) ** 2))
-    peak = max(float(np.abs(ref[sel]).max()), 1e-12)
-    psnr = float("inf") if mse == 0 else 10.0 * np.log10(peak * peak / mse)
-    return {"value": 1.0 / frame_s, "unit": "iter/s", "cores": 1, "kind": "port",
-            "sample": f"{k} of {len(counts)} tiles ({chosen} of {total} intersects), "
-                      f"FP32 oracle fwd+bwd {elapsed:.1f} s, scaled by intersect share",
-            "image_psnr_db_vs_f64_oracle": round(psnr, 2),
-            "psnr_sample": f"{int(sel.sum())} pixels of 16 sampled tiles, out_img + out_texture"}
+        share = min(1.0, share * max(2.0, 0.5 * target_seconds / max(t_render, 1e-3)))
+    t_sh = 0.0
+    if "sh_coeffs" in p2 and q["colors"].grad is not None:
+        t0 = time.perf_counter()
+        col.backward(q["colors"].grad)
+        t_sh = time.perf_counter() - t0
+    frame_s = t_pre + t_sh + t_render * total / max(chosen, 1)
+    cpu = {"value": 1.0 / frame_s, "unit": "iter/s", "cores": threads, "kind": "port",
+           "sample": f"PyTorch CPU path (autograd), {threads} threads: full-frame SH + "
+                     f"projection + binning {t_pre:.2f} s, render fwd+bwd of {k} of "
+                     f"{len(counts)} tiles ({chosen} of {total} intersects) {t_render:.1f} s, "
+                     f"scaled by intersect share",
+           "os_cpu_count": host["os_cpu_count"], "affinity_cpus": host["affinity_cpus"],
+           "cpu_model": host["cpu_model"],
+           "c1_256px_1k_fwd_bwd_median_ms": round(c1_ms, 2),
+           "c1_256px_1k_iter_per_s": round(1e3 / c1_ms, 3),
+           "c1_timing": "3 warm-up, median of 20"}
+    del p2, q, ids, bins
+    checks = {}
+    if jobs:
+        O.build()
+        for name, job in jobs.items():
+            checks[name] = _oracle_quality(O, job.get("scene", scene), job)
+    return cpu, checks
 
 
 def main():
@@ -411,8 +617,12 @@ def main():
     X, C = scene.texture.shape
     kern = {k: timer.mean_ms(k) for k in ("raster_forward", "raster_backward")}
     dom = max(kern, key=kern.get)
-    nbytes = raster_bytes(dom, I, T, P, X, C, N)
+    nbytes = raster_bytes(dom, I, T, P, X, C)
     achieved = nbytes / (kern[dom] * 1e-3) / 1e9
+    rec_bytes = raster_bytes_record_model(dom, I, T, P, X, C, N)
+    D = (scene.sh_degree + 1) ** 2 if scene.sh_degree >= 0 else 0
+    sbytes = step_bytes(N, I, P, T, X, C, D)
+    step_gbs = sbytes / (elapsed / args.steps) / 1e9
     traffic, traffic_src = (measured_traffic(args.workload, dom) if args.num_points is None
                             else (None, None))
     util, share = (measured_valu(args.workload, dom) if args.num_points is None
@@ -420,12 +630,20 @@ def main():
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": traffic, "traffic_source": traffic_src, "algorithmic_bytes": int(nbytes),
+            "bytes_per_unit": ("I*108 + P*72 + T*8 + 2*X*C*4" if dom == "raster_backward"
+                               else "I*108 + P*68 + T*8 + X*C*4"),
+            "record_model": {"algorithmic_bytes": int(rec_bytes),
+                             "achieved": round(rec_bytes / (kern[dom] * 1e-3) / 1e9, 2),
+                             "frac": round(rec_bytes / (kern[dom] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                           5)},
+            "step": {"algorithmic_bytes": int(sbytes), "achieved": round(step_gbs, 2),
+                     "frac": round(step_gbs / HBM_PEAK_GBS, 5)},
             "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
             "valu_issue_share": share, "valu_lane_utilisation": util}
-    variants = {}
+    variants, jobs = {}, {}
     if args.batched_views > 0:
-        b = batched_variant(scene, dev, args.batched_views, max(args.steps // 4, 3),
-                            max(args.warmup // 2, 1))
+        b, jobs["bf16_batched_graph"] = batched_variant(
+            scene, dev, args.batched_views, max(args.steps // 4, 3), max(args.warmup // 2, 1))
         b["vs_plain"] = round(b["views_per_s"] / value, 3)
         variants["bf16_batched_graph"] = b
     if args.train_steps > 0:
@@ -434,14 +652,26 @@ def main():
         variants["train_step_adam"] = t
     if args.edit_sample:
         variants["texture_edit_and_sample"] = edit_sample_variant(scene, step, dev)
+    # precision check of the benched view (after every use of its lists)
+    jobs["headline"] = quality_job(step, target, seed=0)
+    del step
+    torch.cuda.empty_cache()
     if args.config_steps > 0:
         w = max(args.warmup // 2, 1)
         variants["c3_regularised_1600x1200_1m"] = regularised_variant(dev, args.config_steps, w)
-        variants["c4_orbit_16_views_1m_graph"] = multiview_variant(dev, 4, args.config_steps, w)
-        variants["c5_bf16_8x1080p_3m_graph"] = multiview_variant(dev, 5, args.config_steps, w)
-    cpu = None
+        variants["c4_orbit_16_views_1m_graph"], _ = multiview_variant(dev, 4, args.config_steps,
+                                                                      w)
+        c5, c5_job = multiview_variant(dev, 5, args.config_steps, w)
+        variants["c5_bf16_8x1080p_3m_graph"] = c5
+        if c5_job is not None:
+            jobs["c5"] = c5_job
+    cpu, quality = None, {}
     if args.cpu_baseline_seconds > 0:
-        cpu = cpu_baseline(scene, step, args.cpu_baseline_seconds)
+        cpu, quality = cpu_baseline(scene, args.cpu_baseline_seconds, jobs)
+    if "bf16_batched_graph" in quality:
+        variants["bf16_batched_graph"]["vs_f64_oracle"] = quality["bf16_batched_graph"]
+    if "c5" in quality:
+        variants["c5_bf16_8x1080p_3m_graph"]["vs_f64_oracle"] = quality["c5"]
     line = {"metric": METRIC, "value": round(value, 4), "unit": "iter/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": round(1e3 * elapsed / args.steps, 4), "higher_is_better": True,
@@ -451,7 +681,8 @@ def main():
                                    f"FP32 plain path",
                        "num_gaussians": N, "resolution": f"{scene.img_w}x{scene.img_h}",
                        "texels": X, "channels": C, "intersects": I, "block_width": 16},
-            "roofline": roof, "cpu_baseline": cpu, "variants": variants}
+            "roofline": roof, "cpu_baseline": cpu, "vs_f64_oracle": quality.get("headline"),
+            "variants": variants}
     print(json.dumps(line), flush=True)
 
 

@@ -89,9 +89,27 @@ _SIGNATURES = {
                                P, P, P, I, F, P, P, P, P,  # opac, means, scales, N, g, quats, uv
                                P, P, F, F, F, F, I, P,  # viewmat, c2w, intrinsics, settings, bg
                                P, P, Z, P]),  # out, workspace, bytes, stream
+    "gstex_views_table_size": (Z, [I, I]),
+    "gstex_views_setup": (I, [I, P, I, I, I, I, P, I, I, P, P, P, Z, P]),
+    "gstex_views_release": (I, [P]),
+    "gstex_views_raster_forward": (I, [P, P]),
+    "gstex_views_raster_backward": (I, [P, P]),
     "gstex_texture_sample_forward": (I, [I, I, I, P, P, I, P, I, P, P]),
     "gstex_texture_sample_backward": (I, [I, I, I, P, P, I, P, I, P, P, P]),
 }
+
+
+class View(C.Structure):
+    """``gstex_view`` of include/gstex_hip.h: one view of a multi-view raster pass."""
+
+    _fields_ = [("img_width", I), ("img_height", I), ("fx", F), ("fy", F), ("cx", F),
+                ("cy", F), ("viewmat", P), ("c2w", P), ("records", P),
+                ("gaussian_ids_sorted", P), ("num_ids", I), ("tile_bins", P),
+                ("wave_masks", P), ("wave_masks_bytes", Z),
+                ("out_img", P), ("out_depth", P), ("out_reg", P), ("out_texture", P),
+                ("out_normal", P), ("final_Ts", P), ("final_idx", P), ("depth_idx", P),
+                ("out_reg_s", P), ("v_img", P), ("v_depth", P), ("v_reg", P), ("v_alpha", P),
+                ("v_texture_out", P), ("v_normal", P), ("grad_records", P)]
 
 
 class GstexError(RuntimeError):

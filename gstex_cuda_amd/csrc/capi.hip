@@ -4,10 +4,13 @@
 #include <stdio.h>
 
 #include <mutex>
+#include <unordered_map>
+#include <vector>
 
 #include "../../include/gstex_hip.h"
 #include "gstex_common.hpp"
 #include "gstex_kernels.hpp"
+#include "raster_common.hpp"
 
 namespace {
 
@@ -734,6 +737,170 @@ int gstex_adam_step(size_t n, float* param, const float* grad, float* exp_avg, f
                                       step, grad_mode, values,
                                       values_format == GSTEX_TEXTURE_BF16, as_stream(stream)),
                      "adam_step");
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- multi-view raster pass
+namespace {
+
+struct ViewsHost {
+    int num_views, num_items, channels, settings, bf16;
+};
+std::mutex g_views_mu;
+std::unordered_map<const void*, ViewsHost> g_views;
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+size_t views_hdr_bytes() { return align256(sizeof(gstex::ViewsHeader)); }
+size_t views_table_bytes(int v) { return align256((size_t)v * sizeof(gstex::ViewRaster)); }
+
+int persistent_blocks() {
+    static int blocks[256] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 256) return 2048;
+    if (blocks[dev] == 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+        // more workgroups than can be resident: the extra ones start once
+        // others leave and find the queue drained
+        blocks[dev] = cus * 8;
+    }
+    return blocks[dev];
+}
+
+int views_lookup(const void* table, ViewsHost& h) {
+    std::lock_guard<std::mutex> lock(g_views_mu);
+    auto it = g_views.find(table);
+    REQUIRE(it != g_views.end(), "views table %p was not set up by gstex_views_setup", table);
+    h = it->second;
+    return GSTEX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t gstex_views_table_size(int num_views, int total_tiles) {
+    if (num_views <= 0 || total_tiles < 0) return 0;
+    return views_hdr_bytes() + views_table_bytes(num_views) + align256((size_t)total_tiles * 4);
+}
+
+int gstex_views_setup(int num_views, const gstex_view* views, int block_width, int channels,
+                      int settings, int num_points, const void* texture, int num_texels,
+                      int texture_format, const float* background, float* v_texture, void* table,
+                      size_t table_bytes, void* stream) {
+    REQUIRE(num_views >= 1 && num_views < 2048, "num_views must be in [1, 2047], got %d",
+            num_views);
+    REQUIRE(views && table && background, "null pointer argument");
+    REQUIRE(texture_format == GSTEX_TEXTURE_F32 || texture_format == GSTEX_TEXTURE_BF16,
+            "texture_format must be GSTEX_TEXTURE_F32 or GSTEX_TEXTURE_BF16");
+    REQUIRE(channels == 0 || texture, "null texture");
+    REQUIRE(channels == 0 || v_texture, "null v_texture");
+    REQUIRE(((uintptr_t)table % 256) == 0, "the views table must be 256-byte aligned");
+    std::vector<gstex::ViewRaster> host((size_t)num_views);
+    long long total = 0;
+    for (int v = 0; v < num_views; ++v) {
+        const gstex_view& w = views[v];
+        const int tx = tiles_of(w.img_width > 0 ? w.img_width : 1, block_width > 0 ? block_width : 1);
+        const int ty = tiles_of(w.img_height > 0 ? w.img_height : 1, block_width > 0 ? block_width : 1);
+        int rc = check_raster_common(tx, ty, block_width, w.img_width, w.img_height, 1, channels,
+                                     num_points, w.num_ids, num_texels, settings);
+        if (rc) {
+            char msg[sizeof(g_err)];
+            snprintf(msg, sizeof(msg), "%s", g_err);
+            return fail(rc, "view %d: %s", v, msg);
+        }
+        REQUIRE((long long)tx * ty < (1ll << gstex::kViewItemTileBits),
+                "view %d: %d x %d tiles exceed the work-item packing", v, tx, ty);
+        REQUIRE(w.viewmat && w.c2w && w.tile_bins && w.out_img && w.out_depth && w.out_reg &&
+                    w.out_normal && w.final_Ts && w.final_idx && w.depth_idx && w.out_reg_s &&
+                    (channels == 0 || w.out_texture),
+                "view %d: null pointer argument", v);
+        REQUIRE(w.num_ids == 0 || (w.gaussian_ids_sorted && num_points > 0),
+                "view %d: null gaussian_ids_sorted", v);
+        REQUIRE(num_points == 0 || w.records, "view %d: null records", v);
+        REQUIRE(w.v_img && w.v_depth && w.v_reg && w.v_alpha && w.v_normal && w.grad_records &&
+                    (channels == 0 || w.v_texture_out),
+                "view %d: null backward input", v);
+        const size_t mask_need = (size_t)gstex::kMaxTileWaves *
+                                 gstex::raster_wave_mask_words(w.num_ids, tx * ty) * 8;
+        REQUIRE(!w.wave_masks || w.wave_masks_bytes >= mask_need,
+                "view %d: wave masks of %zu bytes, need %zu", v, w.wave_masks_bytes, mask_need);
+        gstex::RasterCall c{w.img_width, w.img_height, block_width, channels, settings,
+                            num_points, w.num_ids, num_texels, 1.f, w.fx, w.fy, w.cx, w.cy,
+                            nullptr, w.gaussian_ids_sorted, w.tile_bins, nullptr, nullptr,
+                            nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, texture,
+                            w.viewmat, w.c2w, background,
+                            static_cast<uint64_t*>(w.wave_masks),
+                            texture_format == GSTEX_TEXTURE_BF16};
+        gstex::ViewRaster& r = host[(size_t)v];
+        r.a = gstex::make_raster_args(c, static_cast<const float*>(w.records));
+        r.o = gstex::FwdOut{w.out_img, w.out_depth, w.out_reg, w.out_texture, w.out_normal,
+                            w.final_Ts, w.final_idx, w.depth_idx, w.out_reg_s};
+        r.b = gstex::BwdIn{w.final_Ts, w.final_idx, w.depth_idx, w.out_reg_s, w.v_img,
+                           w.v_depth, w.v_reg, w.v_alpha, w.v_texture_out, w.v_normal,
+                           w.grad_records, v_texture};
+        total += (long long)tx * ty;
+    }
+    REQUIRE(total < (1ll << 30), "too many tiles over all views: %lld", total);
+    const size_t need = gstex_views_table_size(num_views, (int)total);
+    REQUIRE(table_bytes >= need, "views table too small: %zu < %zu", table_bytes, need);
+    gstex::ViewsHeader hdr{num_views, (int)total, gstex::raster_block_threads(block_width) / 64,
+                           0};
+    char* base = static_cast<char*>(table);
+    hipStream_t s = as_stream(stream);
+    hipError_t e = hipMemcpyAsync(base, &hdr, sizeof(hdr), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(base + views_hdr_bytes(), host.data(),
+                           host.size() * sizeof(gstex::ViewRaster), hipMemcpyHostToDevice, s);
+    // the host copies go out of scope on return
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    int rc = check_hip(e, "views_setup");
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g_views_mu);
+    g_views[table] = ViewsHost{num_views, (int)total, channels, settings,
+                               texture_format == GSTEX_TEXTURE_BF16 ? 1 : 0};
+    return GSTEX_OK;
+}
+
+int gstex_views_release(void* table) {
+    std::lock_guard<std::mutex> lock(g_views_mu);
+    g_views.erase(table);
+    return GSTEX_OK;
+}
+
+int gstex_views_raster_forward(void* table, void* stream) {
+    ViewsHost h;
+    int rc = views_lookup(table, h);
+    if (rc) return rc;
+    char* base = static_cast<char*>(table);
+    auto* hdr = reinterpret_cast<gstex::ViewsHeader*>(base);
+    auto* views = reinterpret_cast<const gstex::ViewRaster*>(base + views_hdr_bytes());
+    auto* items = reinterpret_cast<int*>(base + views_hdr_bytes() + views_table_bytes(h.num_views));
+    hipStream_t s = as_stream(stream);
+    rc = check_hip(gstex::views_order(views, hdr, items, s), "views_raster_forward: order");
+    if (rc) return rc;
+    return check_hip(gstex::raster_forward_views(views, hdr, items, h.channels, h.settings,
+                                                 h.bf16 != 0, persistent_blocks(), s),
+                     "views_raster_forward");
+}
+
+int gstex_views_raster_backward(void* table, void* stream) {
+    ViewsHost h;
+    int rc = views_lookup(table, h);
+    if (rc) return rc;
+    char* base = static_cast<char*>(table);
+    auto* hdr = reinterpret_cast<gstex::ViewsHeader*>(base);
+    auto* views = reinterpret_cast<const gstex::ViewRaster*>(base + views_hdr_bytes());
+    auto* items = reinterpret_cast<int*>(base + views_hdr_bytes() + views_table_bytes(h.num_views));
+    hipStream_t s = as_stream(stream);
+    rc = check_hip(gstex::fill_i32(&hdr->head, 1, 0, s), "views_raster_backward: queue reset");
+    if (rc) return rc;
+    return check_hip(gstex::raster_backward_views(views, hdr, items, h.channels, h.settings,
+                                                  h.bf16 != 0, persistent_blocks(), s),
+                     "views_raster_backward");
 }
 
 }  // extern "C"

@@ -204,6 +204,17 @@ hipError_t adam_step(size_t n, float* param, const float* grad, float* exp_avg, 
                      double lr, double beta1, double beta2, double eps, int step, int grad_mode,
                      void* values, int values_bf16, hipStream_t s);
 
+// multi-view raster pass (the table types are in raster_common.hpp)
+struct ViewRaster;
+struct ViewsHeader;
+hipError_t views_order(const ViewRaster* views, ViewsHeader* hdr, int* items, hipStream_t s);
+hipError_t raster_forward_views(const ViewRaster* views, ViewsHeader* hdr, const int* items,
+                                int channels, int settings, bool bf16, int blocks,
+                                hipStream_t s);
+hipError_t raster_backward_views(const ViewRaster* views, ViewsHeader* hdr, const int* items,
+                                 int channels, int settings, bool bf16, int blocks,
+                                 hipStream_t s);
+
 // texture edit (texture_edit.cu semantics; out is X x C with C >= 5, zeroed here)
 size_t texture_edit_workspace_bytes(int n);
 hipError_t texture_edit(const RasterCall& c, const float* upd_img, const float* upd_alpha,

@@ -20,14 +20,18 @@ namespace gstex {
 // MODE 2 is MODE 0 plus the float64 edge-on solve.  TB selects bfloat16
 // texels.  CM <= 4 is the exact channel count; above 4, CM bounds a runtime
 // count.
+//
+// raster_forward_wave runs one wave's share of a tile: wave `wave` of the
+// tile's raster_block_threads(bw) / 64 waves (an 8x8 quadrant for bw = 16).
 template <int CM, int MODE, bool TB>
-__global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) {
+__device__ __forceinline__ void raster_forward_wave(const RasterArgs& a, const FwdOut& o,
+                                                    const int tile, const int wave) {
     constexpr bool GENERAL = MODE == 1, PRECISE = MODE != 0;
-    const int tile = blockIdx.x;
     const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
+    const int lane = threadIdx.x & (kWave - 1);
     int lx, ly;
     bool in_tile;
-    lane_pixel(a.bw, threadIdx.x, lx, ly, in_tile);
+    lane_pixel(a.bw, wave * kWave + lane, lx, ly, in_tile);
     const int j = tx * a.bw + lx, i = ty * a.bw + ly;
     const bool inside = in_tile && i < a.img_h && j < a.img_w;
     const PixelRay pr = pixel_ray(a.c2w, a.viewmat, a.fx, a.fy, a.cx, a.cy, j, i);
@@ -60,9 +64,8 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
     // box meets this wave's pixels, then the candidates front to back.  The
     // composited entries of each window form one word of the wave masks
     // (every window the walk reaches is written, zeros included).
-    const int lane = threadIdx.x & (kWave - 1);
     uint64_t* const mplane =
-        a.wave_masks ? a.wave_masks + (size_t)wave_index() * a.mask_plane : nullptr;
+        a.wave_masks ? a.wave_masks + (size_t)wave * a.mask_plane : nullptr;
     const int mword0 = (r0 >> 6) + tile, mentry0 = r0 & ~63;
     const WaveRect rect = wave_rect(inside, j, i);
 
@@ -242,6 +245,90 @@ __global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) 
         if (c < C) o.out_texture[(size_t)pid * C + c] = tex[c];
 }
 
+template <int CM, int MODE, bool TB>
+__global__ __launch_bounds__(256) void k_raster_forward(RasterArgs a, FwdOut o) {
+    raster_forward_wave<CM, MODE, TB>(a, o, blockIdx.x, wave_index());
+}
+
+// ------------------------------------------------------------------ multi-view queue
+// Orders the (view, tile) items of a multi-view table longest list first (64
+// length classes, two per octave; order inside a class is arbitrary): a
+// counting sort in one workgroup.
+__global__ __launch_bounds__(1024) void k_views_order(const ViewRaster* __restrict__ views,
+                                                      ViewsHeader* __restrict__ hdr,
+                                                      int* __restrict__ items) {
+    __shared__ int hist[64];
+    const int tid = threadIdx.x;
+    if (tid < 64) hist[tid] = 0;
+    __syncthreads();
+    const int nv = hdr->num_views;
+    auto cls = [&](const RasterArgs& a, int t) {
+        int r0, r1;
+        tile_range(a, t, r0, r1);
+        const unsigned n = (unsigned)(r1 - r0) + 1u;
+        const int lg = 31 - __builtin_clz(n);
+        const int half = lg > 0 ? (int)((n >> (lg - 1)) & 1u) : 0;
+        return 63 - (2 * lg + half);  // descending length
+    };
+    for (int v = 0; v < nv; ++v) {
+        const RasterArgs& a = views[v].a;
+        const int nt = a.tiles_x * a.tiles_y;
+        for (int t = tid; t < nt; t += blockDim.x) atomicAdd(&hist[cls(a, t)], 1);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int k = 0; k < 64; ++k) {
+            const int c = hist[k];
+            hist[k] = run;
+            run += c;
+        }
+        hdr->head = 0;
+    }
+    __syncthreads();
+    for (int v = 0; v < nv; ++v) {
+        const RasterArgs& a = views[v].a;
+        const int nt = a.tiles_x * a.tiles_y;
+        for (int t = tid; t < nt; t += blockDim.x) {
+            const int pos = atomicAdd(&hist[cls(a, t)], 1);
+            items[pos] = (v << kViewItemTileBits) | t;
+        }
+    }
+}
+
+// Takes the next work index of a queue pass (wave-uniform): lane 0 advances
+// the header's head.
+__device__ __forceinline__ int queue_next(ViewsHeader* hdr) {
+    int k = 0;
+    if ((threadIdx.x & (kWave - 1)) == 0) k = atomicAdd(&hdr->head, 1);
+    return __builtin_amdgcn_readfirstlane(__shfl(k, 0));
+}
+
+// Persistent forward over every view of a table: each wave takes (view,
+// tile, wave-in-tile) work from one queue ordered longest list first, so the
+// longest tiles of all views start first and the rest fill the machine
+// around them.  Every wave leaves once the queue is drained.
+template <int CM, int MODE, bool TB>
+__global__ __launch_bounds__(256) void k_raster_forward_queue(const ViewRaster* __restrict__ views,
+                                                              ViewsHeader* __restrict__ hdr,
+                                                              const int* __restrict__ items) {
+    const int wpt = hdr->waves_per_tile, n_work = hdr->num_items * wpt;
+    const int nv = hdr->num_views;
+    for (;;) {
+        const int k = queue_next(hdr);
+        if (k >= n_work) break;
+        const int it = k / wpt, wave = k - it * wpt;
+        const int item = __builtin_amdgcn_readfirstlane(as_const_space(items)[it]);
+        const int v = item >> kViewItemTileBits, tile = item & ((1 << kViewItemTileBits) - 1);
+        if (v >= nv) continue;
+        const GSTEX_CONST ViewRaster* vr = as_const_space(views) + v;
+        const RasterArgs a = load_const(&vr->a);
+        const FwdOut o = load_const(&vr->o);
+        if (tile >= a.tiles_x * a.tiles_y) continue;
+        raster_forward_wave<CM, MODE, TB>(a, o, tile, wave);
+    }
+}
+
 // ------------------------------------------------------------------ launchers
 template <int CM>
 static hipError_t launch_fwd_cm(const RasterArgs& a, const FwdOut& o, hipStream_t s) {
@@ -260,6 +347,42 @@ static hipError_t launch_fwd_cm(const RasterArgs& a, const FwdOut& o, hipStream_
         else k_raster_forward<CM, 0, false><<<grid, block, 0, s>>>(a, o);
     }
     return hipGetLastError();
+}
+
+template <int CM>
+static hipError_t launch_fwd_queue_cm(const ViewRaster* views, ViewsHeader* hdr, const int* items,
+                                      int mode, bool bf16, int blocks, hipStream_t s) {
+    const dim3 grid(blocks), block(256);
+    if (bf16) {
+        if (mode == 1) k_raster_forward_queue<CM, 1, true><<<grid, block, 0, s>>>(views, hdr, items);
+        else if (mode == 2) k_raster_forward_queue<CM, 2, true><<<grid, block, 0, s>>>(views, hdr, items);
+        else k_raster_forward_queue<CM, 0, true><<<grid, block, 0, s>>>(views, hdr, items);
+    } else {
+        if (mode == 1) k_raster_forward_queue<CM, 1, false><<<grid, block, 0, s>>>(views, hdr, items);
+        else if (mode == 2) k_raster_forward_queue<CM, 2, false><<<grid, block, 0, s>>>(views, hdr, items);
+        else k_raster_forward_queue<CM, 0, false><<<grid, block, 0, s>>>(views, hdr, items);
+    }
+    return hipGetLastError();
+}
+
+hipError_t views_order(const ViewRaster* views, ViewsHeader* hdr, int* items, hipStream_t s) {
+    k_views_order<<<1, 1024, 0, s>>>(views, hdr, items);
+    return hipGetLastError();
+}
+
+hipError_t raster_forward_views(const ViewRaster* views, ViewsHeader* hdr, const int* items,
+                                int channels, int settings, bool bf16, int blocks,
+                                hipStream_t s) {
+    const int mode = (settings & (kSetBlur | kSetNdc | kSetAlphaViz)) ? 1
+                     : raster_precise_edge_enabled()                 ? 2
+                                                                     : 0;
+    if (channels == 1) return launch_fwd_queue_cm<1>(views, hdr, items, mode, bf16, blocks, s);
+    if (channels == 2) return launch_fwd_queue_cm<2>(views, hdr, items, mode, bf16, blocks, s);
+    if (channels == 3) return launch_fwd_queue_cm<3>(views, hdr, items, mode, bf16, blocks, s);
+    if (channels == 4) return launch_fwd_queue_cm<4>(views, hdr, items, mode, bf16, blocks, s);
+    if (channels <= 8) return launch_fwd_queue_cm<8>(views, hdr, items, mode, bf16, blocks, s);
+    if (channels <= 16) return launch_fwd_queue_cm<16>(views, hdr, items, mode, bf16, blocks, s);
+    return launch_fwd_queue_cm<64>(views, hdr, items, mode, bf16, blocks, s);
 }
 
 static bool g_raster_culling = true;

@@ -39,23 +39,74 @@ __device__ __forceinline__ float swap16_sum(float a, float b) {
     return __uint_as_float(r[0]) + __uint_as_float(r[1]);
 }
 
-// Reduce-scatter 32 per-lane values over the wave: permlane swaps over lane
-// bits 5 and 4 (32 -> 8 values), DPP halvings with partner masks 15, 7 and 2
-// selecting on bits 3, 2 and 1 (8 -> 1), then a DPP add over bit 0.  Each
-// partner mask leaves the bits already selected on unchanged and together they
-// span all six lane bits, so lane L ends with the wave total of value
-// 16*b5 + 8*b4 + 4*b3 + 2*b2 + b1 = L >> 1 (lanes 2j and 2j+1 both hold j).
-__device__ __forceinline__ float wave_reduce_scatter(float (&v)[32], int lane) {
-    float w[16];
+// Reduce-scatter of N <= 64 per-lane values over the wave in six halving
+// rounds: permlane swaps over lane bits 5 and 4, then DPP lane permutes with
+// partner masks 15, 7, 2 and 1 selecting on bits 3, 2, 1 and 0.  Each partner
+// mask leaves the bits already selected on unchanged, and together they span
+// all six lane bits, so every lane ends with the wave total of one value.  A
+// round over n values pairs value k with value k + n - n/2 (the lane with the
+// bit clear keeps the first, the lane with it set the second); with n odd the
+// middle value is summed by both lanes and kept by both.  Which value a lane
+// ends with, and whether it is the lowest lane holding it, is rs_slot<N>.
+template <int N, int ROUND>
+struct ReduceScatter {
+    static constexpr int H = N / 2, NP = N - H;
+    __device__ __forceinline__ static float run(const float (&v)[N], int lane) {
+        float w[NP];
+        if constexpr (ROUND == 0) {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) w[k] = swap32_sum(v[k], v[k + 16]);
-    float x[8];
+            for (int k = 0; k < H; ++k) w[k] = swap32_sum(v[k], v[k + NP]);
+            if constexpr (NP > H) w[H] = swap32_sum(v[H], v[H]);
+        } else if constexpr (ROUND == 1) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) x[k] = swap16_sum(w[k], w[k + 8]);
-    halve<kDppRowMirror, 8, 4>(x, lane & 8);
-    halve<kDppRowHalfMirror, 8, 2>(x, lane & 4);
-    halve<kDppQuadXor2, 8, 1>(x, lane & 2);
-    return x[0] + dpp<kDppQuadXor1>(x[0]);
+            for (int k = 0; k < H; ++k) w[k] = swap16_sum(v[k], v[k + NP]);
+            if constexpr (NP > H) w[H] = swap16_sum(v[H], v[H]);
+        } else {
+            constexpr int CTRL = ROUND == 2   ? kDppRowMirror
+                                 : ROUND == 3 ? kDppRowHalfMirror
+                                 : ROUND == 4 ? kDppQuadXor2
+                                              : kDppQuadXor1;
+            const bool upper = (lane >> (5 - ROUND)) & 1;
+#pragma unroll
+            for (int k = 0; k < H; ++k) {
+                const float lo = v[k] + dpp<CTRL>(v[k]);
+                const float hi = v[k + NP] + dpp<CTRL>(v[k + NP]);
+                w[k] = upper ? hi : lo;
+            }
+            if constexpr (NP > H) w[H] = v[H] + dpp<CTRL>(v[H]);
+        }
+        if constexpr (ROUND == 5) {
+            static_assert(NP == 1, "at most 64 values");
+            return w[0];
+        } else {
+            return ReduceScatter<NP, ROUND + 1>::run(w, lane);
+        }
+    }
+};
+
+template <int N>
+__device__ __forceinline__ float wave_reduce_scatter(const float (&v)[N], int lane) {
+    return ReduceScatter<N, 0>::run(v, lane);
+}
+
+// The value index lane ends with after wave_reduce_scatter<N>, and whether
+// it is the lowest lane holding that index (bit 6 set when it is not):
+// traced back from the last round to the first.
+template <int N>
+__device__ __forceinline__ int rs_slot(int lane) {
+    int ns[6];
+    ns[0] = N;
+#pragma unroll
+    for (int r = 1; r < 6; ++r) ns[r] = ns[r - 1] - ns[r - 1] / 2;
+    int pos = 0, dup = 0;
+#pragma unroll
+    for (int r = 5; r >= 0; --r) {
+        const int h = ns[r] / 2, np = ns[r] - h;
+        const int bit = (lane >> (5 - r)) & 1;
+        if (pos < h) pos += bit ? np : 0;
+        else dup |= bit;
+    }
+    return pos | (dup << 6);
 }
 
 // ------------------------------------------------------------------ backward
@@ -65,17 +116,20 @@ __device__ __forceinline__ float wave_reduce_scatter(float (&v)[32], int lane) {
 // texels.  CM <= 4 is the exact channel count; above 4, CM bounds a runtime
 // count.  The plain kernels for CM <= 4 are held to 96 VGPRs (5 waves per
 // SIMD), which they need at most one register more than.
+//
+// raster_backward_wave runs one wave's share of a tile (wave `wave` of the
+// tile, as in the forward); `acc` is this hardware wave's LDS texel
+// accumulator, zero on entry and left zero.
 template <int CM, int MODE, bool TB>
-__global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(MODE == 0 && CM <= 4 ? 5 : 1)))
-void k_raster_backward(RasterArgs a, BwdIn b) {
+__device__ __forceinline__ void raster_backward_wave(const RasterArgs& a, const BwdIn& b,
+                                                     const int tile, const int wave,
+                                                     float* const acc) {
     constexpr bool GENERAL = MODE == 1, PRECISE = MODE != 0;
-    const int tile = blockIdx.x;
     const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
     const int lane = threadIdx.x & (kWave - 1);
     int lx, ly;
     bool in_tile;
-    lane_pixel(a.bw, threadIdx.x, lx, ly, in_tile);
+    lane_pixel(a.bw, wave * kWave + lane, lx, ly, in_tile);
     const int j = tx * a.bw + lx, i = ty * a.bw + ly;
     const bool inside = in_tile && i < a.img_h && j < a.img_w;
     const PixelRay pr = pixel_ray(a.c2w, a.viewmat, a.fx, a.fy, a.cx, a.cy, j, i);
@@ -120,13 +174,6 @@ void k_raster_backward(RasterArgs a, BwdIn b) {
     const int e_hi = min(r1 - 1, wave_last);
     const float tn = 0.01f, tf = 1000.0f;
 
-    // Wave-private texel-gradient accumulator (one chart at a time), kept zero
-    // between Gaussians by the flush below.
-    __shared__ float tex_acc[256 / kWave][kTexAccFloats];
-    float* acc = tex_acc[wave_index()];
-    for (int k = lane; k < kTexAccFloats; k += kWave) acc[k] = 0.f;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-
     // One list entry (wave-uniform index e, Gaussian g with record s).
     auto entry = [&](const int e, const int g, const Splat& s) {
         // The pair math and the taps run for every lane: they are finite for
@@ -145,12 +192,16 @@ void k_raster_backward(RasterArgs a, BwdIn b) {
         const float vis = valid ? p.alpha * T : 0.f;
         float u, v, wt[4], fu, fv;
         int row[4];  // byte offsets inside the chart
+        int iu, iv, ju, jv;  // tap rows (u) and columns (v) of the chart
         splat_uv(s, p.delta, u, v);
-        chart_taps(s, ch, u, v, bilinear, wt, row, fu, fv);
+        chart_taps_ij(s, ch, u, v, bilinear, wt, row, fu, fv, iu, iv, ju, jv);
         // How the texel gradients reach HBM (wave-uniform choice):
         //   single  1x1 chart, C <= 4: C values in the reduction
-        //   quad32  every valid lane samples the same texel quad, C <= 3, no
-        //           blur: 4 taps x C values in the reduction
+        //   quad    every valid lane samples the same texel quad, C <= 3:
+        //           4 taps x C values in the reduction
+        //   window  bilinear, C <= 4, every valid lane's taps inside one 3x3
+        //           texel window: the 9 C separable weight products in the
+        //           reduction
         //   lds     chart of <= kTexAccFloats values: LDS accumulation + flush
         //   direct  one atomic per tap
         const int first = __ffsll((unsigned long long)vmask) - 1;
@@ -160,21 +211,33 @@ void k_raster_backward(RasterArgs a, BwdIn b) {
         const int row1 = __builtin_amdgcn_readlane(row[1], first);
         const int row2 = __builtin_amdgcn_readlane(row[2], first);
         const int row3 = __builtin_amdgcn_readlane(row[3], first);
-        const bool quad32 = ch.ok && !tex_single && C >= 1 && C <= 3 && !use_blur &&
+        const bool quad32 = ch.ok && !tex_single && C >= 1 && C <= 3 &&
                             !__any(valid && row[0] != row0);
-        const bool tex_lds = ch.ok && !tex_single && !quad32 && C > 0 && hw * C <= kTexAccFloats;
+        // window origin (ou, ov): the anchor lane's tap row / column, one
+        // lower when some valid lane lies below it
+        const int au = __builtin_amdgcn_readlane(iu, first);
+        const int av = __builtin_amdgcn_readlane(iv, first);
+        const int ou = au - (__any(valid && iu < au) ? 1 : 0);
+        const int ov = av - (__any(valid && iv < av) ? 1 : 0);
+        const bool win3 = CM <= 4 && ch.ok && !tex_single && !quad32 && bilinear && C >= 1 &&
+                          !__any(valid && (iu < ou || iu > ou + 1 || iv < ov || iv > ov + 1));
+        const bool tex_lds = ch.ok && !tex_single && !quad32 && !win3 && C > 0 &&
+                             hw * C <= kTexAccFloats;
         float* const vtex_chart = b.v_texture + (size_t)(ch.ok ? s.start : 0) * C;
 
-        float gr[kGradFloats];
-#pragma unroll
-        for (int k = G_EXTRA; k < kGradFloats; ++k) gr[k] = 0.f;  // route and blur slots
+        // The per-Gaussian values (GradSlot 0..19, then the two blur values
+        // in the general kernels) and the texel terms of the routes.
+        constexpr int NB = G_EXTRA + (GENERAL ? 2 : 0);
+        float gr[NB];
+        float vval[CM];  // vis * dL/dtexture_out per channel
+        float wsum;
         {
             gr[G_RGB + 0] = vis * vo.x;
             gr[G_RGB + 1] = vis * vo.y;
             gr[G_RGB + 2] = vis * vo.z;
             float vvis = dot3(s.rgb, vo) + dot3(s.a3, vn);
             float vu = 0.f, vv = 0.f;
-            const float wsum = (wt[0] + wt[1]) + (wt[2] + wt[3]);
+            wsum = (wt[0] + wt[1]) + (wt[2] + wt[3]);
             // texels in chunks of up to 4 channels, all taps loaded before use
             constexpr int CK = CM < 4 ? CM : 4;
 #pragma unroll
@@ -194,26 +257,20 @@ void k_raster_backward(RasterArgs a, BwdIn b) {
                     if (c >= C) break;
                     const float c0 = tv[0][cc], c1 = tv[1][cc], c2 = tv[2][cc], c3 = tv[3][cc];
                     const float val = wt[0] * c0 + wt[1] * c1 + wt[2] * c2 + wt[3] * c3;
-                    const float vval = vis * vtex[c];
-                    if (tex_single) {
-                        if (c < 4) gr[G_EXTRA + c] = wsum * vval;
-                    } else if (quad32) {
-                        if (c < 3) {
-#pragma unroll
-                            for (int k = 0; k < 4; ++k) gr[G_EXTRA + 3 * k + c] = wt[k] * vval;
-                        }
-                    } else if (tex_lds) {
-                        if constexpr (CM > 4) {
+                    vval[c] = vis * vtex[c];
+                    if constexpr (CM > 4) {
+                        if (tex_lds) {
 #pragma unroll
                             for (int k = 0; k < 4; ++k) {
-                                const float add = wt[k] * vval;
+                                const float add = wt[k] * vval[c];
                                 if (add != 0.f) atomicAdd(acc + (row[k] >> kES) + c, add);
                             }
                         }
-                    } else if (ch.ok) {
+                    }
+                    if (!tex_single && !quad32 && !win3 && !tex_lds && ch.ok) {
 #pragma unroll
                         for (int k = 0; k < 4; ++k) {
-                            const float add = wt[k] * vval;
+                            const float add = wt[k] * vval[c];
                             if (add != 0.f) unsafeAtomicAdd(vtex_chart + (row[k] >> kES) + c, add);
                         }
                     }
@@ -221,8 +278,8 @@ void k_raster_backward(RasterArgs a, BwdIn b) {
                         // d val / d(u, v) through the bilinear weights
                         // (texture_helpers.cuh:262-299), tap differences first so
                         // coincident taps give exactly zero
-                        vu += (float)s.h * (vval * ((1.f - fv) * (c2 - c0) + fv * (c3 - c1)));
-                        vv += (float)s.w * (vval * ((1.f - fu) * (c1 - c0) + fu * (c3 - c2)));
+                        vu += (float)s.h * (vval[c] * ((1.f - fv) * (c2 - c0) + fv * (c3 - c1)));
+                        vv += (float)s.w * (vval[c] * ((1.f - fu) * (c1 - c0) + fu * (c3 - c2)));
                     }
                     vvis += val * vtex[c];
                 }
@@ -272,13 +329,88 @@ void k_raster_backward(RasterArgs a, BwdIn b) {
             gr[G_AX3X + 0] = vden * pr.r.x + vis * vn.x;
             gr[G_AX3X + 1] = vden * pr.r.y + vis * vn.y;
             gr[G_AX3X + 2] = vden * pr.r.z + vis * vn.z;
-            if (use_blur) {
-                const float vsb = p.blur_sel ? -s.opac * p.Gb * valpha : 0.f;
-                gr[G_PIX + 0] = 2.0f * vsb * (s.pix_x - pr.px);
-                gr[G_PIX + 1] = 2.0f * vsb * (s.pix_y - pr.py);
+            if constexpr (GENERAL) {
+                float px = 0.f, py = 0.f;
+                if (use_blur) {
+                    const float vsb = p.blur_sel ? -s.opac * p.Gb * valpha : 0.f;
+                    px = 2.0f * vsb * (s.pix_x - pr.px);
+                    py = 2.0f * vsb * (s.pix_y - pr.py);
+                }
+                gr[G_EXTRA] = px;
+                gr[G_EXTRA + 1] = py;
             }
         }
+
+        // Reduce and add: lane L holds the wave total of value slot(L), and
+        // the lowest lane holding a value adds it.  The per-Gaussian values
+        // and the route's texel values are reduced separately (about the same
+        // VALU as one reduction of both, with half the live registers).
+        {
+            float* const grec = b.grad_rec + (size_t)g * kGradFloats;
+            const float tot = wave_reduce_scatter(gr, lane);
+            const int code = rs_slot<NB>(lane);
+            if (code < 64 && tot != 0.f)
+                unsafeAtomicAdd(grec + (code < G_EXTRA ? code : G_PIX + (code - G_EXTRA)), tot);
+        }
         if constexpr (CM <= 4) {
+            if (tex_single) {
+                float vals[CM];
+#pragma unroll
+                for (int c = 0; c < CM; ++c) vals[c] = wsum * vval[c];
+                const float tot = wave_reduce_scatter(vals, lane);
+                const int code = rs_slot<CM>(lane);
+                if (code < 64 && tot != 0.f) unsafeAtomicAdd(vtex_chart + code, tot);
+                return;
+            }
+            if (quad32) {
+                constexpr int QC = CM <= 3 ? CM : 3;
+                float vals[4 * QC];
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+#pragma unroll
+                    for (int c = 0; c < QC; ++c) vals[QC * k + c] = wt[k] * vval[c];
+                const float tot = wave_reduce_scatter(vals, lane);
+                const int code = rs_slot<4 * QC>(lane);
+                if (code < 64 && tot != 0.f) {
+                    const int k = code / QC, c = code - QC * k;
+                    const int rr = k == 0 ? row0 : k == 1 ? row1 : k == 2 ? row2 : row3;
+                    unsafeAtomicAdd(vtex_chart + (rr >> kES) + c, tot);
+                }
+                return;
+            }
+            if (win3) {
+                // separable weights: tap (row r, column q) has weight
+                // wa[r] * wb[q]; rows iu, ju and columns iv, jv relative to
+                // the window origin lie in 0..2 (ju = iu at the chart edge)
+                const float wa0 = 1.f - fu, wa1 = fu, wb0 = 1.f - fv, wb1 = fv;
+                const int ri = iu - ou, rj = ju - ou, ci = iv - ov, cj = jv - ov;
+                float A[3], Bw[3];
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    A[r] = (ri == r ? wa0 : 0.f) + (rj == r ? wa1 : 0.f);
+                    Bw[r] = (ci == r ? wb0 : 0.f) + (cj == r ? wb1 : 0.f);
+                }
+                float vals[9 * CM];
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        const float ab = A[r] * Bw[q];
+#pragma unroll
+                        for (int c = 0; c < CM; ++c) vals[(3 * r + q) * CM + c] = ab * vval[c];
+                    }
+                const float tot = wave_reduce_scatter(vals, lane);
+                const int code = rs_slot<9 * CM>(lane);
+                if (code < 64 && tot != 0.f) {
+                    const int rq = code / CM, c = code - CM * rq;
+                    const int r = rq / 3, q = rq - 3 * r;
+                    // a window row or column past the chart edge carries
+                    // no weight (taps clamp inside the chart)
+                    if (ou + r < s.h && ov + q < s.w)
+                        unsafeAtomicAdd(vtex_chart + ((ou + r) * s.w + (ov + q)) * CM + c, tot);
+                }
+                return;
+            }
             if (tex_lds) {
                 // merge lanes on the same texel quad before the LDS adds
                 const QuadMerge qm = quad_merge_setup(valid ? row[0] : -1, lane);
@@ -288,27 +420,13 @@ void k_raster_backward(RasterArgs a, BwdIn b) {
 #pragma unroll
                 for (int c = 0; c < CM; ++c) {
                     if (c >= C) break;
-                    const float vv_c = vis * vtex[c];
                     float q[4];
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) q[k] = quad_merge(wt[k] * vv_c, qm);
+                    for (int k = 0; k < 4; ++k) q[k] = quad_merge(wt[k] * vval[c], qm);
 #pragma unroll
                     for (int k = 0; k < 4; ++k)
                         if (q[k] != 0.f) atomicAdd(acc + lofs[k] + c, q[k]);
                 }
-            }
-        }
-        const float tot = wave_reduce_scatter(gr, lane);
-        const int slot = lane >> 1;
-        if ((lane & 1) == 0 && tot != 0.f) {
-            if (slot < G_EXTRA || (use_blur && slot >= G_PIX)) {
-                unsafeAtomicAdd(b.grad_rec + (size_t)g * kGradFloats + slot, tot);
-            } else if (tex_single) {
-                if (slot - G_EXTRA < C) unsafeAtomicAdd(vtex_chart + (slot - G_EXTRA), tot);
-            } else if (quad32) {
-                const int t = slot - G_EXTRA, k = t / 3, c = t - 3 * k;
-                const int rr = k == 0 ? row0 : k == 1 ? row1 : k == 2 ? row2 : row3;
-                if (c < C) unsafeAtomicAdd(vtex_chart + (rr >> kES) + c, tot);
             }
         }
         if (tex_lds) {
@@ -345,7 +463,7 @@ void k_raster_backward(RasterArgs a, BwdIn b) {
         // The mask words of 64 windows are read at once, lane k holding window
         // top - k, so windows this wave never composited (long runs of them in
         // lists the forward culled or passed over) cost no read of their ids.
-        const size_t plane = (size_t)wave_index() * a.mask_plane + mword0;
+        const size_t plane = (size_t)wave * a.mask_plane + mword0;
         const uint64_t* mplane = a.wave_masks + plane;
         const GSTEX_CONST uint64_t* mplane_s = as_const_space(a.wave_masks) + plane;
         const int top = e_hi >= r0 ? ((e_hi & ~63) - mentry0) >> 6 : -1;
@@ -383,6 +501,56 @@ void k_raster_backward(RasterArgs a, BwdIn b) {
             const int g = __builtin_amdgcn_readlane(w.gid, bit);
             entry(base + bit, g, splat_record(a, g));
         }
+    }
+}
+
+// Wave-private texel-gradient accumulator (one chart at a time), zeroed once
+// and kept zero between Gaussians by the route's flush.
+__device__ __forceinline__ float* zeroed_tex_acc(float (&tex_acc)[256 / kWave][kTexAccFloats]) {
+    float* acc = tex_acc[wave_index()];
+    for (int k = threadIdx.x & (kWave - 1); k < kTexAccFloats; k += kWave) acc[k] = 0.f;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    return acc;
+}
+
+// The plain kernels for CM <= 4 are held to 5 waves per SIMD.
+template <int CM, int MODE, bool TB>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(MODE == 0 && CM <= 4 ? 5 : 1)))
+void k_raster_backward(RasterArgs a, BwdIn b) {
+    __shared__ float tex_acc[256 / kWave][kTexAccFloats];
+    float* const acc = zeroed_tex_acc(tex_acc);
+    raster_backward_wave<CM, MODE, TB>(a, b, blockIdx.x, wave_index(), acc);
+}
+
+// Persistent backward over every view of a table, in the queue order the
+// forward pass left in `items` (longest list first).
+template <int CM, int MODE, bool TB>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(MODE == 0 && CM <= 4 ? 5 : 1)))
+void k_raster_backward_queue(const ViewRaster* __restrict__ views, ViewsHeader* __restrict__ hdr,
+                             const int* __restrict__ items) {
+    __shared__ float tex_acc[256 / kWave][kTexAccFloats];
+    float* const acc = zeroed_tex_acc(tex_acc);
+    const int wpt = hdr->waves_per_tile, n_work = hdr->num_items * wpt;
+    const int nv = hdr->num_views;
+    for (;;) {
+        int k = 0;
+        if ((threadIdx.x & (kWave - 1)) == 0) k = atomicAdd(&hdr->head, 1);
+        k = __builtin_amdgcn_readfirstlane(__shfl(k, 0));
+        if (k >= n_work) break;
+        const int it = k / wpt, wave = k - it * wpt;
+        const int item = __builtin_amdgcn_readfirstlane(as_const_space(items)[it]);
+        const int v = item >> kViewItemTileBits, tile = item & ((1 << kViewItemTileBits) - 1);
+        if (v >= nv) continue;
+        const GSTEX_CONST ViewRaster* vr = as_const_space(views) + v;
+        RasterArgs a = load_const(&vr->a);
+        const BwdIn b = load_const(&vr->b);
+        if (tile >= a.tiles_x * a.tiles_y) continue;
+        // the backward differentiates the Gaussian alpha whatever the alpha
+        // visualisation bits say; masks such a forward wrote mark other entries
+        if (a.settings & kSetAlphaViz) a.wave_masks = nullptr;
+        raster_backward_wave<CM, MODE, TB>(a, b, tile, wave, acc);
     }
 }
 
@@ -573,6 +741,35 @@ static hipError_t launch_bwd_cm(const RasterArgs& a, const BwdIn& b, hipStream_t
         else k_raster_backward<CM, 0, false><<<grid, block, 0, s>>>(a, b);
     }
     return hipGetLastError();
+}
+
+template <int CM>
+static hipError_t launch_bwd_queue_cm(const ViewRaster* views, ViewsHeader* hdr, const int* items,
+                                      int mode, bool bf16, int blocks, hipStream_t s) {
+    const dim3 grid(blocks), block(256);
+    if (bf16) {
+        if (mode == 1) k_raster_backward_queue<CM, 1, true><<<grid, block, 0, s>>>(views, hdr, items);
+        else if (mode == 2) k_raster_backward_queue<CM, 2, true><<<grid, block, 0, s>>>(views, hdr, items);
+        else k_raster_backward_queue<CM, 0, true><<<grid, block, 0, s>>>(views, hdr, items);
+    } else {
+        if (mode == 1) k_raster_backward_queue<CM, 1, false><<<grid, block, 0, s>>>(views, hdr, items);
+        else if (mode == 2) k_raster_backward_queue<CM, 2, false><<<grid, block, 0, s>>>(views, hdr, items);
+        else k_raster_backward_queue<CM, 0, false><<<grid, block, 0, s>>>(views, hdr, items);
+    }
+    return hipGetLastError();
+}
+
+hipError_t raster_backward_views(const ViewRaster* views, ViewsHeader* hdr, const int* items,
+                                 int channels, int settings, bool bf16, int blocks,
+                                 hipStream_t s) {
+    const int mode = (settings & (kSetBlur | kSetNdc)) ? 1 : raster_precise_edge_enabled() ? 2 : 0;
+    if (channels == 1) return launch_bwd_queue_cm<1>(views, hdr, items, mode, bf16, blocks, s);
+    if (channels == 2) return launch_bwd_queue_cm<2>(views, hdr, items, mode, bf16, blocks, s);
+    if (channels == 3) return launch_bwd_queue_cm<3>(views, hdr, items, mode, bf16, blocks, s);
+    if (channels == 4) return launch_bwd_queue_cm<4>(views, hdr, items, mode, bf16, blocks, s);
+    if (channels <= 8) return launch_bwd_queue_cm<8>(views, hdr, items, mode, bf16, blocks, s);
+    if (channels <= 16) return launch_bwd_queue_cm<16>(views, hdr, items, mode, bf16, blocks, s);
+    return launch_bwd_queue_cm<64>(views, hdr, items, mode, bf16, blocks, s);
 }
 
 size_t raster_backward_workspace_bytes(int n) {

@@ -70,6 +70,23 @@ struct BwdIn {
     float* v_texture;  // (X, C)
 };
 
+// One view of a multi-view raster pass: its launch arguments, forward
+// outputs and backward inputs, in a device table the queue kernels read.
+struct ViewRaster {
+    RasterArgs a;
+    FwdOut o;
+    BwdIn b;
+};
+
+// Header of a multi-view table (device copy; the host keeps its own).
+struct ViewsHeader {
+    int num_views, num_items, waves_per_tile;
+    int head;  // the work queue's next index, reset before each pass
+};
+
+// A work item packs (view, tile): tile in the low kViewItemTileBits bits.
+constexpr int kViewItemTileBits = 20;
+
 __host__ __device__ inline int raster_block_threads(int bw) {
     return bw == 16 ? 256 : ((bw * bw + kWave - 1) / kWave) * kWave;
 }
@@ -124,6 +141,19 @@ struct Splat {
 template <typename T>
 __device__ __forceinline__ const GSTEX_CONST T* as_const_space(const T* p) {
     return (const GSTEX_CONST T*)p;
+}
+
+// A struct read through the constant address space (scalar loads for a
+// wave-uniform address), dword by dword.
+template <typename T>
+__device__ __forceinline__ T load_const(const GSTEX_CONST T* p) {
+    static_assert(sizeof(T) % 4 == 0, "dword-sized struct");
+    T out;
+    const GSTEX_CONST uint32_t* src = reinterpret_cast<const GSTEX_CONST uint32_t*>(p);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); ++k) dst[k] = src[k];
+    return out;
 }
 
 // Adjacent float reads of one record merge into wide loads (SMEM dwordx8/x16
@@ -269,15 +299,19 @@ __device__ __forceinline__ float chart_texel(const Chart& ch, int off) {
 // (ju, jv) inside the chart, with the replicate-padded neighbour indices
 // (texture_helpers.cuh:162-215).  For a chart inside the texture tu = h u
 // lies in [0, h], so only the upper clamps can act.
-__device__ __forceinline__ void chart_taps(const Splat& s, const Chart& ch, float u, float v,
-                                           bool bilinear, float wt[4], int off[4], float& fu,
-                                           float& fv) {
+// Also returns the texel row / column indices of the taps: rows iu, ju and
+// columns iv, jv (tap k is at row k < 2 ? iu : ju, column k & 1 ? jv : iv).
+__device__ __forceinline__ void chart_taps_ij(const Splat& s, const Chart& ch, float u, float v,
+                                              bool bilinear, float wt[4], int off[4], float& fu,
+                                              float& fv, int& iu, int& iv, int& ju, int& jv) {
     const float tu = (float)s.h * u, tv = (float)s.w * v;
     const int iu0 = f2i_trunc(tu), iv0 = f2i_trunc(tv);
     fu = tu - (float)iu0;
     fv = tv - (float)iv0;
-    const int iu = min(iu0, s.h - 1), iv = min(iv0, s.w - 1);
-    const int ju = min(iu0 + 1, s.h - 1), jv = min(iv0 + 1, s.w - 1);
+    iu = min(iu0, s.h - 1);
+    iv = min(iv0, s.w - 1);
+    ju = min(iu0 + 1, s.h - 1);
+    jv = min(iv0 + 1, s.w - 1);
     const float wii = (1.f - fu) * (1.f - fv), wij = (1.f - fu) * fv, wji = fu * (1.f - fv),
                 wjj = fu * fv;
     if (bilinear) {
@@ -295,6 +329,13 @@ __device__ __forceinline__ void chart_taps(const Splat& s, const Chart& ch, floa
     off[1] = off[0] + dv;
     off[2] = off[0] + (ju != iu ? ch.wce : 0);
     off[3] = off[2] + dv;
+}
+
+__device__ __forceinline__ void chart_taps(const Splat& s, const Chart& ch, float u, float v,
+                                           bool bilinear, float wt[4], int off[4], float& fu,
+                                           float& fv) {
+    int iu, iv, ju, jv;
+    chart_taps_ij(s, ch, u, v, bilinear, wt, off, fu, fv, iu, iv, ju, jv);
 }
 
 __device__ __forceinline__ void splat_uv(const Splat& s, V3 delta, float& u, float& v) {

@@ -185,6 +185,12 @@ class RenderStep:
         return _p(self.wave_masks), self.wave_masks.numel()
 
     def forward(self, timer: Optional[StageTimer] = None):
+        self.forward_pre()
+        self.forward_raster(timer)
+
+    def forward_pre(self):
+        """SH colours, projection, binning and splat records (everything of
+        the forward before the raster launch)."""
         p, cam, s = self.p, self.cam, self._stream()
         n = self.n
         if self.sh_degree >= 0:
@@ -217,6 +223,10 @@ class RenderStep:
              _p(p["quats"]), _p(self.colors), _p(p["opacities"]), _p(p["uv0"]), _p(p["umap"]),
              _p(p["vmap"]), _p(p["texture_dims"]), _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy,
              cam.cx, cam.cy, _p(self.records), s)
+
+    def forward_raster(self, timer: Optional[StageTimer] = None):
+        p, cam, s = self.p, self.cam, self._stream()
+        n, m = self.n, self._list_len
         if timer:
             timer.start("raster_forward")
         self._raster("gstex_raster_forward", cam.img_w, cam.img_h, self.bw, self.C, self.settings,
@@ -381,7 +391,8 @@ class MultiViewStep:
     def __init__(self, params: Dict[str, torch.Tensor], cams, targets, block_width: int = 16,
                  settings: int = 1 << 8, background: Optional[torch.Tensor] = None,
                  sh_degree: int = -1, capacity_factor: float = 1.1,
-                 use_wave_masks: bool = True, streams: int = 4, balance: bool = True):
+                 use_wave_masks: bool = True, streams: int = 4, balance: bool = True,
+                 queue: bool = False):
         self.targets = list(targets)
         kw = dict(block_width=block_width, settings=settings, background=background,
                   sh_degree=sh_degree, use_wave_masks=use_wave_masks)
@@ -420,6 +431,10 @@ class MultiViewStep:
             v.sh_backward = False
             if sh_degree >= 0:
                 v._viewdirs = self._viewdirs[i]
+        self.queue = bool(queue)
+        self._table = None
+        if self.queue:
+            self._setup_queue(block_width, settings)
         k = min(max(int(streams), 1), len(self.views))
         self.streams = [torch.cuda.Stream(dev) for _ in range(k)] if k > 1 else []
         # views -> streams: longest first onto the least loaded stream, by the
@@ -445,9 +460,69 @@ class MultiViewStep:
                 self.assign.append((i, st))
             self.view_ms = cost
 
+    def _setup_queue(self, block_width: int, settings: int):
+        """The launch table of the multi-view raster pass (gstex_views_setup)."""
+        lib = _lib.lib()
+        v0 = self.views[0]
+        arr = (_lib.View * len(self.views))()
+        total = 0
+        for i, v in enumerate(self.views):
+            cam = v.cam
+            total += v.tiles[0] * v.tiles[1]
+            masks, mbytes = v._masks()
+            arr[i] = _lib.View(
+                cam.img_w, cam.img_h, cam.fx, cam.fy, cam.cx, cam.cy, _p(cam.viewmat),
+                _p(cam.c2w), _p(v.records), _p(v.ids), int(v.cap), _p(v.tile_bins), masks,
+                mbytes, _p(v.out_img), _p(v.out_depth), _p(v.out_reg), _p(v.out_texture),
+                _p(v.out_normal), _p(v.final_Ts), _p(v.final_idx), _p(v.depth_idx),
+                _p(v.out_reg_s), _p(v.v_img), _p(v.v_zero1), _p(v.v_zero1), _p(v.v_zero1),
+                _p(v.v_tex), _p(v.v_zero3), _p(v.grad_records))
+        self._views_arr = arr
+        nbytes = lib.gstex_views_table_size(len(self.views), total)
+        self._table = torch.empty((nbytes + 256,), dtype=torch.uint8, device=v0.dev)
+        base = self._table.data_ptr()
+        self._table_ptr = (base + 255) // 256 * 256
+        tex = self.params["texture"]
+        call("gstex_views_setup", len(self.views), arr, block_width, v0.C, settings, v0.n,
+             _p(tex), v0.X, v0.texture_format, _p(v0.background), _p(self.grads["v_texture"]),
+             self._table_ptr, nbytes, v0._stream())
+
+    def __del__(self):
+        if getattr(self, "_table", None) is not None:
+            try:
+                _lib.lib().gstex_views_release(self._table_ptr)
+            except Exception:
+                pass
+
+    def _step_queue(self):
+        """One batch through the multi-view raster pass: the views' SH,
+        projection, binning and records (on the streams), one forward launch
+        over every view's tiles, the loss cotangents, one backward launch."""
+        dev = self.views[0].dev
+        main = torch.cuda.current_stream(dev)
+        if self.streams:
+            for st in self.streams:
+                st.wait_stream(main)
+            for i, st in self.assign:
+                with torch.cuda.stream(self.streams[st]):
+                    self.views[i].forward_pre()
+            for st in self.streams:
+                main.wait_stream(st)
+        else:
+            for v in self.views:
+                v.forward_pre()
+        call("gstex_views_raster_forward", self._table_ptr, main.cuda_stream)
+        for v, tgt in zip(self.views, self.targets):
+            v.l2_loss_grad(tgt)
+        call("gstex_fill_f32", _p(self._grad_records), self._grad_records.numel(), 0.0,
+             main.cuda_stream)
+        call("gstex_views_raster_backward", self._table_ptr, main.cuda_stream)
+
     def step(self):
         self.grads["v_texture"].zero_()
-        if self.streams:
+        if self.queue:
+            self._step_queue()
+        elif self.streams:
             main = torch.cuda.current_stream(self.views[0].dev)
             for st in self.streams:
                 st.wait_stream(main)
@@ -459,6 +534,9 @@ class MultiViewStep:
         else:
             for v, tgt in zip(self.views, self.targets):
                 v.step(tgt)
+        self._batch_gradients()
+
+    def _batch_gradients(self):
         v0, g, p = self.views[0], self.grads, self.params
         call("gstex_finalize_gradients_views", v0.n, len(self.views), _p(self._grad_records),
              _p(self._records), self._rec_stride, _p(p["quats"]), _p(p["scales"]),

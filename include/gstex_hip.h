@@ -174,6 +174,50 @@ int gstex_texture_forward(int tiles_x, int tiles_y, int block_width, int img_wid
                           int32_t *depth_idx, float *out_reg_s, void *workspace,
                           size_t workspace_bytes, void *stream);
 
+/* ---- Multi-view raster pass ------------------------------------------------
+ * No counterpart in the reference, whose trainer renders one view per launch
+ * (texture.py:187-193, texture.cu:864).  For a batch of views of one scene
+ * (shared Gaussians, texels and settings; each view with its own camera,
+ * splat records from gstex_build_splat_records, sorted lists, wave masks,
+ * outputs and backward inputs), one forward launch and one backward launch
+ * raster every tile of every view.  The (view, tile) work is ordered longest
+ * list first and taken by persistent waves from one queue, so the longest
+ * tiles of all views start first.  Results equal gstex_raster_forward /
+ * gstex_raster_backward per view (texel-gradient and gradient-record sums
+ * in another order).
+ *
+ * gstex_views_setup validates the views and writes their launch table into
+ * `table` (gstex_views_table_size bytes, 256-byte aligned, caller-owned);
+ * it synchronises the stream and must not be captured.  The table stays
+ * valid while every buffer it names does; gstex_views_release forgets it.
+ * gstex_views_raster_forward / _backward only launch (graph-capturable);
+ * the backward adds into each view's grad_records and into v_texture, which
+ * the caller zeroes first, and needs the forward of the same lists before it. */
+typedef struct gstex_view {
+    int img_width, img_height;
+    float fx, fy, cx, cy;
+    const float *viewmat, *c2w;        /* (4,4) row-major, device */
+    const void *records;               /* gstex_splat_records_size(num_points) bytes */
+    const int32_t *gaussian_ids_sorted;
+    int num_ids;                       /* length of gaussian_ids_sorted */
+    const int32_t *tile_bins;          /* (tiles, 2) */
+    void *wave_masks;                  /* optional: gstex_raster_wave_masks_size bytes */
+    size_t wave_masks_bytes;
+    float *out_img, *out_depth, *out_reg, *out_texture, *out_normal, *final_Ts;
+    int32_t *final_idx, *depth_idx;
+    float *out_reg_s;
+    const float *v_img, *v_depth, *v_reg, *v_alpha, *v_texture_out, *v_normal;
+    float *grad_records;               /* (num_points, 32), see gstex_raster_backward */
+} gstex_view;
+size_t gstex_views_table_size(int num_views, int total_tiles);
+int gstex_views_setup(int num_views, const gstex_view *views, int block_width, int channels,
+                      int settings, int num_points, const void *texture, int num_texels,
+                      int texture_format, const float *background, float *v_texture, void *table,
+                      size_t table_bytes, void *stream);
+int gstex_views_release(void *table);
+int gstex_views_raster_forward(void *table, void *stream);
+int gstex_views_raster_backward(void *table, void *stream);
+
 /* No counterpart in the reference.  gstex_texture_forward without a
  * workspace argument, for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
