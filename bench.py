@@ -323,13 +323,23 @@ def multiview_variant(dev, workload, steps, warmup):
     from gstex_cuda_amd import synthetic
     from gstex_cuda_amd.pipeline import scene_to_multiview
 
+    from gstex_cuda_amd.autotune import BatchConfig, tune_multiview
+
     bf16 = workload == 5
     scene = synthetic.make_scene(workload)
     views = synthetic.workload_views(workload, scene)
     prev = _C.set_precise_edge_on(bf16)
     job = None
     try:
-        mv = scene_to_multiview(scene, views, dev, bf16=bf16)
+        # the batch's launch configuration, chosen by timing the candidates
+        # (the 16-view workload also tries 8-pixel tiles: its tile size is tuned)
+        widths = (16, 8) if workload == 4 else (16,)
+        streams = (4, 8) if workload == 4 else (4,)
+        cands = [BatchConfig(q, st, bw) for bw in widths for q in (False, True) for st in streams]
+        best, tried = tune_multiview(scene, views, dev, bf16=bf16, candidates=cands)
+        scene.block_width = best.block_width
+        mv = scene_to_multiview(scene, views, dev, bf16=bf16, queue=best.queue,
+                                streams=best.streams)
         mv.capture()
         elapsed = _timed(mv.replay, steps, warmup)
         ok = mv.capacity_ok()
@@ -351,9 +361,17 @@ def multiview_variant(dev, workload, steps, warmup):
            "views_per_s": round(len(views) * steps / elapsed, 3),
            "ms_per_batch": round(1e3 * elapsed / steps, 4),
            "intersects_per_batch": int(sum(isects)), "intersects_min_max": [min(isects), max(isects)],
-           "graph": True, "capacity_ok": bool(ok), "steps": steps}
+           "graph": True, "capacity_ok": bool(ok), "steps": steps,
+           "tuned": {"chosen": {"queue": best.queue, "streams": best.streams,
+                                "block_width": best.block_width},
+                     "candidates_ms_per_batch": tried}}
     del mv
     torch.cuda.empty_cache()
+    # the plain FP32 single-view path on the same cameras, one view at a time
+    scene.block_width = 16
+    plain = plain_same_cameras(scene, views, dev, steps, warmup)
+    out["plain_same_cameras_views_per_s"] = round(plain, 3)
+    out["vs_plain_same_cameras"] = round(out["views_per_s"] / plain, 3)
     return out, job
 
 

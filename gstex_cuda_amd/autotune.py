@@ -17,6 +17,14 @@ A trial is the HIP-event duration of one whole step (SH, projection, binning,
 raster forward, loss gradient, raster backward, finalize) after a warm-up step.  Results are cached per
 (N, width, height, channels, settings, texel format) in memory and, when
 ``cache_path`` is given, in a JSON file.
+
+``tune_multiview`` does the same for a multi-view batch captured as one graph,
+over the batch's own knobs:
+
+* ``queue`` -- one raster launch per direction over every view's tiles from a
+  longest-first queue, or each view's own launches on the streams;
+* ``streams`` -- HIP streams the views' stages run on;
+* ``block_width`` -- tile side (only when the caller lists more than one).
 """
 from __future__ import annotations
 
@@ -111,3 +119,48 @@ def tuned_step(params, cam: Camera, target: torch.Tensor, **kw) -> RenderStep:
 
 def last_timings() -> Dict[str, dict]:
     return dict(_CACHE)
+
+
+@dataclass(frozen=True)
+class BatchConfig:
+    queue: bool = False
+    streams: int = 4
+    block_width: int = 16
+
+
+def tune_multiview(scene, views, device, bf16: bool = False,
+                   candidates: Optional[Iterable[BatchConfig]] = None, trials: int = 3):
+    """Time each candidate configuration of a multi-view batch of ``scene``
+    seen from ``views`` (graph capture, one warm replay, ``trials`` timed
+    replays) and return (fastest config, {config name: ms per batch})."""
+    from .pipeline import scene_to_multiview
+
+    if candidates is None:
+        candidates = [BatchConfig(q, st, scene.block_width) for q in (False, True)
+                      for st in (4, 8)]
+    bw0 = scene.block_width
+    timings = {}
+    best, best_ms = None, float("inf")
+    try:
+        for cfg in candidates:
+            scene.block_width = cfg.block_width
+            mv = scene_to_multiview(scene, views, device, bf16=bf16, queue=cfg.queue,
+                                    streams=cfg.streams)
+            mv.capture()
+            mv.replay_checked()
+            t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0.record()
+            for _ in range(trials):
+                mv.replay()
+            t1.record()
+            t1.synchronize()
+            ms = t0.elapsed_time(t1) / trials
+            name = f"queue{int(cfg.queue)}_streams{cfg.streams}_bw{cfg.block_width}"
+            timings[name] = round(ms, 4)
+            if ms < best_ms:
+                best, best_ms = cfg, ms
+            del mv
+            torch.cuda.empty_cache()
+    finally:
+        scene.block_width = bw0
+    return best, timings
