@@ -95,7 +95,8 @@ _SIGNATURES = {
     "gstex_views_raster_forward": (I, [P, P]),
     "gstex_views_raster_backward": (I, [P, P]),
     "gstex_texture_sample_forward": (I, [I, I, I, P, P, I, P, I, P, P]),
-    "gstex_texture_sample_backward": (I, [I, I, I, P, P, I, P, I, P, P, P]),
+    "gstex_texture_sample_backward_workspace_size": (Z, [I]),
+    "gstex_texture_sample_backward": (I, [I, I, I, P, P, I, P, I, P, P, P, Z, P]),
 }
 
 

@@ -560,6 +560,15 @@ hipError_t radix_sort_pairs(int n, const K* keys_in, const uint32_t* vals_in, K*
     return hipSuccess;
 }
 
+size_t sort_u32_pairs_workspace_bytes(int n) { return radix_workspace_bytes<uint32_t>(n); }
+
+hipError_t sort_u32_pairs(int n, const uint32_t* keys_in, const uint32_t* vals_in,
+                          uint32_t* keys_out, uint32_t* vals_out, int end_bit, void* ws,
+                          size_t ws_bytes, hipStream_t s) {
+    return radix_sort_pairs<uint32_t>(n, keys_in, vals_in, keys_out, vals_out, end_bit, ws,
+                                      ws_bytes, s);
+}
+
 size_t sort_isects_workspace_bytes(int n) { return radix_workspace_bytes<uint64_t>(n); }
 
 hipError_t sort_isects(int n, const int64_t* keys_in, const int* ids_in, int64_t* keys_out,

@@ -644,10 +644,15 @@ int gstex_texture_sample_forward(int num_charts, int num_probs, int channels,
                      "texture_sample_forward");
 }
 
+size_t gstex_texture_sample_backward_workspace_size(int num_queries) {
+    return gstex::sample_backward_workspace_bytes(num_queries);
+}
+
 int gstex_texture_sample_backward(int num_charts, int num_probs, int channels,
                                   const int32_t* texture_dims, const float* uvs, int num_queries,
                                   const float* texture, int num_texels, const float* v_output,
-                                  float* v_texture, void* stream) {
+                                  float* v_texture, void* workspace, size_t workspace_bytes,
+                                  void* stream) {
     (void)num_charts;
     (void)num_probs;
     (void)texture;
@@ -655,8 +660,13 @@ int gstex_texture_sample_backward(int num_charts, int num_probs, int channels,
     if (num_texels == 0 || channels == 0) return GSTEX_OK;
     REQUIRE(v_texture && (num_queries == 0 || (texture_dims && uvs && v_output)),
             "null pointer argument");
+    REQUIRE(num_queries == 0 || workspace, "null workspace");
+    REQUIRE(workspace_bytes >= gstex::sample_backward_workspace_bytes(num_queries),
+            "workspace too small: %zu < %zu", workspace_bytes,
+            gstex::sample_backward_workspace_bytes(num_queries));
     return check_hip(gstex::sample_backward(num_queries, channels, num_texels, texture_dims, uvs,
-                                            v_output, v_texture, as_stream(stream)),
+                                            v_output, v_texture, workspace, workspace_bytes,
+                                            as_stream(stream)),
                      "texture_sample_backward");
 }
 

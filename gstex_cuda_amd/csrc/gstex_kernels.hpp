@@ -63,6 +63,11 @@ hipError_t sort_isects(int n, const int64_t* keys_in, const int* ids_in, int64_t
                        int* ids_out, int end_bit, void* ws, size_t ws_bytes, hipStream_t s);
 hipError_t tile_bin_edges(int m, const int64_t* keys_sorted, int num_tiles, int* bins,
                           hipStream_t s);
+// Stable LSD radix sort of (uint32 key, uint32 value) pairs over key bits [0, end_bit).
+size_t sort_u32_pairs_workspace_bytes(int n);
+hipError_t sort_u32_pairs(int n, const uint32_t* keys_in, const uint32_t* vals_in,
+                          uint32_t* keys_out, uint32_t* vals_out, int end_bit, void* ws,
+                          size_t ws_bytes, hipStream_t s);
 size_t bin_gaussians_workspace_bytes(int n, int m);
 // Depth-first binning into a list of capacity cap >= the intersect total m =
 // sum(max(counts, 0)), which is computed on the device (and copied to *m_out
@@ -224,7 +229,9 @@ hipError_t texture_edit(const RasterCall& c, const float* upd_img, const float* 
 // sampling
 hipError_t sample_forward(int q, int C, int num_texels, const int* dims, const float* uvs,
                           const float* texture, float* out, hipStream_t s);
+size_t sample_backward_workspace_bytes(int q);
 hipError_t sample_backward(int q, int C, int num_texels, const int* dims, const float* uvs,
-                           const float* v_out, float* v_texture, hipStream_t s);
+                           const float* v_out, float* v_texture, void* ws, size_t ws_bytes,
+                           hipStream_t s);
 
 }  // namespace gstex

@@ -523,9 +523,11 @@ def texture_sample_backward(texture_info, texture_dims, uvs, texture, v_output):
     C = int(texture_info[2])
     v_texture = torch.empty((texture.size(0), C), dtype=torch.float32, device=dev)
     with torch.cuda.device(dev):
+        ws, nb = _workspace(_lib.lib().gstex_texture_sample_backward_workspace_size(uvs.size(0)),
+                            dev)
         call("gstex_texture_sample_backward", int(texture_info[0]), int(texture_info[1]), C,
              _ptr(texture_dims), _ptr(uvs), uvs.size(0), _ptr(texture), texture.size(0),
-             _ptr(v_output), _ptr(v_texture), _stream(dev))
+             _ptr(v_output), _ptr(v_texture), _ptr(ws), nb, _stream(dev))
     return v_texture
 
 

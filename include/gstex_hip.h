@@ -384,11 +384,15 @@ int gstex_texture_sample_forward(int num_charts, int num_probs, int channels,
 
 /* Replaces texture_sample_backward_tensor (cuda/csrc/texture_sample.cu:107-141)
  * with the adjoint w.r.t. the texels computed from v_output (the reference
- * kernel reads v_texture there, texture_sample.cu:58).  v_texture fully written. */
+ * kernel reads v_texture there, texture_sample.cu:58).  v_texture fully written.
+ * The queries are sorted by chart in the workspace
+ * (gstex_texture_sample_backward_workspace_size(num_queries) bytes). */
+size_t gstex_texture_sample_backward_workspace_size(int num_queries);
 int gstex_texture_sample_backward(int num_charts, int num_probs, int channels,
                                   const int32_t *texture_dims, const float *uvs, int num_queries,
                                   const float *texture, int num_texels, const float *v_output,
-                                  float *v_texture, void *stream);
+                                  float *v_texture, void *workspace, size_t workspace_bytes,
+                                  void *stream);
 
 /* ---- Training-step glue ------------------------------------------------
  * The reference does these steps with torch ops in its training loop

@@ -369,6 +369,34 @@ def test_texture_sample_adjoint_one_texel_charts(oracle):
     np.testing.assert_allclose(np_(fwd), oracle.texture_sample(qd, uvs, tex, "f64"), atol=1e-5)
 
 
+@pytest.mark.parametrize("C,max_side,n_charts,q", [(3, 16, 20000, 300000), (40, 16, 3000, 60000),
+                                                   (1, 4, 50, 70000)])
+def test_texture_sample_adjoint_sorted_windows(oracle, C, max_side, n_charts, q):
+    # the adjoint sorts the queries by chart and accumulates each workgroup's
+    # run of charts in an LDS window: many workgroups, charts of 1x1 up to
+    # 16x16, channel counts whose charts overflow the window (40 x 256 values)
+    # and heavily shared charts (50 charts, 70k queries), uv outside [0, 1]
+    from gstex_cuda_amd import cuda as _C
+
+    rng = np.random.default_rng(C)
+    h = rng.integers(1, max_side + 1, n_charts)
+    w = rng.integers(1, max_side + 1, n_charts)
+    start = np.concatenate([[0], np.cumsum(h * w)[:-1]])
+    dims = np.stack([h, w, start], 1).astype(np.int32)
+    X = int((h * w).sum())
+    tex = rng.uniform(0, 1, (X, C)).astype(np.float32)
+    qd = dims[rng.integers(0, n_charts, q)]
+    uvs = rng.uniform(-0.2, 1.2, (q, 2)).astype(np.float32)
+    v = rng.standard_normal((q, C)).astype(np.float32)
+    got = np_(_C.texture_sample_backward((n_charts, 1, C), to_dev(qd), to_dev(uvs), to_dev(tex),
+                                         to_dev(v)))
+    want = oracle.texture_sample_vjp(qd, uvs, v, X, "f64")
+    scale = max(float(np.abs(want).max()), 1.0)
+    log(f"C={C} X={X} q={q}: max err {np.abs(got - want).max():.2e} (scale {scale:.1f})")
+    np.testing.assert_allclose(got, want, atol=2e-6 * scale * (q / n_charts) ** 0.5 + 1e-5)
+    assert np.isfinite(got).all()
+
+
 def test_texture_sample_affine_kat():
     from gstex_cuda_amd import cuda as _C
 
