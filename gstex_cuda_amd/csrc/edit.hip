@@ -49,7 +49,10 @@ __global__ __launch_bounds__(256) void k_texture_edit(RasterArgs a, EditIn ed) {
     tile_range(a, tile, r0, r1);
     bool done = !inside;
     float T = 1.f;
-    const WaveRect rect = wave_rect(inside, j, i);
+    // culling rectangle of the live pixels (shrinks as pixels finish, as in
+    // k_raster_forward)
+    WaveRect rect = wave_rect(inside, j, i);
+    int live = __popcll(__ballot(inside));
     // wave-private texel accumulator, kept zero between Gaussians by the flush
     __shared__ float tex_acc[256 / kWave][kTexAccFloats];
     float* acc = tex_acc[wave_index()];
@@ -63,6 +66,13 @@ __global__ __launch_bounds__(256) void k_texture_edit(RasterArgs a, EditIn ed) {
     const bool cull = a.boxes != nullptr;
     for (int base = r0 & ~63; base < r1; base += 64) {
         if (!__any(!done)) break;
+        {
+            const int now = __popcll(__ballot(!done));
+            if (now < live) {
+                live = now;
+                rect = wave_rect(!done, j, i);
+            }
+        }
         const Window w = window_candidates(a, base, r0, r1 - 1, rect, lane, cull);
         uint64_t cand = w.cand;
         while (cand != 0) {

@@ -67,7 +67,12 @@ __device__ __forceinline__ void raster_forward_wave(const RasterArgs& a, const F
     uint64_t* const mplane =
         a.wave_masks ? a.wave_masks + (size_t)wave * a.mask_plane : nullptr;
     const int mword0 = (r0 >> 6) + tile, mentry0 = r0 & ~63;
-    const WaveRect rect = wave_rect(inside, j, i);
+    // Culling rectangle: the wave's live pixels.  A pixel that is done can
+    // take nothing more, so once pixels finish the rectangle shrinks to the
+    // ones still live (recomputed when their count drops), and an entry that
+    // meets none of them is passed over like any culled entry.
+    WaveRect rect = wave_rect(inside, j, i);
+    int live = __popcll(__ballot(inside));
 
     // A pixel whose T (1 - 1/255) <= 1e-4 can composite nothing more: every
     // entry with alpha >= 1/255 ends it (rounding is monotonic), and ending
@@ -185,6 +190,13 @@ __device__ __forceinline__ void raster_forward_wave(const RasterArgs& a, const F
             ev_noop = !any_comp;
         }
         if (mplane && lane == 0) mplane[mword0 + ((base - mentry0) >> 6)] = mbits;
+        {
+            const int now = __popcll(__ballot(!done));
+            if (now < live) {
+                live = now;
+                rect = wave_rect(!done, j, i);
+            }
+        }
         // what a copy of the window's last entry would do: as the evaluation
         // of that entry found, or nothing if it was culled
         bool dup_run = false;
