@@ -492,12 +492,320 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     }
 }
 
+// ------------------------------------------------------------------ one-sweep sort
+// LSD radix sort in which every pass is one launch: the digit counts of all
+// passes come from one read of the keys beforehand (k_onesweep_hist), and each
+// block finds where its digits start by a decoupled look-back over the
+// blocks before it (k_onesweep_scatter).  A block's tile is the one its
+// ticket names (an atomic counter taken when the block starts), so a block
+// only ever waits for blocks that are already running; the wait ends when
+// the block before it has published its inclusive count.
+//
+// Look-back state of one (block, digit), 64 bits: status in bits 62-63
+// (0 not yet written, aggregate = this block's count, inclusive = the count
+// of this block and every earlier one), count in bits 0-61.
+constexpr uint64_t kLbAggregate = 1ull << 62;
+constexpr uint64_t kLbInclusive = 2ull << 62;
+constexpr uint64_t kLbCount = (1ull << 62) - 1ull;
+// Bound on the polls one thread makes over a whole look-back.  A wait that
+// ran this long would mean a block stopped without publishing: the thread
+// then records it in *err and goes on with the partial sum, which can only
+// be smaller than the true one, so every store stays in bounds.
+constexpr int kLbPollLimit = 1 << 20;
+
+__device__ __forceinline__ uint64_t lb_load(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void lb_store(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum of the counts of blocks 0 .. blk-1 in the chain states[j * stride + d],
+// for one thread: kLbWindow states are read per round trip (blocks j, j-1, ...)
+// and consumed in order up to the first inclusive one, or up to the first
+// not yet written, which is then polled again.  Block 0's states are
+// inclusive, so the walk never passes it.
+constexpr int kLbWindow = 8;
+__device__ __forceinline__ uint64_t lb_exclusive(const uint64_t* states, int blk, int stride,
+                                                 int d, uint32_t* err) {
+    uint64_t sum = 0;
+    int j = blk - 1, polls = 0;
+    while (j >= 0) {
+        uint64_t s[kLbWindow];
+#pragma unroll
+        for (int w = 0; w < kLbWindow; ++w)
+            s[w] = j - w >= 0 ? lb_load(states + (size_t)(j - w) * stride + d) : kLbInclusive;
+        int used = 0;
+        bool done = false, stalled = false;
+#pragma unroll
+        for (int w = 0; w < kLbWindow; ++w) {
+            if (done || stalled) continue;
+            const uint64_t st = s[w] & ~kLbCount;
+            if (st == 0) {
+                stalled = true;
+                continue;
+            }
+            sum += s[w] & kLbCount;
+            ++used;
+            done = st == kLbInclusive;
+        }
+        if (done) break;
+        j -= used;
+        if (stalled) {
+            if (++polls > kLbPollLimit) {
+                atomicOr(err, 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    return sum;
+}
+
+// The same sum for a whole wave at once (one chain, stride 1): lane l reads
+// block j - l, so one round trip covers 64 blocks.  Every lane returns the sum.
+__device__ __forceinline__ uint64_t lb_exclusive_wave(const uint64_t* states, int blk,
+                                                      uint32_t* err) {
+    const int lane = threadIdx.x & (kWave - 1);
+    uint64_t sum = 0;
+    int j = blk - 1, polls = 0;
+    while (j >= 0) {
+        const uint64_t s = j - lane >= 0 ? lb_load(states + (j - lane)) : kLbInclusive;
+        const uint64_t st = s & ~kLbCount;
+        const uint64_t not_ready = __ballot(st == 0), incl = __ballot(st == kLbInclusive);
+        // lanes before the first unwritten state, through the first inclusive one
+        const int first_nr = not_ready ? __ffsll((unsigned long long)not_ready) - 1 : kWave;
+        const int first_in = incl ? __ffsll((unsigned long long)incl) - 1 : kWave;
+        const int take = first_in < first_nr ? first_in + 1 : first_nr;
+        uint64_t part = lane < take ? (s & kLbCount) : 0ull;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) part += __shfl_xor(part, off);
+        sum += part;
+        if (first_in < first_nr) break;
+        j -= take;
+        if (take < kWave) {
+            if (++polls > kLbPollLimit) {
+                if (lane == 0) atomicOr(err, 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    return sum;
+}
+
+constexpr int kOsMaxPasses = 4;  // 32-bit keys, digits of up to 8 bits
+constexpr int kOsMaxBlocks = 1024;  // block tiles up to which the one-sweep passes are used
+constexpr int kOsHistItems = 2048;
+
+// Global digit counts of every pass in one read: hist[p * 256 + d] += items
+// whose pass-p digit is d (hist zeroed beforehand).
+__global__ __launch_bounds__(256) void k_onesweep_hist(int n, const uint32_t* __restrict__ keys,
+                                                       int passes, int bits_per, int end_bit,
+                                                       int per_block,
+                                                       uint32_t* __restrict__ hist) {
+    __shared__ uint32_t cnt[kOsMaxPasses * 256];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < passes * 256; i += 256) cnt[i] = 0;
+    __syncthreads();
+    const size_t lo = (size_t)blockIdx.x * per_block;
+    const size_t hi = min(lo + (size_t)per_block, (size_t)n);
+    for (size_t i = lo + tid; i < hi; i += 256) {
+        const uint32_t k = keys[i];
+        for (int p = 0; p < passes; ++p) {
+            const int shift = p * bits_per;
+            const int bits = min(bits_per, end_bit - shift);
+            atomicAdd(&cnt[p * 256 + ((k >> shift) & ((1u << bits) - 1u))], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < passes * 256; i += 256) {
+        const uint32_t c = cnt[i];
+        if (c) atomicAdd(&hist[i], c);
+    }
+}
+
+// One pass of the one-sweep sort over block tiles of ITEMS * 256 items.  The
+// tile is read, ranked and placed in LDS as in k_radix_scatter; its digit
+// counts are published before the look-back, and each digit's run goes to
+// base(digit) + (items of the digit in earlier tiles) onwards.
+template <int ITEMS>
+__global__ __launch_bounds__(kSortThreads) void k_onesweep_scatter(
+    int n, const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int shift, int bits,
+    const uint32_t* __restrict__ digit_counts, uint64_t* __restrict__ states,
+    uint32_t* __restrict__ ticket, uint32_t* __restrict__ err) {
+    constexpr int kTile = kSortThreads * ITEMS;
+    constexpr int kWaves = kSortThreads / kWave;
+    __shared__ uint32_t gbase[256];
+    __shared__ uint32_t lstart[256];
+    __shared__ uint32_t wrun[kWaves][256];
+    __shared__ uint32_t skeys[kTile];
+    __shared__ uint32_t svals[kTile];
+    __shared__ uint32_t s_blk;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const uint32_t mask = (1u << bits) - 1u;
+    const int ndig = 1 << bits;
+    if (tid == 0) s_blk = atomicAdd(ticket, 1u);
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) wrun[w][tid] = 0;
+    __syncthreads();
+    const int blk = (int)s_blk;
+    const size_t base = (size_t)blk * kTile;
+    const int count = (int)min((size_t)kTile, (size_t)n - base);
+    const int wbase = wave * ITEMS * kWave;
+
+    uint32_t key[ITEMS], val[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        const int li = wbase + r * kWave + lane;
+        key[r] = li < count ? keys_in[base + li] : 0u;
+        val[r] = li < count ? vals_in[base + li] : 0u;
+    }
+    const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    uint32_t rank[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        const bool valid = wbase + r * kWave + lane < count;
+        const uint32_t d = valid ? (key[r] >> shift) & mask : 0u;
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < bits; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t before = (uint32_t)__popcll(peers & lt_mask);
+        const uint32_t seen = valid ? wrun[wave][d] : 0u;
+        if (valid && before == 0) wrun[wave][d] = seen + (uint32_t)__popcll(peers);
+        rank[r] = seen + before;
+    }
+    __syncthreads();
+    uint32_t c = 0, dbase;
+    {
+        // this tile's count of digit tid, published at once for later tiles
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t t = wrun[w][tid];
+            wrun[w][tid] = c;
+            c += t;
+        }
+        if (tid < ndig)
+            lb_store(states + (size_t)blk * 256 + tid,
+                     (blk == 0 ? kLbInclusive : kLbAggregate) | (uint64_t)c);
+        // exclusive scans over the digits of the tile counts (LDS run
+        // starts) and of the global counts (digit bases)
+        const uint32_t g = tid < ndig ? digit_counts[tid] : 0u;
+        uint32_t v = c, gv = g;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const uint32_t t = __shfl_up(v, off), tg = __shfl_up(gv, off);
+            if (lane >= off) {
+                v += t;
+                gv += tg;
+            }
+        }
+        lstart[tid] = v;
+        gbase[tid] = gv;
+        __syncthreads();
+        uint32_t add = 0, gadd = 0;
+        for (int w = 0; w < wave; ++w) {
+            add += lstart[w * kWave + kWave - 1];
+            gadd += gbase[w * kWave + kWave - 1];
+        }
+        __syncthreads();
+        lstart[tid] = v + add - c;
+        dbase = gv + gadd - g;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        if (wbase + r * kWave + lane < count) {
+            const uint32_t d = (key[r] >> shift) & mask;
+            const uint32_t pos = lstart[d] + wrun[wave][d] + rank[r];
+            skeys[pos] = key[r];
+            svals[pos] = val[r];
+        }
+    }
+    if (tid < ndig) {
+        uint64_t before = 0;
+        if (blk > 0) {
+            before = lb_exclusive(states, blk, 256, tid, err);
+            lb_store(states + (size_t)blk * 256 + tid, kLbInclusive | (before + c));
+        }
+        gbase[tid] = dbase + (uint32_t)before;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        const int p = r * kSortThreads + tid;
+        if (p < count) {
+            const uint32_t k = skeys[p];
+            const uint32_t d = (k >> shift) & mask;
+            const uint32_t gp = gbase[d] + (uint32_t)p - lstart[d];
+            keys_out[gp] = k;
+            vals_out[gp] = svals[p];
+        }
+    }
+}
+
+// Workspace of the one-sweep sort after the key and value buffers, in 32-bit
+// words: digit counts (kOsMaxPasses x 256), kOsMaxPasses tickets and the
+// error word (padded to kOsHead words), then the look-back states
+// (kOsMaxPasses x blocks x 256, 8 bytes each), so that one fill clears
+// everything a sort uses.
+constexpr int kOsHead = kOsMaxPasses * 256 + 16;
+static size_t onesweep_state_words(int n) {
+    if (sort_blocks(n) > kOsMaxBlocks) return 0;
+    return (size_t)kOsHead + 2 * (size_t)kOsMaxPasses * sort_blocks(n) * 256;
+}
+
 template <typename K>
 size_t radix_workspace_bytes(int n) {
     const size_t nb = (size_t)sort_blocks(n);
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    return up(sizeof(K) * (size_t)n) + up(sizeof(uint32_t) * (size_t)n) +
-           up(sizeof(uint32_t) * 256 * nb) + up(sizeof(uint32_t) * scan_partials_count((int)(256 * nb))) + 256;
+    const size_t lsd = up(sizeof(uint32_t) * 256 * nb) +
+                       up(sizeof(uint32_t) * scan_partials_count((int)(256 * nb)));
+    const size_t os = sizeof(K) == 4 ? up(sizeof(uint32_t) * onesweep_state_words(n)) : 0;
+    return up(sizeof(K) * (size_t)n) + up(sizeof(uint32_t) * (size_t)n) + std::max(lsd, os) + 256;
+}
+
+// One-sweep sort of (uint32 key, uint32 value) pairs over key bits [0, end_bit).
+static hipError_t onesweep_sort_u32(int n, const uint32_t* keys_in, const uint32_t* vals_in,
+                                    uint32_t* keys_out, uint32_t* vals_out, int end_bit,
+                                    uint32_t* tk, uint32_t* tv, uint32_t* state, hipStream_t s) {
+    const int passes = (end_bit + 7) / 8;
+    const int bits_per = (end_bit + passes - 1) / passes;
+    const int nblocks = sort_blocks(n);
+    const bool small = sort_items(n) == 8;
+    uint32_t* hist = state;
+    uint32_t* tickets = state + kOsMaxPasses * 256;
+    uint32_t* err = tickets + kOsMaxPasses;
+    uint64_t* states = (uint64_t*)(state + kOsHead);
+    hipError_t e = fill<uint32_t>(state, (size_t)kOsHead + 2 * (size_t)passes * nblocks * 256,
+                                  0u, s);
+    if (e != hipSuccess) return e;
+    const int hb = std::min((n + kOsHistItems - 1) / kOsHistItems, 1024);
+    const int per = (n + hb - 1) / hb;
+    k_onesweep_hist<<<hb, 256, 0, s>>>(n, keys_in, passes, bits_per, end_bit, per, hist);
+    const uint32_t* sk = keys_in;
+    const uint32_t* sv = vals_in;
+    for (int pass = 0; pass < passes; ++pass) {
+        const int shift = pass * bits_per;
+        const int bits = std::min(bits_per, end_bit - shift);
+        const bool to_out = ((passes - 1 - pass) % 2) == 0;
+        uint32_t* dk = to_out ? keys_out : tk;
+        uint32_t* dv = to_out ? vals_out : tv;
+        uint64_t* st = states + (size_t)pass * nblocks * 256;
+        if (small)
+            k_onesweep_scatter<8><<<nblocks, kSortThreads, 0, s>>>(
+                n, sk, sv, dk, dv, shift, bits, hist + pass * 256, st, tickets + pass, err);
+        else
+            k_onesweep_scatter<16><<<nblocks, kSortThreads, 0, s>>>(
+                n, sk, sv, dk, dv, shift, bits, hist + pass * 256, st, tickets + pass, err);
+        sk = dk;
+        sv = dv;
+    }
+    return hipGetLastError();
 }
 
 // Stable LSD sort of (key, value) pairs over key bits [0, end_bit).  With all 64
@@ -528,6 +836,15 @@ hipError_t radix_sort_pairs(int n, const K* keys_in, const uint32_t* vals_in, K*
         return hipMemcpyAsync(vals_out, vals_in, sizeof(uint32_t) * (size_t)n,
                               hipMemcpyDeviceToDevice, s);
     }
+    // The one-sweep passes win while the look-back chains are short; over many
+    // block tiles the serialised round trips between a block and the ones
+    // before it (coherent across the chip's L2s) cost more than a separate
+    // histogram and scan.  Measured at 1080p / 500k: the 500k-key depth sort
+    // (245 tiles) 108 -> 78 us, the 11.5M-key tile sort (2805 tiles) 173 -> 197 us.
+    if constexpr (sizeof(K) == 4)
+        if (nblocks <= kOsMaxBlocks)
+            return onesweep_sort_u32(n, (const uint32_t*)keys_in, vals_in, (uint32_t*)keys_out,
+                                     vals_out, end_bit, (uint32_t*)tk, tv, hist, s);
     const int bits_per = (end_bit + passes - 1) / passes;  // balanced digit widths <= 8
     const K flip = (sizeof(K) == 8 && end_bit == 64) ? (K)((K)1 << (8 * sizeof(K) - 1)) : (K)0;
     const K* sk = keys_in;
@@ -631,12 +948,57 @@ __global__ __launch_bounds__(256) void k_bin_prepare(
     order[g] = (uint32_t)g;
 }
 
-__global__ __launch_bounds__(256) void k_gather_written(int n, const uint32_t* __restrict__ order,
-                                                        const int* __restrict__ written,
-                                                        uint32_t* __restrict__ out) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    out[p] = (uint32_t)written[order[p]];
+// offs[p] = sum of written[order[q]] over q < p, and *total = the sum over
+// every p, in one launch: tiles of kWrScanTile positions, 16 consecutive per
+// thread, each tile adding the sum of the tiles before it by a decoupled
+// look-back (ticket order, as in the one-sweep sort).  states holds one
+// look-back state per tile, then the ticket and error words, all zeroed.
+constexpr int kWrScanPer = 16;
+constexpr int kWrScanTile = 256 * kWrScanPer;
+static int wr_scan_tiles(int n) { return (n + kWrScanTile - 1) / kWrScanTile; }
+
+__global__ __launch_bounds__(256) void k_scan_written(int n, const uint32_t* __restrict__ order,
+                                                      const int* __restrict__ written,
+                                                      uint32_t* __restrict__ offs,
+                                                      uint32_t* __restrict__ total,
+                                                      uint64_t* __restrict__ states, int tiles) {
+    __shared__ uint32_t lds[256 / kWave];
+    __shared__ uint32_t s_blk, s_before;
+    uint32_t* const ticket = (uint32_t*)(states + tiles);
+    const int tid = threadIdx.x;
+    if (tid == 0) s_blk = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const int blk = (int)s_blk;
+    const size_t base = (size_t)blk * kWrScanTile + (size_t)tid * kWrScanPer;
+    uint32_t v[kWrScanPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < kWrScanPer; ++j) {
+        v[j] = base + j < (size_t)n ? (uint32_t)written[order[base + j]] : 0u;
+        sum += v[j];
+    }
+    uint32_t tot;
+    const uint32_t excl = block_exclusive_scan(sum, lds, tot);
+    if (tid < kWave) {
+        if (tid == 0)
+            lb_store(states + blk, (blk == 0 ? kLbInclusive : kLbAggregate) | (uint64_t)tot);
+        uint64_t before = 0;
+        if (blk > 0) {
+            before = lb_exclusive_wave(states, blk, ticket + 1);
+            if (tid == 0) lb_store(states + blk, kLbInclusive | (before + tot));
+        }
+        if (tid == 0) {
+            s_before = (uint32_t)before;
+            if ((size_t)(blk + 1) * kWrScanTile >= (size_t)n) *total = (uint32_t)(before + tot);
+        }
+    }
+    __syncthreads();
+    uint32_t run = s_before + excl;
+#pragma unroll
+    for (int j = 0; j < kWrScanPer; ++j) {
+        if (base + j < (size_t)n) offs[base + j] = run;
+        run += v[j];
+    }
 }
 
 // Emission of (tile, id) for every depth-ordered Gaussian after the zero
@@ -648,21 +1010,23 @@ __global__ __launch_bounds__(256) void k_gather_written(int n, const uint32_t* _
 constexpr int kEmitOut = 2048;
 
 // starts[b] = the depth-ordered position whose output range holds output
-// b * kEmitOut, and starts[ceil(total / kEmitOut)] = the last writing
-// position.  Output ranges [offs[p], offs[p] + w) tile [0, total) (writing
-// Gaussians all come before the ones that write nothing), so every entry is
-// written by exactly one lane.  Entries past max_b are not needed.
+// b * out, and starts[ceil(total / out)] = the last writing position.  Output
+// ranges [offs[p], offs[p] + w) tile [0, total) (writing Gaussians all come
+// before the ones that write nothing), so every entry is written by exactly
+// one lane.  Entries past max_b are not needed.
 __global__ __launch_bounds__(256) void k_bin_mark_starts(int n, const uint32_t* __restrict__ offs,
                                                          const uint32_t* __restrict__ total_written,
-                                                         int max_b, uint32_t* __restrict__ starts) {
+                                                         int max_b, int out,
+                                                         uint32_t* __restrict__ starts) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const uint32_t tw = *total_written, o = offs[p];
     const uint32_t end = p + 1 < n ? offs[p + 1] : tw;
     if (end <= o) return;
-    const uint32_t b0 = (o + kEmitOut - 1) / kEmitOut, b1 = (end - 1) / kEmitOut;
+    const uint32_t uo = (uint32_t)out;
+    const uint32_t b0 = (o + uo - 1) / uo, b1 = (end - 1) / uo;
     for (uint32_t b = b0; b <= b1 && b <= (uint32_t)max_b; ++b) starts[b] = (uint32_t)p;
-    const uint32_t last = (tw + kEmitOut - 1) / kEmitOut;
+    const uint32_t last = (tw + uo - 1) / uo;
     if (end == tw && last <= (uint32_t)max_b) starts[last] = (uint32_t)p;
 }
 
@@ -808,7 +1172,7 @@ static BinLayout bin_layout(int n, int m) {
         up(4 * (size_t)n) * 2,  // depth keys in/out
         up(4 * (size_t)n) * 2,  // order in/out
         up(4 * (size_t)n),      // written
-        up(4 * (size_t)n + 8) + up(4 * scan_partials_count(n)),  // offsets (+ totals), partials
+        up(4 * (size_t)n + 8) + up(8 * ((size_t)wr_scan_tiles(n) + 1)),  // offsets (+ totals), scan states
         up(4 * (size_t)m) * 2,  // tile keys in/out
         up(4 * (size_t)m),      // ids in (pre-sort)
         up(std::max(radix_workspace_bytes<uint32_t>(n), radix_workspace_bytes<uint32_t>(m))),
@@ -873,14 +1237,16 @@ hipError_t bin_gaussians(int n, int cap, const float* centers, const float* exte
     if ((e = hipGetLastError()) != hipSuccess) return e;
     e = radix_sort_pairs<uint32_t>(n, dkey, ord, dkey2, ord2, 32, rws, rws_bytes, s);
     if (e != hipSuccess) return e;
-    k_gather_written<<<gn, 256, 0, s>>>(n, ord2, written, offs);
-    if ((e = scan_exclusive(offs, n, total, (uint32_t*)(b + L.off[3] + up(4 * (size_t)n + 8)),
-                            s)) != hipSuccess)
-        return e;
+    {
+        const int st = wr_scan_tiles(n);
+        uint64_t* states = (uint64_t*)(b + L.off[3] + up(4 * (size_t)n + 8));
+        if ((e = fill<uint64_t>(states, (size_t)st + 1, 0ull, s)) != hipSuccess) return e;
+        k_scan_written<<<st, 256, 0, s>>>(n, ord2, written, offs, total, states, st);
+    }
     k_zero_slots<<<std::min(gm, 1024), 256, 0, s>>>(cap, m_total, total, num_tiles, tkey, tid_in);
     const int eb = emit_blocks(cap);
     uint32_t* starts = (uint32_t*)(b + L.off[7]);
-    k_bin_mark_starts<<<gn, 256, 0, s>>>(n, offs, total, eb, starts);
+    k_bin_mark_starts<<<gn, 256, 0, s>>>(n, offs, total, eb, kEmitOut, starts);
     k_bin_emit<<<eb, 256, 0, s>>>(n, cap, m_total, ord2, offs, total, starts, centers, extents,
                                   tiles_x, tiles_y, bw, tkey, tid_in);
     if ((e = hipGetLastError()) != hipSuccess) return e;

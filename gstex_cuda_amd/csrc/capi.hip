@@ -241,6 +241,25 @@ int gstex_sort_intersects(int num_intersects, int end_bit, const int64_t* isect_
                      "sort_intersects");
 }
 
+size_t gstex_sort_pairs_u32_workspace_size(int n) {
+    return n > 0 ? gstex::sort_u32_pairs_workspace_bytes(n) : 0;
+}
+
+int gstex_sort_pairs_u32(int n, int end_bit, const uint32_t* keys, const uint32_t* values,
+                         uint32_t* keys_sorted, uint32_t* values_sorted, void* workspace,
+                         size_t workspace_bytes, void* stream) {
+    REQUIRE(n >= 0, "n must be >= 0");
+    REQUIRE(end_bit >= 0 && end_bit <= 32, "end_bit must be in [0, 32], got %d", end_bit);
+    if (n == 0) return GSTEX_OK;
+    REQUIRE(keys && values && keys_sorted && values_sorted && workspace, "null pointer argument");
+    REQUIRE(workspace_bytes >= gstex::sort_u32_pairs_workspace_bytes(n),
+            "workspace too small: %zu < %zu", workspace_bytes,
+            gstex::sort_u32_pairs_workspace_bytes(n));
+    return check_hip(gstex::sort_u32_pairs(n, keys, values, keys_sorted, values_sorted, end_bit,
+                                           workspace, workspace_bytes, as_stream(stream)),
+                     "sort_pairs_u32");
+}
+
 int gstex_get_tile_bin_edges(int num_intersects, const int64_t* isect_ids_sorted, int tiles_x,
                              int tiles_y, int32_t* tile_bins, void* stream) {
     REQUIRE(num_intersects >= 0 && tiles_x >= 0 && tiles_y >= 0, "negative sizes");

@@ -22,7 +22,8 @@ __all__ = [
     "get_tile_bin_edges", "get_aabb_2d", "texture_forward", "texture_backward",
     "texture_sample_forward", "texture_sample_backward", "texture_edit",
     # additions beyond the reference's ten
-    "project_gaussians", "project_points", "num_tiles_hit", "sort_intersects", "bin_gaussians",
+    "project_gaussians", "project_points", "num_tiles_hit", "sort_intersects", "sort_pairs_u32",
+    "bin_gaussians",
     "build_splat_records", "raster_forward", "raster_backward", "finalize_gradients",
     "set_raster_culling", "set_precise_edge_on",
 ]
@@ -200,6 +201,26 @@ def sort_intersects(isect_ids, gaussian_ids, end_bit=64):
         call("gstex_sort_intersects", m, int(end_bit), _ptr(isect_ids), _ptr(gaussian_ids),
              _ptr(keys_out), _ptr(ids_out), _ptr(ws), nb, _stream(dev))
     return keys_out, ids_out
+
+
+def sort_pairs_u32(keys, values, end_bit=32):
+    """Stable radix sort of (uint32 key, uint32 value) pairs over key bits
+    [0, end_bit); int32 / uint32 tensors of equal length."""
+    _check(keys, "keys")
+    _check(values, "values")
+    n = keys.numel()
+    if values.numel() != n:
+        raise ValueError("keys and values must have the same length")
+    dev = keys.device
+    keys_out = torch.empty_like(keys)
+    values_out = torch.empty_like(values)
+    if n == 0:
+        return keys_out, values_out
+    with torch.cuda.device(dev):
+        ws, nb = _workspace(_lib.lib().gstex_sort_pairs_u32_workspace_size(n), dev)
+        call("gstex_sort_pairs_u32", n, int(end_bit), _ptr(keys), _ptr(values), _ptr(keys_out),
+             _ptr(values_out), _ptr(ws), nb, _stream(dev))
+    return keys_out, values_out
 
 
 def get_tile_bin_edges(num_intersects, isect_ids_sorted, tile_bounds):

@@ -128,6 +128,16 @@ int gstex_sort_intersects(int num_intersects, int end_bit, const int64_t *isect_
                           int32_t *gaussian_ids_sorted, void *workspace, size_t workspace_bytes,
                           void *stream);
 
+/* No counterpart in the reference: the sort that binning and the
+ * texture_sample adjoint use, for callers that sort their own 32-bit keys.
+ * Stable ascending radix sort of (uint32 key, uint32 value) pairs over key
+ * bits [0, end_bit) (end_bit <= 32; higher key bits are carried, not
+ * compared), one launch per 8-bit digit plus one histogram launch. */
+size_t gstex_sort_pairs_u32_workspace_size(int n);
+int gstex_sort_pairs_u32(int n, int end_bit, const uint32_t *keys, const uint32_t *values,
+                         uint32_t *keys_sorted, uint32_t *values_sorted, void *workspace,
+                         size_t workspace_bytes, void *stream);
+
 /* Replaces get_tile_bin_edges_tensor (cuda/csrc/bindings.cu:123-141).
  * tile_bins (tiles_x*tiles_y, 2) int32 is fully written. */
 int gstex_get_tile_bin_edges(int num_intersects, const int64_t *isect_ids_sorted, int tiles_x,

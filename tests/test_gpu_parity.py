@@ -190,6 +190,39 @@ def test_sort_full_64bit_signed():
     assert np.array_equal(np_(is_), ids[order])
 
 
+@pytest.mark.parametrize("n,end_bit,kind", [
+    (1, 32, "random"), (777, 7, "random"), (300_000, 32, "random"), (300_000, 13, "few"),
+    (300_000, 25, "high_bits"), (5_000_000, 14, "few"), (5_000_000, 32, "random"),
+    (4_194_303, 32, "sorted"), (4_194_304, 9, "reversed"), (65_537, 0, "random")])
+def test_sort_pairs_u32_stable(n, end_bit, kind):
+    # the one-sweep sort: tiles of 2048 (n < 4M) and 4096 items, digit widths
+    # 7 to 8, one to four passes, duplicate-heavy keys (stability), keys with
+    # bits above end_bit (carried, not compared) and presorted inputs
+    from gstex_cuda_amd import cuda as _C
+
+    rng = np.random.default_rng(n + end_bit)
+    hi = 1 << end_bit
+    if kind == "few":
+        keys = rng.integers(0, min(hi, 40), n, dtype=np.uint64).astype(np.uint32)
+    elif kind == "sorted":
+        keys = np.sort(rng.integers(0, hi, n, dtype=np.uint64)).astype(np.uint32)
+    elif kind == "reversed":
+        keys = np.sort(rng.integers(0, hi, n, dtype=np.uint64))[::-1].astype(np.uint32)
+    else:
+        keys = rng.integers(0, hi, n, dtype=np.uint64).astype(np.uint32)
+    if kind == "high_bits":
+        keys |= rng.integers(0, 1 << (32 - end_bit), n, dtype=np.uint64).astype(np.uint32) << end_bit
+    vals = rng.permutation(n).astype(np.uint32)
+    log(f"n={n} end_bit={end_bit} {kind}")
+    ks, vs = _C.sort_pairs_u32(to_dev(keys.view(np.int32)), to_dev(vals.view(np.int32)),
+                               end_bit=end_bit)
+    order = np.argsort(keys & np.uint32(hi - 1 if end_bit < 32 else 0xFFFFFFFF), kind="stable")
+    if end_bit == 0:
+        order = np.arange(n)
+    assert np.array_equal(np_(ks).view(np.uint32), keys[order])
+    assert np.array_equal(np_(vs).view(np.uint32), vals[order])
+
+
 def test_empty_image_and_no_intersects(oracle):
     from gstex_cuda_amd.texture import texture_gaussians
 
