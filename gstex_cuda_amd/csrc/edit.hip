@@ -7,8 +7,10 @@
 // of the pair's texel quad (always bilinear, replicate padding) times
 // (r*a, g*a, b*a, a, 1) are added to channels 0..4 of the output, a being the
 // pixel's edit alpha.  The walk is the forward's: quadrant waves, 64-entry
-// windows with screen-box culling, SMEM records; lanes on the same texel quad
-// are merged with DPP before the float atomics.
+// windows with screen-box culling, SMEM records.  The values reach the output
+// by the texel routes of the raster backward: wave reduce-scatters for one
+// texel, a shared texel quad or a 3x3 texel window, else DPP-merged lanes into
+// an LDS accumulator or direct float atomics.
 #include "raster_common.hpp"
 
 namespace gstex {
@@ -84,6 +86,8 @@ __global__ __launch_bounds__(256) void k_texture_edit(RasterArgs a, EditIn ed) {
             bool hit = false;
             float wt[4] = {0.f, 0.f, 0.f, 0.f};
             int row[4] = {0, 0, 0, 0};  // byte offsets inside the chart (float values)
+            float fu = 0.f, fv = 0.f;
+            int iu = 0, iv = 0, ju = 0, jv = 0;  // tap rows (u) and columns (v)
             // out has the texture's (X, C) float layout; charts outside it get nothing
             const Chart ch = chart_of<false>(ed.out, s, C, a.num_texels);
             if (!done) {
@@ -94,9 +98,9 @@ __global__ __launch_bounds__(256) void k_texture_edit(RasterArgs a, EditIn ed) {
                     const float t_view = p.t * pr.view_depth;
                     if (t_view >= lo && t_view <= hi) {
                         hit = true;
-                        float u, v, fu, fv;
+                        float u, v;
                         splat_uv(s, p.delta, u, v);
-                        chart_taps(s, ch, u, v, true, wt, row, fu, fv);
+                        chart_taps_ij(s, ch, u, v, true, wt, row, fu, fv, iu, iv, ju, jv);
                     }
                     T = next_T;
                     if (T * kKeep <= 1e-4f) done = true;
@@ -104,6 +108,87 @@ __global__ __launch_bounds__(256) void k_texture_edit(RasterArgs a, EditIn ed) {
             }
             if (!__any(hit) || !ch.ok) continue;
             float* const out_chart = ed.out + (size_t)s.start * C;
+            // How the (taps x 5) values reach the output (wave-uniform choice,
+            // as for the texel gradients of the raster backward):
+            //   single  1x1 chart: 5 values in a wave reduce-scatter
+            //   quad    every hitting lane samples the same texel quad: 20 values
+            //   window  every hitting lane's taps lie in one 3x3 texel window:
+            //           the 9 separable weight products x 5 values
+            //   lds     charts of up to kTexAccFloats / 5 texels: merged lanes
+            //           add into the wave's LDS accumulator, then flush
+            //   direct  one atomic per tap and value
+            // Lanes that do not hit have zero weights, so they add nothing.
+            const int first = __ffsll((unsigned long long)__ballot(hit)) - 1;
+            if (s.h * s.w == 1) {
+                const float ws = (wt[0] + wt[1]) + (wt[2] + wt[3]);
+                float v5[5];
+#pragma unroll
+                for (int k = 0; k < 5; ++k) v5[k] = ws * val[k];
+                const float tot = wave_reduce_scatter(v5, lane);
+                const int code = rs_slot<5>(lane);
+                if (code < 64 && tot != 0.f) unsafeAtomicAdd(out_chart + code, tot);
+                continue;
+            }
+            const int row0 = __builtin_amdgcn_readlane(row[0], first);
+            if (!__any(hit && row[0] != row0)) {
+                const int row1 = __builtin_amdgcn_readlane(row[1], first);
+                const int row2 = __builtin_amdgcn_readlane(row[2], first);
+                const int row3 = __builtin_amdgcn_readlane(row[3], first);
+                float v20[20];
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) v20[5 * t + k] = wt[t] * val[k];
+                const float tot = wave_reduce_scatter(v20, lane);
+                const int code = rs_slot<20>(lane);
+                if (code < 64 && tot != 0.f) {
+                    const int t = code / 5, k = code - 5 * t;
+                    const int rr = t == 0 ? row0 : t == 1 ? row1 : t == 2 ? row2 : row3;
+                    unsafeAtomicAdd(out_chart + (rr >> 2) + k, tot);
+                }
+                continue;
+            }
+            {
+                // window origin: the first hitting lane's tap row / column, one
+                // lower when some hitting lane lies below it
+                const int au = __builtin_amdgcn_readlane(iu, first);
+                const int av = __builtin_amdgcn_readlane(iv, first);
+                const int ou = au - (__any(hit && iu < au) ? 1 : 0);
+                const int ov = av - (__any(hit && iv < av) ? 1 : 0);
+                if (!__any(hit && (iu < ou || iu > ou + 1 || iv < ov || iv > ov + 1))) {
+                    // tap (row r, column q) has weight A[r] * B[q]; rows iu, ju and
+                    // columns iv, jv relative to the origin lie in 0..2
+                    const float h1 = hit ? 1.f : 0.f;
+                    const float wa0 = h1 * (1.f - fu), wa1 = h1 * fu, wb0 = 1.f - fv, wb1 = fv;
+                    const int ri = iu - ou, rj = ju - ou, ci = iv - ov, cj = jv - ov;
+                    float A[3], Bw[3];
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                        A[r] = (ri == r ? wa0 : 0.f) + (rj == r ? wa1 : 0.f);
+                        Bw[r] = (ci == r ? wb0 : 0.f) + (cj == r ? wb1 : 0.f);
+                    }
+                    float v45[45];
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+#pragma unroll
+                        for (int q = 0; q < 3; ++q) {
+                            const float ab = A[r] * Bw[q];
+#pragma unroll
+                            for (int k = 0; k < 5; ++k) v45[(3 * r + q) * 5 + k] = ab * val[k];
+                        }
+                    const float tot = wave_reduce_scatter(v45, lane);
+                    const int code = rs_slot<45>(lane);
+                    if (code < 64 && tot != 0.f) {
+                        const int rq = code / 5, k = code - 5 * rq;
+                        const int r = rq / 3, q = rq - 3 * r;
+                        // a window row or column past the chart edge has no weight
+                        if (ou + r < s.h && ov + q < s.w)
+                            unsafeAtomicAdd(out_chart + ((size_t)(ou + r) * s.w + (ov + q)) * C + k,
+                                            tot);
+                    }
+                    continue;
+                }
+            }
             const QuadMerge qm = quad_merge_setup(hit ? row[0] : -1, lane);
             if (s.h * s.w * 5 <= kTexAccFloats) {
                 // charts of up to kTexAccFloats / 5 texels: merged lanes add

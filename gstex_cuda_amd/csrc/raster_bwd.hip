@@ -1,113 +1,17 @@
 // Back-to-front adjoint of the tile rasteriser (texture.cu:331-760 semantics)
 // and the per-Gaussian gradient finalize.
 //
-// Per (wave, Gaussian) the lanes' contributions are summed with one butterfly
-// reduce-scatter over 32 values built from permlane swaps and DPP lane
-// permutes (all VALU, no LDS traffic); every other lane then holds the wave
-// total of one value and issues its atomic.  The 32 values are 20 per-Gaussian scalars from which the finalize
-// kernel rebuilds every parameter gradient, plus up to 12 texel-gradient
-// values.  Using delta = l1*ax1 + l2*ax2 (the ray hits the splat plane), each
-// outer product sum_pixels v * delta becomes ax1 * sum(v*l1) + ax2 * sum(v*l2).
+// Per (wave, Gaussian) the lanes' contributions are summed with butterfly
+// reduce-scatters built from permlane swaps and DPP lane permutes (all VALU,
+// no LDS traffic), after which a lane holds the wave total of one value and
+// issues its atomic: one over the 20 per-Gaussian scalars from which the
+// finalize kernel rebuilds every parameter gradient, one over the texel
+// values of the chart's route.  Using delta = l1*ax1 + l2*ax2 (the ray hits
+// the splat plane), each outer product sum_pixels v * delta becomes
+// ax1 * sum(v*l1) + ax2 * sum(v*l2).
 #include "raster_common.hpp"
 
 namespace gstex {
-
-// ------------------------------------------------------------------ wave reduction
-// One halving round: partners exchange the half of the values they give up.
-template <int CTRL, int NV, int H>
-__device__ __forceinline__ void halve(float (&v)[NV], bool upper) {
-#pragma unroll
-    for (int k = 0; k < H; ++k) {
-        const float lo = v[k] + dpp<CTRL>(v[k]);
-        const float hi = v[k + H] + dpp<CTRL>(v[k + H]);
-        v[k] = upper ? hi : lo;
-    }
-}
-
-// One halving round over lane bit 5 (bit 4) by a permlane swap: lanes with
-// the bit clear end with a summed with the partner lane's a, lanes with it set
-// with b summed with the partner's b.  The swap moves the halves each lane
-// gives up in one instruction, so a pair costs a swap and an add.
-__device__ __forceinline__ float swap32_sum(float a, float b) {
-    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b),
-                                                    false, false);
-    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
-}
-__device__ __forceinline__ float swap16_sum(float a, float b) {
-    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b),
-                                                    false, false);
-    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
-}
-
-// Reduce-scatter of N <= 64 per-lane values over the wave in six halving
-// rounds: permlane swaps over lane bits 5 and 4, then DPP lane permutes with
-// partner masks 15, 7, 2 and 1 selecting on bits 3, 2, 1 and 0.  Each partner
-// mask leaves the bits already selected on unchanged, and together they span
-// all six lane bits, so every lane ends with the wave total of one value.  A
-// round over n values pairs value k with value k + n - n/2 (the lane with the
-// bit clear keeps the first, the lane with it set the second); with n odd the
-// middle value is summed by both lanes and kept by both.  Which value a lane
-// ends with, and whether it is the lowest lane holding it, is rs_slot<N>.
-template <int N, int ROUND>
-struct ReduceScatter {
-    static constexpr int H = N / 2, NP = N - H;
-    __device__ __forceinline__ static float run(const float (&v)[N], int lane) {
-        float w[NP];
-        if constexpr (ROUND == 0) {
-#pragma unroll
-            for (int k = 0; k < H; ++k) w[k] = swap32_sum(v[k], v[k + NP]);
-            if constexpr (NP > H) w[H] = swap32_sum(v[H], v[H]);
-        } else if constexpr (ROUND == 1) {
-#pragma unroll
-            for (int k = 0; k < H; ++k) w[k] = swap16_sum(v[k], v[k + NP]);
-            if constexpr (NP > H) w[H] = swap16_sum(v[H], v[H]);
-        } else {
-            constexpr int CTRL = ROUND == 2   ? kDppRowMirror
-                                 : ROUND == 3 ? kDppRowHalfMirror
-                                 : ROUND == 4 ? kDppQuadXor2
-                                              : kDppQuadXor1;
-            const bool upper = (lane >> (5 - ROUND)) & 1;
-#pragma unroll
-            for (int k = 0; k < H; ++k) {
-                const float lo = v[k] + dpp<CTRL>(v[k]);
-                const float hi = v[k + NP] + dpp<CTRL>(v[k + NP]);
-                w[k] = upper ? hi : lo;
-            }
-            if constexpr (NP > H) w[H] = v[H] + dpp<CTRL>(v[H]);
-        }
-        if constexpr (ROUND == 5) {
-            static_assert(NP == 1, "at most 64 values");
-            return w[0];
-        } else {
-            return ReduceScatter<NP, ROUND + 1>::run(w, lane);
-        }
-    }
-};
-
-template <int N>
-__device__ __forceinline__ float wave_reduce_scatter(const float (&v)[N], int lane) {
-    return ReduceScatter<N, 0>::run(v, lane);
-}
-
-// The value index lane ends with after wave_reduce_scatter<N>, and whether
-// it is the lowest lane holding that index (bit 6 set when it is not):
-// traced back from the last round to the first.
-template <int N>
-__device__ __forceinline__ int rs_slot(int lane) {
-    int ns[6];
-    ns[0] = N;
-#pragma unroll
-    for (int r = 1; r < 6; ++r) ns[r] = ns[r - 1] - ns[r - 1] / 2;
-    int pos = 0, dup = 0;
-#pragma unroll
-    for (int r = 5; r >= 0; --r) {
-        const int h = ns[r] / 2, np = ns[r] - h;
-        const int bit = (lane >> (5 - r)) & 1;
-        if (pos < h) pos += bit ? np : 0;
-        else dup |= bit;
-    }
-    return pos | (dup << 6);
-}
 
 // ------------------------------------------------------------------ backward
 // MODE 0 compiles out the blur (settings bit 9), NDC distortion (bit 10)

@@ -419,6 +419,92 @@ __device__ __forceinline__ float quad_merge(float q, const QuadMerge& m) {
     return q;
 }
 
+// ------------------------------------------------------------------ wave reduction
+// One halving round over lane bit 5 (bit 4) by a permlane swap: lanes with
+// the bit clear end with a summed with the partner lane's a, lanes with it set
+// with b summed with the partner's b.  The swap moves the halves each lane
+// gives up in one instruction, so a pair costs a swap and an add.
+__device__ __forceinline__ float swap32_sum(float a, float b) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b),
+                                                    false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float swap16_sum(float a, float b) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b),
+                                                    false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+// Reduce-scatter of N <= 64 per-lane values over the wave in six halving
+// rounds: permlane swaps over lane bits 5 and 4, then DPP lane permutes with
+// partner masks 15, 7, 2 and 1 selecting on bits 3, 2, 1 and 0.  Each partner
+// mask leaves the bits already selected on unchanged, and together they span
+// all six lane bits, so every lane ends with the wave total of one value.  A
+// round over n values pairs value k with value k + n - n/2 (the lane with the
+// bit clear keeps the first, the lane with it set the second); with n odd the
+// middle value is summed by both lanes and kept by both.  Which value a lane
+// ends with, and whether it is the lowest lane holding it, is rs_slot<N>.
+template <int N, int ROUND>
+struct ReduceScatter {
+    static constexpr int H = N / 2, NP = N - H;
+    __device__ __forceinline__ static float run(const float (&v)[N], int lane) {
+        float w[NP];
+        if constexpr (ROUND == 0) {
+#pragma unroll
+            for (int k = 0; k < H; ++k) w[k] = swap32_sum(v[k], v[k + NP]);
+            if constexpr (NP > H) w[H] = swap32_sum(v[H], v[H]);
+        } else if constexpr (ROUND == 1) {
+#pragma unroll
+            for (int k = 0; k < H; ++k) w[k] = swap16_sum(v[k], v[k + NP]);
+            if constexpr (NP > H) w[H] = swap16_sum(v[H], v[H]);
+        } else {
+            constexpr int CTRL = ROUND == 2   ? kDppRowMirror
+                                 : ROUND == 3 ? kDppRowHalfMirror
+                                 : ROUND == 4 ? kDppQuadXor2
+                                              : kDppQuadXor1;
+            const bool upper = (lane >> (5 - ROUND)) & 1;
+#pragma unroll
+            for (int k = 0; k < H; ++k) {
+                const float lo = v[k] + dpp<CTRL>(v[k]);
+                const float hi = v[k + NP] + dpp<CTRL>(v[k + NP]);
+                w[k] = upper ? hi : lo;
+            }
+            if constexpr (NP > H) w[H] = v[H] + dpp<CTRL>(v[H]);
+        }
+        if constexpr (ROUND == 5) {
+            static_assert(NP == 1, "at most 64 values");
+            return w[0];
+        } else {
+            return ReduceScatter<NP, ROUND + 1>::run(w, lane);
+        }
+    }
+};
+
+template <int N>
+__device__ __forceinline__ float wave_reduce_scatter(const float (&v)[N], int lane) {
+    return ReduceScatter<N, 0>::run(v, lane);
+}
+
+// The value index lane ends with after wave_reduce_scatter<N>, and whether
+// it is the lowest lane holding that index (bit 6 set when it is not):
+// traced back from the last round to the first.
+template <int N>
+__device__ __forceinline__ int rs_slot(int lane) {
+    int ns[6];
+    ns[0] = N;
+#pragma unroll
+    for (int r = 1; r < 6; ++r) ns[r] = ns[r - 1] - ns[r - 1] / 2;
+    int pos = 0, dup = 0;
+#pragma unroll
+    for (int r = 5; r >= 0; --r) {
+        const int h = ns[r] / 2, np = ns[r] - h;
+        const int bit = (lane >> (5 - r)) & 1;
+        if (pos < h) pos += bit ? np : 0;
+        else dup |= bit;
+    }
+    return pos | (dup << 6);
+}
+
 // Index of this wave within the workgroup, as a wave-uniform (SGPR) value:
 // divergence analysis treats any function of threadIdx.x as per-lane, which
 // would turn loads addressed by it into vector loads.

@@ -24,6 +24,9 @@ CASES = {
     # 1x1..16x16 charts of workload 2 take both routes (h w 8 <= 1024 or not)
     "w2_c8": (2, 3000, 160, 90, 16, 0, 8),
     "w1_c8_bw8": (1, 1500, 80, 56, 8, 0, 8),
+    # charts of 2 or 3 texels a side: most pairs take the shared-quad and
+    # 3x3-window routes (wave reduce-scatters over 20 and 45 values)
+    "w2_small_charts": (2, 3000, 160, 90, 16, 0, 5),
 }
 
 
@@ -34,8 +37,13 @@ def _case(name):
     spec = dataclasses.replace(synthetic.WORKLOADS[wl], block_width=bw)
     s = synthetic.make_scene(wl, seed=20 + sum(map(ord, name)), num_points=n, img_w=W, img_h=H,
                              spec=spec)
-    s.texture = np.zeros((s.texture.shape[0], C), np.float32)
     rng = np.random.default_rng(sum(map(ord, name)))
+    if name.endswith("small_charts"):
+        hw = rng.integers(2, 4, (s.num_points, 2))
+        start = np.concatenate([[0], np.cumsum(hw[:, 0] * hw[:, 1])[:-1]])
+        s.texture_dims = np.stack([hw[:, 0], hw[:, 1], start], 1).astype(np.int32)
+        s.texture = np.zeros((int((hw[:, 0] * hw[:, 1]).sum()), C), np.float32)
+    s.texture = np.zeros((s.texture.shape[0], C), np.float32)
     img = rng.random((H, W, 3)).astype(np.float32)
     al = (rng.random((H, W)) * (rng.random((H, W)) < 0.7)).astype(np.float32)
     lo = rng.uniform(0.0, 4.0, (H, W)).astype(np.float32)
