@@ -75,7 +75,8 @@ class RenderStep:
     def __init__(self, params: Dict[str, torch.Tensor], cam: Camera, block_width: int = 16,
                  settings: int = 1 << 8, background: Optional[torch.Tensor] = None,
                  sh_degree: int = -1, use_wave_masks: bool = True,
-                 capacity: Optional[int] = None, culling: Optional[bool] = None):
+                 capacity: Optional[int] = None, culling: Optional[bool] = None,
+                 overlap: bool = True):
         self.p = params
         self.culling = culling
         self.cam = cam
@@ -151,6 +152,14 @@ class RenderStep:
         self.cap = capacity
         self.m_dev = torch.zeros((1,), **i32)
         self.graph = None
+        # With overlap, the work that needs neither the projection nor the
+        # binning (SH colours, splat records, zeroing the gradient buffers)
+        # runs on a second stream (default priority) beside them; the raster
+        # launches wait for it through events.
+        self._side = torch.cuda.Stream(dev) if overlap else None
+        self._ev_records = torch.cuda.Event() if overlap else None
+        self._ev_zeroed = torch.cuda.Event() if overlap else None
+        self._zeroed = False
         if capacity is not None:
             self._ensure_bins(int(capacity))
 
@@ -188,17 +197,43 @@ class RenderStep:
         self.forward_pre()
         self.forward_raster(timer)
 
-    def forward_pre(self):
-        """SH colours, projection, binning and splat records (everything of
-        the forward before the raster launch)."""
-        p, cam, s = self.p, self.cam, self._stream()
-        n = self.n
+    def _zero_grads(self, s):
+        call("gstex_fill_f32", _p(self.grad_records), self.grad_records.numel(), 0.0, s)
+        if self.zero_texture_grad:
+            call("gstex_fill_f32", _p(self.grads["v_texture"]), self.grads["v_texture"].numel(),
+                 0.0, s)
+
+    def _colors_and_records(self, s):
+        p, cam, n = self.p, self.cam, self.n
         if self.sh_degree >= 0:
             torch.sub(p["means"], cam.c2w[:3, 3], out=self._viewdirs)
             fn = ("gstex_compute_sh_forward_bf16" if p["sh_coeffs"].dtype == torch.bfloat16
                   else "gstex_compute_sh_forward")
             call(fn, n, self.sh_degree, self.sh_degree, _p(self._viewdirs), _p(p["sh_coeffs"]),
                  _p(self.colors), s)
+        call("gstex_build_splat_records", n, _p(p["means"]), _p(p["scales"]), 1.0,
+             _p(p["quats"]), _p(self.colors), _p(p["opacities"]), _p(p["uv0"]), _p(p["umap"]),
+             _p(p["vmap"]), _p(p["texture_dims"]), _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy,
+             cam.cx, cam.cy, _p(self.records), s)
+
+    def forward_pre(self):
+        """SH colours, projection, binning and splat records (everything of
+        the forward before the raster launch)."""
+        p, cam, s = self.p, self.cam, self._stream()
+        n = self.n
+        main = torch.cuda.current_stream(self.dev)
+        side = self._side
+        if side is not None:
+            # SH colours, splat records and the gradient zeroing beside the
+            # projection and binning; the side stream first waits for
+            # everything already queued (parameter updates, the last step)
+            side.wait_stream(main)
+            with torch.cuda.stream(side):
+                self._colors_and_records(side.cuda_stream)
+                self._ev_records.record(side)
+                self._zero_grads(side.cuda_stream)
+                self._ev_zeroed.record(side)
+            self._zeroed = True
         call("gstex_project_gaussians", n, _p(p["means"]), _p(p["scales"]), 1.0, _p(p["quats"]),
              _p(cam.viewmat), cam.fx, cam.fy, cam.cx, cam.cy, cam.img_h, cam.img_w, self.bw,
              _p(self.centers), _p(self.extents), _p(self.depths), _p(self.tiles_hit), s)
@@ -219,10 +254,10 @@ class RenderStep:
             else:
                 self.tile_bins.zero_()
         self._list_len = m
-        call("gstex_build_splat_records", n, _p(p["means"]), _p(p["scales"]), 1.0,
-             _p(p["quats"]), _p(self.colors), _p(p["opacities"]), _p(p["uv0"]), _p(p["umap"]),
-             _p(p["vmap"]), _p(p["texture_dims"]), _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy,
-             cam.cx, cam.cy, _p(self.records), s)
+        if side is None:
+            self._colors_and_records(s)
+        else:
+            main.wait_event(self._ev_records)
 
     def forward_raster(self, timer: Optional[StageTimer] = None):
         p, cam, s = self.p, self.cam, self._stream()
@@ -281,9 +316,12 @@ class RenderStep:
                  v_alpha=None, v_normal=None):
         p, cam, s, g = self.p, self.cam, self._stream(), self.grads
         n, m = self.n, self._list_len
-        call("gstex_fill_f32", _p(self.grad_records), self.grad_records.numel(), 0.0, s)
-        if self.zero_texture_grad:
-            call("gstex_fill_f32", _p(g["v_texture"]), g["v_texture"].numel(), 0.0, s)
+        if self._zeroed:
+            # zeroed on the side stream during the forward
+            torch.cuda.current_stream(self.dev).wait_event(self._ev_zeroed)
+            self._zeroed = False
+        else:
+            self._zero_grads(s)
         v_depth = self.v_zero1 if v_depth is None else v_depth
         v_reg = self.v_zero1 if v_reg is None else v_reg
         v_alpha = self.v_zero1 if v_alpha is None else v_alpha
@@ -394,8 +432,10 @@ class MultiViewStep:
                  use_wave_masks: bool = True, streams: int = 4, balance: bool = True,
                  queue: bool = False):
         self.targets = list(targets)
+        # the views' concurrency comes from the batch's streams, not from
+        # each view's own side stream
         kw = dict(block_width=block_width, settings=settings, background=background,
-                  sh_degree=sh_degree, use_wave_masks=use_wave_masks)
+                  sh_degree=sh_degree, use_wave_masks=use_wave_masks, overlap=False)
         self.views = []
         for cam, tgt in zip(cams, self.targets):
             probe = RenderStep(params, cam, **kw)
