@@ -196,14 +196,19 @@ __device__ __forceinline__ uint32_t digit_of(K key, int shift, uint32_t mask) {
 
 // Per-block digit histogram of one block tile (ITEMS * 256 items), stored
 // digit-major (hist[d * nblocks + b]) so one exclusive scan yields scatter
-// bases.
+// bases.  Block 0 also zeroes `nzero` words at `zero`.
 template <typename K, int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(int n, const K* __restrict__ keys,
                                                              K flip, int shift, int bits,
                                                              uint32_t* __restrict__ hist,
-                                                             int nblocks) {
+                                                             int nblocks,
+                                                             uint64_t* __restrict__ zero,
+                                                             int nzero) {
     __shared__ uint32_t cnt[256];
     const int tid = threadIdx.x;
+    // block 0 also clears the look-back states of the scan that follows
+    if (blockIdx.x == 0)
+        for (int i = tid; i < nzero; i += kSortThreads) zero[i] = 0ull;
     cnt[tid] = 0;
     __syncthreads();
     const uint32_t mask = (1u << bits) - 1u;
@@ -748,6 +753,75 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep_scatter(
     }
 }
 
+// Tiles of the single-pass look-back scans: kWrScanPer consecutive values per
+// thread, 256 threads.
+constexpr int kWrScanPer = 16;
+constexpr int kWrScanTile = 256 * kWrScanPer;
+static int wr_scan_tiles(int n) { return (n + kWrScanTile - 1) / kWrScanTile; }
+
+// In-place exclusive scan of n uint32 values in one launch: tiles of
+// kWrScanTile values, 16 consecutive per thread, each tile adding the sum of
+// the tiles before it by a wave-wide decoupled look-back (ticket order).
+// states: one look-back state per tile, then the ticket and error words, all
+// zeroed beforehand.
+__global__ __launch_bounds__(256) void k_scan_lookback(uint32_t* __restrict__ a, int n,
+                                                       uint64_t* __restrict__ states, int tiles) {
+    __shared__ uint32_t lds[256 / kWave];
+    __shared__ uint32_t s_blk, s_before;
+    uint32_t* const ticket = (uint32_t*)(states + tiles);
+    const int tid = threadIdx.x;
+    if (tid == 0) s_blk = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const int blk = (int)s_blk;
+    const size_t base = (size_t)blk * kWrScanTile + (size_t)tid * kWrScanPer;
+    uint32_t v[kWrScanPer];
+    uint32_t sum = 0;
+    if (base + kWrScanPer <= (size_t)n) {
+#pragma unroll
+        for (int q = 0; q < kWrScanPer / 4; ++q) {
+            const uint4 x = reinterpret_cast<const uint4*>(a + base)[q];
+            v[4 * q] = x.x;
+            v[4 * q + 1] = x.y;
+            v[4 * q + 2] = x.z;
+            v[4 * q + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kWrScanPer; ++j) v[j] = base + j < (size_t)n ? a[base + j] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kWrScanPer; ++j) sum += v[j];
+    uint32_t tot;
+    const uint32_t excl = block_exclusive_scan(sum, lds, tot);
+    if (tid < kWave) {
+        if (tid == 0)
+            lb_store(states + blk, (blk == 0 ? kLbInclusive : kLbAggregate) | (uint64_t)tot);
+        uint64_t before = 0;
+        if (blk > 0) {
+            before = lb_exclusive_wave(states, blk, ticket + 1);
+            if (tid == 0) lb_store(states + blk, kLbInclusive | (before + tot));
+        }
+        if (tid == 0) s_before = (uint32_t)before;
+    }
+    __syncthreads();
+    uint32_t run = s_before + excl;
+    if (base + kWrScanPer <= (size_t)n) {
+#pragma unroll
+        for (int q = 0; q < kWrScanPer / 4; ++q) {
+            const uint32_t o0 = run, o1 = o0 + v[4 * q], o2 = o1 + v[4 * q + 1],
+                           o3 = o2 + v[4 * q + 2];
+            reinterpret_cast<uint4*>(a + base)[q] = make_uint4(o0, o1, o2, o3);
+            run = o3 + v[4 * q + 3];
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kWrScanPer; ++j) {
+            if (base + j < (size_t)n) a[base + j] = run;
+            run += v[j];
+        }
+    }
+}
+
 // Workspace of the one-sweep sort after the key and value buffers, in 32-bit
 // words: digit counts (kOsMaxPasses x 256), kOsMaxPasses tickets and the
 // error word (padded to kOsHead words), then the look-back states
@@ -764,7 +838,7 @@ size_t radix_workspace_bytes(int n) {
     const size_t nb = (size_t)sort_blocks(n);
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t lsd = up(sizeof(uint32_t) * 256 * nb) +
-                       up(sizeof(uint32_t) * scan_partials_count((int)(256 * nb)));
+                       up(sizeof(uint64_t) * scan_partials_count((int)(256 * nb)));
     const size_t os = sizeof(K) == 4 ? up(sizeof(uint32_t) * onesweep_state_words(n)) : 0;
     return up(sizeof(K) * (size_t)n) + up(sizeof(uint32_t) * (size_t)n) + std::max(lsd, os) + 256;
 }
@@ -855,14 +929,24 @@ hipError_t radix_sort_pairs(int n, const K* keys_in, const uint32_t* vals_in, K*
         const bool to_out = ((passes - 1 - pass) % 2) == 0;
         K* dk = to_out ? keys_out : tk;
         uint32_t* dv = to_out ? vals_out : tv;
+        // past one workgroup's reach, the digit counts are scanned in one
+        // launch with a look-back (its states cleared by the histogram kernel)
+        const int hcount = (1 << bits) * nblocks;
+        const bool lb = hcount > kScanSingleMax;
+        const int lbt = wr_scan_tiles(hcount);
+        uint64_t* lbs = lb ? reinterpret_cast<uint64_t*>(partials) : nullptr;
         if (small)
             k_radix_hist<K, 8><<<nblocks, kSortThreads, 0, s>>>(n, sk, flip, shift, bits, hist,
-                                                                 nblocks);
+                                                                 nblocks, lbs, lb ? lbt + 1 : 0);
         else
             k_radix_hist<K, 16><<<nblocks, kSortThreads, 0, s>>>(n, sk, flip, shift, bits, hist,
-                                                                  nblocks);
-        hipError_t es = scan_exclusive(hist, (1 << bits) * nblocks, nullptr, partials, s);
-        if (es != hipSuccess) return es;
+                                                                  nblocks, lbs, lb ? lbt + 1 : 0);
+        if (lb) {
+            k_scan_lookback<<<lbt, 256, 0, s>>>(hist, hcount, lbs, lbt);
+        } else {
+            hipError_t es = scan_exclusive(hist, hcount, nullptr, partials, s);
+            if (es != hipSuccess) return es;
+        }
         if (small)
             k_radix_scatter<K, 8><<<nblocks, kSortThreads, 0, s>>>(n, sk, sv, dk, dv, flip, shift,
                                                                     bits, hist, nblocks);
@@ -953,9 +1037,6 @@ __global__ __launch_bounds__(256) void k_bin_prepare(
 // thread, each tile adding the sum of the tiles before it by a decoupled
 // look-back (ticket order, as in the one-sweep sort).  states holds one
 // look-back state per tile, then the ticket and error words, all zeroed.
-constexpr int kWrScanPer = 16;
-constexpr int kWrScanTile = 256 * kWrScanPer;
-static int wr_scan_tiles(int n) { return (n + kWrScanTile - 1) / kWrScanTile; }
 
 __global__ __launch_bounds__(256) void k_scan_written(int n, const uint32_t* __restrict__ order,
                                                       const int* __restrict__ written,
