@@ -41,6 +41,7 @@ _SIGNATURES = {
     "gstex_sort_intersects_workspace_size": (Z, [I]),
     "gstex_sort_intersects": (I, [I, I, P, P, P, P, P, Z, P]),
     "gstex_sort_pairs_u32_workspace_size": (Z, [I]),
+    "gstex_count_intersects": (I, [I, P, P, P]),
     "gstex_sort_pairs_u32": (I, [I, I, P, P, P, P, P, Z, P]),
     "gstex_get_tile_bin_edges": (I, [I, P, I, I, P, P]),
     "gstex_bin_gaussians_workspace_size": (Z, [I, I]),

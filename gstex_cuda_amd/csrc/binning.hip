@@ -1271,6 +1271,13 @@ static BinLayout bin_layout(int n, int m) {
 
 size_t bin_gaussians_workspace_bytes(int n, int m) { return bin_layout(n, m).total; }
 
+hipError_t count_intersects(int n, const int* counts, unsigned* out, hipStream_t s) {
+    hipError_t e = fill<uint32_t>(out, 1, 0u, s);
+    if (e != hipSuccess || n <= 0) return e;
+    k_count_total<<<std::min((n + 255) / 256, kCountBlocks), 256, 0, s>>>(n, counts, out);
+    return hipGetLastError();
+}
+
 hipError_t bin_gaussians(int n, int cap, const float* centers, const float* extents,
                          const float* depths, const int* counts, int tiles_x, int tiles_y, int bw,
                          int* ids_sorted, int* tile_bins, unsigned* m_out, void* ws,

@@ -298,6 +298,15 @@ int gstex_bin_gaussians(int num_points, int num_intersects, const float* centers
                      "bin_gaussians");
 }
 
+int gstex_count_intersects(int num_points, const int32_t* num_tiles_hit,
+                           uint32_t* num_intersects_out, void* stream) {
+    REQUIRE(num_points >= 0, "negative sizes");
+    REQUIRE(num_intersects_out && (num_points == 0 || num_tiles_hit), "null pointer argument");
+    return check_hip(gstex::count_intersects(num_points, num_tiles_hit, num_intersects_out,
+                                             as_stream(stream)),
+                     "count_intersects");
+}
+
 int gstex_bin_gaussians_capped(int num_points, int capacity, const float* centers,
                                const float* extents, const float* depths,
                                const int32_t* num_tiles_hit, int tiles_x, int tiles_y,

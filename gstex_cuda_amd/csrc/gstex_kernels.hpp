@@ -69,6 +69,7 @@ hipError_t sort_u32_pairs(int n, const uint32_t* keys_in, const uint32_t* vals_i
                           uint32_t* keys_out, uint32_t* vals_out, int end_bit, void* ws,
                           size_t ws_bytes, hipStream_t s);
 size_t bin_gaussians_workspace_bytes(int n, int m);
+hipError_t count_intersects(int n, const int* counts, unsigned* out, hipStream_t s);
 // Depth-first binning into a list of capacity cap >= the intersect total m =
 // sum(max(counts, 0)), which is computed on the device (and copied to *m_out
 // when given).  Slots past m hold sentinel entries outside every bin; entries

@@ -22,7 +22,8 @@ __all__ = [
     "get_tile_bin_edges", "get_aabb_2d", "texture_forward", "texture_backward",
     "texture_sample_forward", "texture_sample_backward", "texture_edit",
     # additions beyond the reference's ten
-    "project_gaussians", "project_points", "num_tiles_hit", "sort_intersects", "sort_pairs_u32",
+    "project_gaussians", "project_points", "num_tiles_hit", "count_intersects", "sort_intersects",
+    "sort_pairs_u32",
     "bin_gaussians",
     "build_splat_records", "raster_forward", "raster_backward", "finalize_gradients",
     "set_raster_culling", "set_precise_edge_on",
@@ -201,6 +202,18 @@ def sort_intersects(isect_ids, gaussian_ids, end_bit=64):
         call("gstex_sort_intersects", m, int(end_bit), _ptr(isect_ids), _ptr(gaussian_ids),
              _ptr(keys_out), _ptr(ids_out), _ptr(ws), nb, _stream(dev))
     return keys_out, ids_out
+
+
+def count_intersects(num_tiles_hit) -> int:
+    """Total intersect count sum(max(num_tiles_hit, 0)), read back to the host
+    (the total of compute_cumulative_intersects, utils.py:40-59)."""
+    _check(num_tiles_hit, "num_tiles_hit")
+    dev = num_tiles_hit.device
+    out = torch.zeros((1,), dtype=torch.int32, device=dev)
+    with torch.cuda.device(dev):
+        call("gstex_count_intersects", num_tiles_hit.numel(), _ptr(num_tiles_hit), _ptr(out),
+             _stream(dev))
+    return int(out.item())
 
 
 def sort_pairs_u32(keys, values, end_bit=32):

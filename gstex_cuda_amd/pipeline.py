@@ -244,7 +244,8 @@ class RenderStep:
                  _p(self.ids), _p(self.tile_bins), _p(self.m_dev), _p(self.bin_ws),
                  self.bin_ws.numel(), s)
         else:
-            m = int(self.tiles_hit.sum(dtype=torch.int64).item())
+            call("gstex_count_intersects", n, _p(self.tiles_hit), _p(self.m_dev), s)
+            m = int(self.m_dev.item())
             self.num_intersects = m
             self._ensure_bins(m)
             if m > 0:

@@ -143,6 +143,13 @@ int gstex_sort_pairs_u32(int n, int end_bit, const uint32_t *keys, const uint32_
 int gstex_get_tile_bin_edges(int num_intersects, const int64_t *isect_ids_sorted, int tiles_x,
                              int tiles_y, int32_t *tile_bins, void *stream);
 
+/* Replaces the total of compute_cumulative_intersects (utils.py:40-59,
+ * num_intersects = cum_tiles_hit[-1]) where only the total is needed:
+ * sum(max(num_tiles_hit, 0)) written to *num_intersects_out (device memory);
+ * the caller reads it back. */
+int gstex_count_intersects(int num_points, const int32_t *num_tiles_hit,
+                           uint32_t *num_intersects_out, void *stream);
+
 /* Replaces bin_and_sort_gaussians (utils.py:106-162) where only the sorted ids
  * and tile bins are consumed (texture.py:207-222): identical outputs, computed
  * by sorting the Gaussians by depth once and stable-sorting the emitted

@@ -112,6 +112,19 @@ def test_binning_1080p_200k(oracle):
     assert _binning_case(oracle, s, 16) > 1_000_000
 
 
+def test_count_intersects(oracle):
+    # gstex_count_intersects: the intersect total of compute_cumulative_intersects
+    # at full C2 size, and for no Gaussians
+    from gstex_cuda_amd import cuda as _C
+
+    s = scene(2)
+    pb = project_and_bin(s)
+    c, e = oracle.get_aabb_2d(s.means, s.scales, 1.0, s.quats, s.viewmat, s.fx, s.fy, s.cx, s.cy)
+    want = int(oracle.num_tiles_hit(c, e, s.img_h, s.img_w, 16).astype(np.int64).sum())
+    assert _C.count_intersects(pb["num_tiles_hit"]) == want == pb["num_intersects"]
+    assert _C.count_intersects(torch.zeros((0,), dtype=torch.int32, device=DEV)) == 0
+
+
 @pytest.mark.parametrize("view", [0, 3, 6, 13])
 def test_projection_bit_exact_rotated_cameras(oracle, view):
     # orbit cameras with general rotations (products and sums that a fused
