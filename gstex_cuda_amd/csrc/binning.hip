@@ -1127,7 +1127,8 @@ __global__ __launch_bounds__(256) void k_bin_emit(
     constexpr int kStage = kEmitOut + 1;
     constexpr int kPer = kEmitOut / 256;
     __shared__ uint32_t s_off[kStage], s_g[kStage], s_xy[kStage], s_wd[kStage];
-    __shared__ uint32_t s_tile[kEmitOut], s_id[kEmitOut];
+    __shared__ __align__(16) uint32_t s_tile[kEmitOut];
+    __shared__ __align__(16) uint32_t s_id[kEmitOut];
     const int tid = threadIdx.x;
     const uint32_t tw = *total_written, m = *m_total;
     const uint32_t k0 = (uint32_t)blockIdx.x * kEmitOut;
@@ -1164,14 +1165,40 @@ __global__ __launch_bounds__(256) void k_bin_emit(
             else hi = mid - 1;
         }
         const uint32_t kend = min(kt + (uint32_t)kPer, k1);
-        for (uint32_t k = kt; k < kend; ++k) {
-            while (lo + 1 < cnt && s_off[lo + 1] <= k) ++lo;
-            const uint32_t li = k - s_off[lo];
-            const uint32_t wd = s_wd[lo], xy = s_xy[lo];
-            const uint32_t ty = (xy >> 16) + li / wd;
-            const uint32_t tx = (xy & 0xffffu) + li % wd;
-            s_tile[k - k0] = ty * (uint32_t)tiles_x + tx;
-            s_id[k - k0] = s_g[lo];
+        // the first output's place in its Gaussian's rectangle by one
+        // division, then row-major steps.  Moving on to the next Gaussian
+        // starts at its rectangle's corner: output k - 1 belonged to an
+        // earlier Gaussian, so the one holding k starts at k.
+        uint32_t wd = s_wd[lo], xy = s_xy[lo], gid = s_g[lo];
+        const uint32_t li = kt - s_off[lo];
+        uint32_t ry = li / wd, rx = li - ry * wd;
+        // the thread's kPer results leave as 16-byte LDS stores (slots past
+        // k1 hold stale values and are never read)
+        uint32_t ot[kPer], oi[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const uint32_t k = kt + (uint32_t)j;
+            if (k < kend && lo + 1 < cnt && s_off[lo + 1] <= k) {
+                while (lo + 1 < cnt && s_off[lo + 1] <= k) ++lo;
+                wd = s_wd[lo];
+                xy = s_xy[lo];
+                gid = s_g[lo];
+                ry = 0;
+                rx = 0;
+            }
+            ot[j] = ((xy >> 16) + ry) * (uint32_t)tiles_x + (xy & 0xffffu) + rx;
+            oi[j] = gid;
+            if (++rx == wd) {
+                rx = 0;
+                ++ry;
+            }
+        }
+        uint4* const dt = reinterpret_cast<uint4*>(s_tile + kPer * tid);
+        uint4* const di = reinterpret_cast<uint4*>(s_id + kPer * tid);
+#pragma unroll
+        for (int q = 0; q < kPer / 4; ++q) {
+            dt[q] = make_uint4(ot[4 * q], ot[4 * q + 1], ot[4 * q + 2], ot[4 * q + 3]);
+            di[q] = make_uint4(oi[4 * q], oi[4 * q + 1], oi[4 * q + 2], oi[4 * q + 3]);
         }
     }
     __syncthreads();
@@ -1318,8 +1345,15 @@ hipError_t bin_gaussians(int n, int cap, const float* centers, const float* exte
                                          radix_workspace_bytes<uint32_t>(cap)));
     const int gn = (n + 255) / 256, gm = (cap + 255) / 256;
 
-    if ((e = fill<uint32_t>(m_total, 1, 0u, s)) != hipSuccess) return e;
-    k_count_total<<<std::min(gn, kCountBlocks), 256, 0, s>>>(n, counts, m_total);
+    if (m_out) {
+        // capacity mode: the total is counted on the device
+        if ((e = fill<uint32_t>(m_total, 1, 0u, s)) != hipSuccess) return e;
+        k_count_total<<<std::min(gn, kCountBlocks), 256, 0, s>>>(n, counts, m_total);
+    } else {
+        // exact size: the caller's num_intersects is that total (a wrong one
+        // still keeps every write inside the list and the zero-slot range)
+        if ((e = fill<uint32_t>(m_total, 1, (uint32_t)cap, s)) != hipSuccess) return e;
+    }
     k_bin_prepare<<<gn, 256, 0, s>>>(n, centers, extents, depths, counts, tiles_x, tiles_y, bw,
                                      dkey, ord, written);
     if ((e = hipGetLastError()) != hipSuccess) return e;
