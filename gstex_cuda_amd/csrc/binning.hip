@@ -157,21 +157,40 @@ __global__ __launch_bounds__(256) void k_bin_edges64(int m, const int64_t* __res
 }
 
 // Same edges on sorted 32-bit tile ids, four per thread (one 16-byte load
-// when the ids are 16-byte aligned, plus the id before them).
+// when the ids are 16-byte aligned, plus the id before them).  With id_bits
+// > 0 the keys are packed (tile << id_bits | Gaussian id): the tile is the
+// high part, and the ids go out to ids_out.
 __global__ __launch_bounds__(256) void k_bin_edges32(int m, const uint32_t* __restrict__ tiles,
                                                      int num_tiles, int* __restrict__ bins,
-                                                     bool vec) {
+                                                     bool vec, int id_bits,
+                                                     int* __restrict__ ids_out) {
     const int i0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
     if (i0 >= m) return;
-    int cur[4];
-    if (vec && i0 + 3 < m) {
+    uint32_t raw[4];
+    const bool full = vec && i0 + 3 < m;
+    if (full) {
         const uint4 v = *reinterpret_cast<const uint4*>(tiles + i0);
-        cur[0] = (int)v.x; cur[1] = (int)v.y; cur[2] = (int)v.z; cur[3] = (int)v.w;
+        raw[0] = v.x; raw[1] = v.y; raw[2] = v.z; raw[3] = v.w;
     } else {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) cur[k] = i0 + k < m ? (int)tiles[i0 + k] : -1;
+        for (int k = 0; k < 4; ++k) raw[k] = i0 + k < m ? tiles[i0 + k] : 0u;
     }
-    int prev = i0 > 0 ? (int)tiles[i0 - 1] : -1;
+    int cur[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) cur[k] = i0 + k < m ? (int)(raw[k] >> id_bits) : -1;
+    if (id_bits > 0) {
+        const uint32_t idm = (1u << id_bits) - 1u;
+        if (full) {
+            *reinterpret_cast<int4*>(ids_out + i0) =
+                make_int4((int)(raw[0] & idm), (int)(raw[1] & idm), (int)(raw[2] & idm),
+                          (int)(raw[3] & idm));
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (i0 + k < m) ids_out[i0 + k] = (int)(raw[k] & idm);
+        }
+    }
+    int prev = i0 > 0 ? (int)(tiles[i0 - 1] >> id_bits) : -1;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int i = i0 + k;
@@ -419,11 +438,12 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
 
     K key[ITEMS];
     uint32_t val[ITEMS];
+    const bool has_vals = vals_in != nullptr;  // keys only when null
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
         const int li = wbase + r * kWave + lane;
         key[r] = li < count ? keys_in[base + li] : (K)0;
-        val[r] = li < count ? vals_in[base + li] : 0u;
+        val[r] = has_vals && li < count ? vals_in[base + li] : 0u;
     }
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) wrun[w][tid] = 0;
@@ -479,7 +499,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
             const uint32_t d = digit_of(key[r] ^ flip, shift, mask);
             const uint32_t pos = lstart[d] + wrun[wave][d] + rank[r];
             skeys[pos] = key[r];
-            svals[pos] = val[r];
+            if (has_vals) svals[pos] = val[r];
         }
     }
     __syncthreads();
@@ -492,7 +512,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
             const uint32_t d = digit_of(k ^ flip, shift, mask);
             const uint32_t gp = gbase[d] + (uint32_t)p - lstart[d];
             keys_out[gp] = k;
-            vals_out[gp] = svals[p];
+            if (has_vals) vals_out[gp] = svals[p];
         }
     }
 }
@@ -606,8 +626,8 @@ constexpr int kOsHistItems = 2048;
 // Global digit counts of every pass in one read: hist[p * 256 + d] += items
 // whose pass-p digit is d (hist zeroed beforehand).
 __global__ __launch_bounds__(256) void k_onesweep_hist(int n, const uint32_t* __restrict__ keys,
-                                                       int passes, int bits_per, int end_bit,
-                                                       int per_block,
+                                                       int passes, int start_bit, int bits_per,
+                                                       int end_bit, int per_block,
                                                        uint32_t* __restrict__ hist) {
     __shared__ uint32_t cnt[kOsMaxPasses * 256];
     const int tid = threadIdx.x;
@@ -618,7 +638,7 @@ __global__ __launch_bounds__(256) void k_onesweep_hist(int n, const uint32_t* __
     for (size_t i = lo + tid; i < hi; i += 256) {
         const uint32_t k = keys[i];
         for (int p = 0; p < passes; ++p) {
-            const int shift = p * bits_per;
+            const int shift = start_bit + p * bits_per;
             const int bits = min(bits_per, end_bit - shift);
             atomicAdd(&cnt[p * 256 + ((k >> shift) & ((1u << bits) - 1u))], 1u);
         }
@@ -661,11 +681,12 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep_scatter(
     const int wbase = wave * ITEMS * kWave;
 
     uint32_t key[ITEMS], val[ITEMS];
+    const bool has_vals = vals_in != nullptr;  // keys only when null
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
         const int li = wbase + r * kWave + lane;
         key[r] = li < count ? keys_in[base + li] : 0u;
-        val[r] = li < count ? vals_in[base + li] : 0u;
+        val[r] = has_vals && li < count ? vals_in[base + li] : 0u;
     }
     const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     uint32_t rank[ITEMS];
@@ -728,7 +749,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep_scatter(
             const uint32_t d = (key[r] >> shift) & mask;
             const uint32_t pos = lstart[d] + wrun[wave][d] + rank[r];
             skeys[pos] = key[r];
-            svals[pos] = val[r];
+            if (has_vals) svals[pos] = val[r];
         }
     }
     if (tid < ndig) {
@@ -748,7 +769,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep_scatter(
             const uint32_t d = (k >> shift) & mask;
             const uint32_t gp = gbase[d] + (uint32_t)p - lstart[d];
             keys_out[gp] = k;
-            vals_out[gp] = svals[p];
+            if (has_vals) vals_out[gp] = svals[p];
         }
     }
 }
@@ -845,10 +866,11 @@ size_t radix_workspace_bytes(int n) {
 
 // One-sweep sort of (uint32 key, uint32 value) pairs over key bits [0, end_bit).
 static hipError_t onesweep_sort_u32(int n, const uint32_t* keys_in, const uint32_t* vals_in,
-                                    uint32_t* keys_out, uint32_t* vals_out, int end_bit,
-                                    uint32_t* tk, uint32_t* tv, uint32_t* state, hipStream_t s) {
-    const int passes = (end_bit + 7) / 8;
-    const int bits_per = (end_bit + passes - 1) / passes;
+                                    uint32_t* keys_out, uint32_t* vals_out, int start_bit,
+                                    int end_bit, uint32_t* tk, uint32_t* tv, uint32_t* state,
+                                    hipStream_t s) {
+    const int passes = (end_bit - start_bit + 7) / 8;
+    const int bits_per = (end_bit - start_bit + passes - 1) / passes;
     const int nblocks = sort_blocks(n);
     const bool small = sort_items(n) == 8;
     uint32_t* hist = state;
@@ -860,15 +882,16 @@ static hipError_t onesweep_sort_u32(int n, const uint32_t* keys_in, const uint32
     if (e != hipSuccess) return e;
     const int hb = std::min((n + kOsHistItems - 1) / kOsHistItems, 1024);
     const int per = (n + hb - 1) / hb;
-    k_onesweep_hist<<<hb, 256, 0, s>>>(n, keys_in, passes, bits_per, end_bit, per, hist);
+    k_onesweep_hist<<<hb, 256, 0, s>>>(n, keys_in, passes, start_bit, bits_per, end_bit, per,
+                                       hist);
     const uint32_t* sk = keys_in;
     const uint32_t* sv = vals_in;
     for (int pass = 0; pass < passes; ++pass) {
-        const int shift = pass * bits_per;
+        const int shift = start_bit + pass * bits_per;
         const int bits = std::min(bits_per, end_bit - shift);
         const bool to_out = ((passes - 1 - pass) % 2) == 0;
         uint32_t* dk = to_out ? keys_out : tk;
-        uint32_t* dv = to_out ? vals_out : tv;
+        uint32_t* dv = vals_in ? (to_out ? vals_out : tv) : nullptr;
         uint64_t* st = states + (size_t)pass * nblocks * 256;
         if (small)
             k_onesweep_scatter<8><<<nblocks, kSortThreads, 0, s>>>(
@@ -882,13 +905,14 @@ static hipError_t onesweep_sort_u32(int n, const uint32_t* keys_in, const uint32
     return hipGetLastError();
 }
 
-// Stable LSD sort of (key, value) pairs over key bits [0, end_bit).  With all 64
-// bits of a 64-bit key the sign bit is flipped in the digits, giving the
-// signed (int64) order.
+// Stable LSD sort of (key, value) pairs over key bits [start_bit, end_bit)
+// (bits outside are carried); with vals_in null the keys are sorted alone.
+// With all 64 bits of a 64-bit key the sign bit is flipped in the digits,
+// giving the signed (int64) order.
 template <typename K>
 hipError_t radix_sort_pairs(int n, const K* keys_in, const uint32_t* vals_in, K* keys_out,
                             uint32_t* vals_out, int end_bit, void* ws, size_t ws_bytes,
-                            hipStream_t s) {
+                            hipStream_t s, int start_bit = 0) {
     if (n <= 0) return hipSuccess;
     if (ws_bytes < radix_workspace_bytes<K>(n)) return hipErrorInvalidValue;
     const int nblocks = sort_blocks(n);
@@ -902,11 +926,11 @@ hipError_t radix_sort_pairs(int n, const K* keys_in, const uint32_t* vals_in, K*
     uint32_t* hist = (uint32_t*)p;
     p += up(sizeof(uint32_t) * 256 * (size_t)nblocks);
     uint32_t* partials = (uint32_t*)p;
-    const int passes = (end_bit + 7) / 8;
-    if (passes == 0) {
+    const int passes = (end_bit - start_bit + 7) / 8;
+    if (passes <= 0) {
         hipError_t e = hipMemcpyAsync(keys_out, keys_in, sizeof(K) * (size_t)n,
                                       hipMemcpyDeviceToDevice, s);
-        if (e != hipSuccess) return e;
+        if (e != hipSuccess || !vals_in) return e;
         return hipMemcpyAsync(vals_out, vals_in, sizeof(uint32_t) * (size_t)n,
                               hipMemcpyDeviceToDevice, s);
     }
@@ -918,17 +942,17 @@ hipError_t radix_sort_pairs(int n, const K* keys_in, const uint32_t* vals_in, K*
     if constexpr (sizeof(K) == 4)
         if (nblocks <= kOsMaxBlocks)
             return onesweep_sort_u32(n, (const uint32_t*)keys_in, vals_in, (uint32_t*)keys_out,
-                                     vals_out, end_bit, (uint32_t*)tk, tv, hist, s);
-    const int bits_per = (end_bit + passes - 1) / passes;  // balanced digit widths <= 8
+                                     vals_out, start_bit, end_bit, (uint32_t*)tk, tv, hist, s);
+    const int bits_per = (end_bit - start_bit + passes - 1) / passes;  // balanced digit widths <= 8
     const K flip = (sizeof(K) == 8 && end_bit == 64) ? (K)((K)1 << (8 * sizeof(K) - 1)) : (K)0;
     const K* sk = keys_in;
     const uint32_t* sv = vals_in;
     for (int pass = 0; pass < passes; ++pass) {
-        const int shift = pass * bits_per;
+        const int shift = start_bit + pass * bits_per;
         const int bits = min(bits_per, end_bit - shift);
         const bool to_out = ((passes - 1 - pass) % 2) == 0;
         K* dk = to_out ? keys_out : tk;
-        uint32_t* dv = to_out ? vals_out : tv;
+        uint32_t* dv = vals_in ? (to_out ? vals_out : tv) : nullptr;
         // past one workgroup's reach, the digit counts are scanned in one
         // launch with a look-back (its states cleared by the histogram kernel)
         const int hcount = (1 << bits) * nblocks;
@@ -1118,12 +1142,13 @@ __global__ __launch_bounds__(256) void k_bin_mark_starts(int n, const uint32_t* 
 // Each thread takes kEmitOut / 256 consecutive outputs: a binary search in
 // LDS for the first, then a forward walk over the staged offsets.  A
 // Gaussian's tiles go row by row over its rectangle, capped at its count.
+// With id_bits > 0 each entry is one packed key (tile << id_bits | id).
 __global__ __launch_bounds__(256) void k_bin_emit(
     int n, int cap, const uint32_t* __restrict__ m_total, const uint32_t* __restrict__ order,
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ total_written,
     const uint32_t* __restrict__ starts, const float* __restrict__ centers,
     const float* __restrict__ extents, int tiles_x, int tiles_y, int bw,
-    uint32_t* __restrict__ tiles_out, uint32_t* __restrict__ ids_out) {
+    uint32_t* __restrict__ tiles_out, uint32_t* __restrict__ ids_out, int id_bits) {
     constexpr int kStage = kEmitOut + 1;
     constexpr int kPer = kEmitOut / 256;
     __shared__ uint32_t s_off[kStage], s_g[kStage], s_xy[kStage], s_wd[kStage];
@@ -1207,8 +1232,12 @@ __global__ __launch_bounds__(256) void k_bin_emit(
         const uint32_t idx = (uint32_t)(r * 256 + tid), k = k0 + idx;
         // entries past the capacity are dropped (the caller sees m > cap)
         if (k < k1 && zbase + k < limit) {
-            tiles_out[zbase + k] = s_tile[idx];
-            ids_out[zbase + k] = s_id[idx];
+            if (id_bits > 0) {
+                tiles_out[zbase + k] = (s_tile[idx] << id_bits) | s_id[idx];
+            } else {
+                tiles_out[zbase + k] = s_tile[idx];
+                ids_out[zbase + k] = s_id[idx];
+            }
         }
     }
 }
@@ -1240,23 +1269,20 @@ __global__ __launch_bounds__(256) void k_count_total(int n, const int* __restric
 // (tile 0, id 0) first, then the emitted entries (k_bin_emit), and past the
 // intersect total m (if m < cap) sentinel entries with tile = num_tiles that
 // sort last and belong to no bin.  Only the zero and sentinel ranges are
-// written: a grid-stride walk over their concatenation.
+// written: a grid-stride walk over their concatenation (packed keys when
+// id_bits > 0).
 __global__ __launch_bounds__(256) void k_zero_slots(int cap, const uint32_t* __restrict__ m_total,
                                                     const uint32_t* __restrict__ total_written,
                                                     int num_tiles, uint32_t* __restrict__ tiles_out,
-                                                    uint32_t* __restrict__ ids_out) {
+                                                    uint32_t* __restrict__ ids_out, int id_bits) {
     const uint32_t m = *m_total, tw = *total_written, c = (uint32_t)cap;
     const uint32_t z = min(tw <= m ? m - tw : 0u, c);
     const uint32_t tail = m < c ? c - m : 0u;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < z + tail;
          k += gridDim.x * blockDim.x) {
-        if (k < z) {
-            tiles_out[k] = 0u;
-            ids_out[k] = 0u;
-        } else {
-            tiles_out[m + (k - z)] = (uint32_t)num_tiles;
-            ids_out[m + (k - z)] = 0u;
-        }
+        const uint32_t at = k < z ? k : m + (k - z);
+        tiles_out[at] = k < z ? 0u : (uint32_t)num_tiles << id_bits;
+        if (id_bits == 0) ids_out[at] = 0u;
     }
 }
 
@@ -1365,19 +1391,32 @@ hipError_t bin_gaussians(int n, int cap, const float* centers, const float* exte
         if ((e = fill<uint64_t>(states, (size_t)st + 1, 0ull, s)) != hipSuccess) return e;
         k_scan_written<<<st, 256, 0, s>>>(n, ord2, written, offs, total, states, st);
     }
-    k_zero_slots<<<std::min(gm, 1024), 256, 0, s>>>(cap, m_total, total, num_tiles, tkey, tid_in);
+    // the sentinel tile (num_tiles) needs one more key value; when the tile
+    // and the Gaussian id fit one 32-bit word together, the entries are sorted
+    // as packed keys alone (half the bytes per pass), and the ids come out of
+    // the bin-edge pass
+    const int tile_bits = bits_for(num_tiles + 1), id_bits = bits_for(n);
+    const int pk = (id_bits > 0 && tile_bits + id_bits <= 32) ? id_bits : 0;
+    k_zero_slots<<<std::min(gm, 1024), 256, 0, s>>>(cap, m_total, total, num_tiles, tkey, tid_in,
+                                                    pk);
     const int eb = emit_blocks(cap);
     uint32_t* starts = (uint32_t*)(b + L.off[7]);
     k_bin_mark_starts<<<gn, 256, 0, s>>>(n, offs, total, eb, kEmitOut, starts);
     k_bin_emit<<<eb, 256, 0, s>>>(n, cap, m_total, ord2, offs, total, starts, centers, extents,
-                                  tiles_x, tiles_y, bw, tkey, tid_in);
+                                  tiles_x, tiles_y, bw, tkey, tid_in, pk);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    // the sentinel tile (num_tiles) needs one more key value
-    e = radix_sort_pairs<uint32_t>(cap, tkey, tid_in, tkey2, (uint32_t*)ids_sorted,
-                                   bits_for(num_tiles + 1), rws, rws_bytes, s);
+    if (pk)
+        e = radix_sort_pairs<uint32_t>(cap, tkey, nullptr, tkey2, nullptr, pk + tile_bits, rws,
+                                       rws_bytes, s, pk);
+    else
+        e = radix_sort_pairs<uint32_t>(cap, tkey, tid_in, tkey2, (uint32_t*)ids_sorted, tile_bits,
+                                       rws, rws_bytes, s);
     if (e != hipSuccess) return e;
     k_bin_edges32<<<(cap + 1023) / 1024, 256, 0, s>>>(
-        cap, tkey2, num_tiles, tile_bins, (reinterpret_cast<uintptr_t>(tkey2) & 15) == 0);
+        cap, tkey2, num_tiles, tile_bins,
+        (reinterpret_cast<uintptr_t>(tkey2) & 15) == 0 &&
+            (reinterpret_cast<uintptr_t>(ids_sorted) & 15) == 0,
+        pk, ids_sorted);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (m_out) return hipMemcpyAsync(m_out, m_total, sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
     return hipSuccess;
