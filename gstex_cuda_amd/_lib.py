@@ -32,6 +32,8 @@ _SIGNATURES = {
     "gstex_num_tiles_hit": (I, [I, P, P, I, I, I, P, P]),
     "gstex_num_tiles_hit_wrapped": (I, [I, P, P, I, P, P]),
     "gstex_project_gaussians": (I, [I, P, P, F, P, P, F, F, F, F, I, I, I, P, P, P, P, P]),
+    "gstex_project_gaussians_total": (I, [I, P, P, F, P, P, F, F, F, F, I, I, I, P, P, P, P, P,
+                                          P]),
     "gstex_project_points": (I, [I, P, P, F, F, F, F, I, P, P, P]),
     "gstex_compute_sh_forward": (I, [I, I, I, P, P, P, P]),
     "gstex_compute_sh_forward_bf16": (I, [I, I, I, P, P, P, P]),

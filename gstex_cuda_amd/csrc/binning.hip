@@ -40,6 +40,9 @@ hipError_t fill(T* p, size_t n, T v, hipStream_t s) {
 
 hipError_t fill_f32(float* p, size_t n, float v, hipStream_t s) { return fill<float>(p, n, v, s); }
 hipError_t fill_i32(int* p, size_t n, int v, hipStream_t s) { return fill<int>(p, n, v, s); }
+hipError_t fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
+    return fill<uint32_t>(p, n, v, s);
+}
 
 // ------------------------------------------------------------------ tile rectangle
 // get_tile_bbox (helpers.cuh:37-51,75-92): [min, max) in tile units, clamped.

@@ -125,9 +125,29 @@ int gstex_project_gaussians(int num_points, const float* means, const float* sca
                                                  viewmat, fx, fy, cx, cy, block_width,
                                                  tiles_of(img_width, block_width),
                                                  tiles_of(img_height, block_width), centers,
-                                                 extents, depths, num_tiles_hit,
+                                                 extents, depths, num_tiles_hit, nullptr,
                                                  as_stream(stream)),
                      "project_gaussians");
+}
+
+int gstex_project_gaussians_total(int num_points, const float* means, const float* scales,
+                                  float glob_scale, const float* quats, const float* viewmat,
+                                  float fx, float fy, float cx, float cy, int img_height,
+                                  int img_width, int block_width, float* centers, float* extents,
+                                  float* depths, int32_t* num_tiles_hit,
+                                  uint32_t* num_intersects_out, void* stream) {
+    REQUIRE(num_points >= 0 && block_width > 0, "invalid sizes");
+    REQUIRE(num_intersects_out, "null pointer argument");
+    REQUIRE(num_points == 0 || (means && scales && quats && viewmat && centers && extents &&
+                                depths && num_tiles_hit),
+            "null pointer argument");
+    return check_hip(gstex::launch_project_fused(num_points, means, scales, glob_scale, quats,
+                                                 viewmat, fx, fy, cx, cy, block_width,
+                                                 tiles_of(img_width, block_width),
+                                                 tiles_of(img_height, block_width), centers,
+                                                 extents, depths, num_tiles_hit,
+                                                 num_intersects_out, as_stream(stream)),
+                     "project_gaussians_total");
 }
 
 int gstex_project_points(int num_points, const float* points, const float* viewmat, float fx,

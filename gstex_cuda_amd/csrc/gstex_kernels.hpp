@@ -20,7 +20,7 @@ hipError_t launch_project_fused(int n, const float* means, const float* scales, 
                                 const float* quats, const float* viewmat, float fx, float fy,
                                 float cx, float cy, int bw, int tiles_x, int tiles_y,
                                 float* centers, float* extents, float* depths, int* tiles_hit,
-                                hipStream_t stream);
+                                uint32_t* total, hipStream_t stream);
 hipError_t launch_num_tiles_hit(int n, const float* centers, const float* extents, int bw,
                                 int tiles_x, int tiles_y, int* out, hipStream_t stream);
 hipError_t launch_project_points(int n, const float* points, const float* viewmat, float fx,
@@ -51,6 +51,7 @@ int sh_num_bases(int degree);
 // fills
 hipError_t fill_f32(float* p, size_t n, float v, hipStream_t s);
 hipError_t fill_i32(int* p, size_t n, int v, hipStream_t s);
+hipError_t fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 
 // binning
 hipError_t map_intersects(int n, int num_isects, const float* centers, const float* extents,

@@ -87,19 +87,41 @@ __global__ __launch_bounds__(256) void k_get_aabb_2d(int n, const float* __restr
     reinterpret_cast<float2*>(extents)[i] = make_float2(ex, ey);
 }
 
+// Centres, extents, depths and tile counts in one pass; with `total`, also
+// sum(tiles_hit) added into *total (zeroed beforehand).
 __global__ __launch_bounds__(256) void k_project_fused(
     int n, const float* __restrict__ means, const float* __restrict__ scales, float glob_scale,
     const float* __restrict__ quats, const float* __restrict__ viewmat, float fx, float fy,
     float cx, float cy, int bw, int tiles_x, int tiles_y, float* __restrict__ centers,
-    float* __restrict__ extents, float* __restrict__ depths, int* __restrict__ tiles_hit) {
+    float* __restrict__ extents, float* __restrict__ depths, int* __restrict__ tiles_hit,
+    uint32_t* __restrict__ total) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float ccx, ccy, ex, ey, depth;
-    aabb_one(means, scales, glob_scale, quats, viewmat, fx, fy, cx, cy, i, ccx, ccy, ex, ey, depth);
-    reinterpret_cast<float2*>(centers)[i] = make_float2(ccx, ccy);
-    reinterpret_cast<float2*>(extents)[i] = make_float2(ex, ey);
-    depths[i] = depth;
-    tiles_hit[i] = tiles_hit_one(ccx, ccy, ex, ey, bw, tiles_x, tiles_y);
+    int hit = 0;
+    if (i < n) {
+        float ccx, ccy, ex, ey, depth;
+        aabb_one(means, scales, glob_scale, quats, viewmat, fx, fy, cx, cy, i, ccx, ccy, ex, ey,
+                 depth);
+        reinterpret_cast<float2*>(centers)[i] = make_float2(ccx, ccy);
+        reinterpret_cast<float2*>(extents)[i] = make_float2(ex, ey);
+        depths[i] = depth;
+        hit = tiles_hit_one(ccx, ccy, ex, ey, bw, tiles_x, tiles_y);
+        tiles_hit[i] = hit;
+    }
+    if (total) {
+        // the intersect total: a workgroup sum, one atomic per workgroup
+        __shared__ uint32_t wsum[256 / kWave];
+        uint32_t v = (uint32_t)max(hit, 0);
+#pragma unroll
+        for (int off = kWave / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+        if ((threadIdx.x & (kWave - 1)) == 0) wsum[threadIdx.x / kWave] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+#pragma unroll
+            for (int w = 0; w < 256 / kWave; ++w) t += wsum[w];
+            if (t) atomicAdd(total, t);
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_num_tiles_hit(int n, const float* __restrict__ centers,
@@ -373,12 +395,16 @@ hipError_t launch_project_fused(int n, const float* means, const float* scales, 
                                 const float* quats, const float* viewmat, float fx, float fy,
                                 float cx, float cy, int bw, int tiles_x, int tiles_y,
                                 float* centers, float* extents, float* depths, int* tiles_hit,
-                                hipStream_t stream) {
+                                uint32_t* total, hipStream_t stream) {
+    if (total) {
+        hipError_t e = fill_u32(total, 1, 0u, stream);
+        if (e != hipSuccess) return e;
+    }
     if (n <= 0) return hipSuccess;
     k_project_fused<<<(n + 255) / 256, 256, 0, stream>>>(n, means, scales, glob_scale, quats,
                                                         viewmat, fx, fy, cx, cy, bw, tiles_x,
                                                         tiles_y, centers, extents, depths,
-                                                        tiles_hit);
+                                                        tiles_hit, total);
     return hipGetLastError();
 }
 

@@ -234,9 +234,14 @@ class RenderStep:
                 self._zero_grads(side.cuda_stream)
                 self._ev_zeroed.record(side)
             self._zeroed = True
-        call("gstex_project_gaussians", n, _p(p["means"]), _p(p["scales"]), 1.0, _p(p["quats"]),
-             _p(cam.viewmat), cam.fx, cam.fy, cam.cx, cam.cy, cam.img_h, cam.img_w, self.bw,
-             _p(self.centers), _p(self.extents), _p(self.depths), _p(self.tiles_hit), s)
+        proj = (_p(p["means"]), _p(p["scales"]), 1.0, _p(p["quats"]), _p(cam.viewmat), cam.fx,
+                cam.fy, cam.cx, cam.cy, cam.img_h, cam.img_w, self.bw, _p(self.centers),
+                _p(self.extents), _p(self.depths), _p(self.tiles_hit))
+        if self.cap is not None:
+            call("gstex_project_gaussians", n, *proj, s)
+        else:
+            # the intersect total comes out of the projection launch itself
+            call("gstex_project_gaussians_total", n, *proj, _p(self.m_dev), s)
         if self.cap is not None:
             m = int(self.cap)
             call("gstex_bin_gaussians_capped", n, m, _p(self.centers), _p(self.extents),
@@ -244,7 +249,6 @@ class RenderStep:
                  _p(self.ids), _p(self.tile_bins), _p(self.m_dev), _p(self.bin_ws),
                  self.bin_ws.numel(), s)
         else:
-            call("gstex_count_intersects", n, _p(self.tiles_hit), _p(self.m_dev), s)
             m = int(self.m_dev.item())
             self.num_intersects = m
             self._ensure_bins(m)

@@ -66,6 +66,16 @@ int gstex_project_gaussians(int num_points, const float *means, const float *sca
                             float fy, float cx, float cy, int img_height, int img_width,
                             int block_width, float *centers, float *extents, float *depths,
                             int32_t *num_tiles_hit, void *stream);
+/* gstex_project_gaussians plus the total of compute_cumulative_intersects
+ * (utils.py:40-59): sum(num_tiles_hit) written to *num_intersects_out (device
+ * memory) by the same launch, for callers that read it back to size the
+ * list. */
+int gstex_project_gaussians_total(int num_points, const float *means, const float *scales,
+                                  float glob_scale, const float *quats, const float *viewmat,
+                                  float fx, float fy, float cx, float cy, int img_height,
+                                  int img_width, int block_width, float *centers, float *extents,
+                                  float *depths, int32_t *num_tiles_hit,
+                                  uint32_t *num_intersects_out, void *stream);
 
 /* Replaces the torch function project_points (get_aabb_2d.py:22-32): pixel
  * coordinates (N,2) and view-space depth (N,) of points (N,3) under the first

@@ -123,6 +123,14 @@ def test_count_intersects(oracle):
     want = int(oracle.num_tiles_hit(c, e, s.img_h, s.img_w, 16).astype(np.int64).sum())
     assert _C.count_intersects(pb["num_tiles_hit"]) == want == pb["num_intersects"]
     assert _C.count_intersects(torch.zeros((0,), dtype=torch.int32, device=DEV)) == 0
+    # the exact-size step takes the total from gstex_project_gaussians_total
+    from gstex_cuda_amd.pipeline import scene_to_step
+
+    step = scene_to_step(s, DEV)
+    step.forward_pre()
+    torch.cuda.synchronize()
+    assert step.num_intersects == want
+    assert np.array_equal(np_(step.tiles_hit), np_(pb["num_tiles_hit"]))
 
 
 @pytest.mark.parametrize("view", [0, 3, 6, 13])
