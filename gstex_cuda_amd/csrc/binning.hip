@@ -216,6 +216,31 @@ __device__ __forceinline__ uint32_t digit_of(K key, int shift, uint32_t mask) {
     return (uint32_t)(key >> shift) & mask;
 }
 
+// The lanes of the wave that hold the same digit d (of up to 8 bits) as this
+// one, among the valid lanes: one ballot per digit bit, each lane keeping the
+// lanes that agree with its own bit.  With t all ones when the lane's bit is
+// set, the lanes that agree are ~(ballot ^ t): peers & ~(ballot ^ t) is one
+// v_bitop3 per 32-bit half (truth table 0x90 over (peers, ballot, t)).
+__device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid, int bits) {
+    const uint64_t v = __ballot(valid);
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        if (b >= bits) break;
+        const uint32_t t = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+        const uint64_t bal = __ballot((int32_t)t < 0);
+        lo = __builtin_amdgcn_bitop3_b32(lo, (uint32_t)bal, t, 0x90);
+        hi = __builtin_amdgcn_bitop3_b32(hi, (uint32_t)(bal >> 32), t, 0x90);
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Set bits of m in the lanes below this one.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // Per-block digit histogram of one block tile (ITEMS * 256 items), stored
 // digit-major (hist[d * nblocks + b]) so one exclusive scan yields scatter
 // bases.  Block 0 also zeroes `nzero` words at `zero`.
@@ -420,21 +445,109 @@ static hipError_t scan_exclusive(uint32_t* a, int n, uint32_t* total, uint32_t* 
 // the tile, the items are placed in an LDS copy of the tile sorted by digit,
 // and lanes write consecutive LDS positions out, so each digit's run goes to
 // consecutive global addresses (hist[digit][block] onwards).
+struct ScatterLds {
+    uint32_t gbase[256];                      // global start of each digit's run for this block
+    uint32_t lstart[256];                     // start of each digit's run in the LDS tile
+    uint32_t wrun[kSortThreads / kWave][256]; // per wave: running count, then start offset
+};
+
+// The rank / place / write part of the scatter, for a tile already held in
+// registers in (wave, round, lane) order: item r of lane l of wave w is item
+// w * ITEMS * 64 + r * 64 + l of the tile, and items at or past count are
+// absent.  skeys / svals are LDS arrays of the tile's size (svals unused
+// without values); gb is thread t's digit base (the global start of digit
+// t's run for this tile, 0 for t past the digits), loaded by the caller
+// ahead of time, and writes stay below out_n.
+template <typename K, int ITEMS>
+__device__ __forceinline__ void tile_rank_scatter(const K (&key)[ITEMS],
+                                                  const uint32_t (&val)[ITEMS], int count,
+                                                  bool has_vals, K flip, int shift, int bits,
+                                                  uint32_t gb, K* __restrict__ keys_out,
+                                                  uint32_t* __restrict__ vals_out, uint32_t out_n,
+                                                  ScatterLds& sl, K* skeys,
+                                                  uint32_t* svals) {
+    constexpr int kWaves = kSortThreads / kWave;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const uint32_t mask = (1u << bits) - 1u;
+    const int wbase = wave * ITEMS * kWave;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) sl.wrun[w][tid] = 0;
+    __syncthreads();
+
+    // wave-private ranking: rank[r] = items of digit d before item r in this wave
+    uint32_t rank[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        const bool valid = wbase + r * kWave + lane < count;
+        const uint32_t d = valid ? digit_of(key[r] ^ flip, shift, mask) : 0u;
+        const uint64_t peers = digit_peers(d, valid, bits);
+        const uint32_t before = lanes_below(peers);
+        const uint32_t seen = valid ? sl.wrun[wave][d] : 0u;
+        if (valid && before == 0) sl.wrun[wave][d] = seen + (uint32_t)__popcll(peers);
+        rank[r] = seen + before;
+    }
+    __syncthreads();
+    // per digit (one per thread): each wave's start inside the digit's run,
+    // and the exclusive scan of the digit totals over the tile
+    {
+        uint32_t c = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t t = sl.wrun[w][tid];
+            sl.wrun[w][tid] = c;
+            c += t;
+        }
+        uint32_t v = c;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const uint32_t t = __shfl_up(v, off);
+            if (lane >= off) v += t;
+        }
+        sl.lstart[tid] = v;  // inclusive within the wave for now
+        sl.gbase[tid] = gb;
+        __syncthreads();
+        uint32_t add = 0;
+        for (int w = 0; w < wave; ++w) add += sl.lstart[w * kWave + kWave - 1];
+        __syncthreads();
+        sl.lstart[tid] = v + add - c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        if (wbase + r * kWave + lane < count) {
+            const uint32_t d = digit_of(key[r] ^ flip, shift, mask);
+            const uint32_t pos = sl.lstart[d] + sl.wrun[wave][d] + rank[r];
+            skeys[pos] = key[r];
+            if (has_vals) svals[pos] = val[r];
+        }
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        const int p = r * kSortThreads + tid;
+        if (p < count) {
+            const K k = skeys[p];
+            const uint32_t d = digit_of(k ^ flip, shift, mask);
+            const uint32_t gp = sl.gbase[d] + (uint32_t)p - sl.lstart[d];
+            if (gp < out_n) {  // always, when the digit counts are those of the keys
+                keys_out[gp] = k;
+                if (has_vals) vals_out[gp] = svals[p];
+            }
+        }
+    }
+}
+
 template <typename K, int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     int n, const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     K* __restrict__ keys_out, uint32_t* __restrict__ vals_out, K flip, int shift, int bits,
     const uint32_t* __restrict__ hist, int nblocks) {
     constexpr int kTile = kSortThreads * ITEMS;
-    constexpr int kWaves = kSortThreads / kWave;
-    __shared__ uint32_t gbase[256];   // global start of each digit's run for this block
-    __shared__ uint32_t lstart[256];  // start of each digit's run in the LDS tile
-    __shared__ uint32_t wrun[kWaves][256];  // per wave: running count, then start offset
+    __shared__ ScatterLds sl;
     __shared__ K skeys[kTile];
     __shared__ uint32_t svals[kTile];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-    const uint32_t mask = (1u << bits) - 1u;
-    const int ndig = 1 << bits;
     const size_t base = (size_t)blockIdx.x * kTile;
     const int count = (int)min((size_t)kTile, (size_t)n - base);
     const int wbase = wave * ITEMS * kWave;
@@ -448,76 +561,9 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
         key[r] = li < count ? keys_in[base + li] : (K)0;
         val[r] = has_vals && li < count ? vals_in[base + li] : 0u;
     }
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) wrun[w][tid] = 0;
-    __syncthreads();
-
-    // wave-private ranking: rank[r] = items of digit d before item r in this wave
-    const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-    uint32_t rank[ITEMS];
-#pragma unroll
-    for (int r = 0; r < ITEMS; ++r) {
-        const bool valid = wbase + r * kWave + lane < count;
-        const uint32_t d = valid ? digit_of(key[r] ^ flip, shift, mask) : 0u;
-        uint64_t peers = __ballot(valid);
-        for (int b = 0; b < bits; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t bal = __ballot(bit);
-            peers &= bit ? bal : ~bal;
-        }
-        const uint32_t before = (uint32_t)__popcll(peers & lt_mask);
-        const uint32_t seen = valid ? wrun[wave][d] : 0u;
-        if (valid && before == 0) wrun[wave][d] = seen + (uint32_t)__popcll(peers);
-        rank[r] = seen + before;
-    }
-    __syncthreads();
-    // per digit (one per thread): each wave's start inside the digit's run,
-    // and the exclusive scan of the digit totals over the tile
-    {
-        uint32_t c = 0;
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) {
-            const uint32_t t = wrun[w][tid];
-            wrun[w][tid] = c;
-            c += t;
-        }
-        uint32_t v = c;
-#pragma unroll
-        for (int off = 1; off < kWave; off <<= 1) {
-            const uint32_t t = __shfl_up(v, off);
-            if (lane >= off) v += t;
-        }
-        lstart[tid] = v;  // inclusive within the wave for now
-        gbase[tid] = tid < ndig ? hist[(size_t)tid * nblocks + blockIdx.x] : 0u;
-        __syncthreads();
-        uint32_t add = 0;
-        for (int w = 0; w < wave; ++w) add += lstart[w * kWave + kWave - 1];
-        __syncthreads();
-        lstart[tid] = v + add - c;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < ITEMS; ++r) {
-        if (wbase + r * kWave + lane < count) {
-            const uint32_t d = digit_of(key[r] ^ flip, shift, mask);
-            const uint32_t pos = lstart[d] + wrun[wave][d] + rank[r];
-            skeys[pos] = key[r];
-            if (has_vals) svals[pos] = val[r];
-        }
-    }
-    __syncthreads();
-
-#pragma unroll
-    for (int r = 0; r < ITEMS; ++r) {
-        const int p = r * kSortThreads + tid;
-        if (p < count) {
-            const K k = skeys[p];
-            const uint32_t d = digit_of(k ^ flip, shift, mask);
-            const uint32_t gp = gbase[d] + (uint32_t)p - lstart[d];
-            keys_out[gp] = k;
-            if (has_vals) vals_out[gp] = svals[p];
-        }
-    }
+    const uint32_t gb = tid < (1 << bits) ? hist[(size_t)tid * nblocks + blockIdx.x] : 0u;
+    tile_rank_scatter<K, ITEMS>(key, val, count, has_vals, flip, shift, bits, gb, keys_out,
+                                vals_out, (uint32_t)n, sl, skeys, svals);
 }
 
 // ------------------------------------------------------------------ one-sweep sort
@@ -691,19 +737,13 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep_scatter(
         key[r] = li < count ? keys_in[base + li] : 0u;
         val[r] = has_vals && li < count ? vals_in[base + li] : 0u;
     }
-    const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     uint32_t rank[ITEMS];
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
         const bool valid = wbase + r * kWave + lane < count;
         const uint32_t d = valid ? (key[r] >> shift) & mask : 0u;
-        uint64_t peers = __ballot(valid);
-        for (int b = 0; b < bits; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t bal = __ballot(bit);
-            peers &= bit ? bal : ~bal;
-        }
-        const uint32_t before = (uint32_t)__popcll(peers & lt_mask);
+        const uint64_t peers = digit_peers(d, valid, bits);
+        const uint32_t before = lanes_below(peers);
         const uint32_t seen = valid ? wrun[wave][d] : 0u;
         if (valid && before == 0) wrun[wave][d] = seen + (uint32_t)__popcll(peers);
         rank[r] = seen + before;
@@ -1034,24 +1074,28 @@ hipError_t tile_bin_edges(int m, const int64_t* keys_sorted, int num_tiles, int*
 // id 0) first in tile 0.  So: sort the N Gaussians by depth once, emit their
 // tiles in that order after the zero slots, and stable-sort by tile only.
 
-// Per Gaussian: entries it writes (rectangle, capped by its count) and its
-// depth key; Gaussians that write nothing sort last.
+// Per Gaussian: its depth key, and info[g] = (entries it writes (rectangle,
+// capped by its count), rectangle corner x0 | y0 << 16, rectangle width, 0);
+// Gaussians that write nothing sort last.
 __global__ __launch_bounds__(256) void k_bin_prepare(
     int n, const float* __restrict__ centers, const float* __restrict__ extents,
     const float* __restrict__ depths, const int* __restrict__ counts, int tiles_x, int tiles_y,
     int bw, uint32_t* __restrict__ depth_keys, uint32_t* __restrict__ order,
-    int* __restrict__ written) {
+    uint4* __restrict__ info) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const float2 c = reinterpret_cast<const float2*>(centers)[g];
     const float2 e = reinterpret_cast<const float2*>(extents)[g];
     int w = 0;
+    uint32_t xy = 0, wd = 1;
     if (writes_intersects(e.x, e.y)) {
         int x0, y0, x1, y1;
         tile_rect(c.x, c.y, e.x, e.y, bw, tiles_x, tiles_y, x0, y0, x1, y1);
         w = min((x1 - x0) * (y1 - y0), max(counts[g], 0));
+        xy = (uint32_t)x0 | ((uint32_t)y0 << 16);  // tile indices < 65536
+        wd = (uint32_t)max(x1 - x0, 1);
     }
-    written[g] = w;
+    info[g] = make_uint4((uint32_t)w, xy, wd, 0u);
     // 0xFFFFFFFF is kept for Gaussians that write nothing, so they all sort
     // after every writing one (a writing Gaussian's depth is a positive float,
     // whose bits are below 0x80000000; the clamp only touches a NaN pattern).
@@ -1059,15 +1103,18 @@ __global__ __launch_bounds__(256) void k_bin_prepare(
     order[g] = (uint32_t)g;
 }
 
-// offs[p] = sum of written[order[q]] over q < p, and *total = the sum over
+// offs[p] = sum of w[order[q]] over q < p (w = info.x), and *total = the sum over
 // every p, in one launch: tiles of kWrScanTile positions, 16 consecutive per
 // thread, each tile adding the sum of the tiles before it by a decoupled
 // look-back (ticket order, as in the one-sweep sort).  states holds one
 // look-back state per tile, then the ticket and error words, all zeroed.
+// rec[p] = (offs[p], order[p], its rectangle corner, its rectangle width):
+// what the emission needs of position p, in one 16-byte read.
 
 __global__ __launch_bounds__(256) void k_scan_written(int n, const uint32_t* __restrict__ order,
-                                                      const int* __restrict__ written,
+                                                      const uint4* __restrict__ info,
                                                       uint32_t* __restrict__ offs,
+                                                      uint4* __restrict__ rec,
                                                       uint32_t* __restrict__ total,
                                                       uint64_t* __restrict__ states, int tiles) {
     __shared__ uint32_t lds[256 / kWave];
@@ -1078,11 +1125,15 @@ __global__ __launch_bounds__(256) void k_scan_written(int n, const uint32_t* __r
     __syncthreads();
     const int blk = (int)s_blk;
     const size_t base = (size_t)blk * kWrScanTile + (size_t)tid * kWrScanPer;
-    uint32_t v[kWrScanPer];
+    uint32_t v[kWrScanPer], gs[kWrScanPer], xys[kWrScanPer], wds[kWrScanPer];
     uint32_t sum = 0;
 #pragma unroll
     for (int j = 0; j < kWrScanPer; ++j) {
-        v[j] = base + j < (size_t)n ? (uint32_t)written[order[base + j]] : 0u;
+        gs[j] = base + j < (size_t)n ? order[base + j] : 0u;
+        const uint4 f = base + j < (size_t)n ? info[gs[j]] : make_uint4(0u, 0u, 1u, 0u);
+        v[j] = f.x;
+        xys[j] = f.y;
+        wds[j] = f.z;
         sum += v[j];
     }
     uint32_t tot;
@@ -1104,145 +1155,45 @@ __global__ __launch_bounds__(256) void k_scan_written(int n, const uint32_t* __r
     uint32_t run = s_before + excl;
 #pragma unroll
     for (int j = 0; j < kWrScanPer; ++j) {
-        if (base + j < (size_t)n) offs[base + j] = run;
+        if (base + j < (size_t)n) {
+            offs[base + j] = run;
+            rec[base + j] = make_uint4(run, gs[j], xys[j], wds[j]);
+        }
         run += v[j];
     }
 }
 
-// Emission of (tile, id) for every depth-ordered Gaussian after the zero
-// slots, which occupy [0, m - total_written) and read as (tile 0, id 0).
-// Work is split by output, not by Gaussian: workgroup b writes outputs
-// [b * kEmitOut, (b + 1) * kEmitOut) of the depth-ordered stream, so a few
-// screen-sized splats near the camera (thousands of tiles each) spread over
-// many workgroups instead of stalling one.
+// The (tile, id) list before the tile sort holds, by position: the zero
+// slots (tile 0, id 0), which occupy [0, m - total_written), then every
+// depth-ordered Gaussian's tiles.  Work on it is split by position, not by
+// Gaussian: workgroup b handles positions [b * kEmitOut, (b + 1) * kEmitOut),
+// so a few screen-sized splats near the camera (thousands of tiles each)
+// spread over many workgroups instead of stalling one.
 constexpr int kEmitOut = 2048;
 
-// starts[b] = the depth-ordered position whose output range holds output
-// b * out, and starts[ceil(total / out)] = the last writing position.  Output
-// ranges [offs[p], offs[p] + w) tile [0, total) (writing Gaussians all come
-// before the ones that write nothing), so every entry is written by exactly
-// one lane.  Entries past max_b are not needed.
+// starts[b] = the depth-ordered Gaussian whose outputs hold list position
+// b * out, and starts[ceil((z + total) / out)] = the last writing Gaussian,
+// where z = m - total_written is the number of zero slots.  Output ranges
+// [offs[p], offs[p] + w) tile [0, total) (writing Gaussians all come before
+// the ones that write nothing), so every entry is written by exactly one
+// lane.  Entries past max_b are not needed.
 __global__ __launch_bounds__(256) void k_bin_mark_starts(int n, const uint32_t* __restrict__ offs,
                                                          const uint32_t* __restrict__ total_written,
+                                                         const uint32_t* __restrict__ m_total,
                                                          int max_b, int out,
                                                          uint32_t* __restrict__ starts) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    const uint32_t tw = *total_written, o = offs[p];
-    const uint32_t end = p + 1 < n ? offs[p + 1] : tw;
+    const uint32_t tw = *total_written, m = *m_total;
+    const uint32_t z = tw <= m ? m - tw : 0u;
+    const uint32_t o = z + offs[p];
+    const uint32_t end = z + (p + 1 < n ? offs[p + 1] : tw);
     if (end <= o) return;
     const uint32_t uo = (uint32_t)out;
     const uint32_t b0 = (o + uo - 1) / uo, b1 = (end - 1) / uo;
     for (uint32_t b = b0; b <= b1 && b <= (uint32_t)max_b; ++b) starts[b] = (uint32_t)p;
-    const uint32_t last = (tw + uo - 1) / uo;
-    if (end == tw && last <= (uint32_t)max_b) starts[last] = (uint32_t)p;
-}
-
-// Workgroup b: its Gaussians are positions starts[b] .. starts[b + 1] (at
-// most kEmitOut + 1 of them, each with at least one output in the block or
-// at its end), staged in LDS with their output offsets and tile rectangles.
-// Output k belongs to the last staged Gaussian whose first output is <= k.
-// Each thread takes kEmitOut / 256 consecutive outputs: a binary search in
-// LDS for the first, then a forward walk over the staged offsets.  A
-// Gaussian's tiles go row by row over its rectangle, capped at its count.
-// With id_bits > 0 each entry is one packed key (tile << id_bits | id).
-__global__ __launch_bounds__(256) void k_bin_emit(
-    int n, int cap, const uint32_t* __restrict__ m_total, const uint32_t* __restrict__ order,
-    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ total_written,
-    const uint32_t* __restrict__ starts, const float* __restrict__ centers,
-    const float* __restrict__ extents, int tiles_x, int tiles_y, int bw,
-    uint32_t* __restrict__ tiles_out, uint32_t* __restrict__ ids_out, int id_bits) {
-    constexpr int kStage = kEmitOut + 1;
-    constexpr int kPer = kEmitOut / 256;
-    __shared__ uint32_t s_off[kStage], s_g[kStage], s_xy[kStage], s_wd[kStage];
-    __shared__ __align__(16) uint32_t s_tile[kEmitOut];
-    __shared__ __align__(16) uint32_t s_id[kEmitOut];
-    const int tid = threadIdx.x;
-    const uint32_t tw = *total_written, m = *m_total;
-    const uint32_t k0 = (uint32_t)blockIdx.x * kEmitOut;
-    if (k0 >= tw) return;
-    const uint32_t k1 = min(k0 + (uint32_t)kEmitOut, tw);
-    const uint32_t zbase = tw <= m ? m - tw : 0u;
-    const uint32_t limit = min(m, (uint32_t)cap);
-    const uint32_t plo = starts[blockIdx.x];
-    const uint32_t phi = min(starts[blockIdx.x + 1], (uint32_t)n - 1u);
-    const int cnt = phi >= plo ? (int)min(phi - plo + 1u, (uint32_t)kStage) : 1;
-    for (int i = tid; i < cnt; i += 256) {
-        const uint32_t p = plo + (uint32_t)i;
-        const uint32_t g = order[p];
-        const float2 c = reinterpret_cast<const float2*>(centers)[g];
-        const float2 e = reinterpret_cast<const float2*>(extents)[g];
-        int x0, y0, x1, y1;
-        tile_rect(c.x, c.y, e.x, e.y, bw, tiles_x, tiles_y, x0, y0, x1, y1);
-        s_off[i] = offsets[p];
-        s_g[i] = g;
-        s_xy[i] = (uint32_t)x0 | ((uint32_t)y0 << 16);  // tile indices < 65536
-        s_wd[i] = (uint32_t)max(x1 - x0, 1);
-    }
-    __syncthreads();
-    // Each thread resolves kPer consecutive outputs: a binary search in LDS
-    // for the first, then a forward walk over the staged offsets (a
-    // Gaussian's outputs are consecutive, so the walk mostly stays put).
-    // The results go to LDS and leave in consecutive-lane order.
-    const uint32_t kt = k0 + (uint32_t)(kPer * tid);
-    if (kt < k1) {
-        int lo = 0, hi = cnt - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_off[mid] <= kt) lo = mid;
-            else hi = mid - 1;
-        }
-        const uint32_t kend = min(kt + (uint32_t)kPer, k1);
-        // the first output's place in its Gaussian's rectangle by one
-        // division, then row-major steps.  Moving on to the next Gaussian
-        // starts at its rectangle's corner: output k - 1 belonged to an
-        // earlier Gaussian, so the one holding k starts at k.
-        uint32_t wd = s_wd[lo], xy = s_xy[lo], gid = s_g[lo];
-        const uint32_t li = kt - s_off[lo];
-        uint32_t ry = li / wd, rx = li - ry * wd;
-        // the thread's kPer results leave as 16-byte LDS stores (slots past
-        // k1 hold stale values and are never read)
-        uint32_t ot[kPer], oi[kPer];
-#pragma unroll
-        for (int j = 0; j < kPer; ++j) {
-            const uint32_t k = kt + (uint32_t)j;
-            if (k < kend && lo + 1 < cnt && s_off[lo + 1] <= k) {
-                while (lo + 1 < cnt && s_off[lo + 1] <= k) ++lo;
-                wd = s_wd[lo];
-                xy = s_xy[lo];
-                gid = s_g[lo];
-                ry = 0;
-                rx = 0;
-            }
-            ot[j] = ((xy >> 16) + ry) * (uint32_t)tiles_x + (xy & 0xffffu) + rx;
-            oi[j] = gid;
-            if (++rx == wd) {
-                rx = 0;
-                ++ry;
-            }
-        }
-        uint4* const dt = reinterpret_cast<uint4*>(s_tile + kPer * tid);
-        uint4* const di = reinterpret_cast<uint4*>(s_id + kPer * tid);
-#pragma unroll
-        for (int q = 0; q < kPer / 4; ++q) {
-            dt[q] = make_uint4(ot[4 * q], ot[4 * q + 1], ot[4 * q + 2], ot[4 * q + 3]);
-            di[q] = make_uint4(oi[4 * q], oi[4 * q + 1], oi[4 * q + 2], oi[4 * q + 3]);
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < kPer; ++r) {
-        const uint32_t idx = (uint32_t)(r * 256 + tid), k = k0 + idx;
-        // entries past the capacity are dropped (the caller sees m > cap)
-        if (k < k1 && zbase + k < limit) {
-            if (id_bits > 0) {
-                tiles_out[zbase + k] = (s_tile[idx] << id_bits) | s_id[idx];
-            } else {
-                tiles_out[zbase + k] = s_tile[idx];
-                ids_out[zbase + k] = s_id[idx];
-            }
-        }
-    }
+    const uint32_t last = (z + tw + uo - 1) / uo;
+    if (end == z + tw && last <= (uint32_t)max_b) starts[last] = (uint32_t)p;
 }
 
 // Total intersect count sum(max(counts, 0)) into *m_total (zeroed beforehand by
@@ -1268,25 +1219,238 @@ __global__ __launch_bounds__(256) void k_count_total(int n, const int* __restric
     }
 }
 
-// Slots [0, cap) of the (tile, id) list before the tile sort: the zero slots
-// (tile 0, id 0) first, then the emitted entries (k_bin_emit), and past the
-// intersect total m (if m < cap) sentinel entries with tile = num_tiles that
-// sort last and belong to no bin.  Only the zero and sentinel ranges are
-// written: a grid-stride walk over their concatenation (packed keys when
-// id_bits > 0).
-__global__ __launch_bounds__(256) void k_zero_slots(int cap, const uint32_t* __restrict__ m_total,
-                                                    const uint32_t* __restrict__ total_written,
-                                                    int num_tiles, uint32_t* __restrict__ tiles_out,
-                                                    uint32_t* __restrict__ ids_out, int id_bits) {
-    const uint32_t m = *m_total, tw = *total_written, c = (uint32_t)cap;
-    const uint32_t z = min(tw <= m ? m - tw : 0u, c);
-    const uint32_t tail = m < c ? c - m : 0u;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < z + tail;
-         k += gridDim.x * blockDim.x) {
-        const uint32_t at = k < z ? k : m + (k - z);
-        tiles_out[at] = k < z ? 0u : (uint32_t)num_tiles << id_bits;
-        if (id_bits == 0) ids_out[at] = 0u;
+// The list before the tile sort, by position in [0, cap): zero slots in
+// [0, zpos), emitted output k at position z + k in [zpos, limit), sentinels
+// in [limit, cap).  z = m - total_written (the zero slots a Gaussian counts
+// but does not write), limit = min(m, cap).
+struct ListSpan {
+    uint32_t z, zpos, limit;
+};
+__device__ __forceinline__ ListSpan list_span(uint32_t m, uint32_t tw, uint32_t cap) {
+    ListSpan sp;
+    sp.z = tw <= m ? m - tw : 0u;
+    sp.limit = min(m, cap);
+    sp.zpos = min(sp.z, sp.limit);
+    return sp;
+}
+
+// Fused emission and first tile-sort pass.  The list is never written in
+// position order: k_tile_hist1 counts the first pass's digits (the low bits
+// of the tile) for each block of kEmitOut positions from the Gaussians'
+// rectangles, and k_emit_scatter1 generates each block's entries in LDS and
+// scatters them as the first pass would.  Blocks are the k_bin_mark_starts
+// blocks over positions (with m_total), so block b stages the depth-ordered
+// Gaussians starts[b] .. starts[b + 1] (from 0 when the block holds zero
+// slots).
+
+// Digit counts of block b: zero slots count for tile 0, sentinels for tile
+// num_tiles.  A Gaussian's outputs inside the block are a row-major stretch of
+// its tile rectangle, cut into one run of consecutive tiles per rectangle row;
+// a run of L tiles from tile t adds L >> bits to every digit and one to each
+// of the (L mod 2^bits) digits from t mod 2^bits on, cyclically, entered in a
+// difference array in LDS.  hist[d * nblocks + b] gets the count of digit d.
+// Block 0 also zeroes nzero words at zero (the scan's look-back states).
+__global__ __launch_bounds__(256) void k_tile_hist1(
+    int n, int cap, const uint32_t* __restrict__ m_total,
+    const uint32_t* __restrict__ total_written, const uint4* __restrict__ rec,
+    const uint32_t* __restrict__ offs, const uint32_t* __restrict__ starts, int tiles_x,
+    int num_tiles, int bits, uint32_t* __restrict__ hist, int nblocks,
+    uint64_t* __restrict__ zero, int nzero) {
+    __shared__ uint32_t diff[257];
+    __shared__ uint32_t lds[256 / kWave];
+    __shared__ uint32_t s_all;
+    const int tid = threadIdx.x;
+    if (blockIdx.x == 0)
+        for (int i = tid; i < nzero; i += 256) zero[i] = 0ull;
+    const uint32_t D = 1u << bits, dm = D - 1u;
+    diff[tid] = 0u;
+    if (tid == 0) {
+        diff[256] = 0u;
+        s_all = 0u;
     }
+    __syncthreads();
+    // the block's start entries are read with the totals (they are valid
+    // whenever the block holds emitted entries)
+    const uint32_t st0 = starts[blockIdx.x], st1 = starts[blockIdx.x + 1];
+    const uint32_t tw = *total_written;
+    const ListSpan sp = list_span(*m_total, tw, (uint32_t)cap);
+    const uint32_t p0 = blockIdx.x * (uint32_t)kEmitOut;
+    const uint32_t p1 = min(p0 + (uint32_t)kEmitOut, (uint32_t)cap);
+    if (tid == 0) {
+        const uint32_t ze = min(p1, sp.zpos);
+        if (ze > p0) {
+            atomicAdd(&diff[0], ze - p0);
+            atomicAdd(&diff[1], p0 - ze);
+        }
+        const uint32_t s0 = max(p0, sp.limit);
+        if (p1 > s0) {
+            const uint32_t sd = (uint32_t)num_tiles & dm;
+            atomicAdd(&diff[sd], p1 - s0);
+            atomicAdd(&diff[sd + 1], s0 - p1);
+        }
+    }
+    const uint32_t e0 = max(p0, sp.zpos), e1 = min(p1, sp.limit);
+    uint32_t all = 0;
+    if (e1 > e0) {
+        const uint32_t k0 = e0 - sp.z, k1 = e1 - sp.z;
+        const uint32_t plo = p0 <= sp.z ? 0u : st0;
+        const uint32_t phi = min(st1, (uint32_t)n - 1u);
+        const int cnt = phi >= plo ? (int)min(phi - plo + 1u, (uint32_t)kEmitOut + 1u) : 0;
+        for (int i = tid; i < cnt; i += 256) {
+            const uint32_t p = plo + (uint32_t)i;
+            const uint4 r = rec[p];
+            const uint32_t o = r.x;
+            const uint32_t end = p + 1 < (uint32_t)n ? offs[p + 1] : tw;
+            const uint32_t a = max(o, k0), bq = min(end, k1);
+            if (a >= bq) continue;
+            const uint32_t wd = r.w, x0 = r.z & 0xffffu, y0 = r.z >> 16;
+            const uint32_t la = a - o, lb = bq - 1u - o;
+            const uint32_t ra = la / wd, rb = lb / wd;
+            const uint32_t ca = la - ra * wd, cb = lb - rb * wd;
+            for (uint32_t row = ra; row <= rb; ++row) {
+                const uint32_t cs = row == ra ? ca : 0u, ce = row == rb ? cb : wd - 1u;
+                const uint32_t t0 = (y0 + row) * (uint32_t)tiles_x + x0 + cs;
+                const uint32_t len = ce - cs + 1u;
+                all += len >> bits;
+                const uint32_t rem = len & dm;
+                if (rem) {
+                    const uint32_t d0 = t0 & dm, d1 = d0 + rem;
+                    atomicAdd(&diff[d0], 1u);
+                    if (d1 <= D) {
+                        atomicAdd(&diff[d1], ~0u);
+                    } else {  // wraps: digits d0 .. D - 1 and 0 .. d1 - D - 1
+                        atomicAdd(&diff[0], 1u);
+                        atomicAdd(&diff[d1 - D], ~0u);
+                    }
+                }
+            }
+        }
+    }
+    if (all) atomicAdd(&s_all, all);
+    __syncthreads();
+    const uint32_t v = tid < (int)D ? diff[tid] : 0u;
+    uint32_t tot;
+    const uint32_t incl = block_exclusive_scan(v, lds, tot) + v;
+    if (tid < (int)D) hist[(size_t)tid * nblocks + blockIdx.x] = incl + s_all;
+}
+
+// Block b generates the entries of positions [b * kEmitOut, (b + 1) *
+// kEmitOut) as k_bin_emit would write them (a zero key, the emitted key, or
+// the sentinel key num_tiles << id_bits; with PACKED each emitted key is
+// tile << id_bits | id, else the key is the tile and the id goes with it as
+// its value), eight consecutive positions per thread, into LDS, then reads
+// them back in (wave, round, lane) order and scatters them by the digit
+// [id_bits, id_bits + bits) with bases from the scanned k_tile_hist1 counts.
+template <bool PACKED>
+__global__ __launch_bounds__(256) void k_emit_scatter1(
+    int n, int cap, const uint32_t* __restrict__ m_total,
+    const uint32_t* __restrict__ total_written, const uint4* __restrict__ rec,
+    const uint32_t* __restrict__ starts, int tiles_x, int num_tiles, int id_bits, int bits,
+    const uint32_t* __restrict__ hist, int nblocks, uint32_t* __restrict__ keys_out,
+    uint32_t* __restrict__ ids_out) {
+    constexpr int kStage = kEmitOut + 1;
+    constexpr int kPer = kEmitOut / 256;
+    static_assert(kPer == 8 && kEmitOut == kSortThreads * 8, "one sort tile per block");
+    __shared__ ScatterLds sl;
+    // staging; afterwards s_off / s_g hold the digit-sorted tile
+    __shared__ uint32_t s_off[kStage], s_g[kStage], s_xy[kStage], s_wd[kStage];
+    __shared__ __align__(16) uint32_t s_key[kEmitOut];
+    __shared__ __align__(16) uint32_t s_id[PACKED ? 4 : kEmitOut];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const int ndig = 1 << bits;
+    // independent loads first: this thread's digit base, the block's start
+    // entries (valid whenever the block holds emitted entries) and the totals
+    const uint32_t gb = tid < ndig ? hist[(size_t)tid * nblocks + blockIdx.x] : 0u;
+    const uint32_t st0 = starts[blockIdx.x], st1 = starts[blockIdx.x + 1];
+    const uint32_t tw = *total_written;
+    const ListSpan sp = list_span(*m_total, tw, (uint32_t)cap);
+    const uint32_t p0 = blockIdx.x * (uint32_t)kEmitOut;
+    const int count = (int)min((uint32_t)kEmitOut, (uint32_t)cap - p0);
+    const uint32_t e0 = max(p0, sp.zpos), e1 = min(p0 + (uint32_t)count, sp.limit);
+    int cnt = 0;
+    if (e1 > e0) {
+        const uint32_t plo = p0 <= sp.z ? 0u : st0;
+        const uint32_t phi = min(st1, (uint32_t)n - 1u);
+        cnt = phi >= plo ? (int)min(phi - plo + 1u, (uint32_t)kStage) : 1;
+        for (int i = tid; i < cnt; i += 256) {
+            const uint4 r = rec[plo + (uint32_t)i];
+            s_off[i] = r.x;
+            s_g[i] = r.y;
+            s_xy[i] = r.z;
+            s_wd[i] = r.w;
+        }
+    }
+    __syncthreads();
+    const uint32_t sentinel = (uint32_t)num_tiles << id_bits;
+    const uint32_t q0 = p0 + (uint32_t)(kPer * tid);
+    uint32_t ot[kPer], oi[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        ot[j] = q0 + (uint32_t)j < sp.zpos ? 0u : sentinel;
+        oi[j] = 0u;
+    }
+    // this thread's emitted positions [a0, a1): the first one's Gaussian by a
+    // binary search over the staged offsets, then a forward walk
+    const uint32_t a0 = max(q0, e0), a1 = min(q0 + (uint32_t)kPer, e1);
+    if (a0 < a1) {
+        const uint32_t kt = a0 - sp.z;
+        int lo = 0, hi = cnt - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= kt) lo = mid;
+            else hi = mid - 1;
+        }
+        uint32_t wd = s_wd[lo], xy = s_xy[lo], gid = s_g[lo];
+        const uint32_t li = kt - s_off[lo];
+        uint32_t ry = li / wd, rx = li - ry * wd;
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const uint32_t q = q0 + (uint32_t)j;
+            if (q >= a0 && q < a1) {
+                const uint32_t k = q - sp.z;
+                if (lo + 1 < cnt && s_off[lo + 1] <= k) {
+                    while (lo + 1 < cnt && s_off[lo + 1] <= k) ++lo;
+                    wd = s_wd[lo];
+                    xy = s_xy[lo];
+                    gid = s_g[lo];
+                    ry = 0;
+                    rx = 0;
+                }
+                const uint32_t tile = ((xy >> 16) + ry) * (uint32_t)tiles_x + (xy & 0xffffu) + rx;
+                if (PACKED) {
+                    ot[j] = (tile << id_bits) | gid;
+                } else {
+                    ot[j] = tile;
+                    oi[j] = gid;
+                }
+                if (++rx == wd) {
+                    rx = 0;
+                    ++ry;
+                }
+            }
+        }
+    }
+    {
+        uint4* const dk = reinterpret_cast<uint4*>(s_key + kPer * tid);
+        dk[0] = make_uint4(ot[0], ot[1], ot[2], ot[3]);
+        dk[1] = make_uint4(ot[4], ot[5], ot[6], ot[7]);
+        if (!PACKED) {
+            uint4* const di = reinterpret_cast<uint4*>(s_id + kPer * tid);
+            di[0] = make_uint4(oi[0], oi[1], oi[2], oi[3]);
+            di[1] = make_uint4(oi[4], oi[5], oi[6], oi[7]);
+        }
+    }
+    __syncthreads();
+    uint32_t key[kPer], val[kPer];
+    const int wbase = wave * kPer * kWave;
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const int li = wbase + r * kWave + lane;
+        key[r] = s_key[li];
+        val[r] = PACKED ? 0u : s_id[li];
+    }
+    tile_rank_scatter<uint32_t, kPer>(key, val, count, !PACKED, 0u, id_bits, bits, gb, keys_out,
+                                      ids_out, (uint32_t)cap, sl, s_off, s_g);
 }
 
 static inline int bits_for(int v) {  // bits needed to represent 0..v-1
@@ -1296,28 +1460,33 @@ static inline int bits_for(int v) {  // bits needed to represent 0..v-1
 }
 
 struct BinLayout {
-    size_t off[9];
+    size_t off[10];
     size_t total;
 };
 
 static int emit_blocks(int cap) { return (cap + kEmitOut - 1) / kEmitOut; }
 
+// first tile-pass digit counts: up to 256 digits per emission block, then the
+// look-back states of their scan
+static size_t hist1_words(int cap) { return (size_t)256 * emit_blocks(cap); }
+
 static BinLayout bin_layout(int n, int m) {
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     BinLayout L;
-    size_t sizes[9] = {
+    size_t sizes[10] = {
         up(4 * (size_t)n) * 2,  // depth keys in/out
         up(4 * (size_t)n) * 2,  // order in/out
-        up(4 * (size_t)n),      // written
+        up(16 * (size_t)n) * 2,  // per-Gaussian info, per-position records
         up(4 * (size_t)n + 8) + up(8 * ((size_t)wr_scan_tiles(n) + 1)),  // offsets (+ totals), scan states
         up(4 * (size_t)m) * 2,  // tile keys in/out
-        up(4 * (size_t)m),      // ids in (pre-sort)
+        up(4 * (size_t)m),      // ids after the first tile pass
         up(std::max(radix_workspace_bytes<uint32_t>(n), radix_workspace_bytes<uint32_t>(m))),
         up(4 * ((size_t)emit_blocks(m) + 2)),  // emission block starts
+        up(4 * hist1_words(m)) + up(8 * ((size_t)wr_scan_tiles((int)hist1_words(m)) + 1)),
         256,
     };
     size_t acc = 0;
-    for (int i = 0; i < 9; ++i) {
+    for (int i = 0; i < 10; ++i) {
         L.off[i] = acc;
         acc += sizes[i];
     }
@@ -1362,7 +1531,8 @@ hipError_t bin_gaussians(int n, int cap, const float* centers, const float* exte
     uint32_t* dkey2 = (uint32_t*)(b + L.off[0] + up(4 * (size_t)n));
     uint32_t* ord = (uint32_t*)(b + L.off[1]);
     uint32_t* ord2 = (uint32_t*)(b + L.off[1] + up(4 * (size_t)n));
-    int* written = (int*)(b + L.off[2]);
+    uint4* info = (uint4*)(b + L.off[2]);
+    uint4* rec = (uint4*)(b + L.off[2] + up(16 * (size_t)n));
     uint32_t* offs = (uint32_t*)(b + L.off[3]);
     uint32_t* total = offs + n;       // entries written by Gaussians
     uint32_t* m_total = offs + n + 1; // intersect total (written + zero slots)
@@ -1372,7 +1542,7 @@ hipError_t bin_gaussians(int n, int cap, const float* centers, const float* exte
     void* rws = (void*)(b + L.off[6]);
     const size_t rws_bytes = up(std::max(radix_workspace_bytes<uint32_t>(n),
                                          radix_workspace_bytes<uint32_t>(cap)));
-    const int gn = (n + 255) / 256, gm = (cap + 255) / 256;
+    const int gn = (n + 255) / 256;
 
     if (m_out) {
         // capacity mode: the total is counted on the device
@@ -1384,7 +1554,7 @@ hipError_t bin_gaussians(int n, int cap, const float* centers, const float* exte
         if ((e = fill<uint32_t>(m_total, 1, (uint32_t)cap, s)) != hipSuccess) return e;
     }
     k_bin_prepare<<<gn, 256, 0, s>>>(n, centers, extents, depths, counts, tiles_x, tiles_y, bw,
-                                     dkey, ord, written);
+                                     dkey, ord, info);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     e = radix_sort_pairs<uint32_t>(n, dkey, ord, dkey2, ord2, 32, rws, rws_bytes, s);
     if (e != hipSuccess) return e;
@@ -1392,7 +1562,7 @@ hipError_t bin_gaussians(int n, int cap, const float* centers, const float* exte
         const int st = wr_scan_tiles(n);
         uint64_t* states = (uint64_t*)(b + L.off[3] + up(4 * (size_t)n + 8));
         if ((e = fill<uint64_t>(states, (size_t)st + 1, 0ull, s)) != hipSuccess) return e;
-        k_scan_written<<<st, 256, 0, s>>>(n, ord2, written, offs, total, states, st);
+        k_scan_written<<<st, 256, 0, s>>>(n, ord2, info, offs, rec, total, states, st);
     }
     // the sentinel tile (num_tiles) needs one more key value; when the tile
     // and the Gaussian id fit one 32-bit word together, the entries are sorted
@@ -1400,21 +1570,46 @@ hipError_t bin_gaussians(int n, int cap, const float* centers, const float* exte
     // the bin-edge pass
     const int tile_bits = bits_for(num_tiles + 1), id_bits = bits_for(n);
     const int pk = (id_bits > 0 && tile_bits + id_bits <= 32) ? id_bits : 0;
-    k_zero_slots<<<std::min(gm, 1024), 256, 0, s>>>(cap, m_total, total, num_tiles, tkey, tid_in,
-                                                    pk);
+    // the emission is fused with the first tile pass (digit width bits1, as
+    // the sort would split tile_bits); later passes run as a sort from bit
+    // pk + bits1
+    const int tpasses = (tile_bits + 7) / 8;
+    const int bits1 = (tile_bits + tpasses - 1) / tpasses;
     const int eb = emit_blocks(cap);
     uint32_t* starts = (uint32_t*)(b + L.off[7]);
-    k_bin_mark_starts<<<gn, 256, 0, s>>>(n, offs, total, eb, kEmitOut, starts);
-    k_bin_emit<<<eb, 256, 0, s>>>(n, cap, m_total, ord2, offs, total, starts, centers, extents,
-                                  tiles_x, tiles_y, bw, tkey, tid_in, pk);
+    uint32_t* hist1 = (uint32_t*)(b + L.off[8]);
+    uint64_t* hstates = (uint64_t*)(b + L.off[8] + up(4 * hist1_words(cap)));
+    const int hcount = (1 << bits1) * eb;
+    const bool hlb = hcount > kScanSingleMax;
+    const int hlbt = wr_scan_tiles(hcount);
+    k_bin_mark_starts<<<gn, 256, 0, s>>>(n, offs, total, m_total, eb, kEmitOut, starts);
+    k_tile_hist1<<<eb, 256, 0, s>>>(n, cap, m_total, total, rec, offs, starts, tiles_x, num_tiles,
+                                    bits1, hist1, eb, hlb ? hstates : nullptr, hlb ? hlbt + 1 : 0);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (hlb) {
+        k_scan_lookback<<<hlbt, 256, 0, s>>>(hist1, hcount, hstates, hlbt);
+    } else if ((e = scan_exclusive(hist1, hcount, nullptr, nullptr, s)) != hipSuccess) {
+        return e;
+    }
+    const bool more = tile_bits > bits1;
+    uint32_t* k1 = more ? tkey : tkey2;
     if (pk)
-        e = radix_sort_pairs<uint32_t>(cap, tkey, nullptr, tkey2, nullptr, pk + tile_bits, rws,
-                                       rws_bytes, s, pk);
+        k_emit_scatter1<true><<<eb, 256, 0, s>>>(n, cap, m_total, total, rec, starts, tiles_x,
+                                                 num_tiles, pk, bits1, hist1, eb, k1, nullptr);
     else
-        e = radix_sort_pairs<uint32_t>(cap, tkey, tid_in, tkey2, (uint32_t*)ids_sorted, tile_bits,
-                                       rws, rws_bytes, s);
-    if (e != hipSuccess) return e;
+        k_emit_scatter1<false><<<eb, 256, 0, s>>>(n, cap, m_total, total, rec, starts, tiles_x,
+                                                  num_tiles, 0, bits1, hist1, eb, k1,
+                                                  more ? tid_in : (uint32_t*)ids_sorted);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (more) {
+        if (pk)
+            e = radix_sort_pairs<uint32_t>(cap, tkey, nullptr, tkey2, nullptr, pk + tile_bits,
+                                           rws, rws_bytes, s, pk + bits1);
+        else
+            e = radix_sort_pairs<uint32_t>(cap, tkey, tid_in, tkey2, (uint32_t*)ids_sorted,
+                                           tile_bits, rws, rws_bytes, s, bits1);
+        if (e != hipSuccess) return e;
+    }
     k_bin_edges32<<<(cap + 1023) / 1024, 256, 0, s>>>(
         cap, tkey2, num_tiles, tile_bins,
         (reinterpret_cast<uintptr_t>(tkey2) & 15) == 0 &&

@@ -112,6 +112,14 @@ def test_binning_1080p_200k(oracle):
     assert _binning_case(oracle, s, 16) > 1_000_000
 
 
+def test_binning_unpacked_keys(oracle):
+    # 1080p at 8-pixel tiles (32400 tiles, 15 key bits) with 300k Gaussians
+    # (19 id bits): tile and id do not fit one 32-bit key, so the list is
+    # sorted as (tile, id) pairs, over two passes of 8 and 7 bits
+    s = scene(2, num_points=300_000, seed=21)
+    assert _binning_case(oracle, s, 8) > 1_000_000
+
+
 def test_count_intersects(oracle):
     # gstex_count_intersects: the intersect total of compute_cumulative_intersects
     # at full C2 size, and for no Gaussians
