@@ -251,22 +251,31 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(int n, const K* __r
                                                              int nblocks,
                                                              uint64_t* __restrict__ zero,
                                                              int nzero) {
-    __shared__ uint32_t cnt[256];
-    const int tid = threadIdx.x;
+    // one count per digit and wave, so that the waves' LDS atomics do not
+    // contend with each other
+    constexpr int kWaves = kSortThreads / kWave;
+    __shared__ uint32_t cnt[kWaves][256];
+    const int tid = threadIdx.x, wave = tid / kWave;
     // block 0 also clears the look-back states of the scan that follows
     if (blockIdx.x == 0)
         for (int i = tid; i < nzero; i += kSortThreads) zero[i] = 0ull;
-    cnt[tid] = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) cnt[w][tid] = 0;
     __syncthreads();
     const uint32_t mask = (1u << bits) - 1u;
     const size_t base = (size_t)blockIdx.x * (kSortThreads * ITEMS);
 #pragma unroll 4
     for (int r = 0; r < ITEMS; ++r) {
         const size_t idx = base + (size_t)r * kSortThreads + tid;
-        if (idx < (size_t)n) atomicAdd(&cnt[digit_of(keys[idx] ^ flip, shift, mask)], 1u);
+        if (idx < (size_t)n) atomicAdd(&cnt[wave][digit_of(keys[idx] ^ flip, shift, mask)], 1u);
     }
     __syncthreads();
-    if (tid < (1 << bits)) hist[(size_t)tid * nblocks + blockIdx.x] = cnt[tid];
+    if (tid < (1 << bits)) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) c += cnt[w][tid];
+        hist[(size_t)tid * nblocks + blockIdx.x] = c;
+    }
 }
 
 // Exclusive scan of up to 64K uint32 values in place by one 1024-lane
@@ -538,7 +547,8 @@ __device__ __forceinline__ void tile_rank_scatter(const K (&key)[ITEMS],
     }
 }
 
-template <typename K, int ITEMS>
+// VALS false: keys only (vals_in / vals_out unused), and no LDS for values.
+template <typename K, int ITEMS, bool VALS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     int n, const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     K* __restrict__ keys_out, uint32_t* __restrict__ vals_out, K flip, int shift, int bits,
@@ -546,7 +556,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     constexpr int kTile = kSortThreads * ITEMS;
     __shared__ ScatterLds sl;
     __shared__ K skeys[kTile];
-    __shared__ uint32_t svals[kTile];
+    __shared__ uint32_t svals[VALS ? kTile : 1];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const size_t base = (size_t)blockIdx.x * kTile;
     const int count = (int)min((size_t)kTile, (size_t)n - base);
@@ -554,7 +564,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
 
     K key[ITEMS];
     uint32_t val[ITEMS];
-    const bool has_vals = vals_in != nullptr;  // keys only when null
+    constexpr bool has_vals = VALS;
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
         const int li = wbase + r * kWave + lane;
@@ -1014,12 +1024,18 @@ hipError_t radix_sort_pairs(int n, const K* keys_in, const uint32_t* vals_in, K*
             hipError_t es = scan_exclusive(hist, hcount, nullptr, partials, s);
             if (es != hipSuccess) return es;
         }
-        if (small)
-            k_radix_scatter<K, 8><<<nblocks, kSortThreads, 0, s>>>(n, sk, sv, dk, dv, flip, shift,
-                                                                    bits, hist, nblocks);
+        if (small && sv)
+            k_radix_scatter<K, 8, true><<<nblocks, kSortThreads, 0, s>>>(
+                n, sk, sv, dk, dv, flip, shift, bits, hist, nblocks);
+        else if (small)
+            k_radix_scatter<K, 8, false><<<nblocks, kSortThreads, 0, s>>>(
+                n, sk, sv, dk, dv, flip, shift, bits, hist, nblocks);
+        else if (sv)
+            k_radix_scatter<K, 16, true><<<nblocks, kSortThreads, 0, s>>>(
+                n, sk, sv, dk, dv, flip, shift, bits, hist, nblocks);
         else
-            k_radix_scatter<K, 16><<<nblocks, kSortThreads, 0, s>>>(n, sk, sv, dk, dv, flip,
-                                                                     shift, bits, hist, nblocks);
+            k_radix_scatter<K, 16, false><<<nblocks, kSortThreads, 0, s>>>(
+                n, sk, sv, dk, dv, flip, shift, bits, hist, nblocks);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         sk = dk;
@@ -1348,14 +1364,26 @@ __global__ __launch_bounds__(256) void k_emit_scatter1(
     const uint32_t* __restrict__ starts, int tiles_x, int num_tiles, int id_bits, int bits,
     const uint32_t* __restrict__ hist, int nblocks, uint32_t* __restrict__ keys_out,
     uint32_t* __restrict__ ids_out) {
-    constexpr int kStage = kEmitOut + 1;
+    // Gaussians with an output in the block: at most one per position (of
+    // starts[b] .. starts[b + 1], the last one is left out when there are
+    // more: it then starts after the block)
+    constexpr int kStage = kEmitOut;
     constexpr int kPer = kEmitOut / 256;
     static_assert(kPer == 8 && kEmitOut == kSortThreads * 8, "one sort tile per block");
-    __shared__ ScatterLds sl;
-    // staging; afterwards s_off / s_g hold the digit-sorted tile
-    __shared__ uint32_t s_off[kStage], s_g[kStage], s_xy[kStage], s_wd[kStage];
-    __shared__ __align__(16) uint32_t s_key[kEmitOut];
-    __shared__ __align__(16) uint32_t s_id[PACKED ? 4 : kEmitOut];
+    // One LDS buffer (40 KB with packed keys, four workgroups per CU): the
+    // staged Gaussians (offset, id, corner, width) and the generated keys (and
+    // ids).  Once generated, s_off / s_g hold the digit-sorted tile and the
+    // scatter's counters take the place of s_xy / s_wd.
+    constexpr int kWords = 4 * kStage + (PACKED ? 1 : 2) * kEmitOut;
+    __shared__ __align__(16) uint32_t lds[kWords];
+    uint32_t* const s_off = lds;
+    uint32_t* const s_g = lds + kStage;
+    uint32_t* const s_xy = lds + 2 * kStage;
+    uint32_t* const s_wd = lds + 3 * kStage;
+    uint32_t* const s_key = lds + 4 * kStage;
+    uint32_t* const s_id = s_key + kEmitOut;  // without PACKED
+    static_assert(sizeof(ScatterLds) <= 2 * kStage * sizeof(uint32_t), "counters fit s_xy, s_wd");
+    ScatterLds& sl = *reinterpret_cast<ScatterLds*>(s_xy);
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const int ndig = 1 << bits;
     // independent loads first: this thread's digit base, the block's start
