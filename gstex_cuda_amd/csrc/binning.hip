@@ -251,29 +251,49 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(int n, const K* __r
                                                              int nblocks,
                                                              uint64_t* __restrict__ zero,
                                                              int nzero) {
-    // one count per digit and wave, so that the waves' LDS atomics do not
-    // contend with each other
-    constexpr int kWaves = kSortThreads / kWave;
-    __shared__ uint32_t cnt[kWaves][256];
-    const int tid = threadIdx.x, wave = tid / kWave;
+    // 16 copies of the counts (one per wave and lane quarter, 257 words
+    // apart so that a digit's copies sit in different banks): lanes that
+    // meet on one digit mostly add into different words
+    constexpr int kCopies = 16, kStride = 257;
+    __shared__ uint32_t cnt_all[kCopies * kStride];
+    const int tid = threadIdx.x;
+    uint32_t* const cnt = cnt_all + (tid >> 4) * kStride;
     // block 0 also clears the look-back states of the scan that follows
     if (blockIdx.x == 0)
         for (int i = tid; i < nzero; i += kSortThreads) zero[i] = 0ull;
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) cnt[w][tid] = 0;
+    for (int i = tid; i < kCopies * kStride; i += kSortThreads) cnt_all[i] = 0;
     __syncthreads();
     const uint32_t mask = (1u << bits) - 1u;
-    const size_t base = (size_t)blockIdx.x * (kSortThreads * ITEMS);
+    constexpr int kTile = kSortThreads * ITEMS;
+    constexpr int kVec = 16 / sizeof(K);  // keys per 16-byte load
+    const size_t base = (size_t)blockIdx.x * kTile;
+    if (base + kTile <= (size_t)n && (reinterpret_cast<uintptr_t>(keys) & 15) == 0) {
+        // a full tile: every load issued up front, 16 bytes per lane
+        using V = uint4;
+        V v[ITEMS / kVec];
+        const V* kv = reinterpret_cast<const V*>(keys + base);
+#pragma unroll
+        for (int q = 0; q < ITEMS / kVec; ++q) v[q] = kv[q * kSortThreads + tid];
+#pragma unroll
+        for (int q = 0; q < ITEMS / kVec; ++q) {
+            const K* k = reinterpret_cast<const K*>(&v[q]);
+#pragma unroll
+            for (int j = 0; j < kVec; ++j)
+                atomicAdd(&cnt[digit_of(k[j] ^ flip, shift, mask)], 1u);
+        }
+    } else {
 #pragma unroll 4
-    for (int r = 0; r < ITEMS; ++r) {
-        const size_t idx = base + (size_t)r * kSortThreads + tid;
-        if (idx < (size_t)n) atomicAdd(&cnt[wave][digit_of(keys[idx] ^ flip, shift, mask)], 1u);
+        for (int r = 0; r < ITEMS; ++r) {
+            const size_t idx = base + (size_t)r * kSortThreads + tid;
+            if (idx < (size_t)n)
+                atomicAdd(&cnt[digit_of(keys[idx] ^ flip, shift, mask)], 1u);
+        }
     }
     __syncthreads();
     if (tid < (1 << bits)) {
         uint32_t c = 0;
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) c += cnt[w][tid];
+        for (int w = 0; w < kCopies; ++w) c += cnt_all[w * kStride + tid];
         hist[(size_t)tid * nblocks + blockIdx.x] = c;
     }
 }
@@ -1092,13 +1112,19 @@ hipError_t tile_bin_edges(int m, const int64_t* keys_sorted, int num_tiles, int*
 
 // Per Gaussian: its depth key, and info[g] = (entries it writes (rectangle,
 // capped by its count), rectangle corner x0 | y0 << 16, rectangle width, 0);
-// Gaussians that write nothing sort last.
+// Gaussians that write nothing sort last.  The launch also clears the tile
+// bins (nbins ints) and the offset scan's look-back states (nstates words),
+// and with m_exact >= 0 sets *m_total = m_exact.
 __global__ __launch_bounds__(256) void k_bin_prepare(
     int n, const float* __restrict__ centers, const float* __restrict__ extents,
     const float* __restrict__ depths, const int* __restrict__ counts, int tiles_x, int tiles_y,
     int bw, uint32_t* __restrict__ depth_keys, uint32_t* __restrict__ order,
-    uint4* __restrict__ info) {
+    uint4* __restrict__ info, int* __restrict__ bins, int nbins, uint64_t* __restrict__ states,
+    int nstates, uint32_t* __restrict__ m_total, int m_exact) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int i = g; i < nbins; i += gridDim.x * blockDim.x) bins[i] = 0;
+    for (int i = g; i < nstates; i += gridDim.x * blockDim.x) states[i] = 0ull;
+    if (g == 0 && m_exact >= 0) *m_total = (uint32_t)m_exact;
     if (g >= n) return;
     const float2 c = reinterpret_cast<const float2*>(centers)[g];
     const float2 e = reinterpret_cast<const float2*>(extents)[g];
@@ -1536,7 +1562,8 @@ hipError_t bin_gaussians(int n, int cap, const float* centers, const float* exte
                          int* ids_sorted, int* tile_bins, unsigned* m_out, void* ws,
                          size_t ws_bytes, hipStream_t s) {
     const int num_tiles = tiles_x * tiles_y;
-    hipError_t e = fill<int>(tile_bins, (size_t)num_tiles * 2, 0, s);
+    hipError_t e = hipSuccess;
+    if (n <= 0 || cap <= 0) e = fill<int>(tile_bins, (size_t)num_tiles * 2, 0, s);
     if (e != hipSuccess || n <= 0) return e;
     if (cap <= 0) {
         if (m_out) {
@@ -1572,26 +1599,22 @@ hipError_t bin_gaussians(int n, int cap, const float* centers, const float* exte
                                          radix_workspace_bytes<uint32_t>(cap)));
     const int gn = (n + 255) / 256;
 
+    // capacity mode: the total is counted on the device; exact size: the
+    // caller's num_intersects is that total (a wrong one still keeps every
+    // write inside the list and the zero-slot range), set by k_bin_prepare
     if (m_out) {
-        // capacity mode: the total is counted on the device
         if ((e = fill<uint32_t>(m_total, 1, 0u, s)) != hipSuccess) return e;
         k_count_total<<<std::min(gn, kCountBlocks), 256, 0, s>>>(n, counts, m_total);
-    } else {
-        // exact size: the caller's num_intersects is that total (a wrong one
-        // still keeps every write inside the list and the zero-slot range)
-        if ((e = fill<uint32_t>(m_total, 1, (uint32_t)cap, s)) != hipSuccess) return e;
     }
+    const int st = wr_scan_tiles(n);
+    uint64_t* states = (uint64_t*)(b + L.off[3] + up(4 * (size_t)n + 8));
     k_bin_prepare<<<gn, 256, 0, s>>>(n, centers, extents, depths, counts, tiles_x, tiles_y, bw,
-                                     dkey, ord, info);
+                                     dkey, ord, info, tile_bins, 2 * num_tiles, states, st + 1,
+                                     m_total, m_out ? -1 : cap);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     e = radix_sort_pairs<uint32_t>(n, dkey, ord, dkey2, ord2, 32, rws, rws_bytes, s);
     if (e != hipSuccess) return e;
-    {
-        const int st = wr_scan_tiles(n);
-        uint64_t* states = (uint64_t*)(b + L.off[3] + up(4 * (size_t)n + 8));
-        if ((e = fill<uint64_t>(states, (size_t)st + 1, 0ull, s)) != hipSuccess) return e;
-        k_scan_written<<<st, 256, 0, s>>>(n, ord2, info, offs, rec, total, states, st);
-    }
+    k_scan_written<<<st, 256, 0, s>>>(n, ord2, info, offs, rec, total, states, st);
     // the sentinel tile (num_tiles) needs one more key value; when the tile
     // and the Gaussian id fit one 32-bit word together, the entries are sorted
     // as packed keys alone (half the bytes per pass), and the ids come out of
