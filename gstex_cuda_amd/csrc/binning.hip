@@ -1376,9 +1376,9 @@ __global__ __launch_bounds__(256) void k_tile_hist1(
     if (tid < (int)D) hist[(size_t)tid * nblocks + blockIdx.x] = incl + s_all;
 }
 
-// Block b generates the entries of positions [b * kEmitOut, (b + 1) *
-// kEmitOut) as k_bin_emit would write them (a zero key, the emitted key, or
-// the sentinel key num_tiles << id_bits; with PACKED each emitted key is
+// Block b generates the list entries of positions [b * kEmitOut, (b + 1) *
+// kEmitOut) (a zero key, the emitted key, or the sentinel key
+// num_tiles << id_bits; with PACKED each emitted key is
 // tile << id_bits | id, else the key is the tile and the id goes with it as
 // its value), eight consecutive positions per thread, into LDS, then reads
 // them back in (wave, round, lane) order and scatters them by the digit
