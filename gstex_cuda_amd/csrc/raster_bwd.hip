@@ -21,6 +21,37 @@ namespace gstex {
 // count.  The plain kernels for CM <= 4 are held to 96 VGPRs (5 waves per
 // SIMD), which they need at most one register more than.
 //
+// Texel gradients of the window route over the R x Q texels from chart row
+// ou, column ov (R, Q <= 3): window tap (row r, column q) has weight
+// A[r] * B[q], and rows and columns from R, Q on carry zero weight on every
+// valid lane.  One reduction of the R * Q * CM values; the lane holding a
+// wave total adds it to its texel.
+template <int R, int Q, int CM>
+__device__ __forceinline__ void window_texel_add(const float (&A)[3], const float (&B)[3],
+                                                 const float (&vval)[CM], int lane,
+                                                 float* vtex_chart, int ou, int ov, int h,
+                                                 int w) {
+    float vals[R * Q * CM];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const float ab = A[r] * B[q];
+#pragma unroll
+            for (int c = 0; c < CM; ++c) vals[(Q * r + q) * CM + c] = ab * vval[c];
+        }
+    const float tot = wave_reduce_scatter(vals, lane);
+    const int code = rs_slot<R * Q * CM>(lane);
+    if (code < 64 && tot != 0.f) {
+        const int rq = code / CM, c = code - CM * rq;
+        const int r = rq / Q, q = rq - Q * r;
+        // a window row or column past the chart edge carries no weight
+        // (taps clamp inside the chart)
+        if (ou + r < h && ov + q < w)
+            unsafeAtomicAdd(vtex_chart + ((ou + r) * w + (ov + q)) * CM + c, tot);
+    }
+}
+
 // raster_backward_wave runs one wave's share of a tile (wave `wave` of the
 // tile, as in the forward); `acc` is this hardware wave's LDS texel
 // accumulator, zero on entry and left zero.
@@ -294,25 +325,22 @@ __device__ __forceinline__ void raster_backward_wave(const RasterArgs& a, const 
                     A[r] = (ri == r ? wa0 : 0.f) + (rj == r ? wa1 : 0.f);
                     Bw[r] = (ci == r ? wb0 : 0.f) + (cj == r ? wb1 : 0.f);
                 }
-                float vals[9 * CM];
-#pragma unroll
-                for (int r = 0; r < 3; ++r)
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        const float ab = A[r] * Bw[q];
-#pragma unroll
-                        for (int c = 0; c < CM; ++c) vals[(3 * r + q) * CM + c] = ab * vval[c];
+                // when no valid lane's taps reach window row (column) 2, its
+                // weights are zero on every lane and the window is 2 x 3 (3 x 2);
+                // not for 4 channels, whose kernel has no registers to spare
+                if constexpr (CM <= 3) {
+                    if (!__any(valid && rj == 2)) {
+                        window_texel_add<2, 3, CM>(A, Bw, vval, lane, vtex_chart, ou, ov, s.h,
+                                                   s.w);
+                        return;
                     }
-                const float tot = wave_reduce_scatter(vals, lane);
-                const int code = rs_slot<9 * CM>(lane);
-                if (code < 64 && tot != 0.f) {
-                    const int rq = code / CM, c = code - CM * rq;
-                    const int r = rq / 3, q = rq - 3 * r;
-                    // a window row or column past the chart edge carries
-                    // no weight (taps clamp inside the chart)
-                    if (ou + r < s.h && ov + q < s.w)
-                        unsafeAtomicAdd(vtex_chart + ((ou + r) * s.w + (ov + q)) * CM + c, tot);
+                    if (!__any(valid && cj == 2)) {
+                        window_texel_add<3, 2, CM>(A, Bw, vval, lane, vtex_chart, ou, ov, s.h,
+                                                   s.w);
+                        return;
+                    }
                 }
+                window_texel_add<3, 3, CM>(A, Bw, vval, lane, vtex_chart, ou, ov, s.h, s.w);
                 return;
             }
             if (tex_lds) {
