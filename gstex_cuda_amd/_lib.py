@@ -79,6 +79,11 @@ _SIGNATURES = {
     "gstex_raster_forward": (I, [I, I, I, I, I, P, I, P, I, P, P, I, I, P, P, F, F, F, F, P,
                                  P, P, P, P, P, P, P, P, P,  # nine outputs
                                  P, Z, P]),  # wave_masks, bytes, stream
+    "gstex_raster_backward_l2": (I, [I, I, I, I, I, P, I, P, I, P, P, I, I, P, P, F, F, F, F, P,
+                                     P, P, P, P,  # final_Ts, final_idx, depth_idx, final_s
+                                     P, P, P, F,  # out_img, out_texture, target, scale
+                                     P, P, P, P,  # v_output_depth, _reg, _alpha, _normal
+                                     P, P, P, Z, P]),  # grad_records, v_texture, masks, bytes, stream
     "gstex_raster_backward": (I, [I, I, I, I, I, P, I, P, I, P, P, I, I, P, P, F, F, F, F, P,
                                   P, P, P, P,  # final_Ts, final_idx, depth_idx, final_s
                                   P, P, P, P, P, P,  # six output gradients

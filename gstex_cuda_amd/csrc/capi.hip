@@ -584,20 +584,20 @@ int gstex_raster_forward(int img_width, int img_height, int block_width, int cha
                      "raster_forward");
 }
 
-int gstex_raster_backward(int img_width, int img_height, int block_width, int channels,
-                          int settings, const void* records, int num_points,
-                          const int32_t* gaussian_ids_sorted, int num_ids,
-                          const int32_t* tile_bins, const void* texture, int num_texels,
-                          int texture_format,
-                          const float* viewmat, const float* c2w, float fx, float fy, float cx,
-                          float cy, const float* background, const float* final_Ts,
-                          const int32_t* final_idx, const int32_t* depth_idx,
-                          const float* final_s, const float* v_output,
-                          const float* v_output_depth, const float* v_output_reg,
-                          const float* v_output_alpha, const float* v_output_texture,
-                          const float* v_output_normal, float* grad_records, float* v_texture,
-                          const uint64_t* wave_masks, size_t wave_masks_bytes,
-                          void* stream) {
+static int raster_backward_impl(int img_width, int img_height, int block_width, int channels,
+                                int settings, const void* records, int num_points,
+                                const int32_t* gaussian_ids_sorted, int num_ids,
+                                const int32_t* tile_bins, const void* texture, int num_texels,
+                                int texture_format, const float* viewmat, const float* c2w,
+                                float fx, float fy, float cx, float cy, const float* background,
+                                const float* final_Ts, const int32_t* final_idx,
+                                const int32_t* depth_idx, const float* final_s,
+                                const float* v_output, const float* v_output_depth,
+                                const float* v_output_reg, const float* v_output_alpha,
+                                const float* v_output_texture, const float* v_output_normal,
+                                float* grad_records, float* v_texture,
+                                const uint64_t* wave_masks, size_t wave_masks_bytes,
+                                void* stream, const float* loss_target, float loss_scale) {
     const int tx = img_width > 0 && block_width > 0 ? tiles_of(img_width, block_width) : 0;
     const int ty = img_height > 0 && block_width > 0 ? tiles_of(img_height, block_width) : 0;
     int rc = check_raster_common(tx, ty, block_width, img_width, img_height, 1, channels,
@@ -625,8 +625,56 @@ int gstex_raster_backward(int img_width, int img_height, int block_width, int ch
                                                   depth_idx, final_s, v_output, v_output_depth,
                                                   v_output_reg, v_output_alpha, v_output_texture,
                                                   v_output_normal, grad_records, v_texture,
-                                                  as_stream(stream)),
+                                                  as_stream(stream), loss_target, loss_scale),
                      "raster_backward");
+}
+
+int gstex_raster_backward(int img_width, int img_height, int block_width, int channels,
+                          int settings, const void* records, int num_points,
+                          const int32_t* gaussian_ids_sorted, int num_ids,
+                          const int32_t* tile_bins, const void* texture, int num_texels,
+                          int texture_format,
+                          const float* viewmat, const float* c2w, float fx, float fy, float cx,
+                          float cy, const float* background, const float* final_Ts,
+                          const int32_t* final_idx, const int32_t* depth_idx,
+                          const float* final_s, const float* v_output,
+                          const float* v_output_depth, const float* v_output_reg,
+                          const float* v_output_alpha, const float* v_output_texture,
+                          const float* v_output_normal, float* grad_records, float* v_texture,
+                          const uint64_t* wave_masks, size_t wave_masks_bytes,
+                          void* stream) {
+    return raster_backward_impl(img_width, img_height, block_width, channels, settings, records,
+                                num_points, gaussian_ids_sorted, num_ids, tile_bins, texture,
+                                num_texels, texture_format, viewmat, c2w, fx, fy, cx, cy,
+                                background, final_Ts, final_idx, depth_idx, final_s, v_output,
+                                v_output_depth, v_output_reg, v_output_alpha, v_output_texture,
+                                v_output_normal, grad_records, v_texture, wave_masks,
+                                wave_masks_bytes, stream, nullptr, 0.f);
+}
+
+int gstex_raster_backward_l2(int img_width, int img_height, int block_width, int channels,
+                             int settings, const void* records, int num_points,
+                             const int32_t* gaussian_ids_sorted, int num_ids,
+                             const int32_t* tile_bins, const void* texture, int num_texels,
+                             int texture_format, const float* viewmat, const float* c2w,
+                             float fx, float fy, float cx, float cy, const float* background,
+                             const float* final_Ts, const int32_t* final_idx,
+                             const int32_t* depth_idx, const float* final_s,
+                             const float* out_img, const float* out_texture,
+                             const float* target, float scale, const float* v_output_depth,
+                             const float* v_output_reg, const float* v_output_alpha,
+                             const float* v_output_normal, float* grad_records,
+                             float* v_texture, const uint64_t* wave_masks,
+                             size_t wave_masks_bytes, void* stream) {
+    REQUIRE(channels == 3, "gstex_raster_backward_l2 needs 3 channels, got %d", channels);
+    REQUIRE(out_img && out_texture && target, "null pointer argument");
+    return raster_backward_impl(img_width, img_height, block_width, channels, settings, records,
+                                num_points, gaussian_ids_sorted, num_ids, tile_bins, texture,
+                                num_texels, texture_format, viewmat, c2w, fx, fy, cx, cy,
+                                background, final_Ts, final_idx, depth_idx, final_s, out_img,
+                                v_output_depth, v_output_reg, v_output_alpha, out_texture,
+                                v_output_normal, grad_records, v_texture, wave_masks,
+                                wave_masks_bytes, stream, target, scale);
 }
 
 int gstex_finalize_gradients(int num_points, const float* grad_records, const void* records,

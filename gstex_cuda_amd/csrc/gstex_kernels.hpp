@@ -136,7 +136,9 @@ hipError_t raster_backward(const RasterCall& c, const float* final_Ts, const int
                            float* v_uv0, float* v_umap, float* v_vmap, float* v_texture, void* ws,
                            size_t ws_bytes, hipStream_t s);
 
-// per-stage raster entry points (records built by launch_build_records)
+// per-stage raster entry points (records built by launch_build_records); with
+// loss_target (three channels) the backward takes out_img / out_texture in
+// place of v_img / v_texture_out and forms the L2 image-loss cotangents itself
 hipError_t raster_forward_stage(const RasterCall& c, const float* records, float* out_img,
                                 float* out_depth, float* out_reg, float* out_texture,
                                 float* out_normal, float* final_Ts, int* final_idx,
@@ -146,7 +148,8 @@ hipError_t raster_backward_stage(const RasterCall& c, const float* records,
                                  const int* depth_idx, const float* final_s, const float* v_img,
                                  const float* v_depth, const float* v_reg, const float* v_alpha,
                                  const float* v_texture_out, const float* v_normal,
-                                 float* grad_rec, float* v_texture, hipStream_t s);
+                                 float* grad_rec, float* v_texture, hipStream_t s,
+                                 const float* loss_target = nullptr, float loss_scale = 0.f);
 hipError_t grad_finalize(int n, const float* grad_rec, const float* records, const float* quats,
                          const float* scales, const float* viewmat, const float* c2w, float fx,
                          float fy, float* v_colors, float* v_opacity, float* v_means,

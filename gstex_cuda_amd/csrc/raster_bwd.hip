@@ -93,11 +93,23 @@ __device__ __forceinline__ void raster_backward_wave(const RasterArgs& a, const 
         vn = v3(b.v_normal[3 * pid], b.v_normal[3 * pid + 1], b.v_normal[3 * pid + 2]);
         vdep = b.v_depth[pid];
         vreg = b.v_reg[pid];
-        vT = a.background[0] * vo.x + a.background[1] * vo.y + a.background[2] * vo.z -
-             b.v_alpha[pid];
 #pragma unroll
         for (int c = 0; c < CM; ++c)
             if (c < C) vtex[c] = b.v_texture_out[(size_t)pid * C + c];
+        if constexpr (CM == 3) {
+            if (b.loss_target) {  // vo, vtex hold out_img, out_texture
+                const float* tg = b.loss_target + 3 * pid;
+                const float g0 = l2_cotangent(vo.x, vtex[0], tg[0], b.loss_scale);
+                const float g1 = l2_cotangent(vo.y, vtex[1], tg[1], b.loss_scale);
+                const float g2 = l2_cotangent(vo.z, vtex[2], tg[2], b.loss_scale);
+                vo = v3(g0, g1, g2);
+                vtex[0] = g0;
+                vtex[1] = g1;
+                vtex[2] = g2;
+            }
+        }
+        vT = a.background[0] * vo.x + a.background[1] * vo.y + a.background[2] * vo.z -
+             b.v_alpha[pid];
     }
     int r0, r1;
     tile_range(a, tile, r0, r1);
@@ -713,14 +725,16 @@ hipError_t raster_backward_stage(const RasterCall& c, const float* records,
                                  const int* depth_idx, const float* final_s, const float* v_img,
                                  const float* v_depth, const float* v_reg, const float* v_alpha,
                                  const float* v_texture_out, const float* v_normal,
-                                 float* grad_rec, float* v_texture, hipStream_t s) {
+                                 float* grad_rec, float* v_texture, hipStream_t s,
+                                 const float* loss_target, float loss_scale) {
     RasterArgs a = make_raster_args(c, records);
     // The backward differentiates the Gaussian alpha whatever the alpha
     // visualisation bits say (texture.cu:381-387 reads only bits 2, 8, 9, 10);
     // masks written by a visualisation forward would mark other entries.
     if (c.settings & kSetAlphaViz) a.wave_masks = nullptr;
+    if (loss_target && c.channels != 3) return hipErrorInvalidValue;
     const BwdIn b{final_Ts, final_idx, depth_idx, final_s, v_img, v_depth, v_reg, v_alpha,
-                  v_texture_out, v_normal, grad_rec, v_texture};
+                  v_texture_out, v_normal, grad_rec, v_texture, loss_target, loss_scale};
     if (a.tiles_x * a.tiles_y <= 0) return hipSuccess;
     if (c.channels == 1) return launch_bwd_cm<1>(a, b, s);
     if (c.channels == 2) return launch_bwd_cm<2>(a, b, s);

@@ -68,7 +68,20 @@ struct BwdIn {
     const float* v_normal;
     float* grad_rec;   // (N, 32) accumulators
     float* v_texture;  // (X, C)
+    // optional, three channels only: v_img and v_texture_out then point at
+    // the forward's out_img and out_texture, and the image cotangents are the
+    // L2 loss's, ((out_img + out_texture) - loss_target) * loss_scale
+    const float* loss_target;
+    float loss_scale;
 };
+
+// One pixel channel of that cotangent, each step rounded as a separate
+// float32 operation (as gstex_image_loss_grad rounds it).
+__device__ __forceinline__ float l2_cotangent(float img, float tex, float target, float scale) {
+#pragma clang fp contract(off)
+    const float d = img + tex;
+    return (d - target) * scale;
+}
 
 // One view of a multi-view raster pass: its launch arguments, forward
 // outputs and backward inputs, in a device table the queue kernels read.

@@ -318,7 +318,12 @@ class RenderStep:
         self.backward(timer, v_reg=v_reg, v_normal=v_normal)
 
     def backward(self, timer: Optional[StageTimer] = None, v_depth=None, v_reg=None,
-                 v_alpha=None, v_normal=None):
+                 v_alpha=None, v_normal=None, loss_target: Optional[torch.Tensor] = None):
+        """The raster backward and the parameter gradients.  The image and
+        texture cotangents are ``v_img`` / ``v_tex``, or with ``loss_target``
+        (three channels, float32, contiguous, H x W x 3) those of
+        ``l2_loss_grad(loss_target)``, formed inside the raster backward
+        (``v_img`` / ``v_tex`` are then not written)."""
         p, cam, s, g = self.p, self.cam, self._stream(), self.grads
         n, m = self.n, self._list_len
         if self._zeroed:
@@ -333,13 +338,20 @@ class RenderStep:
         v_normal = self.v_zero3 if v_normal is None else v_normal
         if timer:
             timer.start("raster_backward")
-        self._raster("gstex_raster_backward", cam.img_w, cam.img_h, self.bw, self.C, self.settings,
-             _p(self.records), n, _p(self.ids), m, _p(self.tile_bins), _p(p["texture"]), self.X,
-             self.texture_format, _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy, cam.cx, cam.cy,
-             _p(self.background),
-             _p(self.final_Ts), _p(self.final_idx), _p(self.depth_idx), _p(self.out_reg_s),
-             _p(self.v_img), _p(v_depth), _p(v_reg), _p(v_alpha), _p(self.v_tex), _p(v_normal),
-             _p(self.grad_records), _p(g["v_texture"]), *self._masks(), s)
+        head = (cam.img_w, cam.img_h, self.bw, self.C, self.settings,
+                _p(self.records), n, _p(self.ids), m, _p(self.tile_bins), _p(p["texture"]),
+                self.X, self.texture_format, _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy,
+                cam.cx, cam.cy, _p(self.background),
+                _p(self.final_Ts), _p(self.final_idx), _p(self.depth_idx), _p(self.out_reg_s))
+        if loss_target is None:
+            self._raster("gstex_raster_backward", *head, _p(self.v_img), _p(v_depth), _p(v_reg),
+                         _p(v_alpha), _p(self.v_tex), _p(v_normal), _p(self.grad_records),
+                         _p(g["v_texture"]), *self._masks(), s)
+        else:
+            self._raster("gstex_raster_backward_l2", *head, _p(self.out_img),
+                         _p(self.out_texture), _p(loss_target), 2.0 / loss_target.numel(),
+                         _p(v_depth), _p(v_reg), _p(v_alpha), _p(v_normal),
+                         _p(self.grad_records), _p(g["v_texture"]), *self._masks(), s)
         if timer:
             timer.stop("raster_backward")
         if not self.finalize:
@@ -353,10 +365,21 @@ class RenderStep:
                  _p(self._viewdirs), _p(g["v_colors"]), _p(g["v_sh_coeffs"]), s)
 
     def step(self, target: torch.Tensor, timer: Optional[StageTimer] = None):
-        """Forward, L2 loss gradient against ``target``, backward."""
+        """Forward, L2 loss gradient against ``target``, backward.  With three
+        channels the loss gradient is formed inside the raster backward."""
         self.forward(timer)
-        self.l2_loss_grad(target)
-        self.backward(timer)
+        fused = self._fused_loss_target(target)
+        if fused is None:
+            self.l2_loss_grad(target)
+        self.backward(timer, loss_target=fused)
+
+    def _fused_loss_target(self, target: torch.Tensor) -> Optional[torch.Tensor]:
+        """target when the raster backward can form the L2 cotangents itself."""
+        if (self.C == 3 and target.dtype == torch.float32 and target.is_contiguous()
+                and target.device == self.out_img.device
+                and tuple(target.shape) == (self.cam.img_h, self.cam.img_w, 3)):
+            return target
+        return None
 
     def intersect_total(self) -> int:
         """Intersect count of the last step (a host synchronisation)."""

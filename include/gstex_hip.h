@@ -353,6 +353,29 @@ int gstex_raster_backward(int img_width, int img_height, int block_width, int ch
                           const float *v_output_normal, float *grad_records, float *v_texture,
                           const uint64_t *wave_masks, size_t wave_masks_bytes,
                           void *stream);
+/* gstex_raster_backward for the L2 image loss mean((out_img + out_texture -
+ * target)^2) of a training step: in place of v_output and v_output_texture it
+ * takes the forward's out_img and out_texture (H x W x 3 each), the target
+ * and scale (2 / (H W 3) for that mean), and forms the image and texture
+ * cotangents ((out_img + out_texture) - target) * scale per pixel channel,
+ * each step rounded as gstex_image_loss_grad with with_img = 1 rounds it, so
+ * that the separate loss pass and its buffers are not needed.  Three
+ * channels only.  No counterpart in the reference (its trainer forms the
+ * loss gradient with torch autograd, example.py:188-210). */
+int gstex_raster_backward_l2(int img_width, int img_height, int block_width, int channels,
+                             int settings, const void *records, int num_points,
+                             const int32_t *gaussian_ids_sorted, int num_ids,
+                             const int32_t *tile_bins, const void *texture, int num_texels,
+                             int texture_format, const float *viewmat, const float *c2w,
+                             float fx, float fy, float cx, float cy, const float *background,
+                             const float *final_Ts, const int32_t *final_idx,
+                             const int32_t *depth_idx, const float *final_s,
+                             const float *out_img, const float *out_texture,
+                             const float *target, float scale, const float *v_output_depth,
+                             const float *v_output_reg, const float *v_output_alpha,
+                             const float *v_output_normal, float *grad_records,
+                             float *v_texture, const uint64_t *wave_masks,
+                             size_t wave_masks_bytes, void *stream);
 int gstex_finalize_gradients(int num_points, const float *grad_records, const void *records,
                              const float *quats, const float *scales, const float *viewmat,
                              const float *c2w, float fx, float fy, float *v_colors,

@@ -125,3 +125,46 @@ def test_wave_masks_full_size_1080p_500k():
     for k in GRAD_NAMES + ("v_sh_coeffs",):
         ok, l2, fr = grad_close(out[0][k], out[1][k], rel_l2=1e-4, elem_tol=1e-4, frac=0.9999)
         assert ok, (k, l2, fr)
+
+
+@pytest.mark.parametrize("case", ["w1_small", "w1_bw5", "w1_blur_ndc"])
+def test_backward_l2_equals_separate_loss_pass(case):
+    # gstex_raster_backward_l2 (the L2 cotangents formed in the backward, as
+    # RenderStep.step does for three channels) gives the gradients of
+    # gstex_image_loss_grad followed by gstex_raster_backward: the same
+    # per-pixel cotangents bit for bit, so the same sums up to the order of
+    # the float atomics
+    from gstex_cuda_amd.pipeline import scene_to_step
+
+    s = _scene(case)
+    tgt = torch.from_numpy(np.ascontiguousarray(s.target, dtype=np.float32)).to(DEV)
+    ref = scene_to_step(s, DEV)
+    assert ref._fused_loss_target(tgt) is tgt
+    ref.forward()
+    ref.l2_loss_grad(tgt)
+    ref.backward()
+    fused = scene_to_step(s, DEV)
+    fused.step(tgt)
+    torch.cuda.synchronize()
+    for k in FWD_NAMES:
+        assert np.array_equal(np_(getattr(fused, k)), np_(getattr(ref, k))), k
+    for k in GRAD_NAMES:
+        ok, l2, fr = grad_close(np_(fused.grads[k]), np_(ref.grads[k]), rel_l2=1e-5,
+                                elem_tol=1e-5, frac=0.9999)
+        assert ok, (k, l2, fr)
+    # and the raw per-Gaussian gradient records agree to that order
+    rec_ref = np_(ref.grad_records)
+    rec_fused = np_(fused.grad_records)
+    assert np.allclose(rec_fused, rec_ref, rtol=1e-5, atol=1e-7 * np.abs(rec_ref).max())
+
+
+def test_backward_l2_rejects_other_channel_counts():
+    from gstex_cuda_amd import _lib
+
+    lib = _lib.lib()
+    dummy = torch.zeros(64, device=DEV)
+    p = dummy.data_ptr()
+    rc = lib.gstex_raster_backward_l2(32, 32, 16, 4, 0, p, 1, p, 1, p, p, 16, 0, p, p, 1.0, 1.0,
+                                      16.0, 16.0, p, p, p, p, p, p, p, p, 1.0, p, p, p, p, p,
+                                      p, None, 0, None)
+    assert rc != 0 and b"3 channels" in lib.gstex_last_error()
