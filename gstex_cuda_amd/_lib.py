@@ -48,6 +48,10 @@ _SIGNATURES = {
     "gstex_get_tile_bin_edges": (I, [I, P, I, I, P, P]),
     "gstex_bin_gaussians_workspace_size": (Z, [I, I]),
     "gstex_bin_gaussians": (I, [I, I, P, P, P, P, I, I, I, P, P, P, Z, P]),
+    "gstex_bin_gaussians_depth_workspace_size": (Z, [I]),
+    "gstex_bin_gaussians_tiles_workspace_size": (Z, [I, I]),
+    "gstex_bin_gaussians_depth": (I, [I, P, P, P, P, I, I, I, P, P, Z, P]),
+    "gstex_bin_gaussians_tiles": (I, [I, I, P, I, I, P, Z, P, P, P, Z, P]),
     "gstex_bin_gaussians_capped": (I, [I, I, P, P, P, P, I, I, I, P, P, P, P, Z, P]),
     "gstex_texture_forward_workspace_size": (Z, [I]),
     "gstex_texture_forward": (I, [I, I, I, I, I, I, I, I,  # tiles, bw, img, texture_info

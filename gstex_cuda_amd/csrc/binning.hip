@@ -1513,48 +1513,197 @@ static inline int bits_for(int v) {  // bits needed to represent 0..v-1
     return b;
 }
 
-struct BinLayout {
-    size_t off[10];
-    size_t total;
-};
-
 static int emit_blocks(int cap) { return (cap + kEmitOut - 1) / kEmitOut; }
 
 // first tile-pass digit counts: up to 256 digits per emission block, then the
 // look-back states of their scan
 static size_t hist1_words(int cap) { return (size_t)256 * emit_blocks(cap); }
 
-static BinLayout bin_layout(int n, int m) {
-    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    BinLayout L;
-    size_t sizes[10] = {
-        up(4 * (size_t)n) * 2,  // depth keys in/out
-        up(4 * (size_t)n) * 2,  // order in/out
-        up(16 * (size_t)n) * 2,  // per-Gaussian info, per-position records
-        up(4 * (size_t)n + 8) + up(8 * ((size_t)wr_scan_tiles(n) + 1)),  // offsets (+ totals), scan states
-        up(4 * (size_t)m) * 2,  // tile keys in/out
-        up(4 * (size_t)m),      // ids after the first tile pass
-        up(std::max(radix_workspace_bytes<uint32_t>(n), radix_workspace_bytes<uint32_t>(m))),
-        up(4 * ((size_t)emit_blocks(m) + 2)),  // emission block starts
-        up(4 * hist1_words(m)) + up(8 * ((size_t)wr_scan_tiles((int)hist1_words(m)) + 1)),
-        256,
+static size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Binning runs in two phases.  The depth phase needs only the Gaussians:
+// entry counts and rectangles, the depth sort and the emission offsets and
+// records.  The tile phase needs the list size as well: the fused emission
+// and first tile pass, the remaining passes and the bin edges.  Each has its
+// own workspace (the tile phase reads the depth phase's), so a caller that
+// reads the intersect total back to the host can queue the depth phase first.
+struct DepthWs {
+    uint32_t *dkey, *dkey2, *ord, *ord2;
+    uint4 *info, *rec;
+    uint32_t* offs;     // n offsets, then the written total and the intersect total
+    uint64_t* states;   // offset scan look-back states
+    void* rws;
+    size_t rws_bytes, total;
+};
+
+static DepthWs depth_ws(int n, void* base) {
+    char* b = (char*)base;
+    DepthWs d;
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+        char* p = b ? b + at : nullptr;
+        at += up256(bytes);
+        return p;
     };
-    size_t acc = 0;
-    for (int i = 0; i < 10; ++i) {
-        L.off[i] = acc;
-        acc += sizes[i];
-    }
-    L.total = acc;
-    return L;
+    d.dkey = (uint32_t*)take(4 * (size_t)n);
+    d.dkey2 = (uint32_t*)take(4 * (size_t)n);
+    d.ord = (uint32_t*)take(4 * (size_t)n);
+    d.ord2 = (uint32_t*)take(4 * (size_t)n);
+    d.info = (uint4*)take(16 * (size_t)n);
+    d.rec = (uint4*)take(16 * (size_t)n);
+    d.offs = (uint32_t*)take(4 * (size_t)n + 8);
+    d.states = (uint64_t*)take(8 * ((size_t)wr_scan_tiles(n) + 1));
+    d.rws_bytes = up256(radix_workspace_bytes<uint32_t>(n));
+    d.rws = take(d.rws_bytes);
+    d.total = at;
+    return d;
 }
 
-size_t bin_gaussians_workspace_bytes(int n, int m) { return bin_layout(n, m).total; }
+struct TileWs {
+    uint32_t *tkey, *tkey2, *tid_in, *starts, *hist1, *m_word;
+    uint64_t* hstates;
+    void* rws;
+    size_t rws_bytes, total;
+};
+
+static TileWs tile_ws(int cap, void* base) {
+    char* b = (char*)base;
+    TileWs t;
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+        char* p = b ? b + at : nullptr;
+        at += up256(bytes);
+        return p;
+    };
+    t.tkey = (uint32_t*)take(4 * (size_t)cap);
+    t.tkey2 = (uint32_t*)take(4 * (size_t)cap);
+    t.tid_in = (uint32_t*)take(4 * (size_t)cap);  // ids after the first tile pass
+    t.starts = (uint32_t*)take(4 * ((size_t)emit_blocks(cap) + 2));
+    t.hist1 = (uint32_t*)take(4 * hist1_words(cap));
+    t.hstates = (uint64_t*)take(8 * ((size_t)wr_scan_tiles((int)hist1_words(cap)) + 1));
+    t.m_word = (uint32_t*)take(4);
+    t.rws_bytes = up256(radix_workspace_bytes<uint32_t>(cap));
+    t.rws = take(t.rws_bytes);
+    t.total = at;
+    return t;
+}
+
+size_t bin_depth_workspace_bytes(int n) { return depth_ws(std::max(n, 1), nullptr).total; }
+size_t bin_tiles_workspace_bytes(int cap) { return tile_ws(std::max(cap, 1), nullptr).total; }
+size_t bin_gaussians_workspace_bytes(int n, int m) {
+    return bin_depth_workspace_bytes(n) + bin_tiles_workspace_bytes(m);
+}
 
 hipError_t count_intersects(int n, const int* counts, unsigned* out, hipStream_t s) {
     hipError_t e = fill<uint32_t>(out, 1, 0u, s);
     if (e != hipSuccess || n <= 0) return e;
     k_count_total<<<std::min((n + 255) / 256, kCountBlocks), 256, 0, s>>>(n, counts, out);
     return hipGetLastError();
+}
+
+// Depth phase (n > 0).  Also clears the tile bins; with m_exact >= 0 the
+// intersect total word of the workspace is set to it.
+static hipError_t bin_depth_phase(int n, const float* centers, const float* extents,
+                                  const float* depths, const int* counts, int tiles_x,
+                                  int tiles_y, int bw, int* tile_bins, const DepthWs& d,
+                                  int m_exact, hipStream_t s) {
+    const int gn = (n + 255) / 256, st = wr_scan_tiles(n);
+    uint32_t* total = d.offs + n;  // entries written by Gaussians
+    k_bin_prepare<<<gn, 256, 0, s>>>(n, centers, extents, depths, counts, tiles_x, tiles_y, bw,
+                                     d.dkey, d.ord, d.info, tile_bins, 2 * tiles_x * tiles_y,
+                                     d.states, st + 1, d.offs + n + 1, m_exact);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = radix_sort_pairs<uint32_t>(n, d.dkey, d.ord, d.dkey2, d.ord2, 32, d.rws, d.rws_bytes, s);
+    if (e != hipSuccess) return e;
+    k_scan_written<<<st, 256, 0, s>>>(n, d.ord2, d.info, d.offs, d.rec, total, d.states, st);
+    return hipGetLastError();
+}
+
+// Tile phase (n > 0, cap > 0), with the intersect total (list entries, zero
+// slots included) read from m_total on the device.
+static hipError_t bin_tiles_phase(int n, int cap, const uint32_t* m_total, int tiles_x,
+                                  int tiles_y, const DepthWs& d, int* ids_sorted, int* tile_bins,
+                                  const TileWs& t, hipStream_t s) {
+    const int num_tiles = tiles_x * tiles_y, gn = (n + 255) / 256;
+    const uint32_t* total = d.offs + n;
+    // the sentinel tile (num_tiles) needs one more key value; when the tile
+    // and the Gaussian id fit one 32-bit word together, the entries are sorted
+    // as packed keys alone (half the bytes per pass), and the ids come out of
+    // the bin-edge pass
+    const int tile_bits = bits_for(num_tiles + 1), id_bits = bits_for(n);
+    const int pk = (id_bits > 0 && tile_bits + id_bits <= 32) ? id_bits : 0;
+    // the emission is fused with the first tile pass (digit width bits1, as
+    // the sort would split tile_bits); later passes run as a sort from bit
+    // pk + bits1
+    const int tpasses = (tile_bits + 7) / 8;
+    const int bits1 = (tile_bits + tpasses - 1) / tpasses;
+    const int eb = emit_blocks(cap);
+    const int hcount = (1 << bits1) * eb;
+    const bool hlb = hcount > kScanSingleMax;
+    const int hlbt = wr_scan_tiles(hcount);
+    hipError_t e;
+    k_bin_mark_starts<<<gn, 256, 0, s>>>(n, d.offs, total, m_total, eb, kEmitOut, t.starts);
+    k_tile_hist1<<<eb, 256, 0, s>>>(n, cap, m_total, total, d.rec, d.offs, t.starts, tiles_x,
+                                    num_tiles, bits1, t.hist1, eb, hlb ? t.hstates : nullptr,
+                                    hlb ? hlbt + 1 : 0);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (hlb) {
+        k_scan_lookback<<<hlbt, 256, 0, s>>>(t.hist1, hcount, t.hstates, hlbt);
+    } else if ((e = scan_exclusive(t.hist1, hcount, nullptr, nullptr, s)) != hipSuccess) {
+        return e;
+    }
+    const bool more = tile_bits > bits1;
+    uint32_t* k1 = more ? t.tkey : t.tkey2;
+    if (pk)
+        k_emit_scatter1<true><<<eb, 256, 0, s>>>(n, cap, m_total, total, d.rec, t.starts, tiles_x,
+                                                 num_tiles, pk, bits1, t.hist1, eb, k1, nullptr);
+    else
+        k_emit_scatter1<false><<<eb, 256, 0, s>>>(n, cap, m_total, total, d.rec, t.starts,
+                                                  tiles_x, num_tiles, 0, bits1, t.hist1, eb, k1,
+                                                  more ? t.tid_in : (uint32_t*)ids_sorted);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (more) {
+        if (pk)
+            e = radix_sort_pairs<uint32_t>(cap, t.tkey, nullptr, t.tkey2, nullptr,
+                                           pk + tile_bits, t.rws, t.rws_bytes, s, pk + bits1);
+        else
+            e = radix_sort_pairs<uint32_t>(cap, t.tkey, t.tid_in, t.tkey2, (uint32_t*)ids_sorted,
+                                           tile_bits, t.rws, t.rws_bytes, s, bits1);
+        if (e != hipSuccess) return e;
+    }
+    k_bin_edges32<<<(cap + 1023) / 1024, 256, 0, s>>>(
+        cap, t.tkey2, num_tiles, tile_bins,
+        (reinterpret_cast<uintptr_t>(t.tkey2) & 15) == 0 &&
+            (reinterpret_cast<uintptr_t>(ids_sorted) & 15) == 0,
+        pk, ids_sorted);
+    return hipGetLastError();
+}
+
+hipError_t bin_depth(int n, const float* centers, const float* extents, const float* depths,
+                     const int* counts, int tiles_x, int tiles_y, int bw, int* tile_bins,
+                     void* ws, size_t ws_bytes, hipStream_t s) {
+    if (n <= 0) return fill<int>(tile_bins, (size_t)tiles_x * tiles_y * 2, 0, s);
+    const DepthWs d = depth_ws(n, ws);
+    if (ws_bytes < d.total) return hipErrorInvalidValue;
+    return bin_depth_phase(n, centers, extents, depths, counts, tiles_x, tiles_y, bw, tile_bins, d,
+                           -1, s);
+}
+
+hipError_t bin_tiles(int n, int cap, const unsigned* m_total, int tiles_x, int tiles_y,
+                     const void* depth_ws_base, size_t depth_ws_bytes, int* ids_sorted,
+                     int* tile_bins, void* ws, size_t ws_bytes, hipStream_t s) {
+    if (n <= 0 || cap <= 0) return hipSuccess;  // the depth phase left the bins empty
+    const DepthWs d = depth_ws(n, const_cast<void*>(depth_ws_base));
+    const TileWs t = tile_ws(cap, ws);
+    if (depth_ws_bytes < d.total || ws_bytes < t.total) return hipErrorInvalidValue;
+    const uint32_t* mt = m_total;
+    if (!mt) {
+        hipError_t e = fill<uint32_t>(t.m_word, 1, (uint32_t)cap, s);
+        if (e != hipSuccess) return e;
+        mt = t.m_word;
+    }
+    return bin_tiles_phase(n, cap, mt, tiles_x, tiles_y, d, ids_sorted, tile_bins, t, s);
 }
 
 hipError_t bin_gaussians(int n, int cap, const float* centers, const float* extents,
@@ -1568,9 +1717,8 @@ hipError_t bin_gaussians(int n, int cap, const float* centers, const float* exte
     if (cap <= 0) {
         if (m_out) {
             // the total is still reported, so the caller can size a capacity
-            BinLayout L = bin_layout(n, 1);
-            if (ws_bytes < L.total) return hipErrorInvalidValue;
-            uint32_t* mt = (uint32_t*)((char*)ws + L.off[3]) + n + 1;
+            if (ws_bytes < bin_gaussians_workspace_bytes(n, 1)) return hipErrorInvalidValue;
+            uint32_t* mt = depth_ws(n, ws).offs + n + 1;
             if ((e = fill<uint32_t>(mt, 1, 0u, s)) != hipSuccess) return e;
             k_count_total<<<std::min((n + 255) / 256, kCountBlocks), 256, 0, s>>>(n, counts, mt);
             if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -1578,95 +1726,22 @@ hipError_t bin_gaussians(int n, int cap, const float* centers, const float* exte
         }
         return hipSuccess;
     }
-    BinLayout L = bin_layout(n, cap);
-    if (ws_bytes < L.total) return hipErrorInvalidValue;
-    char* b = (char*)ws;
-    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
-    uint32_t* dkey = (uint32_t*)(b + L.off[0]);
-    uint32_t* dkey2 = (uint32_t*)(b + L.off[0] + up(4 * (size_t)n));
-    uint32_t* ord = (uint32_t*)(b + L.off[1]);
-    uint32_t* ord2 = (uint32_t*)(b + L.off[1] + up(4 * (size_t)n));
-    uint4* info = (uint4*)(b + L.off[2]);
-    uint4* rec = (uint4*)(b + L.off[2] + up(16 * (size_t)n));
-    uint32_t* offs = (uint32_t*)(b + L.off[3]);
-    uint32_t* total = offs + n;       // entries written by Gaussians
-    uint32_t* m_total = offs + n + 1; // intersect total (written + zero slots)
-    uint32_t* tkey = (uint32_t*)(b + L.off[4]);
-    uint32_t* tkey2 = (uint32_t*)(b + L.off[4] + up(4 * (size_t)cap));
-    uint32_t* tid_in = (uint32_t*)(b + L.off[5]);
-    void* rws = (void*)(b + L.off[6]);
-    const size_t rws_bytes = up(std::max(radix_workspace_bytes<uint32_t>(n),
-                                         radix_workspace_bytes<uint32_t>(cap)));
-    const int gn = (n + 255) / 256;
-
+    if (ws_bytes < bin_gaussians_workspace_bytes(n, cap)) return hipErrorInvalidValue;
+    const DepthWs d = depth_ws(n, ws);
+    const TileWs t = tile_ws(cap, (char*)ws + bin_depth_workspace_bytes(n));
+    uint32_t* m_total = d.offs + n + 1;  // intersect total (written + zero slots)
     // capacity mode: the total is counted on the device; exact size: the
     // caller's num_intersects is that total (a wrong one still keeps every
     // write inside the list and the zero-slot range), set by k_bin_prepare
     if (m_out) {
         if ((e = fill<uint32_t>(m_total, 1, 0u, s)) != hipSuccess) return e;
-        k_count_total<<<std::min(gn, kCountBlocks), 256, 0, s>>>(n, counts, m_total);
+        k_count_total<<<std::min((n + 255) / 256, kCountBlocks), 256, 0, s>>>(n, counts, m_total);
     }
-    const int st = wr_scan_tiles(n);
-    uint64_t* states = (uint64_t*)(b + L.off[3] + up(4 * (size_t)n + 8));
-    k_bin_prepare<<<gn, 256, 0, s>>>(n, centers, extents, depths, counts, tiles_x, tiles_y, bw,
-                                     dkey, ord, info, tile_bins, 2 * num_tiles, states, st + 1,
-                                     m_total, m_out ? -1 : cap);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    e = radix_sort_pairs<uint32_t>(n, dkey, ord, dkey2, ord2, 32, rws, rws_bytes, s);
+    e = bin_depth_phase(n, centers, extents, depths, counts, tiles_x, tiles_y, bw, tile_bins, d,
+                        m_out ? -1 : cap, s);
     if (e != hipSuccess) return e;
-    k_scan_written<<<st, 256, 0, s>>>(n, ord2, info, offs, rec, total, states, st);
-    // the sentinel tile (num_tiles) needs one more key value; when the tile
-    // and the Gaussian id fit one 32-bit word together, the entries are sorted
-    // as packed keys alone (half the bytes per pass), and the ids come out of
-    // the bin-edge pass
-    const int tile_bits = bits_for(num_tiles + 1), id_bits = bits_for(n);
-    const int pk = (id_bits > 0 && tile_bits + id_bits <= 32) ? id_bits : 0;
-    // the emission is fused with the first tile pass (digit width bits1, as
-    // the sort would split tile_bits); later passes run as a sort from bit
-    // pk + bits1
-    const int tpasses = (tile_bits + 7) / 8;
-    const int bits1 = (tile_bits + tpasses - 1) / tpasses;
-    const int eb = emit_blocks(cap);
-    uint32_t* starts = (uint32_t*)(b + L.off[7]);
-    uint32_t* hist1 = (uint32_t*)(b + L.off[8]);
-    uint64_t* hstates = (uint64_t*)(b + L.off[8] + up(4 * hist1_words(cap)));
-    const int hcount = (1 << bits1) * eb;
-    const bool hlb = hcount > kScanSingleMax;
-    const int hlbt = wr_scan_tiles(hcount);
-    k_bin_mark_starts<<<gn, 256, 0, s>>>(n, offs, total, m_total, eb, kEmitOut, starts);
-    k_tile_hist1<<<eb, 256, 0, s>>>(n, cap, m_total, total, rec, offs, starts, tiles_x, num_tiles,
-                                    bits1, hist1, eb, hlb ? hstates : nullptr, hlb ? hlbt + 1 : 0);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (hlb) {
-        k_scan_lookback<<<hlbt, 256, 0, s>>>(hist1, hcount, hstates, hlbt);
-    } else if ((e = scan_exclusive(hist1, hcount, nullptr, nullptr, s)) != hipSuccess) {
-        return e;
-    }
-    const bool more = tile_bits > bits1;
-    uint32_t* k1 = more ? tkey : tkey2;
-    if (pk)
-        k_emit_scatter1<true><<<eb, 256, 0, s>>>(n, cap, m_total, total, rec, starts, tiles_x,
-                                                 num_tiles, pk, bits1, hist1, eb, k1, nullptr);
-    else
-        k_emit_scatter1<false><<<eb, 256, 0, s>>>(n, cap, m_total, total, rec, starts, tiles_x,
-                                                  num_tiles, 0, bits1, hist1, eb, k1,
-                                                  more ? tid_in : (uint32_t*)ids_sorted);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (more) {
-        if (pk)
-            e = radix_sort_pairs<uint32_t>(cap, tkey, nullptr, tkey2, nullptr, pk + tile_bits,
-                                           rws, rws_bytes, s, pk + bits1);
-        else
-            e = radix_sort_pairs<uint32_t>(cap, tkey, tid_in, tkey2, (uint32_t*)ids_sorted,
-                                           tile_bits, rws, rws_bytes, s, bits1);
-        if (e != hipSuccess) return e;
-    }
-    k_bin_edges32<<<(cap + 1023) / 1024, 256, 0, s>>>(
-        cap, tkey2, num_tiles, tile_bins,
-        (reinterpret_cast<uintptr_t>(tkey2) & 15) == 0 &&
-            (reinterpret_cast<uintptr_t>(ids_sorted) & 15) == 0,
-        pk, ids_sorted);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    e = bin_tiles_phase(n, cap, m_total, tiles_x, tiles_y, d, ids_sorted, tile_bins, t, s);
+    if (e != hipSuccess) return e;
     if (m_out) return hipMemcpyAsync(m_out, m_total, sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
     return hipSuccess;
 }

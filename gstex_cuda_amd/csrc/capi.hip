@@ -296,6 +296,61 @@ size_t gstex_bin_gaussians_workspace_size(int num_points, int num_intersects) {
     return gstex::bin_gaussians_workspace_bytes(num_points, num_intersects);
 }
 
+size_t gstex_bin_gaussians_depth_workspace_size(int num_points) {
+    return gstex::bin_depth_workspace_bytes(num_points);
+}
+
+size_t gstex_bin_gaussians_tiles_workspace_size(int num_points, int num_intersects) {
+    (void)num_points;
+    return gstex::bin_tiles_workspace_bytes(num_intersects);
+}
+
+int gstex_bin_gaussians_depth(int num_points, const float* centers, const float* extents,
+                              const float* depths, const int32_t* num_tiles_hit, int tiles_x,
+                              int tiles_y, int block_width, int32_t* tile_bins, void* depth_ws,
+                              size_t depth_ws_bytes, void* stream) {
+    REQUIRE(num_points >= 0, "negative sizes");
+    REQUIRE(block_width > 0 && tiles_x >= 0 && tiles_y >= 0, "invalid tile bounds");
+    REQUIRE_TILE_GRID(tiles_x, tiles_y);
+    if (tiles_x * tiles_y == 0) return GSTEX_OK;
+    REQUIRE(tile_bins, "null pointer argument");
+    if (num_points > 0) {
+        REQUIRE(centers && extents && depths && num_tiles_hit && depth_ws,
+                "null pointer argument");
+        const size_t need = gstex::bin_depth_workspace_bytes(num_points);
+        REQUIRE(depth_ws_bytes >= need, "workspace too small: %zu < %zu", depth_ws_bytes, need);
+    }
+    return check_hip(gstex::bin_depth(num_points, centers, extents, depths, num_tiles_hit,
+                                      tiles_x, tiles_y, block_width, tile_bins, depth_ws,
+                                      depth_ws_bytes, as_stream(stream)),
+                     "bin_gaussians_depth");
+}
+
+int gstex_bin_gaussians_tiles(int num_points, int num_intersects,
+                              const uint32_t* num_intersects_dev, int tiles_x, int tiles_y,
+                              const void* depth_ws, size_t depth_ws_bytes,
+                              int32_t* gaussian_ids_sorted, int32_t* tile_bins, void* tiles_ws,
+                              size_t tiles_ws_bytes, void* stream) {
+    REQUIRE(num_points >= 0 && num_intersects >= 0, "negative sizes");
+    REQUIRE(tiles_x >= 0 && tiles_y >= 0, "invalid tile bounds");
+    REQUIRE_TILE_GRID(tiles_x, tiles_y);
+    if (tiles_x * tiles_y == 0) return GSTEX_OK;
+    REQUIRE(tile_bins, "null pointer argument");
+    if (num_points > 0 && num_intersects > 0) {
+        REQUIRE(depth_ws && gaussian_ids_sorted && tiles_ws, "null pointer argument");
+        const size_t need_d = gstex::bin_depth_workspace_bytes(num_points);
+        const size_t need_t = gstex::bin_tiles_workspace_bytes(num_intersects);
+        REQUIRE(depth_ws_bytes >= need_d, "depth workspace too small: %zu < %zu",
+                depth_ws_bytes, need_d);
+        REQUIRE(tiles_ws_bytes >= need_t, "workspace too small: %zu < %zu", tiles_ws_bytes,
+                need_t);
+    }
+    return check_hip(gstex::bin_tiles(num_points, num_intersects, num_intersects_dev, tiles_x,
+                                      tiles_y, depth_ws, depth_ws_bytes, gaussian_ids_sorted,
+                                      tile_bins, tiles_ws, tiles_ws_bytes, as_stream(stream)),
+                     "bin_gaussians_tiles");
+}
+
 int gstex_bin_gaussians(int num_points, int num_intersects, const float* centers,
                         const float* extents, const float* depths, const int32_t* num_tiles_hit,
                         int tiles_x, int tiles_y, int block_width, int32_t* gaussian_ids_sorted,

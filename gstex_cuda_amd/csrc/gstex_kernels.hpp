@@ -79,6 +79,17 @@ hipError_t bin_gaussians(int n, int cap, const float* centers, const float* exte
                          const float* depths, const int* counts, int tiles_x, int tiles_y, int bw,
                          int* ids_sorted, int* tile_bins, unsigned* m_out, void* ws,
                          size_t ws_bytes, hipStream_t s);
+// The same in two phases: the depth phase (n only; clears the tile bins),
+// then the tile phase, which reads the depth phase's workspace and takes the
+// intersect total cap from m_total on the device (nullptr: cap itself).
+size_t bin_depth_workspace_bytes(int n);
+size_t bin_tiles_workspace_bytes(int cap);
+hipError_t bin_depth(int n, const float* centers, const float* extents, const float* depths,
+                     const int* counts, int tiles_x, int tiles_y, int bw, int* tile_bins,
+                     void* ws, size_t ws_bytes, hipStream_t s);
+hipError_t bin_tiles(int n, int cap, const unsigned* m_total, int tiles_x, int tiles_y,
+                     const void* depth_ws, size_t depth_ws_bytes, int* ids_sorted,
+                     int* tile_bins, void* ws, size_t ws_bytes, hipStream_t s);
 
 // raster
 struct RasterCall {

@@ -162,6 +162,14 @@ class RenderStep:
         self._zeroed = False
         if capacity is not None:
             self._ensure_bins(int(capacity))
+        else:
+            # exact size: binning runs as a depth phase (queued before the host
+            # waits for the intersect total) and a tile phase
+            self.depth_ws = torch.empty(
+                (max(_lib.lib().gstex_bin_gaussians_depth_workspace_size(n), 1),),
+                dtype=torch.uint8, device=dev)
+            self.m_host = torch.zeros((1,), dtype=torch.int32, pin_memory=True)
+            self._ev_m = torch.cuda.Event()
 
     def _stream(self):
         return torch.cuda.current_stream(self.dev).cuda_stream
@@ -169,7 +177,10 @@ class RenderStep:
     def _ensure_bins(self, m: int):
         if self.ids.numel() < m:
             self.ids = torch.empty((int(m * 1.25) + 1024,), dtype=torch.int32, device=self.dev)
-        need = _lib.lib().gstex_bin_gaussians_workspace_size(self.n, m)
+        if self.cap is None:
+            need = _lib.lib().gstex_bin_gaussians_tiles_workspace_size(self.n, m)
+        else:
+            need = _lib.lib().gstex_bin_gaussians_workspace_size(self.n, m)
         if self.bin_ws.numel() < need:
             self.bin_ws = torch.empty((int(need * 1.25) + 4096,), dtype=torch.uint8,
                                       device=self.dev)
@@ -249,15 +260,21 @@ class RenderStep:
                  _p(self.ids), _p(self.tile_bins), _p(self.m_dev), _p(self.bin_ws),
                  self.bin_ws.numel(), s)
         else:
-            m = int(self.m_dev.item())
+            # the depth phase needs no total: it is queued behind the total's
+            # copy to the host, and the host's wait for that copy overlaps it
+            self.m_host.copy_(self.m_dev, non_blocking=True)
+            self._ev_m.record(main)
+            call("gstex_bin_gaussians_depth", n, _p(self.centers), _p(self.extents),
+                 _p(self.depths), _p(self.tiles_hit), self.tiles[0], self.tiles[1], self.bw,
+                 _p(self.tile_bins), _p(self.depth_ws), self.depth_ws.numel(), s)
+            self._ev_m.synchronize()
+            m = int(self.m_host[0])
             self.num_intersects = m
             self._ensure_bins(m)
             if m > 0:
-                call("gstex_bin_gaussians", n, m, _p(self.centers), _p(self.extents),
-                     _p(self.depths), _p(self.tiles_hit), self.tiles[0], self.tiles[1], self.bw,
-                     _p(self.ids), _p(self.tile_bins), _p(self.bin_ws), self.bin_ws.numel(), s)
-            else:
-                self.tile_bins.zero_()
+                call("gstex_bin_gaussians_tiles", n, m, _p(self.m_dev), self.tiles[0],
+                     self.tiles[1], _p(self.depth_ws), self.depth_ws.numel(), _p(self.ids),
+                     _p(self.tile_bins), _p(self.bin_ws), self.bin_ws.numel(), s)
         self._list_len = m
         if side is None:
             self._colors_and_records(s)

@@ -169,6 +169,28 @@ int gstex_bin_gaussians(int num_points, int num_intersects, const float *centers
                         const float *extents, const float *depths, const int32_t *num_tiles_hit,
                         int tiles_x, int tiles_y, int block_width, int32_t *gaussian_ids_sorted,
                         int32_t *tile_bins, void *workspace, size_t workspace_bytes, void *stream);
+/* gstex_bin_gaussians as two calls (no counterpart in the
+ * reference; the same outputs), for a caller that reads the intersect total
+ * back to the host: gstex_bin_gaussians_depth needs only the Gaussians (entry
+ * counts, the depth sort, the emission offsets) and can be queued before the
+ * read-back; gstex_bin_gaussians_tiles then takes num_intersects (the list
+ * length) and the depth phase's workspace, unchanged in between, and writes
+ * the sorted ids and the bins.  The depth phase clears tile_bins.  When
+ * num_intersects_dev is given it holds num_intersects on the device (as
+ * written by gstex_project_gaussians_total or gstex_count_intersects), which
+ * saves a launch.  Workspaces: gstex_bin_gaussians_depth_workspace_size(N)
+ * and gstex_bin_gaussians_tiles_workspace_size(N, num_intersects) bytes. */
+size_t gstex_bin_gaussians_depth_workspace_size(int num_points);
+size_t gstex_bin_gaussians_tiles_workspace_size(int num_points, int num_intersects);
+int gstex_bin_gaussians_depth(int num_points, const float *centers, const float *extents,
+                              const float *depths, const int32_t *num_tiles_hit, int tiles_x,
+                              int tiles_y, int block_width, int32_t *tile_bins, void *depth_ws,
+                              size_t depth_ws_bytes, void *stream);
+int gstex_bin_gaussians_tiles(int num_points, int num_intersects,
+                              const uint32_t *num_intersects_dev, int tiles_x, int tiles_y,
+                              const void *depth_ws, size_t depth_ws_bytes,
+                              int32_t *gaussian_ids_sorted, int32_t *tile_bins, void *tiles_ws,
+                              size_t tiles_ws_bytes, void *stream);
 /* Like gstex_bin_gaussians, with no host-side intersect count (no host sync,
  * so a caller can capture the step in a graph): the list has room for
  * `capacity` entries, the intersect total sum(num_tiles_hit) is computed on

@@ -120,6 +120,44 @@ def test_binning_unpacked_keys(oracle):
     assert _binning_case(oracle, s, 8) > 1_000_000
 
 
+@pytest.mark.parametrize("bw,num_points", [(16, 3000), (8, 300_000)])
+def test_binning_two_phase_equals_one_call(bw, num_points):
+    # gstex_bin_gaussians_depth + gstex_bin_gaussians_tiles (the exact-size
+    # step's path) give gstex_bin_gaussians' ids and bins, with the total
+    # passed on the device or taken from num_intersects; packed keys at bw 16,
+    # (tile, id) pairs at 1080p with 8-pixel tiles
+    from gstex_cuda_amd import _lib
+
+    s = scene(2 if num_points > 100_000 else 1, num_points=num_points, seed=5 + bw)
+    pb = project_and_bin(s, bw)
+    n, m = s.num_points, pb["num_intersects"]
+    tx, ty, _ = tile_bounds(s.img_w, s.img_h, bw)
+    lib = _lib.lib()
+    dws = torch.empty(lib.gstex_bin_gaussians_depth_workspace_size(n), dtype=torch.uint8,
+                      device=DEV)
+    tws = torch.empty(lib.gstex_bin_gaussians_tiles_workspace_size(n, m), dtype=torch.uint8,
+                      device=DEV)
+    m_dev = torch.tensor([m], dtype=torch.int32, device=DEV)
+    st = torch.cuda.current_stream().cuda_stream
+    for dev_total in (m_dev, None):
+        ids = torch.full((m,), -1, dtype=torch.int32, device=DEV)
+        bins = torch.full((tx * ty, 2), -5, dtype=torch.int32, device=DEV)
+        assert lib.gstex_bin_gaussians_depth(
+            n, pb["centers"].data_ptr(), pb["extents"].data_ptr(), pb["depths"].data_ptr(),
+            pb["num_tiles_hit"].data_ptr(), tx, ty, bw, bins.data_ptr(), dws.data_ptr(),
+            dws.numel(), st) == 0
+        assert lib.gstex_bin_gaussians_tiles(
+            n, m, None if dev_total is None else dev_total.data_ptr(), tx, ty, dws.data_ptr(),
+            dws.numel(), ids.data_ptr(), bins.data_ptr(), tws.data_ptr(), tws.numel(), st) == 0
+        torch.cuda.synchronize()
+        assert np.array_equal(np_(ids), np_(pb["ids"]))
+        assert np.array_equal(np_(bins), np_(pb["bins"]))
+    # a workspace below the queried size is refused
+    assert lib.gstex_bin_gaussians_tiles(n, m, None, tx, ty, dws.data_ptr(), dws.numel(),
+                                         ids.data_ptr(), bins.data_ptr(), tws.data_ptr(),
+                                         tws.numel() - 1, st) != 0
+
+
 def test_count_intersects(oracle):
     # gstex_count_intersects: the intersect total of compute_cumulative_intersects
     # at full C2 size, and for no Gaussians
