@@ -114,31 +114,72 @@ __global__ __launch_bounds__(256) void k_sh_forward(int n, int nb, const float* 
 
 // v_coeffs[k][c] = basis_k * v_colors[c] for k < bases(D), zero above
 // (sh.cuh:234-253); the output is fully written, so no memset.
+// With staging (v_coeffs 16-byte aligned and dynamic LDS of 256 x (3 nb + 1)
+// floats), a block's rows go through LDS (rows padded to an odd word count,
+// so the row writes do not collide in banks) and leave as 16-byte stores of
+// consecutive addresses: the block's rows are one contiguous span.  Without
+// it each thread stores its own row.
 template <int D>
 __global__ __launch_bounds__(256) void k_sh_backward(int n, int nb, const float* __restrict__ viewdirs,
                                                      const float* __restrict__ v_colors,
-                                                     float* __restrict__ v_coeffs) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+                                                     float* __restrict__ v_coeffs, bool staged) {
+    extern __shared__ float stage[];
+    const int i0 = blockIdx.x * blockDim.x, i = i0 + threadIdx.x;
+    const bool valid = i < n;
+    if (!staged && !valid) return;
     constexpr int used = D <= 0 ? 1 : D == 1 ? 4 : D == 2 ? 9 : D == 3 ? 16 : 25;
     float b[25];
     b[0] = kShC0;
-    if constexpr (D >= 1) {
-        float x, y, z;
-        sh_basis<D>(viewdirs[3 * i], viewdirs[3 * i + 1], viewdirs[3 * i + 2], b, x, y, z);
+    float g0 = 0.f, g1 = 0.f, g2 = 0.f;
+    if (valid) {
+        if constexpr (D >= 1) {
+            float x, y, z;
+            sh_basis<D>(viewdirs[3 * i], viewdirs[3 * i + 1], viewdirs[3 * i + 2], b, x, y, z);
+        }
+        g0 = v_colors[3 * i];
+        g1 = v_colors[3 * i + 1];
+        g2 = v_colors[3 * i + 2];
     }
-    const float g0 = v_colors[3 * i], g1 = v_colors[3 * i + 1], g2 = v_colors[3 * i + 2];
-    float* out = v_coeffs + (size_t)i * nb * 3;
+    const int R = 3 * nb;
+    float* out = staged ? stage + threadIdx.x * (R + 1) : v_coeffs + (size_t)i * R;
 #pragma unroll
     for (int k = 0; k < used; ++k) {
         out[3 * k] = b[k] * g0;
         out[3 * k + 1] = b[k] * g1;
         out[3 * k + 2] = b[k] * g2;
     }
-    for (int k = used; k < nb; ++k) {
-        out[3 * k] = 0.f;
-        out[3 * k + 1] = 0.f;
-        out[3 * k + 2] = 0.f;
+    for (int k = 3 * used; k < R; ++k) out[k] = 0.f;
+    if (!staged) return;
+    __syncthreads();
+    const int cnt = min(n - i0, (int)blockDim.x);
+    const int total = cnt * R;
+    float* dst = v_coeffs + (size_t)i0 * R;
+    // position q of the span is row r, column c; both advance without a
+    // division per element
+    const int step = 4 * (int)blockDim.x, step_r = step / R, step_c = step - step_r * R;
+    int q = 4 * (int)threadIdx.x, r = q / R, c = q - r * R;
+    for (; q < total; q += step) {
+        float v[4];
+        int rr = r, cc = c;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            v[j] = q + j < total ? stage[rr * (R + 1) + cc] : 0.f;
+            if (++cc == R) {
+                cc = 0;
+                ++rr;
+            }
+        }
+        if (q + 4 <= total) {
+            *reinterpret_cast<float4*>(dst + q) = make_float4(v[0], v[1], v[2], v[3]);
+        } else {
+            for (int j = 0; q + j < total; ++j) dst[q + j] = v[j];
+        }
+        r += step_r;
+        c += step_c;
+        if (c >= R) {
+            c -= R;
+            ++r;
+        }
     }
 }
 
@@ -211,12 +252,16 @@ hipError_t launch_sh_backward(int n, int degree, int degrees_to_use, const float
                               const float* v_colors, float* v_coeffs, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     const int nb = num_sh_bases(degree), g = (n + 255) / 256;
+    // staged rows up to 64 KB of LDS per block (degree <= 3)
+    const size_t lds = (size_t)256 * (3 * nb + 1) * sizeof(float);
+    const bool staged = (reinterpret_cast<uintptr_t>(v_coeffs) & 15) == 0 && lds <= 65536;
+    const size_t sh = staged ? lds : 0;
     switch (degrees_to_use) {
-        case 0: k_sh_backward<0><<<g, 256, 0, stream>>>(n, nb, viewdirs, v_colors, v_coeffs); break;
-        case 1: k_sh_backward<1><<<g, 256, 0, stream>>>(n, nb, viewdirs, v_colors, v_coeffs); break;
-        case 2: k_sh_backward<2><<<g, 256, 0, stream>>>(n, nb, viewdirs, v_colors, v_coeffs); break;
-        case 3: k_sh_backward<3><<<g, 256, 0, stream>>>(n, nb, viewdirs, v_colors, v_coeffs); break;
-        default: k_sh_backward<4><<<g, 256, 0, stream>>>(n, nb, viewdirs, v_colors, v_coeffs); break;
+        case 0: k_sh_backward<0><<<g, 256, sh, stream>>>(n, nb, viewdirs, v_colors, v_coeffs, staged); break;
+        case 1: k_sh_backward<1><<<g, 256, sh, stream>>>(n, nb, viewdirs, v_colors, v_coeffs, staged); break;
+        case 2: k_sh_backward<2><<<g, 256, sh, stream>>>(n, nb, viewdirs, v_colors, v_coeffs, staged); break;
+        case 3: k_sh_backward<3><<<g, 256, sh, stream>>>(n, nb, viewdirs, v_colors, v_coeffs, staged); break;
+        default: k_sh_backward<4><<<g, 256, sh, stream>>>(n, nb, viewdirs, v_colors, v_coeffs, staged); break;
     }
     return hipGetLastError();
 }

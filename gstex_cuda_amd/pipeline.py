@@ -159,7 +159,8 @@ class RenderStep:
         self._side = torch.cuda.Stream(dev) if overlap else None
         self._ev_records = torch.cuda.Event() if overlap else None
         self._ev_zeroed = torch.cuda.Event() if overlap else None
-        self._zeroed = False
+        self._zeroed = False       # zeroing queued on the side stream, not yet waited for
+        self._zero_ready = None    # the stream that waited for it
         if capacity is not None:
             self._ensure_bins(int(capacity))
         else:
@@ -279,7 +280,13 @@ class RenderStep:
         if side is None:
             self._colors_and_records(s)
         else:
-            main.wait_event(self._ev_records)
+            # The raster forward needs the records.  The zeroing, queued after
+            # them on the side stream, ends long before the binning does, so
+            # one wait here covers both (each wait is a barrier packet that
+            # the main stream processes between kernels).
+            main.wait_event(self._ev_zeroed)
+            self._zeroed = False
+            self._zero_ready = main
 
     def forward_raster(self, timer: Optional[StageTimer] = None):
         p, cam, s = self.p, self.cam, self._stream()
@@ -343,12 +350,16 @@ class RenderStep:
         (``v_img`` / ``v_tex`` are then not written)."""
         p, cam, s, g = self.p, self.cam, self._stream(), self.grads
         n, m = self.n, self._list_len
-        if self._zeroed:
+        cur = torch.cuda.current_stream(self.dev)
+        if self._zero_ready is not None and self._zero_ready == cur:
+            pass  # zeroed on the side stream; this stream already waited for it
+        elif self._zeroed or self._zero_ready is not None:
             # zeroed on the side stream during the forward
-            torch.cuda.current_stream(self.dev).wait_event(self._ev_zeroed)
-            self._zeroed = False
+            cur.wait_event(self._ev_zeroed)
         else:
             self._zero_grads(s)
+        self._zeroed = False
+        self._zero_ready = None
         v_depth = self.v_zero1 if v_depth is None else v_depth
         v_reg = self.v_zero1 if v_reg is None else v_reg
         v_alpha = self.v_zero1 if v_alpha is None else v_alpha
