@@ -93,6 +93,8 @@ _SIGNATURES = {
                                   P, P, P, P, P, P,  # six output gradients
                                   P, P, P, Z, P]),  # grad_records, v_texture, masks, bytes, stream
     "gstex_finalize_gradients": (I, [I, P, P, P, P, P, P, F, F, P, P, P, P, P, P, P, P, P]),
+    "gstex_finalize_gradients_sh": (I, [I, P, P, P, P, P, P, F, F, P, I, I,  # .., means, degrees
+                                        P, P, P, P, P, P, P, P, P, P]),  # 9 gradients, stream
     "gstex_finalize_gradients_views": (I, [I, I, P, P, Z, P, P, P, P, P, P, P, P, P, P, P, P, P,
                                            P, P]),
     "gstex_image_loss_grad": (I, [I, I, I, P, P, P, F, P, F, P, P, P, P]),

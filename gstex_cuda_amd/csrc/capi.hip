@@ -749,6 +749,34 @@ int gstex_finalize_gradients(int num_points, const float* grad_records, const vo
                      "finalize_gradients");
 }
 
+int gstex_finalize_gradients_sh(int num_points, const float* grad_records, const void* records,
+                                const float* quats, const float* scales, const float* viewmat,
+                                const float* c2w, float fx, float fy, const float* means,
+                                int sh_degree, int degrees_to_use, float* v_colors,
+                                float* v_opacity, float* v_means, float* v_scales,
+                                float* v_quats, float* v_uv0, float* v_umap, float* v_vmap,
+                                float* v_sh_coeffs, void* stream) {
+    REQUIRE(num_points >= 0, "num_points must be >= 0");
+    REQUIRE(sh_degree >= 0 && sh_degree <= 3,
+            "gstex_finalize_gradients_sh takes SH degrees 0..3, got %d", sh_degree);
+    REQUIRE(degrees_to_use >= 0 && degrees_to_use <= sh_degree,
+            "degrees_to_use must be in 0..%d, got %d", sh_degree, degrees_to_use);
+    if (num_points == 0) return GSTEX_OK;
+    REQUIRE(grad_records && records && scales && quats && viewmat && c2w && means && v_colors &&
+                v_opacity && v_means && v_scales && v_quats && v_uv0 && v_umap && v_vmap &&
+                v_sh_coeffs,
+            "null pointer argument");
+    const int nb = (sh_degree + 1) * (sh_degree + 1);
+    REQUIRE(gstex::grad_finalize_sh_supported(nb, v_sh_coeffs),
+            "v_sh_coeffs must be 16-byte aligned");
+    return check_hip(gstex::grad_finalize_sh(num_points, grad_records, (const float*)records,
+                                             quats, scales, viewmat, c2w, fx, fy, v_colors,
+                                             v_opacity, v_means, v_scales, v_quats, v_uv0,
+                                             v_umap, v_vmap, means, nb, degrees_to_use,
+                                             v_sh_coeffs, as_stream(stream)),
+                     "finalize_gradients_sh");
+}
+
 int gstex_finalize_gradients_views(int num_points, int num_views, const float* grad_records,
                                    const void* records, size_t records_stride,
                                    const float* quats, const float* scales,

@@ -166,6 +166,16 @@ hipError_t grad_finalize(int n, const float* grad_rec, const float* records, con
                          float fy, float* v_colors, float* v_opacity, float* v_means,
                          float* v_scales, float* v_quats, float* v_uv0, float* v_umap,
                          float* v_vmap, hipStream_t s);
+// grad_finalize followed by the SH adjoint of the colour gradient (what
+// compute_sh_backward gives with viewdirs = means - c2w translation), for a
+// coefficient stride of nb bases (at most 16) into 16-byte aligned v_coeffs.
+bool grad_finalize_sh_supported(int nb, const float* v_coeffs);
+hipError_t grad_finalize_sh(int n, const float* grad_rec, const float* records, const float* quats,
+                            const float* scales, const float* viewmat, const float* c2w,
+                            float fx, float fy, float* v_colors, float* v_opacity,
+                            float* v_means, float* v_scales, float* v_quats, float* v_uv0,
+                            float* v_umap, float* v_vmap, const float* means, int nb,
+                            int degrees_to_use, float* v_coeffs, hipStream_t s);
 hipError_t grad_finalize_views(int n, int views, const float* grad_rec, const void* records,
                                size_t rec_stride, const float* quats, const float* scales,
                                const float* viewmats, const float* c2ws, const float* focals,

@@ -10,6 +10,7 @@
 // the splat plane), each outer product sum_pixels v * delta becomes
 // ax1 * sum(v*l1) + ax2 * sum(v*l2).
 #include "raster_common.hpp"
+#include "sh_basis.hpp"
 
 namespace gstex {
 
@@ -627,6 +628,43 @@ __global__ __launch_bounds__(256) void k_grad_finalize(
     store_final(o, g, f);
 }
 
+// k_grad_finalize followed by the SH adjoint of its colour gradient (what
+// gstex_compute_sh_backward does with viewdirs = means - camera position,
+// the same float32 subtraction): v_coeffs rows of 3 nb floats, bands up to
+// D.  The block's LDS first holds its gradient records, then its SH rows
+// (sh_rows_staged), so the dynamic LDS is the larger of the two.
+template <int D>
+__global__ __launch_bounds__(256) void k_grad_finalize_sh(
+    int n, const float* __restrict__ grad_rec, const float* __restrict__ records,
+    const float* __restrict__ quats, const float* __restrict__ scales,
+    const float* __restrict__ viewmat, const float* __restrict__ c2w, float fx, float fy,
+    FinalOut o, const float* __restrict__ means, int nb, float* __restrict__ v_coeffs) {
+    extern __shared__ float lds[];
+    GradRows& rows = *reinterpret_cast<GradRows*>(lds);
+    const int g0 = blockIdx.x * blockDim.x, g = g0 + threadIdx.x;
+    stage_grad_rows(grad_rec, g0, n, rows);
+    __syncthreads();
+    const bool valid = g < n;
+    float b[25];
+    b[0] = kShC0;
+    float c0 = 0.f, c1 = 0.f, c2 = 0.f;
+    if (valid) {
+        float f[kFinalFloats];
+        finalize_one(g, rows[threadIdx.x], records, quats, scales, viewmat, c2w, fx, fy, f);
+        store_final(o, g, f);
+        c0 = f[0];
+        c1 = f[1];
+        c2 = f[2];
+        if constexpr (D >= 1) {
+            float x, y, z;
+            sh_basis<D>(means[3 * g] - c2w[3], means[3 * g + 1] - c2w[7],
+                        means[3 * g + 2] - c2w[11], b, x, y, z);
+        }
+    }
+    __syncthreads();  // the gradient records are consumed
+    sh_rows_staged<D>(lds, nb, b, c0, c1, c2, g0, n, v_coeffs);
+}
+
 // The sum over V views of k_grad_finalize: view v's gradient records start at
 // grad_rec + v n 32 and its splat records rec_stride bytes after view v - 1's;
 // cameras are (V, 16) row-major matrices and (V, 2) focal lengths.  Each
@@ -754,6 +792,36 @@ hipError_t grad_finalize(int n, const float* grad_rec, const float* records, con
     const FinalOut o{v_colors, v_opacity, v_means, v_scales, v_quats, v_uv0, v_umap, v_vmap};
     k_grad_finalize<<<(n + 255) / 256, 256, 0, s>>>(n, grad_rec, records, quats, scales, viewmat,
                                                    c2w, fx, fy, o);
+    return hipGetLastError();
+}
+
+bool grad_finalize_sh_supported(int nb, const float* v_coeffs) {
+    return (reinterpret_cast<uintptr_t>(v_coeffs) & 15) == 0 &&
+           std::max(sizeof(GradRows), sh_rows_stage_bytes(nb)) <= 65536;
+}
+
+hipError_t grad_finalize_sh(int n, const float* grad_rec, const float* records, const float* quats,
+                            const float* scales, const float* viewmat, const float* c2w,
+                            float fx, float fy, float* v_colors, float* v_opacity,
+                            float* v_means, float* v_scales, float* v_quats, float* v_uv0,
+                            float* v_umap, float* v_vmap, const float* means, int nb,
+                            int degrees_to_use, float* v_coeffs, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (!grad_finalize_sh_supported(nb, v_coeffs)) return hipErrorInvalidValue;
+    const FinalOut o{v_colors, v_opacity, v_means, v_scales, v_quats, v_uv0, v_umap, v_vmap};
+    const size_t lds = std::max(sizeof(GradRows), sh_rows_stage_bytes(nb));
+    const int g = (n + 255) / 256;
+#define GSTEX_FIN_SH(D)                                                                       \
+    k_grad_finalize_sh<D><<<g, 256, lds, s>>>(n, grad_rec, records, quats, scales, viewmat, c2w, \
+                                              fx, fy, o, means, nb, v_coeffs)
+    switch (degrees_to_use) {
+        case 0: GSTEX_FIN_SH(0); break;
+        case 1: GSTEX_FIN_SH(1); break;
+        case 2: GSTEX_FIN_SH(2); break;
+        case 3: GSTEX_FIN_SH(3); break;
+        default: GSTEX_FIN_SH(4); break;
+    }
+#undef GSTEX_FIN_SH
     return hipGetLastError();
 }
 

@@ -7,69 +7,9 @@
 // stride (bases of the stored degree) stays a runtime value.
 #include "gstex_common.hpp"
 #include "gstex_kernels.hpp"
+#include "sh_basis.hpp"
 
 namespace gstex {
-
-constexpr float kShC0 = 0.28209479177387814f;
-constexpr float kShC1 = 0.4886025119029199f;
-constexpr float kShC2_0 = 1.0925484305920792f, kShC2_1 = -1.0925484305920792f,
-                kShC2_2 = 0.31539156525252005f, kShC2_3 = -1.0925484305920792f,
-                kShC2_4 = 0.5462742152960396f;
-constexpr float kShC3_0 = -0.5900435899266435f, kShC3_1 = 2.890611442640554f,
-                kShC3_2 = -0.4570457994644658f, kShC3_3 = 0.3731763325901154f,
-                kShC3_4 = -0.4570457994644658f, kShC3_5 = 1.445305721320277f,
-                kShC3_6 = -0.5900435899266435f;
-constexpr float kShC4_0 = 2.5033429417967046f, kShC4_1 = -1.7701307697799304f,
-                kShC4_2 = 0.9461746957575601f, kShC4_3 = -0.6690465435572892f,
-                kShC4_4 = 0.10578554691520431f, kShC4_5 = -0.6690465435572892f,
-                kShC4_6 = 0.47308734787878004f, kShC4_7 = -1.7701307697799304f,
-                kShC4_8 = 0.6258357354491761f;
-
-__host__ __device__ inline int num_sh_bases(int degree) {
-    return degree <= 0 ? 1 : degree == 1 ? 4 : degree == 2 ? 9 : degree == 3 ? 16 : 25;
-}
-
-// Basis values 1..24 at the normalised direction for bands up to D.
-template <int D>
-__device__ __forceinline__ void sh_basis(float dx, float dy, float dz, float (&b)[25], float& x,
-                                         float& y, float& z) {
-    const float norm = sqrtf(dx * dx + dy * dy + dz * dz);
-    x = dx / norm;
-    y = dy / norm;
-    z = dz / norm;
-    const float xx = x * x, xy = x * y, xz = x * z, yy = y * y, yz = y * z, zz = z * z;
-    b[0] = kShC0;
-    b[1] = -kShC1 * y;
-    b[2] = kShC1 * z;
-    b[3] = -kShC1 * x;
-    if constexpr (D >= 2) {
-        b[4] = kShC2_0 * xy;
-        b[5] = kShC2_1 * yz;
-        b[6] = kShC2_2 * (2.f * zz - xx - yy);
-        b[7] = kShC2_3 * xz;
-        b[8] = kShC2_4 * (xx - yy);
-    }
-    if constexpr (D >= 3) {
-        b[9] = kShC3_0 * y * (3.f * xx - yy);
-        b[10] = kShC3_1 * xy * z;
-        b[11] = kShC3_2 * y * (4.f * zz - xx - yy);
-        b[12] = kShC3_3 * z * (2.f * zz - 3.f * xx - 3.f * yy);
-        b[13] = kShC3_4 * x * (4.f * zz - xx - yy);
-        b[14] = kShC3_5 * z * (xx - yy);
-        b[15] = kShC3_6 * x * (xx - 3.f * yy);
-    }
-    if constexpr (D >= 4) {
-        b[16] = kShC4_0 * xy * (xx - yy);
-        b[17] = kShC4_1 * yz * (3.f * xx - yy);
-        b[18] = kShC4_2 * xy * (7.f * zz - 1.f);
-        b[19] = kShC4_3 * yz * (7.f * zz - 3.f);
-        b[20] = kShC4_4 * (zz * (35.f * zz - 30.f) + 3.f);
-        b[21] = kShC4_5 * xz * (7.f * zz - 3.f);
-        b[22] = kShC4_6 * (xx - yy) * (7.f * zz - 1.f);
-        b[23] = kShC4_7 * xz * (xx - 3.f * yy);
-        b[24] = kShC4_8 * (xx * (xx - 3.f * yy) - yy * (3.f * xx - yy));
-    }
-}
 
 // Coefficient k of a float32 or bfloat16 (uint16 bits, widened exactly) array.
 __device__ __forceinline__ float coef(const float* c, int k) { return c[k]; }
@@ -140,47 +80,18 @@ __global__ __launch_bounds__(256) void k_sh_backward(int n, int nb, const float*
         g1 = v_colors[3 * i + 1];
         g2 = v_colors[3 * i + 2];
     }
-    const int R = 3 * nb;
-    float* out = staged ? stage + threadIdx.x * (R + 1) : v_coeffs + (size_t)i * R;
+    if (staged) {
+        sh_rows_staged<D>(stage, nb, b, g0, g1, g2, i0, n, v_coeffs);
+        return;
+    }
+    float* out = v_coeffs + (size_t)i * 3 * nb;
 #pragma unroll
     for (int k = 0; k < used; ++k) {
         out[3 * k] = b[k] * g0;
         out[3 * k + 1] = b[k] * g1;
         out[3 * k + 2] = b[k] * g2;
     }
-    for (int k = 3 * used; k < R; ++k) out[k] = 0.f;
-    if (!staged) return;
-    __syncthreads();
-    const int cnt = min(n - i0, (int)blockDim.x);
-    const int total = cnt * R;
-    float* dst = v_coeffs + (size_t)i0 * R;
-    // position q of the span is row r, column c; both advance without a
-    // division per element
-    const int step = 4 * (int)blockDim.x, step_r = step / R, step_c = step - step_r * R;
-    int q = 4 * (int)threadIdx.x, r = q / R, c = q - r * R;
-    for (; q < total; q += step) {
-        float v[4];
-        int rr = r, cc = c;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            v[j] = q + j < total ? stage[rr * (R + 1) + cc] : 0.f;
-            if (++cc == R) {
-                cc = 0;
-                ++rr;
-            }
-        }
-        if (q + 4 <= total) {
-            *reinterpret_cast<float4*>(dst + q) = make_float4(v[0], v[1], v[2], v[3]);
-        } else {
-            for (int j = 0; q + j < total; ++j) dst[q + j] = v[j];
-        }
-        r += step_r;
-        c += step_c;
-        if (c >= R) {
-            c -= R;
-            ++r;
-        }
-    }
+    for (int k = 3 * used; k < 3 * nb; ++k) out[k] = 0.f;
 }
 
 // The sum over V views of k_sh_backward: viewdirs and v_colors are (V, n, 3)
@@ -253,7 +164,7 @@ hipError_t launch_sh_backward(int n, int degree, int degrees_to_use, const float
     if (n <= 0) return hipSuccess;
     const int nb = num_sh_bases(degree), g = (n + 255) / 256;
     // staged rows up to 64 KB of LDS per block (degree <= 3)
-    const size_t lds = (size_t)256 * (3 * nb + 1) * sizeof(float);
+    const size_t lds = sh_rows_stage_bytes(nb);
     const bool staged = (reinterpret_cast<uintptr_t>(v_coeffs) & 15) == 0 && lds <= 65536;
     const size_t sh = staged ? lds : 0;
     switch (degrees_to_use) {

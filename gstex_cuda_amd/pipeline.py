@@ -384,11 +384,18 @@ class RenderStep:
             timer.stop("raster_backward")
         if not self.finalize:
             return
-        call("gstex_finalize_gradients", n, _p(self.grad_records), _p(self.records),
-             _p(p["quats"]), _p(p["scales"]), _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy,
-             _p(g["v_colors"]), _p(g["v_opacity"]), _p(g["v_means"]), _p(g["v_scales"]),
-             _p(g["v_quats"]), _p(g["v_uv0"]), _p(g["v_umap"]), _p(g["v_vmap"]), s)
-        if self.sh_degree >= 0 and self.sh_backward:
+        fin = (n, _p(self.grad_records), _p(self.records), _p(p["quats"]), _p(p["scales"]),
+               _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy)
+        grads = (_p(g["v_colors"]), _p(g["v_opacity"]), _p(g["v_means"]), _p(g["v_scales"]),
+                 _p(g["v_quats"]), _p(g["v_uv0"]), _p(g["v_umap"]), _p(g["v_vmap"]))
+        sh = self.sh_degree >= 0 and self.sh_backward
+        if sh and self.sh_degree <= 3 and g["v_sh_coeffs"].data_ptr() % 16 == 0:
+            # the SH adjoint in the finalize pass (view directions from the means)
+            call("gstex_finalize_gradients_sh", *fin, _p(p["means"]), self.sh_degree,
+                 self.sh_degree, *grads, _p(g["v_sh_coeffs"]), s)
+            return
+        call("gstex_finalize_gradients", *fin, *grads, s)
+        if sh:
             call("gstex_compute_sh_backward", n, self.sh_degree, self.sh_degree,
                  _p(self._viewdirs), _p(g["v_colors"]), _p(g["v_sh_coeffs"]), s)
 

@@ -403,6 +403,20 @@ int gstex_finalize_gradients(int num_points, const float *grad_records, const vo
                              const float *c2w, float fx, float fy, float *v_colors,
                              float *v_opacity, float *v_means, float *v_scales, float *v_quats,
                              float *v_uv0, float *v_umap, float *v_vmap, void *stream);
+/* gstex_finalize_gradients followed by gstex_compute_sh_backward(num_points,
+ * sh_degree, degrees_to_use, viewdirs, v_colors, v_sh_coeffs) with viewdirs =
+ * means - the camera position (c2w[:3, 3], the same float32 subtraction), in
+ * one pass: the same outputs, and v_colors is written too.  No counterpart in
+ * the reference (whose trainer applies the SH adjoint through torch
+ * autograd); for training steps with SH colours.  sh_degree 0..3 (v_sh_coeffs
+ * (num_points, (sh_degree + 1)^2, 3), 16-byte aligned). */
+int gstex_finalize_gradients_sh(int num_points, const float *grad_records, const void *records,
+                                const float *quats, const float *scales, const float *viewmat,
+                                const float *c2w, float fx, float fy, const float *means,
+                                int sh_degree, int degrees_to_use, float *v_colors,
+                                float *v_opacity, float *v_means, float *v_scales,
+                                float *v_quats, float *v_uv0, float *v_umap, float *v_vmap,
+                                float *v_sh_coeffs, void *stream);
 /* The sum over num_views views of gstex_finalize_gradients, for batched
  * multi-view steps (no counterpart in the reference, whose trainer renders one
  * view per step, example.py:227-255).  grad_records holds num_views stacked

@@ -168,3 +168,40 @@ def test_backward_l2_rejects_other_channel_counts():
                                       16.0, 16.0, p, p, p, p, p, p, p, p, 1.0, p, p, p, p, p,
                                       p, None, 0, None)
     assert rc != 0 and b"3 channels" in lib.gstex_last_error()
+
+
+@pytest.mark.parametrize("degree,num_points", [(3, 3001), (1, 700), (0, 300)])
+def test_finalize_sh_equals_separate_passes(degree, num_points):
+    # gstex_finalize_gradients_sh (RenderStep's finalize with SH colours) gives
+    # gstex_finalize_gradients followed by gstex_compute_sh_backward at the
+    # view directions means - camera position, bit for bit
+    from gstex_cuda_amd import _lib, synthetic
+    from gstex_cuda_amd.pipeline import scene_to_step
+
+    spec = dataclasses.replace(synthetic.WORKLOADS[1], sh_degree=degree, texel_budget=0,
+                               texel_side=4)
+    s = synthetic.make_scene(1, seed=70 + degree, num_points=num_points, img_w=96, img_h=64,
+                             spec=spec)
+    tgt = torch.from_numpy(np.ascontiguousarray(s.target, dtype=np.float32)).to(DEV)
+    step = scene_to_step(s, DEV)
+    assert step.sh_degree == degree
+    step.step(tgt)
+    torch.cuda.synchronize()
+    got = {k: np_(v).copy() for k, v in step.grads.items()}
+    lib = _lib.lib()
+    p, cam, n = step.p, step.cam, step.n
+    ref = {k: torch.full_like(v, 7.0) for k, v in step.grads.items()}
+    st = torch.cuda.current_stream().cuda_stream
+    ptr = lambda t: t.data_ptr()
+    assert lib.gstex_finalize_gradients(
+        n, ptr(step.grad_records), ptr(step.records), ptr(p["quats"]), ptr(p["scales"]),
+        ptr(cam.viewmat), ptr(cam.c2w), cam.fx, cam.fy, ptr(ref["v_colors"]),
+        ptr(ref["v_opacity"]), ptr(ref["v_means"]), ptr(ref["v_scales"]), ptr(ref["v_quats"]),
+        ptr(ref["v_uv0"]), ptr(ref["v_umap"]), ptr(ref["v_vmap"]), st) == 0
+    dirs = (p["means"] - cam.c2w[:3, 3]).contiguous()
+    assert lib.gstex_compute_sh_backward(n, degree, degree, ptr(dirs), ptr(ref["v_colors"]),
+                                         ptr(ref["v_sh_coeffs"]), st) == 0
+    torch.cuda.synchronize()
+    for k in ("v_colors", "v_opacity", "v_means", "v_scales", "v_quats", "v_uv0", "v_umap",
+              "v_vmap", "v_sh_coeffs"):
+        assert np.array_equal(got[k], np_(ref[k])), k
