@@ -1383,7 +1383,10 @@ __global__ __launch_bounds__(256) void k_tile_hist1(
 // its value), eight consecutive positions per thread, into LDS, then reads
 // them back in (wave, round, lane) order and scatters them by the digit
 // [id_bits, id_bits + bits) with bases from the scanned k_tile_hist1 counts.
-template <bool PACKED>
+// COMPACT stages each rectangle as one word, first tile << 12 | width (grids
+// below 2^20 tiles with rows of fewer than 4096 tiles): 8 KB less LDS, one
+// more workgroup per CU.
+template <bool PACKED, bool COMPACT>
 __global__ __launch_bounds__(256) void k_emit_scatter1(
     int n, int cap, const uint32_t* __restrict__ m_total,
     const uint32_t* __restrict__ total_written, const uint4* __restrict__ rec,
@@ -1396,20 +1399,22 @@ __global__ __launch_bounds__(256) void k_emit_scatter1(
     constexpr int kStage = kEmitOut;
     constexpr int kPer = kEmitOut / 256;
     static_assert(kPer == 8 && kEmitOut == kSortThreads * 8, "one sort tile per block");
-    // One LDS buffer (40 KB with packed keys, four workgroups per CU): the
-    // staged Gaussians (offset, id, corner, width) and the generated keys (and
-    // ids).  Once generated, s_off / s_g hold the digit-sorted tile and the
-    // scatter's counters take the place of s_xy / s_wd.
-    constexpr int kWords = 4 * kStage + (PACKED ? 1 : 2) * kEmitOut;
+    // One LDS buffer (32 KB with packed keys and COMPACT: five workgroups per
+    // CU): the staged Gaussians (offset, id, rectangle) and the generated keys
+    // (and ids).  Once generated, s_off / s_g hold the digit-sorted tile and
+    // the scatter's counters take the place of the rectangles.
+    constexpr int kRectWords = COMPACT ? 1 : 2;
+    constexpr int kWords = (2 + kRectWords) * kStage + (PACKED ? 1 : 2) * kEmitOut;
     __shared__ __align__(16) uint32_t lds[kWords];
     uint32_t* const s_off = lds;
     uint32_t* const s_g = lds + kStage;
-    uint32_t* const s_xy = lds + 2 * kStage;
-    uint32_t* const s_wd = lds + 3 * kStage;
-    uint32_t* const s_key = lds + 4 * kStage;
+    uint32_t* const s_r0 = lds + 2 * kStage;  // first tile << 12 | width, or corner x0 | y0 << 16
+    uint32_t* const s_r1 = s_r0 + kStage;     // width (without COMPACT)
+    uint32_t* const s_key = lds + (2 + kRectWords) * kStage;
     uint32_t* const s_id = s_key + kEmitOut;  // without PACKED
-    static_assert(sizeof(ScatterLds) <= 2 * kStage * sizeof(uint32_t), "counters fit s_xy, s_wd");
-    ScatterLds& sl = *reinterpret_cast<ScatterLds*>(s_xy);
+    static_assert(sizeof(ScatterLds) <= kRectWords * kStage * sizeof(uint32_t),
+                  "counters fit the rectangles");
+    ScatterLds& sl = *reinterpret_cast<ScatterLds*>(s_r0);
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const int ndig = 1 << bits;
     // independent loads first: this thread's digit base, the block's start
@@ -1430,8 +1435,13 @@ __global__ __launch_bounds__(256) void k_emit_scatter1(
             const uint4 r = rec[plo + (uint32_t)i];
             s_off[i] = r.x;
             s_g[i] = r.y;
-            s_xy[i] = r.z;
-            s_wd[i] = r.w;
+            if (COMPACT) {
+                const uint32_t t0 = (r.z >> 16) * (uint32_t)tiles_x + (r.z & 0xffffu);
+                s_r0[i] = (t0 << 12) | r.w;
+            } else {
+                s_r0[i] = r.z;
+                s_r1[i] = r.w;
+            }
         }
     }
     __syncthreads();
@@ -1454,7 +1464,20 @@ __global__ __launch_bounds__(256) void k_emit_scatter1(
             if (s_off[mid] <= kt) lo = mid;
             else hi = mid - 1;
         }
-        uint32_t wd = s_wd[lo], xy = s_xy[lo], gid = s_g[lo];
+        // the Gaussian's first tile t0 and rectangle width
+        auto rect = [&](int at, uint32_t& t0, uint32_t& w) {
+            if (COMPACT) {
+                const uint32_t v = s_r0[at];
+                t0 = v >> 12;
+                w = v & 0xfffu;
+            } else {
+                const uint32_t xy = s_r0[at];
+                t0 = (xy >> 16) * (uint32_t)tiles_x + (xy & 0xffffu);
+                w = s_r1[at];
+            }
+        };
+        uint32_t t0, wd, gid = s_g[lo];
+        rect(lo, t0, wd);
         const uint32_t li = kt - s_off[lo];
         uint32_t ry = li / wd, rx = li - ry * wd;
 #pragma unroll
@@ -1464,13 +1487,12 @@ __global__ __launch_bounds__(256) void k_emit_scatter1(
                 const uint32_t k = q - sp.z;
                 if (lo + 1 < cnt && s_off[lo + 1] <= k) {
                     while (lo + 1 < cnt && s_off[lo + 1] <= k) ++lo;
-                    wd = s_wd[lo];
-                    xy = s_xy[lo];
+                    rect(lo, t0, wd);
                     gid = s_g[lo];
                     ry = 0;
                     rx = 0;
                 }
-                const uint32_t tile = ((xy >> 16) + ry) * (uint32_t)tiles_x + (xy & 0xffffu) + rx;
+                const uint32_t tile = t0 + ry * (uint32_t)tiles_x + rx;
                 if (PACKED) {
                     ot[j] = (tile << id_bits) | gid;
                 } else {
@@ -1655,13 +1677,17 @@ static hipError_t bin_tiles_phase(int n, int cap, const uint32_t* m_total, int t
     }
     const bool more = tile_bits > bits1;
     uint32_t* k1 = more ? t.tkey : t.tkey2;
-    if (pk)
-        k_emit_scatter1<true><<<eb, 256, 0, s>>>(n, cap, m_total, total, d.rec, t.starts, tiles_x,
-                                                 num_tiles, pk, bits1, t.hist1, eb, k1, nullptr);
-    else
-        k_emit_scatter1<false><<<eb, 256, 0, s>>>(n, cap, m_total, total, d.rec, t.starts,
-                                                  tiles_x, num_tiles, 0, bits1, t.hist1, eb, k1,
-                                                  more ? t.tid_in : (uint32_t*)ids_sorted);
+    // rectangles staged as one word when the tile grid allows it
+    const bool compact = num_tiles < (1 << 20) && tiles_x < 4096;
+    uint32_t* const v1 = pk ? nullptr : (more ? t.tid_in : (uint32_t*)ids_sorted);
+#define GSTEX_EMIT(P, C)                                                                    \
+    k_emit_scatter1<P, C><<<eb, 256, 0, s>>>(n, cap, m_total, total, d.rec, t.starts, tiles_x, \
+                                             num_tiles, pk, bits1, t.hist1, eb, k1, v1)
+    if (pk && compact) GSTEX_EMIT(true, true);
+    else if (pk) GSTEX_EMIT(true, false);
+    else if (compact) GSTEX_EMIT(false, true);
+    else GSTEX_EMIT(false, false);
+#undef GSTEX_EMIT
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (more) {
         if (pk)

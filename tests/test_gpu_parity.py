@@ -120,6 +120,16 @@ def test_binning_unpacked_keys(oracle):
     assert _binning_case(oracle, s, 8) > 1_000_000
 
 
+@pytest.mark.parametrize("num_points", [5000, 70_000])
+def test_binning_wide_tile_rows(oracle, num_points):
+    # an 8200 x 24 image at 2-pixel tiles: 4100 tiles per row, past the
+    # 4095-tile rows whose rectangles the fused emission stages as one word,
+    # so the two-word staging runs (packed keys with 5000 Gaussians, (tile, id)
+    # pairs with 70000)
+    s = scene(1, num_points=num_points, seed=31, img_w=8200, img_h=24)
+    assert _binning_case(oracle, s, 2) > 0
+
+
 @pytest.mark.parametrize("bw,num_points", [(16, 3000), (8, 300_000)])
 def test_binning_two_phase_equals_one_call(bw, num_points):
     # gstex_bin_gaussians_depth + gstex_bin_gaussians_tiles (the exact-size
