@@ -401,7 +401,10 @@ class RenderStep:
 
     def step(self, target: torch.Tensor, timer: Optional[StageTimer] = None):
         """Forward, L2 loss gradient against ``target``, backward.  With three
-        channels the loss gradient is formed inside the raster backward."""
+        channels (float32 target of the image's shape) the loss gradient is
+        formed inside the raster backward, and ``v_img`` / ``v_tex`` are not
+        written; call ``forward``, ``l2_loss_grad`` and ``backward`` to keep
+        them."""
         self.forward(timer)
         fused = self._fused_loss_target(target)
         if fused is None:
