@@ -37,6 +37,7 @@ _SIGNATURES = {
     "gstex_project_points": (I, [I, P, P, F, F, F, F, I, P, P, P]),
     "gstex_compute_sh_forward": (I, [I, I, I, P, P, P, P]),
     "gstex_compute_sh_forward_bf16": (I, [I, I, I, P, P, P, P]),
+    "gstex_compute_sh_forward_means": (I, [I, I, I, P, P, P, I, P, P]),
     "gstex_compute_sh_backward": (I, [I, I, I, P, P, P, P]),
     "gstex_compute_sh_backward_views": (I, [I, I, I, I, P, P, P, P]),
     "gstex_map_gaussian_to_intersects": (I, [I, I, P, P, P, P, I, I, I, I, P, P, P]),

@@ -188,6 +188,24 @@ int gstex_compute_sh_forward_bf16(int num_points, int degree, int degrees_to_use
                      "compute_sh_forward_bf16");
 }
 
+int gstex_compute_sh_forward_means(int num_points, int degree, int degrees_to_use,
+                                   const float* means, const float* c2w, const void* coeffs,
+                                   int coeffs_format, float* colors, void* stream) {
+    REQUIRE(num_points >= 0, "num_points must be >= 0");
+    REQUIRE(degree >= 0 && degree <= 4, "SH degree must be in [0, 4], got %d", degree);
+    REQUIRE(degrees_to_use >= 0 && degrees_to_use <= degree,
+            "degrees_to_use must be in [0, degree], got %d", degrees_to_use);
+    REQUIRE(coeffs_format == GSTEX_TEXTURE_F32 || coeffs_format == GSTEX_TEXTURE_BF16,
+            "coeffs_format must be 0 (float32) or 1 (bfloat16), got %d", coeffs_format);
+    if (num_points == 0) return GSTEX_OK;
+    REQUIRE(means && c2w && coeffs && colors, "null pointer argument");
+    return check_hip(gstex::launch_sh_forward_means(num_points, degree, degrees_to_use, means,
+                                                    c2w, coeffs,
+                                                    coeffs_format == GSTEX_TEXTURE_BF16, colors,
+                                                    as_stream(stream)),
+                     "compute_sh_forward_means");
+}
+
 int gstex_compute_sh_backward(int num_points, int degree, int degrees_to_use,
                               const float* viewdirs, const float* v_colors, float* v_coeffs,
                               void* stream) {

@@ -41,6 +41,10 @@ hipError_t launch_sh_forward(int n, int degree, int degrees_to_use, const float*
 hipError_t launch_sh_forward_bf16(int n, int degree, int degrees_to_use, const float* viewdirs,
                                   const unsigned short* coeffs, float* colors,
                                   hipStream_t stream);
+// SH colours at the directions means - c2w[:3, 3] (float32 or bfloat16 coefficients).
+hipError_t launch_sh_forward_means(int n, int degree, int degrees_to_use, const float* means,
+                                   const float* c2w, const void* coeffs, bool bf16,
+                                   float* colors, hipStream_t stream);
 hipError_t launch_sh_backward(int n, int degree, int degrees_to_use, const float* viewdirs,
                               const float* v_colors, float* v_coeffs, hipStream_t stream);
 hipError_t launch_sh_backward_views(int n, int views, int degree, int degrees_to_use,

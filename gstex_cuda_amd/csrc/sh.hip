@@ -28,7 +28,11 @@ __device__ __forceinline__ float band_sum(const float (&b)[25], const CT* c, int
 // One lane per Gaussian; coefficients are (N, nb, 3) float32 or bfloat16,
 // evaluated up to degree D in float32.
 template <int D, typename CT>
-__global__ __launch_bounds__(256) void k_sh_forward(int n, int nb, const float* __restrict__ viewdirs,
+// With c2w (4 x 4 row-major camera-to-world), dirs holds the means and the
+// direction of Gaussian i is means[i] - c2w[:3, 3] (one float32 subtraction
+// per component).
+__global__ __launch_bounds__(256) void k_sh_forward(int n, int nb, const float* __restrict__ dirs,
+                                                    const float* __restrict__ c2w,
                                                     const CT* __restrict__ coeffs,
                                                     float* __restrict__ colors) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -39,7 +43,13 @@ __global__ __launch_bounds__(256) void k_sh_forward(int n, int nb, const float* 
     for (int ch = 0; ch < 3; ++ch) acc[ch] = kShC0 * coef(c, ch);
     if constexpr (D >= 1) {
         float b[25], x, y, z;
-        sh_basis<D>(viewdirs[3 * i], viewdirs[3 * i + 1], viewdirs[3 * i + 2], b, x, y, z);
+        float dx = dirs[3 * i], dy = dirs[3 * i + 1], dz = dirs[3 * i + 2];
+        if (c2w) {
+            dx -= c2w[3];
+            dy -= c2w[7];
+            dz -= c2w[11];
+        }
+        sh_basis<D>(dx, dy, dz, b, x, y, z);
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
             acc[ch] += kShC1 * (-y * coef(c, 3 + ch) + z * coef(c, 6 + ch) - x * coef(c, 9 + ch));
@@ -133,30 +143,43 @@ __global__ __launch_bounds__(256) void k_sh_backward_views(int n, int nb, int vi
 }
 
 template <typename CT>
-static hipError_t sh_forward_t(int n, int degree, int degrees_to_use, const float* viewdirs,
-                               const CT* coeffs, float* colors, hipStream_t stream) {
+static hipError_t sh_forward_t(int n, int degree, int degrees_to_use, const float* dirs,
+                               const float* c2w, const CT* coeffs, float* colors,
+                               hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     const int nb = num_sh_bases(degree), g = (n + 255) / 256;
     switch (degrees_to_use) {
-        case 0: k_sh_forward<0, CT><<<g, 256, 0, stream>>>(n, nb, viewdirs, coeffs, colors); break;
-        case 1: k_sh_forward<1, CT><<<g, 256, 0, stream>>>(n, nb, viewdirs, coeffs, colors); break;
-        case 2: k_sh_forward<2, CT><<<g, 256, 0, stream>>>(n, nb, viewdirs, coeffs, colors); break;
-        case 3: k_sh_forward<3, CT><<<g, 256, 0, stream>>>(n, nb, viewdirs, coeffs, colors); break;
-        default: k_sh_forward<4, CT><<<g, 256, 0, stream>>>(n, nb, viewdirs, coeffs, colors); break;
+        case 0: k_sh_forward<0, CT><<<g, 256, 0, stream>>>(n, nb, dirs, c2w, coeffs, colors); break;
+        case 1: k_sh_forward<1, CT><<<g, 256, 0, stream>>>(n, nb, dirs, c2w, coeffs, colors); break;
+        case 2: k_sh_forward<2, CT><<<g, 256, 0, stream>>>(n, nb, dirs, c2w, coeffs, colors); break;
+        case 3: k_sh_forward<3, CT><<<g, 256, 0, stream>>>(n, nb, dirs, c2w, coeffs, colors); break;
+        default: k_sh_forward<4, CT><<<g, 256, 0, stream>>>(n, nb, dirs, c2w, coeffs, colors); break;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_sh_forward(int n, int degree, int degrees_to_use, const float* viewdirs,
                              const float* coeffs, float* colors, hipStream_t stream) {
-    return sh_forward_t<float>(n, degree, degrees_to_use, viewdirs, coeffs, colors, stream);
+    return sh_forward_t<float>(n, degree, degrees_to_use, viewdirs, nullptr, coeffs, colors,
+                               stream);
 }
 
 hipError_t launch_sh_forward_bf16(int n, int degree, int degrees_to_use, const float* viewdirs,
                                   const unsigned short* coeffs, float* colors,
                                   hipStream_t stream) {
-    return sh_forward_t<unsigned short>(n, degree, degrees_to_use, viewdirs, coeffs, colors,
-                                        stream);
+    return sh_forward_t<unsigned short>(n, degree, degrees_to_use, viewdirs, nullptr, coeffs,
+                                        colors, stream);
+}
+
+hipError_t launch_sh_forward_means(int n, int degree, int degrees_to_use, const float* means,
+                                   const float* c2w, const void* coeffs, bool bf16,
+                                   float* colors, hipStream_t stream) {
+    if (bf16)
+        return sh_forward_t<unsigned short>(n, degree, degrees_to_use, means, c2w,
+                                            static_cast<const unsigned short*>(coeffs), colors,
+                                            stream);
+    return sh_forward_t<float>(n, degree, degrees_to_use, means, c2w,
+                               static_cast<const float*>(coeffs), colors, stream);
 }
 
 hipError_t launch_sh_backward(int n, int degree, int degrees_to_use, const float* viewdirs,

@@ -215,14 +215,26 @@ class RenderStep:
             call("gstex_fill_f32", _p(self.grads["v_texture"]), self.grads["v_texture"].numel(),
                  0.0, s)
 
+    def _sh_in_finalize(self) -> bool:
+        """True when the SH adjoint runs inside the gradient finalize, which
+        forms the view directions itself (no direction buffer needed)."""
+        return (0 <= self.sh_degree <= 3 and self.sh_backward
+                and self.grads["v_sh_coeffs"].data_ptr() % 16 == 0)
+
     def _colors_and_records(self, s):
         p, cam, n = self.p, self.cam, self.n
         if self.sh_degree >= 0:
-            torch.sub(p["means"], cam.c2w[:3, 3], out=self._viewdirs)
-            fn = ("gstex_compute_sh_forward_bf16" if p["sh_coeffs"].dtype == torch.bfloat16
-                  else "gstex_compute_sh_forward")
-            call(fn, n, self.sh_degree, self.sh_degree, _p(self._viewdirs), _p(p["sh_coeffs"]),
-                 _p(self.colors), s)
+            bf16 = p["sh_coeffs"].dtype == torch.bfloat16
+            if self._sh_in_finalize():
+                # directions formed in the SH kernel: no pass of their own
+                call("gstex_compute_sh_forward_means", n, self.sh_degree, self.sh_degree,
+                     _p(p["means"]), _p(cam.c2w), _p(p["sh_coeffs"]), int(bf16),
+                     _p(self.colors), s)
+            else:
+                torch.sub(p["means"], cam.c2w[:3, 3], out=self._viewdirs)
+                fn = "gstex_compute_sh_forward_bf16" if bf16 else "gstex_compute_sh_forward"
+                call(fn, n, self.sh_degree, self.sh_degree, _p(self._viewdirs),
+                     _p(p["sh_coeffs"]), _p(self.colors), s)
         call("gstex_build_splat_records", n, _p(p["means"]), _p(p["scales"]), 1.0,
              _p(p["quats"]), _p(self.colors), _p(p["opacities"]), _p(p["uv0"]), _p(p["umap"]),
              _p(p["vmap"]), _p(p["texture_dims"]), _p(cam.viewmat), _p(cam.c2w), cam.fx, cam.fy,
@@ -389,7 +401,7 @@ class RenderStep:
         grads = (_p(g["v_colors"]), _p(g["v_opacity"]), _p(g["v_means"]), _p(g["v_scales"]),
                  _p(g["v_quats"]), _p(g["v_uv0"]), _p(g["v_umap"]), _p(g["v_vmap"]))
         sh = self.sh_degree >= 0 and self.sh_backward
-        if sh and self.sh_degree <= 3 and g["v_sh_coeffs"].data_ptr() % 16 == 0:
+        if sh and self._sh_in_finalize():
             # the SH adjoint in the finalize pass (view directions from the means)
             call("gstex_finalize_gradients_sh", *fin, _p(p["means"]), self.sh_degree,
                  self.sh_degree, *grads, _p(g["v_sh_coeffs"]), s)

@@ -97,6 +97,16 @@ int gstex_compute_sh_forward_bf16(int num_points, int degree, int degrees_to_use
                                   const float *viewdirs, const uint16_t *coeffs, float *colors,
                                   void *stream);
 
+/* compute_sh_forward at the view directions means - c2w[:3, 3] (c2w 4 x 4
+ * row-major; the same float32 subtraction as forming the directions first),
+ * with float32 (coeffs_format 0) or bfloat16 (1) coefficients, so the
+ * directions need no pass of their own.  No reference counterpart (its
+ * trainer forms the directions with torch); for training steps with SH
+ * colours. */
+int gstex_compute_sh_forward_means(int num_points, int degree, int degrees_to_use,
+                                   const float *means, const float *c2w, const void *coeffs,
+                                   int coeffs_format, float *colors, void *stream);
+
 /* Replaces compute_sh_backward_tensor (cuda/csrc/bindings.cu:45-75).
  * v_coeffs (N, bases(degree), 3) is fully written. */
 int gstex_compute_sh_backward(int num_points, int degree, int degrees_to_use,

@@ -337,6 +337,34 @@ def test_sh_matches_oracle(oracle, degree, used):
     np.testing.assert_allclose(np_(vco), oracle.sh_backward(degree, used, dirs, vc, "f64"), atol=1e-5)
 
 
+@pytest.mark.parametrize("degree,used,bf16", [(3, 3, False), (3, 3, True), (4, 2, False),
+                                               (1, 1, True), (0, 0, False)])
+def test_sh_forward_means_equals_directions_first(degree, used, bf16):
+    # gstex_compute_sh_forward_means: the colours of compute_sh_forward at
+    # means - c2w[:3, 3], bit for bit, float32 and bfloat16 coefficients
+    from gstex_cuda_amd import _lib
+
+    rng = np.random.default_rng(degree * 11 + used)
+    n, nb = 4099, (degree + 1) ** 2
+    means = to_dev(rng.standard_normal((n, 3)).astype(np.float32) * 3.0)
+    c2w = to_dev(np.array([[0.8, 0.0, 0.6, 0.3], [0.0, 1.0, 0.0, -1.2], [-0.6, 0.0, 0.8, 4.5],
+                           [0.0, 0.0, 0.0, 1.0]], np.float32))
+    co = to_dev((0.3 * rng.standard_normal((n, nb, 3))).astype(np.float32))
+    if bf16:
+        co = co.to(torch.bfloat16)
+    lib = _lib.lib()
+    st = torch.cuda.current_stream().cuda_stream
+    got = torch.full((n, 3), 7.0, device=DEV)
+    assert lib.gstex_compute_sh_forward_means(n, degree, used, means.data_ptr(), c2w.data_ptr(),
+                                              co.data_ptr(), int(bf16), got.data_ptr(), st) == 0
+    dirs = (means - c2w[:3, 3]).contiguous()
+    want = torch.full((n, 3), -7.0, device=DEV)
+    fn = lib.gstex_compute_sh_forward_bf16 if bf16 else lib.gstex_compute_sh_forward
+    assert fn(n, degree, used, dirs.data_ptr(), co.data_ptr(), want.data_ptr(), st) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(got, want)
+
+
 @pytest.mark.parametrize("degree,used,views", [(3, 3, 5), (4, 2, 3), (0, 0, 2), (2, 2, 1)])
 def test_sh_backward_views_sums_single_views(oracle, degree, used, views):
     # gstex_compute_sh_backward_views equals the per-view adjoints added in
