@@ -27,17 +27,58 @@ __device__ __forceinline__ float band_sum(const float (&b)[25], const CT* c, int
 
 // One lane per Gaussian; coefficients are (N, nb, 3) float32 or bfloat16,
 // evaluated up to degree D in float32.
-template <int D, typename CT>
 // With c2w (4 x 4 row-major camera-to-world), dirs holds the means and the
 // direction of Gaussian i is means[i] - c2w[:3, 3] (one float32 subtraction
-// per component).
+// per component).  With staging (float32 coefficients, 16-byte aligned, and
+// dynamic LDS of 256 x (3 nb + 1) floats), the block's coefficient rows, one
+// contiguous span, come in as 16-byte loads of consecutive addresses and
+// each thread reads its row from LDS (rows padded to an odd word count).
+template <int D, typename CT>
 __global__ __launch_bounds__(256) void k_sh_forward(int n, int nb, const float* __restrict__ dirs,
                                                     const float* __restrict__ c2w,
                                                     const CT* __restrict__ coeffs,
-                                                    float* __restrict__ colors) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+                                                    float* __restrict__ colors, bool staged) {
+    extern __shared__ float stage[];
+    const int i0 = blockIdx.x * blockDim.x, i = i0 + threadIdx.x;
     const CT* c = coeffs + (size_t)i * nb * 3;
+    if constexpr (sizeof(CT) == 4) {
+        if (staged) {
+            const int R = 3 * nb, cnt = min(n - i0, (int)blockDim.x), total = cnt * R;
+            const float* src = reinterpret_cast<const float*>(coeffs) + (size_t)i0 * R;
+            const int step = 4 * (int)blockDim.x, step_r = step / R, step_c = step - step_r * R;
+            int q = 4 * (int)threadIdx.x, r = q / R, cc = q - r * R;
+            for (; q < total; q += step) {
+                float v[4];
+                if (q + 4 <= total) {
+                    const float4 t = *reinterpret_cast<const float4*>(src + q);
+                    v[0] = t.x;
+                    v[1] = t.y;
+                    v[2] = t.z;
+                    v[3] = t.w;
+                } else {
+                    for (int j = 0; j < 4; ++j) v[j] = q + j < total ? src[q + j] : 0.f;
+                }
+                int rr = r, c2 = cc;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (q + j < total) stage[rr * (R + 1) + c2] = v[j];
+                    if (++c2 == R) {
+                        c2 = 0;
+                        ++rr;
+                    }
+                }
+                r += step_r;
+                cc += step_c;
+                if (cc >= R) {
+                    cc -= R;
+                    ++r;
+                }
+            }
+            __syncthreads();
+            c = reinterpret_cast<const CT*>(stage + threadIdx.x * (R + 1));
+        }
+    }
+    if (i >= n) return;
     float acc[3];
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) acc[ch] = kShC0 * coef(c, ch);
@@ -148,13 +189,20 @@ static hipError_t sh_forward_t(int n, int degree, int degrees_to_use, const floa
                                hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     const int nb = num_sh_bases(degree), g = (n + 255) / 256;
+    const size_t lds = sh_rows_stage_bytes(nb);
+    const bool staged = sizeof(CT) == 4 && (reinterpret_cast<uintptr_t>(coeffs) & 15) == 0 &&
+                        lds <= 65536;
+    const size_t sh = staged ? lds : 0;
+#define GSTEX_SHF(D) \
+    k_sh_forward<D, CT><<<g, 256, sh, stream>>>(n, nb, dirs, c2w, coeffs, colors, staged)
     switch (degrees_to_use) {
-        case 0: k_sh_forward<0, CT><<<g, 256, 0, stream>>>(n, nb, dirs, c2w, coeffs, colors); break;
-        case 1: k_sh_forward<1, CT><<<g, 256, 0, stream>>>(n, nb, dirs, c2w, coeffs, colors); break;
-        case 2: k_sh_forward<2, CT><<<g, 256, 0, stream>>>(n, nb, dirs, c2w, coeffs, colors); break;
-        case 3: k_sh_forward<3, CT><<<g, 256, 0, stream>>>(n, nb, dirs, c2w, coeffs, colors); break;
-        default: k_sh_forward<4, CT><<<g, 256, 0, stream>>>(n, nb, dirs, c2w, coeffs, colors); break;
+        case 0: GSTEX_SHF(0); break;
+        case 1: GSTEX_SHF(1); break;
+        case 2: GSTEX_SHF(2); break;
+        case 3: GSTEX_SHF(3); break;
+        default: GSTEX_SHF(4); break;
     }
+#undef GSTEX_SHF
     return hipGetLastError();
 }
 

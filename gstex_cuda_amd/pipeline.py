@@ -171,6 +171,10 @@ class RenderStep:
                 dtype=torch.uint8, device=dev)
             self.m_host = torch.zeros((1,), dtype=torch.int32, pin_memory=True)
             self._ev_m = torch.cuda.Event()
+            # the total's copy to the host runs on a stream of its own
+            # (default priority), so the main stream goes on with binning
+            self._copy_stream = torch.cuda.Stream(dev)
+            self._ev_projected = torch.cuda.Event()
 
     def _stream(self):
         return torch.cuda.current_stream(self.dev).cuda_stream
@@ -273,10 +277,14 @@ class RenderStep:
                  _p(self.ids), _p(self.tile_bins), _p(self.m_dev), _p(self.bin_ws),
                  self.bin_ws.numel(), s)
         else:
-            # the depth phase needs no total: it is queued behind the total's
-            # copy to the host, and the host's wait for that copy overlaps it
-            self.m_host.copy_(self.m_dev, non_blocking=True)
-            self._ev_m.record(main)
+            # the depth phase needs no total: it is queued right after the
+            # projection, and the host's wait for the total's copy overlaps it
+            self._ev_projected.record(main)
+            cs = self._copy_stream
+            cs.wait_event(self._ev_projected)
+            with torch.cuda.stream(cs):
+                self.m_host.copy_(self.m_dev, non_blocking=True)
+                self._ev_m.record(cs)
             call("gstex_bin_gaussians_depth", n, _p(self.centers), _p(self.extents),
                  _p(self.depths), _p(self.tiles_hit), self.tiles[0], self.tiles[1], self.bw,
                  _p(self.tile_bins), _p(self.depth_ws), self.depth_ws.numel(), s)
