@@ -480,21 +480,21 @@ struct ScatterLds {
     uint32_t wrun[kSortThreads / kWave][256]; // per wave: running count, then start offset
 };
 
-// The rank / place / write part of the scatter, for a tile already held in
+// The rank and place part of the scatter, for a tile already held in
 // registers in (wave, round, lane) order: item r of lane l of wave w is item
 // w * ITEMS * 64 + r * 64 + l of the tile, and items at or past count are
 // absent.  skeys / svals are LDS arrays of the tile's size (svals unused
 // without values); gb is thread t's digit base (the global start of digit
 // t's run for this tile, 0 for t past the digits), loaded by the caller
-// ahead of time, and writes stay below out_n.
+// ahead of time.  On return skeys holds the tile sorted stably by digit,
+// sl.lstart[d] is where digit d's run starts in it and sl.gbase[d] its
+// global start (all threads synchronised).
 template <typename K, int ITEMS>
-__device__ __forceinline__ void tile_rank_scatter(const K (&key)[ITEMS],
-                                                  const uint32_t (&val)[ITEMS], int count,
-                                                  bool has_vals, K flip, int shift, int bits,
-                                                  uint32_t gb, K* __restrict__ keys_out,
-                                                  uint32_t* __restrict__ vals_out, uint32_t out_n,
-                                                  ScatterLds& sl, K* skeys,
-                                                  uint32_t* svals) {
+__device__ __forceinline__ void tile_rank_place(const K (&key)[ITEMS],
+                                                const uint32_t (&val)[ITEMS], int count,
+                                                bool has_vals, K flip, int shift, int bits,
+                                                uint32_t gb, ScatterLds& sl, K* skeys,
+                                                uint32_t* svals) {
     constexpr int kWaves = kSortThreads / kWave;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const uint32_t mask = (1u << bits) - 1u;
@@ -551,7 +551,23 @@ __device__ __forceinline__ void tile_rank_scatter(const K (&key)[ITEMS],
         }
     }
     __syncthreads();
+}
 
+// The rank / place / write part of the scatter (tile_rank_place, then each
+// digit's run written to consecutive global addresses); writes stay below
+// out_n.
+template <typename K, int ITEMS>
+__device__ __forceinline__ void tile_rank_scatter(const K (&key)[ITEMS],
+                                                  const uint32_t (&val)[ITEMS], int count,
+                                                  bool has_vals, K flip, int shift, int bits,
+                                                  uint32_t gb, K* __restrict__ keys_out,
+                                                  uint32_t* __restrict__ vals_out, uint32_t out_n,
+                                                  ScatterLds& sl, K* skeys,
+                                                  uint32_t* svals) {
+    tile_rank_place<K, ITEMS>(key, val, count, has_vals, flip, shift, bits, gb, sl, skeys,
+                              svals);
+    const int tid = threadIdx.x;
+    const uint32_t mask = (1u << bits) - 1u;
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
         const int p = r * kSortThreads + tid;
@@ -594,6 +610,87 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint32_t gb = tid < (1 << bits) ? hist[(size_t)tid * nblocks + blockIdx.x] : 0u;
     tile_rank_scatter<K, ITEMS>(key, val, count, has_vals, flip, shift, bits, gb, keys_out,
                                 vals_out, (uint32_t)n, sl, skeys, svals);
+}
+
+// The last tile-sort pass of packed binning keys (tile << id_bits | id), by
+// digit [shift, shift + bits): the stable scatter of k_radix_scatter (keys
+// only), writing the ids (key & id mask) at the sorted positions and the bin
+// edges (bins[2 t] first position of tile t, bins[2 t + 1] one past its
+// last; tiles >= num_tiles belong to no bin) of every tile change inside one
+// digit run of the tile, from the LDS-sorted tile.  The keys are written only
+// at the two ends of each run, for k_run_edges, which settles the edges where
+// runs meet.
+template <int ITEMS>
+__global__ __launch_bounds__(kSortThreads) void k_last_pass_edges(
+    int n, const uint32_t* __restrict__ keys_in, uint32_t* __restrict__ keys_out,
+    int* __restrict__ ids_out, int id_bits, int shift, int bits,
+    const uint32_t* __restrict__ hist, int nblocks, int num_tiles, int* __restrict__ bins) {
+    constexpr int kTile = kSortThreads * ITEMS;
+    __shared__ ScatterLds sl;
+    __shared__ uint32_t skeys[kTile];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const size_t base = (size_t)blockIdx.x * kTile;
+    const int count = (int)min((size_t)kTile, (size_t)n - base);
+    const int wbase = wave * ITEMS * kWave;
+    uint32_t key[ITEMS], val[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        const int li = wbase + r * kWave + lane;
+        key[r] = li < count ? keys_in[base + li] : 0u;
+        val[r] = 0u;
+    }
+    const uint32_t gb = tid < (1 << bits) ? hist[(size_t)tid * nblocks + blockIdx.x] : 0u;
+    tile_rank_place<uint32_t, ITEMS>(key, val, count, false, 0u, shift, bits, gb, sl, skeys,
+                                     nullptr);
+    const uint32_t mask = (1u << bits) - 1u, idmask = (1u << id_bits) - 1u;
+    const uint32_t nt = (uint32_t)num_tiles;
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        const int p = r * kSortThreads + tid;
+        if (p < count) {
+            const uint32_t k = skeys[p];
+            const uint32_t d = (k >> shift) & mask;
+            const uint32_t gp = sl.gbase[d] + (uint32_t)p - sl.lstart[d];
+            if (gp < (uint32_t)n) {  // always, when the digit counts are those of the keys
+                ids_out[gp] = (int)(k & idmask);
+                const uint32_t t = k >> id_bits;
+                // the run of digit d ends where the next digit's starts
+                const bool first = (uint32_t)p == sl.lstart[d];
+                const bool last = (uint32_t)(p + 1) == (d < mask ? sl.lstart[d + 1] : (uint32_t)count);
+                if (first || last) keys_out[gp] = k;
+                if (!first) {
+                    const uint32_t tp = skeys[p - 1] >> id_bits;
+                    if (tp != t) {
+                        if (tp < nt) bins[2 * tp + 1] = (int)gp;
+                        if (t < nt) bins[2 * t] = (int)gp;
+                    }
+                } else if (gp == 0 && t < nt) {
+                    bins[2 * t] = 0;
+                }
+                if (gp + 1 == (uint32_t)n && t < nt) bins[2 * t + 1] = n;
+            }
+        }
+    }
+}
+
+// The bin edges where two runs of k_last_pass_edges meet: run r (digit-major,
+// r = d * nblocks + b) starts at hist[r] (the scanned digit counts) and ends
+// where run r + 1 starts; its first key and the previous run's last key were
+// written.
+__global__ __launch_bounds__(256) void k_run_edges(int n, int runs,
+                                                   const uint32_t* __restrict__ hist,
+                                                   const uint32_t* __restrict__ keys, int id_bits,
+                                                   int num_tiles, int* __restrict__ bins) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= runs) return;
+    const uint32_t pos = hist[r], end = r + 1 < runs ? hist[r + 1] : (uint32_t)n;
+    if (end <= pos || pos == 0 || pos >= (uint32_t)n) return;
+    const uint32_t t = keys[pos] >> id_bits, tp = keys[pos - 1] >> id_bits;
+    const uint32_t nt = (uint32_t)num_tiles;
+    if (t != tp) {
+        if (tp < nt) bins[2 * tp + 1] = (int)pos;
+        if (t < nt) bins[2 * t] = (int)pos;
+    }
 }
 
 // ------------------------------------------------------------------ one-sweep sort
@@ -1689,6 +1786,42 @@ static hipError_t bin_tiles_phase(int n, int cap, const uint32_t* m_total, int t
     else GSTEX_EMIT(false, false);
 #undef GSTEX_EMIT
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    const int bits2 = tile_bits - bits1;
+    if (more && pk && bits2 <= 8) {
+        // one pass left: it writes the ids and the bin edges itself
+        // (k_last_pass_edges, k_run_edges), with its digit counts and their
+        // scan in the tile sort's workspace
+        const int nb2 = sort_blocks(cap), shift2 = pk + bits1;
+        const bool small = sort_items(cap) == 8;
+        uint32_t* const hist2 = (uint32_t*)t.rws;
+        uint64_t* const st2 = (uint64_t*)((char*)t.rws + up256(4 * 256 * (size_t)nb2));
+        const int hc2 = (1 << bits2) * nb2;
+        const bool lb2 = hc2 > kScanSingleMax;
+        const int lbt2 = wr_scan_tiles(hc2);
+        if (small)
+            k_radix_hist<uint32_t, 8><<<nb2, kSortThreads, 0, s>>>(
+                cap, t.tkey, 0u, shift2, bits2, hist2, nb2, lb2 ? st2 : nullptr, lb2 ? lbt2 + 1 : 0);
+        else
+            k_radix_hist<uint32_t, 16><<<nb2, kSortThreads, 0, s>>>(
+                cap, t.tkey, 0u, shift2, bits2, hist2, nb2, lb2 ? st2 : nullptr, lb2 ? lbt2 + 1 : 0);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if (lb2) {
+            k_scan_lookback<<<lbt2, 256, 0, s>>>(hist2, hc2, st2, lbt2);
+        } else if ((e = scan_exclusive(hist2, hc2, nullptr, nullptr, s)) != hipSuccess) {
+            return e;
+        }
+        if (small)
+            k_last_pass_edges<8><<<nb2, kSortThreads, 0, s>>>(cap, t.tkey, t.tkey2, ids_sorted, pk,
+                                                              shift2, bits2, hist2, nb2, num_tiles,
+                                                              tile_bins);
+        else
+            k_last_pass_edges<16><<<nb2, kSortThreads, 0, s>>>(cap, t.tkey, t.tkey2, ids_sorted,
+                                                               pk, shift2, bits2, hist2, nb2,
+                                                               num_tiles, tile_bins);
+        k_run_edges<<<(hc2 + 255) / 256, 256, 0, s>>>(cap, hc2, hist2, t.tkey2, pk, num_tiles,
+                                                      tile_bins);
+        return hipGetLastError();
+    }
     if (more) {
         if (pk)
             e = radix_sort_pairs<uint32_t>(cap, t.tkey, nullptr, t.tkey2, nullptr,
