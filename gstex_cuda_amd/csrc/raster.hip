@@ -72,7 +72,7 @@ __device__ __forceinline__ void raster_forward_wave(const RasterArgs& a, const F
     // ones still live (recomputed when their count drops), and an entry that
     // meets none of them is passed over like any culled entry.
     WaveRect rect = wave_rect(inside, j, i);
-    int live = __popcll(__ballot(inside));
+    int live = __popcll(lanes_where(inside));
 
     // A pixel whose T (1 - 1/255) <= 1e-4 can composite nothing more: every
     // entry with alpha >= 1/255 ends it (rounding is monotonic), and ending
@@ -92,14 +92,14 @@ __device__ __forceinline__ void raster_forward_wave(const RasterArgs& a, const F
     int dup_g = -1;
     bool dup_noop = false;
     for (int base = mentry0; base < r1; base += 64) {
-        if (!__any(!done)) break;
+        if (!any_lane(!done)) break;
         const Window w = window_candidates(a, base, r0, r1 - 1, rect, lane, cull);
         int ev_e = -1, ev_g = -1;
         bool ev_noop = false;
         uint64_t cand = w.cand;
         uint64_t mbits = 0;
         while (cand != 0) {
-            if (!__any(!done)) break;
+            if (!any_lane(!done)) break;
             const int bit = __ffsll((unsigned long long)cand) - 1;
             cand &= cand - 1;
             const int e = base + bit;
@@ -183,7 +183,7 @@ __device__ __forceinline__ void raster_forward_wave(const RasterArgs& a, const F
                     if (T * kKeep <= 1e-4f) done = true;
                 }
             }
-            const bool any_comp = __any(comp);
+            const bool any_comp = any_lane(comp);
             if (any_comp) mbits |= 1ull << bit;
             ev_e = e;
             ev_g = g;
@@ -191,7 +191,7 @@ __device__ __forceinline__ void raster_forward_wave(const RasterArgs& a, const F
         }
         if (mplane && lane == 0) mplane[mword0 + ((base - mentry0) >> 6)] = mbits;
         {
-            const int now = __popcll(__ballot(!done));
+            const int now = __popcll(lanes_where(!done));
             if (now < live) {
                 live = now;
                 rect = wave_rect(!done, j, i);
@@ -216,7 +216,7 @@ __device__ __forceinline__ void raster_forward_wave(const RasterArgs& a, const F
         }
         // pass over the windows made only of copies of that entry, eight
         // windows per read
-        if (dup_noop && dup_run && __any(!done)) {
+        if (dup_noop && dup_run && any_lane(!done)) {
             int nb = base + 64;
             while (nb < r1) {
                 const int cnt = min(r1 - nb, 512);
@@ -225,7 +225,7 @@ __device__ __forceinline__ void raster_forward_wave(const RasterArgs& a, const F
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
                     const bool s_k = 64 * k + lane >= cnt || ids[64 * k] == dup_g;
-                    if (__all(s_k)) same |= 1u << k;
+                    if (!any_lane(!s_k)) same |= 1u << k;
                 }
                 const int lead = min(__builtin_ctz(~same), (cnt + 63) >> 6);
                 if (mplane && lane < lead) mplane[mword0 + ((nb - mentry0) >> 6) + lane] = 0ull;

@@ -121,6 +121,7 @@ __device__ __forceinline__ void raster_backward_wave(const RasterArgs& a, const 
     wave_last = __builtin_amdgcn_readfirstlane(wave_last);
     const int e_hi = min(r1 - 1, wave_last);
     const float tn = 0.01f, tf = 1000.0f;
+    const int last_e = inside ? bfin : -1;  // entries up to it may contribute
 
     // One list entry (wave-uniform index e, Gaussian g with record s).
     auto entry = [&](const int e, const int g, const Splat& s) {
@@ -130,8 +131,8 @@ __device__ __forceinline__ void raster_backward_wave(const RasterArgs& a, const 
         // vt, vu, vv), so every gradient value below is zero for it without a
         // branch or per-value selects.
         const Pair p = pair_eval(s, pr, use_blur, PRECISE && a.precise_edge);
-        const bool valid = inside && e <= bfin && !p.skip;
-        const uint64_t vmask = __ballot(valid);
+        const bool valid = e <= last_e && !p.skip;
+        const uint64_t vmask = lanes_where(e <= last_e) & ~skip_lanes(p);
         if (vmask == 0) return;
 
         // Transmittance before this Gaussian (1 - alpha >= 0.01) and the texel taps.
@@ -160,15 +161,15 @@ __device__ __forceinline__ void raster_backward_wave(const RasterArgs& a, const 
         const int row2 = __builtin_amdgcn_readlane(row[2], first);
         const int row3 = __builtin_amdgcn_readlane(row[3], first);
         const bool quad32 = ch.ok && !tex_single && C >= 1 && C <= 3 &&
-                            !__any(valid && row[0] != row0);
+                            !(lanes_where(row[0] != row0) & vmask);
         // window origin (ou, ov): the anchor lane's tap row / column, one
         // lower when some valid lane lies below it
         const int au = __builtin_amdgcn_readlane(iu, first);
         const int av = __builtin_amdgcn_readlane(iv, first);
-        const int ou = au - (__any(valid && iu < au) ? 1 : 0);
-        const int ov = av - (__any(valid && iv < av) ? 1 : 0);
+        const int ou = au - ((lanes_where(iu < au) & vmask) ? 1 : 0);
+        const int ov = av - ((lanes_where(iv < av) & vmask) ? 1 : 0);
         const bool win3 = CM <= 4 && ch.ok && !tex_single && !quad32 && bilinear && C >= 1 &&
-                          !__any(valid && (iu < ou || iu > ou + 1 || iv < ov || iv > ov + 1));
+                          !(lanes_where(((uint32_t)(iu - ou) | (uint32_t)(iv - ov)) > 1u) & vmask);
         const bool tex_lds = ch.ok && !tex_single && !quad32 && !win3 && C > 0 &&
                              hw * C <= kTexAccFloats;
         float* const vtex_chart = b.v_texture + (size_t)(ch.ok ? s.start : 0) * C;
@@ -342,12 +343,12 @@ __device__ __forceinline__ void raster_backward_wave(const RasterArgs& a, const 
                 // weights are zero on every lane and the window is 2 x 3 (3 x 2);
                 // not for 4 channels, whose kernel has no registers to spare
                 if constexpr (CM <= 3) {
-                    if (!__any(valid && rj == 2)) {
+                    if (!(lanes_where(rj == 2) & vmask)) {
                         window_texel_add<2, 3, CM>(A, Bw, vval, lane, vtex_chart, ou, ov, s.h,
                                                    s.w);
                         return;
                     }
-                    if (!__any(valid && cj == 2)) {
+                    if (!(lanes_where(cj == 2) & vmask)) {
                         window_texel_add<3, 2, CM>(A, Bw, vval, lane, vtex_chart, ou, ov, s.h,
                                                    s.w);
                         return;
@@ -418,7 +419,7 @@ __device__ __forceinline__ void raster_backward_wave(const RasterArgs& a, const 
             const int lo = max(r0 - wb, 0), hi = min(e_hi - wb, 63);
             bool any = false;
             if (wi >= 0 && hi >= lo) any = (mplane[wi] & ((~0ull >> (63 - hi)) & (~0ull << lo))) != 0;
-            uint64_t nz = __ballot(any);
+            uint64_t nz = lanes_where(any);
             while (nz != 0) {
                 const int k = __ffsll((unsigned long long)nz) - 1;
                 nz &= nz - 1;

@@ -206,6 +206,15 @@ __device__ __forceinline__ float div_fast(float a, float b) {
     return fmaf(fmaf(-q, b, a), r, q);
 }
 
+// Lane mask of a per-lane condition over the active lanes.  HIP's __ballot,
+// __any and __all take an int, so the compare's lane mask is widened into a
+// register and compared again (two VALU instructions per call); the builtin
+// on a bool takes the mask as it is.
+__device__ __forceinline__ uint64_t lanes_where(bool c) {
+    return __builtin_amdgcn_ballot_w64(c);
+}
+__device__ __forceinline__ bool any_lane(bool c) { return lanes_where(c) != 0; }
+
 struct Pair {
     V3 delta;
     float den, t, l1, l2, G, Gb, alpha;
@@ -260,6 +269,13 @@ __device__ __forceinline__ Pair pair_eval(const Splat& s, const PixelRay& pr, bo
     p.alpha = fminf(0.99f, s.opac * (p.blur_sel ? p.Gb : p.G));
     p.skip = p.t < 0.01f || p.t > 1000.0f || p.alpha < 1.f / 255.f;
     return p;
+}
+
+// lanes_where(p.skip), as a ballot per compare (each folds into its compare's
+// lane mask; the ballot of their union would be widened into a register)
+__device__ __forceinline__ uint64_t skip_lanes(const Pair& p) {
+    return lanes_where(p.t < 0.01f) | lanes_where(p.t > 1000.0f) |
+           lanes_where(p.alpha < 1.f / 255.f);
 }
 
 // Texels of one chart (wave-uniform), read through a raw buffer resource over
@@ -657,8 +673,8 @@ __device__ __forceinline__ Window window_candidates(const RasterArgs& a, int bas
             }
         }
     }
-    w.in_range = __ballot(in_range);
-    w.cand = __ballot(cand);
+    w.in_range = lanes_where(in_range);
+    w.cand = lanes_where(cand);
     return w;
 }
 
