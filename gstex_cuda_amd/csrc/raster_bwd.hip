@@ -187,6 +187,13 @@ __device__ __forceinline__ void raster_backward_wave(const RasterArgs& a, const 
             float vvis = dot3(s.rgb, vo) + dot3(s.a3, vn);
             float vu = 0.f, vv = 0.f;
             wsum = (wt[0] + wt[1]) + (wt[2] + wt[3]);
+            // D[k] = sum over channels of vtex[c] * (tap k's texel, channel c).
+            // The sampled value's share of vvis is sum_k wt[k] D[k], and
+            // d val / d(u, v) through the bilinear weights
+            // (texture_helpers.cuh:262-299) takes tap differences, which are
+            // differences of two D: coincident taps (equal texels) give equal
+            // D and exactly zero.
+            float D[4] = {0.f, 0.f, 0.f, 0.f};
             // texels in chunks of up to 4 channels, all taps loaded before use
             constexpr int CK = CM < 4 ? CM : 4;
 #pragma unroll
@@ -204,8 +211,8 @@ __device__ __forceinline__ void raster_backward_wave(const RasterArgs& a, const 
                 for (int cc = 0; cc < CK; ++cc) {
                     const int c = cb + cc;
                     if (c >= C) break;
-                    const float c0 = tv[0][cc], c1 = tv[1][cc], c2 = tv[2][cc], c3 = tv[3][cc];
-                    const float val = wt[0] * c0 + wt[1] * c1 + wt[2] * c2 + wt[3] * c3;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) D[k] = fmaf(tv[k][cc], vtex[c], D[k]);
                     vval[c] = vis * vtex[c];
                     if constexpr (CM > 4) {
                         if (tex_lds) {
@@ -223,15 +230,12 @@ __device__ __forceinline__ void raster_backward_wave(const RasterArgs& a, const 
                             if (add != 0.f) unsafeAtomicAdd(vtex_chart + (row[k] >> kES) + c, add);
                         }
                     }
-                    if (bilinear && prop_uv) {
-                        // d val / d(u, v) through the bilinear weights
-                        // (texture_helpers.cuh:262-299), tap differences first so
-                        // coincident taps give exactly zero
-                        vu += (float)s.h * (vval[c] * ((1.f - fv) * (c2 - c0) + fv * (c3 - c1)));
-                        vv += (float)s.w * (vval[c] * ((1.f - fu) * (c1 - c0) + fu * (c3 - c2)));
-                    }
-                    vvis += val * vtex[c];
                 }
+            }
+            vvis += (wt[0] * D[0] + wt[1] * D[1]) + (wt[2] * D[2] + wt[3] * D[3]);
+            if (bilinear && prop_uv) {
+                vu = vis * (float)s.h * ((1.f - fv) * (D[2] - D[0]) + fv * (D[3] - D[1]));
+                vv = vis * (float)s.w * ((1.f - fu) * (D[1] - D[0]) + fu * (D[3] - D[2]));
             }
 
             if (!valid) vu = vv = 0.f;  // NaN-safe zero seeds
