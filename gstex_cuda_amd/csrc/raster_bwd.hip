@@ -53,6 +53,21 @@ __device__ __forceinline__ void window_texel_add(const float (&A)[3], const floa
     }
 }
 
+// Window weights along one axis: weight w0 at window index i and w1 at j,
+// summed where they meet, i.e. W[r] = (i == r ? w0 : 0) + (j == r ? w1 : 0).
+// On a lane whose taps lie in the window, i is 0 or 1 and j is i or i + 1
+// (the second tap clamps onto the first at the chart edge), which leaves the
+// four cases below; other lanes get finite weights of no meaning (their
+// texel terms are zero).
+__device__ __forceinline__ void window_weights(float w0, float w1, int i, int j, float (&W)[3]) {
+    const bool same = j == i;
+    const float first = same ? w0 + w1 : w0, second = same ? 0.f : w1;
+    const bool at0 = i == 0;
+    W[0] = at0 ? first : 0.f;
+    W[1] = at0 ? second : first;
+    W[2] = at0 ? 0.f : second;
+}
+
 // raster_backward_wave runs one wave's share of a tile (wave `wave` of the
 // tile, as in the forward); `acc` is this hardware wave's LDS texel
 // accumulator, zero on entry and left zero.
@@ -335,14 +350,10 @@ __device__ __forceinline__ void raster_backward_wave(const RasterArgs& a, const 
                 // separable weights: tap (row r, column q) has weight
                 // wa[r] * wb[q]; rows iu, ju and columns iv, jv relative to
                 // the window origin lie in 0..2 (ju = iu at the chart edge)
-                const float wa0 = 1.f - fu, wa1 = fu, wb0 = 1.f - fv, wb1 = fv;
                 const int ri = iu - ou, rj = ju - ou, ci = iv - ov, cj = jv - ov;
                 float A[3], Bw[3];
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    A[r] = (ri == r ? wa0 : 0.f) + (rj == r ? wa1 : 0.f);
-                    Bw[r] = (ci == r ? wb0 : 0.f) + (cj == r ? wb1 : 0.f);
-                }
+                window_weights(1.f - fu, fu, ri, rj, A);
+                window_weights(1.f - fv, fv, ci, cj, Bw);
                 // when no valid lane's taps reach window row (column) 2, its
                 // weights are zero on every lane and the window is 2 x 3 (3 x 2);
                 // not for 4 channels, whose kernel has no registers to spare
