@@ -230,8 +230,9 @@ __device__ __forceinline__ Pair pair_eval(const Splat& s, const PixelRay& pr, bo
     Pair p;
     float den = dot3(s.a3, pr.r);
     const float eps = 1e-6f;
-    if (0.f <= den && den < eps) den = eps;
-    else if (-eps < den && den <= 0.f) den = -eps;
+    // |den| below eps becomes eps with den's sign, +eps for both zeros
+    // (den + 0 is +0 for -0): den in [0, eps) -> eps, (-eps, 0) -> -eps
+    den = fabsf(den) < eps ? copysignf(eps, den + 0.f) : den;
     p.den = den;
     if (precise_edge && fabsf(den) < kEdgeOnDen) {
         const double ax = s.a3.x, ay = s.a3.y, az = s.a3.z;
@@ -335,8 +336,9 @@ __device__ __forceinline__ void chart_taps_ij(const Splat& s, const Chart& ch, f
                                               float& fv, int& iu, int& iv, int& ju, int& jv) {
     const float tu = (float)s.h * u, tv = (float)s.w * v;
     const int iu0 = f2i_trunc(tu), iv0 = f2i_trunc(tv);
-    fu = tu - (float)iu0;
-    fv = tv - (float)iv0;
+    // tu, tv >= 0: the part above the truncation, exact (tu - floor(tu))
+    fu = __builtin_amdgcn_fractf(tu);
+    fv = __builtin_amdgcn_fractf(tv);
     iu = min(iu0, s.h - 1);
     iv = min(iv0, s.w - 1);
     ju = min(iu0 + 1, s.h - 1);
