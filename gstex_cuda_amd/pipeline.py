@@ -10,6 +10,8 @@ recorded around it.
 """
 from __future__ import annotations
 
+import ctypes
+import sys
 from dataclasses import dataclass
 from typing import Dict, Optional
 
@@ -35,30 +37,78 @@ class Camera:
     img_w: int
 
 
+# hip_runtime_api.h: an event created with hipEventDisableSystemFence skips
+# the system-scope fence when it is recorded, i.e. the cache write-back and
+# invalidation that otherwise leave the GPU idle for a few microseconds
+# between the kernels the event brackets.  Such events only time: nothing
+# waits on them and no host read depends on them.
+_HIP_EVENT_DISABLE_SYSTEM_FENCE = 0x20000000
+
+
+def _hip_fn(name: str, *argtypes):
+    """A HIP runtime entry, resolved through the library's own dependency
+    (the runtime its kernels run on)."""
+    fn = getattr(_lib.lib(), name)
+    fn.restype = ctypes.c_int
+    fn.argtypes = list(argtypes)
+    return fn
+
+
 class StageTimer:
-    """HIP-event pairs per stage, recorded on the current stream."""
+    """HIP-event pairs per stage, recorded on the current stream (timing-only
+    events, without the system-scope fence)."""
 
     def __init__(self):
         self.events: Dict[str, list] = {}
+        vp = ctypes.c_void_p
+        self._create = _hip_fn("hipEventCreateWithFlags", ctypes.POINTER(vp), ctypes.c_uint)
+        self._record = _hip_fn("hipEventRecord", vp, vp)
+        self._sync = _hip_fn("hipEventSynchronize", vp)
+        self._elapsed = _hip_fn("hipEventElapsedTime", ctypes.POINTER(ctypes.c_float), vp, vp)
+        self._destroy = _hip_fn("hipEventDestroy", vp)
+
+    def _mark(self) -> ctypes.c_void_p:
+        ev = ctypes.c_void_p()
+        if self._create(ctypes.byref(ev), _HIP_EVENT_DISABLE_SYSTEM_FENCE) != 0:
+            raise RuntimeError("hipEventCreateWithFlags failed")
+        if self._record(ev, torch.cuda.current_stream().cuda_stream) != 0:
+            self._destroy(ev)
+            raise RuntimeError("hipEventRecord failed")
+        return ev
 
     def start(self, name: str):
-        ev = torch.cuda.Event(enable_timing=True)
-        ev.record()
-        self.events.setdefault(name, []).append([ev, None])
+        self.events.setdefault(name, []).append([self._mark(), None])
 
     def stop(self, name: str):
-        ev = torch.cuda.Event(enable_timing=True)
-        ev.record()
-        self.events[name][-1][1] = ev
+        self.events[name][-1][1] = self._mark()
 
     def mean_ms(self, name: str) -> float:
         pairs = self.events.get(name, [])
         if not pairs:
             return float("nan")
-        return sum(a.elapsed_time(b) for a, b in pairs) / len(pairs)
+        total = 0.0
+        for a, b in pairs:
+            ms = ctypes.c_float()
+            if self._sync(b) != 0 or self._elapsed(ctypes.byref(ms), a, b) != 0:
+                raise RuntimeError("hipEventElapsedTime failed")
+            total += ms.value
+        return total / len(pairs)
 
     def clear(self):
+        for pairs in self.events.values():
+            for pair in pairs:
+                for ev in pair:
+                    if ev is not None:
+                        self._destroy(ev)
         self.events.clear()
+
+    def __del__(self):
+        if sys.is_finalizing():  # the runtime may already be shutting down
+            return
+        try:
+            self.clear()
+        except Exception:
+            pass
 
 
 class RenderStep:
