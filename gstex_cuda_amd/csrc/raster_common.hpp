@@ -215,6 +215,9 @@ __device__ __forceinline__ uint64_t lanes_where(bool c) {
 }
 __device__ __forceinline__ bool any_lane(bool c) { return lanes_where(c) != 0; }
 
+// log2(e) rounded to float32 (0x3fb8aa3b), the factor of __expf
+constexpr float kLog2E = 1.44269502f;
+
 struct Pair {
     V3 delta;
     float den, t, l1, l2, G, Gb, alpha;
@@ -257,11 +260,14 @@ __device__ __forceinline__ Pair pair_eval(const Splat& s, const PixelRay& pr, bo
         p.l1 = dot3(p.delta, s.a1);
         p.l2 = dot3(p.delta, s.a2);
     }
-    const float sigma = 0.5f * (s.iq1 * p.l1 * p.l1 + s.iq2 * p.l2 * p.l2);
-    p.G = __expf(-sigma);
+    // exp(-sigma), sigma = q / 2, as v_exp of q * (-log2(e) / 2): the same
+    // float product as (q / 2) * -log2(e) (halving is exact), one multiply
+    const float q = s.iq1 * p.l1 * p.l1 + s.iq2 * p.l2 * p.l2;
+    p.G = __builtin_amdgcn_exp2f(q * (-0.5f * kLog2E));
     p.Gb = 0.f;
     p.blur_sel = false;
     if (use_blur) {
+        const float sigma = 0.5f * q;
         const float dx = s.pix_x - pr.px, dy = s.pix_y - pr.py;
         const float sigma_blur = 0.5f * 2.0f * (dx * dx + dy * dy);
         p.Gb = __expf(-sigma_blur);
