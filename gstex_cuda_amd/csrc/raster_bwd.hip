@@ -174,9 +174,6 @@ __device__ __forceinline__ void raster_backward_wave(const RasterArgs& a, const 
         const int hw = ch.ok ? s.h * s.w : 0;
         const bool tex_single = hw == 1 && C >= 1 && C <= 4;
         const int row0 = __builtin_amdgcn_readlane(row[0], first);
-        const int row1 = __builtin_amdgcn_readlane(row[1], first);
-        const int row2 = __builtin_amdgcn_readlane(row[2], first);
-        const int row3 = __builtin_amdgcn_readlane(row[3], first);
         const bool quad32 = ch.ok && !tex_single && C >= 1 && C <= 3 &&
                             !(lanes_where(row[0] != row0) & vmask);
         // window origin (ou, ov): the anchor lane's tap row / column, one
@@ -196,14 +193,12 @@ __device__ __forceinline__ void raster_backward_wave(const RasterArgs& a, const 
         constexpr int NB = G_EXTRA + (GENERAL ? 2 : 0);
         float gr[NB];
         float vval[CM];  // vis * dL/dtexture_out per channel
-        float wsum;
         {
             gr[G_RGB + 0] = vis * vo.x;
             gr[G_RGB + 1] = vis * vo.y;
             gr[G_RGB + 2] = vis * vo.z;
             float vvis = dot3(s.rgb, vo) + dot3(s.a3, vn);
             float vu = 0.f, vv = 0.f;
-            wsum = (wt[0] + wt[1]) + (wt[2] + wt[3]);
             // D[k] = sum over channels of vtex[c] * (tap k's texel, channel c).
             // The sampled value's share of vvis is sum_k wt[k] D[k], and
             // d val / d(u, v) through the bilinear weights
@@ -326,6 +321,7 @@ __device__ __forceinline__ void raster_backward_wave(const RasterArgs& a, const 
         if constexpr (CM <= 4) {
             if (tex_single) {
                 float vals[CM];
+                const float wsum = (wt[0] + wt[1]) + (wt[2] + wt[3]);
 #pragma unroll
                 for (int c = 0; c < CM; ++c) vals[c] = wsum * vval[c];
                 const float tot = wave_reduce_scatter(vals, lane);
@@ -342,6 +338,11 @@ __device__ __forceinline__ void raster_backward_wave(const RasterArgs& a, const 
                     for (int c = 0; c < QC; ++c) vals[QC * k + c] = wt[k] * vval[c];
                 const float tot = wave_reduce_scatter(vals, lane);
                 const int code = rs_slot<4 * QC>(lane);
+                // the anchor lane's other tap rows (every valid lane's quad),
+                // read here while every lane is active
+                const int row1 = __builtin_amdgcn_readlane(row[1], first);
+                const int row2 = __builtin_amdgcn_readlane(row[2], first);
+                const int row3 = __builtin_amdgcn_readlane(row[3], first);
                 if (code < 64 && tot != 0.f) {
                     const int k = code / QC, c = code - QC * k;
                     const int rr = k == 0 ? row0 : k == 1 ? row1 : k == 2 ? row2 : row3;
