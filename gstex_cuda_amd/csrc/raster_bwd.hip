@@ -177,13 +177,17 @@ __device__ __forceinline__ void raster_backward_wave(const RasterArgs& a, const 
         const bool quad32 = ch.ok && !tex_single && C >= 1 && C <= 3 &&
                             !(lanes_where(row[0] != row0) & vmask);
         // window origin (ou, ov): the anchor lane's tap row / column, one
-        // lower when some valid lane lies below it
-        const int au = __builtin_amdgcn_readlane(iu, first);
-        const int av = __builtin_amdgcn_readlane(iv, first);
-        const int ou = au - ((lanes_where(iu < au) & vmask) ? 1 : 0);
-        const int ov = av - ((lanes_where(iv < av) & vmask) ? 1 : 0);
-        const bool win3 = CM <= 4 && ch.ok && !tex_single && !quad32 && bilinear && C >= 1 &&
-                          !(lanes_where(((uint32_t)(iu - ou) | (uint32_t)(iv - ov)) > 1u) & vmask);
+        // lower when some valid lane lies below it (formed only where the
+        // window route is open)
+        int ou = 0, ov = 0;
+        bool win3 = false;
+        if (CM <= 4 && ch.ok && !tex_single && !quad32 && bilinear && C >= 1) {
+            const int au = __builtin_amdgcn_readlane(iu, first);
+            const int av = __builtin_amdgcn_readlane(iv, first);
+            ou = au - ((lanes_where(iu < au) & vmask) ? 1 : 0);
+            ov = av - ((lanes_where(iv < av) & vmask) ? 1 : 0);
+            win3 = !(lanes_where(((uint32_t)(iu - ou) | (uint32_t)(iv - ov)) > 1u) & vmask);
+        }
         const bool tex_lds = ch.ok && !tex_single && !quad32 && !win3 && C > 0 &&
                              hw * C <= kTexAccFloats;
         float* const vtex_chart = b.v_texture + (size_t)(ch.ok ? s.start : 0) * C;
